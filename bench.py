@@ -1,0 +1,232 @@
+#!/usr/bin/env python3
+"""Benchmark: certified BM-ADMM solves per second at an absolute gap of 1e-5 on one MI355X.
+
+Workload (the metric's configuration): n = 18 alternatives, 1 % of the 18 * 2^17 (menu,
+alternative) cells observed (config2 of solver/instances.py, seed 18001: ~23.6k cells, p_hat from
+a 200-order Dirichlet mixture plus 500-reply multinomial noise). A step is one certified solve
+through the unchanged surface (Session.solve with abs_tol = 1e-5, rel_tol = 0) from a cold start:
+the bracket [lb, ub] with ub - lb <= 1e-5 is float64 (push and Frank-Wolfe bound) every check.
+
+    python bench.py [--steps K] [--warmup W] [--dtype mixed|float64] [--plain-steps P] [--no-cpu]
+
+Reported beside the value (one JSON line on stdout):
+  plain      the same certified solve with the reference's own schedule in float64 (separate
+             elementwise kernels, four full butterflies per iteration) on this GPU: tuned vs plain
+  roofline   the dominant fused pass: algorithmic bytes per launch / its mean device time, from
+             HIP events on the engine's stream right after the timed region (rumgpu_bm_profile)
+  cpu_baseline  the CPU oracle (oracle/rum_oracle.cpp, one core) on a bounded sample of the same
+             solve: a few float64 iterations plus one push and one lower bound, scaled to a
+             certified solve by the GPU run's iteration and check counts
+Inputs are resident on the GPU before the timed region (the engines are built in warmup); each
+solve call returns after its last stream synchronization, so the host clock brackets the device
+work exactly.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (os.path.join(ROOT, "rum-testing-gpu_amd"), ROOT):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+METRIC = "certified solves/s at abs gap 1e-5 (n=18); bootstrap draws/s, tuned vs plain"
+HBM_PEAK_GBS = 8000.0
+
+
+def _parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--dtype", default="mixed", choices=["mixed", "float64", "float32"])
+    ap.add_argument("--plain-steps", type=int, default=1)
+    ap.add_argument("--abs-tol", type=float, default=1e-5)
+    ap.add_argument("--max-time", type=float, default=120.0)
+    ap.add_argument("--profile-iters", type=int, default=200)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-iters", type=int, default=20)
+    return ap.parse_args()
+
+
+def _pass_bytes(kind, m2, isz, n, M):
+    """Algorithmic HBM bytes of one launch of a fused pass (slab transfers x m2 x itemsize, plus
+    the per-menu observation arrays of the prox and inv_gl of the D pass)."""
+    transfers = {0: 6, 1: 4, 2: 2, 3: 5, 4: 3, 5: 2, 6: 9}[kind]
+    extra = 0
+    if kind == 4:
+        extra = 8 * (1 << n) + 8 * M
+    if kind == 2:
+        extra = isz * (m2 // n)
+    return transfers * m2 * isz + extra
+
+
+KIND_NAMES = {0: "bm_pass SY (A window 1, s/y update, A' window 1)", 1: "bm_pass RHS (A', rhs, Q^T on window 2)",
+              2: "bm_pass D (Q^T, inv_gl, Q on window 1)", 3: "bm_pass XK (Q, xk, A on window 2)",
+              4: "bm_prox (per-menu projection, z update)", 5: "bm_pass T (middle window)",
+              6: "bm_pass FULL (one window)"}
+
+
+def main():
+    a = _parse()
+    if a.gpus != 1:
+        print("bench.py measures one GPU: this build runs on a single MI355X and has no multi-GPU path "
+              "(--gpus %d requested)" % a.gpus, file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    from solver import instances, _lib
+    from solver.rum_solver import RUM
+    from solver.bm_admm.engine import Handle, solve_kadmm
+
+    cd, n = instances.config2()
+    rum = RUM.from_rows(cd, n)
+    p = rum.choice_probability
+    mp = rum._geom.M
+    m2 = n << (n - 1)
+    print("[bench] config2: n=%d, %d cells (%.3f %% of %d), fingerprint %s" % (n, mp, 100.0 * mp / m2, m2,
+                                                                             instances.fingerprint(cd)),
+          file=sys.stderr, flush=True)
+
+    sess = rum.session(method="bm_admm", method_args={"dtype": a.dtype})
+    results = []
+    for i in range(a.warmup):
+        r = sess.solve(p, abs_tol=a.abs_tol, rel_tol=0.0, max_time=a.max_time)
+        print("[bench] warmup %d: flag %d, [%.10f, %.10f], %d it, %.3f s" % (i, r.exit_flag, r.lb, r.ub, r.iterations,
+                                                                            r.time), file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        r = sess.solve(p, abs_tol=a.abs_tol, rel_tol=0.0, max_time=a.max_time)
+        results.append(r)
+        print("[bench] step %d: flag %d, [%.10f, %.10f] gap %.2e, %d it (%s), %.3f s"
+              % (i, r.exit_flag, r.lb, r.ub, r.ub - r.lb, r.iterations,
+                 {k: r.info.get(k) for k in ("iters32", "iters64")}, r.time), file=sys.stderr, flush=True)
+    elapsed = time.perf_counter() - t0
+    certified = all(r.exit_flag in (1, 3) for r in results)
+    value = a.steps / elapsed if certified else 0.0
+    r_last = results[-1]
+
+    # roofline of the dominant fused pass (profiled on the session's handles, same state)
+    eng = sess._engines["bm"].items
+    prof = {}
+    import ctypes
+    for dname, h in eng.items():
+        ms = (ctypes.c_double * 7)()
+        cnt = (ctypes.c_int64 * 7)()
+        _lib.check(h.lib.rumgpu_bm_profile(h.h, a.profile_iters, ms, cnt), "bm_profile")
+        prof[dname] = {k: (ms[k], cnt[k]) for k in range(7) if cnt[k]}
+    # the handle that ran the most iterations in the timed solves
+    it32 = sum(int(r.info.get("iters32") or 0) for r in results)
+    it64 = sum(int(r.info.get("iters64") or 0) for r in results)
+    dom_dtype = ("float32" if it32 >= it64 else "float64") if a.dtype == "mixed" else a.dtype
+    isz = 4 if dom_dtype == "float32" else 8
+    kinds = prof[dom_dtype]
+    dom = max(kinds, key=lambda k: kinds[k][0])
+    avg_ms = kinds[dom][0] / kinds[dom][1]
+    alg = _pass_bytes(dom, m2, isz, n, mp)
+    achieved = alg / (avg_ms * 1e-3) / 1e9
+    iter_ms = sum(v[0] for v in kinds.values()) / a.profile_iters
+    iter_bytes = sum(_pass_bytes(k, m2, isz, n, mp) * v[1] for k, v in kinds.items()) / a.profile_iters
+    passes = {KIND_NAMES[k]: dict(avg_us=round(1e3 * v[0] / v[1], 3), launches=v[1],
+                                  gbs=round(_pass_bytes(k, m2, isz, n, mp) / (v[0] / v[1] * 1e-3) / 1e9, 1))
+              for k, v in kinds.items()}
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "traffic_%s_n18.json" % dom_dtype)
+    if os.path.exists(tf):
+        try:
+            traffic = json.load(open(tf)).get(str(dom))
+        except Exception:
+            traffic = None
+    roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, kernel=KIND_NAMES[dom],
+                    dtype=dom_dtype, alg_bytes_per_launch=alg, avg_launch_us=round(avg_ms * 1e3, 3),
+                    iteration=dict(us=round(iter_ms * 1e3, 2), alg_bytes=int(iter_bytes),
+                                   gbs=round(iter_bytes / (iter_ms * 1e-3) / 1e9, 1)),
+                    passes=passes)
+    sess.close()
+
+    # plain: the reference schedule in float64, same hooks and stops
+    plain = None
+    if a.plain_steps > 0:
+        h = Handle(n, rum._geom.menu, rum._geom.alternative.astype(np.int32), p[rum._rest], dtype="float64",
+                   beta=r_last.method_args["beta"], plain=True)
+        st = dict(ub=math.inf, lb=0.0)
+
+        def push(hh):
+            u = hh.push()
+            st["ub"] = min(st["ub"], u)
+            return u
+
+        def lower(hh):
+            st["lb"] = max(st["lb"], hh.lb())
+            return st["lb"]
+
+        times, iters = [], []
+        for i in range(a.plain_steps):
+            st.update(ub=math.inf, lb=0.0)
+            h.init_cold()
+            tp = time.perf_counter()
+            rp = solve_kadmm(h, max_iter=10 ** 7, check=50, push_fn=push, lb_fn=lower, ub_arm_immediately=True,
+                             ub_running_min=True, gap_abs_tol=a.abs_tol, gap_rel_tol=0.0,
+                             time_limit=a.max_time * 3)
+            times.append(time.perf_counter() - tp)
+            iters.append(rp["iters"])
+            print("[bench] plain %d: %s, [%.10f, %.10f], %d it, %.3f s" % (i, rp["stop_reason"], rp["lb"], rp["ub"],
+                                                                         rp["iters"], times[-1]),
+                  file=sys.stderr, flush=True)
+        h.close()
+        plain = dict(value=round(len(times) / sum(times), 5), unit="certified solves/s", dtype="f64",
+                     schedule="reference (plain kernels)", iterations=iters, seconds=[round(t, 3) for t in times],
+                     certified=bool(rp["stop_reason"] == "gap_abs"))
+
+    cpu = None
+    if not a.no_cpu:
+        cpu = _cpu_baseline(n, rum, p, r_last, a)
+
+    line = dict(metric=METRIC, value=round(value, 5), unit="certified solves/s", n_gpus=1, steps=a.steps,
+                warmup=a.warmup, ms_per_step=round(1e3 * elapsed / a.steps, 2), higher_is_better=True,
+                scaling="weak", vs_baseline=None, dtype={"mixed": "f32+f64", "float64": "f64", "float32": "f32"}[a.dtype],
+                data="synthetic",
+                config=dict(workload="certified BM-ADMM solve (abs gap 1e-5, rel 0), n=18, 1% density "
+                                     "(config2 seed 18001, %d cells)" % mp, n=n, cells=mp, abs_tol=a.abs_tol,
+                            beta=r_last.method_args["beta"], check=50,
+                            iterations=[r.iterations for r in results],
+                            iters32=[r.info.get("iters32") for r in results],
+                            iters64=[r.info.get("iters64") for r in results],
+                            bracket=[r_last.lb, r_last.ub], certified=certified),
+                roofline=roofline, cpu_baseline=cpu, plain=plain,
+                tuned_vs_plain=round(value / plain["value"], 3) if plain and plain["value"] > 0 else None)
+    print(json.dumps(line), flush=True)
+
+
+def _cpu_baseline(n, rum, p, r_last, a):
+    """The oracle on one host core: cpu_iters float64 iterations, one push and one lower bound of
+    the same instance, scaled to one certified solve with the GPU solve's counts."""
+    import numpy as np
+    from oracle import oracle as O
+    g = rum._geom
+    menu = g.menu
+    alt = g.alternative.astype(np.int32)
+    pr = p[rum._rest]
+    st = O.AdmmState(n, menu, alt, pr, r_last.method_args["beta"])
+    t0 = time.perf_counter()
+    st.run(a.cpu_iters)
+    t_it = (time.perf_counter() - t0) / a.cpu_iters
+    t1 = time.perf_counter()
+    O.push_ub64(n, st.x2, st.obs, pr)
+    O.fw_lb64(n, st.obs, st.x2[st.obs], pr)
+    t_chk = time.perf_counter() - t1
+    iters = r_last.iterations
+    checks = max(1, iters // 50)
+    t_solve = iters * t_it + checks * t_chk
+    return dict(value=round(1.0 / t_solve, 8), unit="certified solves/s", cores=1, kind="port",
+                sample="%d float64 iterations + 1 push + 1 lower bound of the n=18 1%% instance (%.1f s on 1 core), "
+                       "scaled to %d iterations and %d checks" % (a.cpu_iters, a.cpu_iters * t_it + t_chk, iters, checks),
+                seconds_per_iteration=round(t_it, 4), seconds_per_check=round(t_chk, 4),
+                host_cpus=os.cpu_count())
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,117 @@
+/* librumgpu: the MI355X (gfx950) BM-ADMM and Flow-CG engines of the RUM distance, as a C ABI.
+ *
+ * Plain pointers and sizes only. Every entry returns an int status:
+ *   0 ok, 1 invalid argument, 2 out of device memory, 3 HIP error, 4 unsupported size;
+ * rumgpu_last_error() returns the message of the last failure in this process.
+ * Host arrays are read or written synchronously before the call returns; device memory stays
+ * behind the opaque handles. One context = one device and one HIP stream; a handle is not
+ * thread-safe. Row arrays: menu = bitmask of the offered set (bit a = alternative a), alt = the
+ * chosen alternative (0-based), one entry per observed non-singleton (menu, alternative) cell.
+ *
+ * Reference interfaces each entry replaces are cited as path:line into the reference
+ * repository (mingxiz/rum-testing-gpu).
+ */
+#ifndef RUMGPU_H
+#define RUMGPU_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct rumgpu_ctx rumgpu_ctx;
+typedef struct rumgpu_bm rumgpu_bm;
+typedef struct rumgpu_cg rumgpu_cg;
+
+/* ---- context, device, errors (replaces CuPy's device/pool handling and machine.py:25-61,
+ *      machine.py:156 gpu_free_bytes) ---- */
+int rumgpu_ctx_create(int device, rumgpu_ctx** out);
+int rumgpu_ctx_destroy(rumgpu_ctx* ctx);
+int rumgpu_mem_info(int device, size_t* free_bytes, size_t* total_bytes);
+int rumgpu_device_props(int device, char* name, int name_len, char* arch, int arch_len, size_t* total_mem,
+                        int* cu_count);
+int rumgpu_device_count(int* count);
+int rumgpu_last_error(char* buf, int len);
+int rumgpu_version(void);
+
+/* ---- BM-ADMM (replaces solve_kadmm_v2, solver/bm_admm/solver.py:148-377, its kernels
+ *      :33-139, bm_admm/kernels.py:23-35 and bm_admm/butterfly2.py:206-247) ----
+ * dtype_bytes 8 = float64 iterates, 4 = float32 iterates. (e0, e1, Q row-major) = eigh of
+ * [[1,-1],[-1,1+p^2]] and gamma = beta2/beta1 (solver.py:206-214). flags bit 0 = the plain
+ * reference schedule (separate elementwise kernels, four full butterflies per iteration)
+ * instead of the fused window passes. The object starts cold: x2 = prox(xhat), s = y = z = 0
+ * (solver.py:264-267). */
+int rumgpu_bm_create(rumgpu_ctx* ctx, int n, const uint64_t* menu, const int32_t* alt, const double* p, int64_t M,
+                     int dtype_bytes, double beta, double alpha, double precond_p, double e0, double e1, double gamma,
+                     const double* Q, int flags, rumgpu_bm** out);
+int rumgpu_bm_destroy(rumgpu_bm* bm);
+/* new probabilities on the same rows (a bootstrap draw): Session.solve(p), rum_solver.py:598 */
+int rumgpu_bm_set_p(rumgpu_bm* bm, const double* p);
+/* cold start: solver.py:264-267 */
+int rumgpu_bm_init_cold(rumgpu_bm* bm);
+/* state (x2, s, y, z) as float64 host slabs of n 2^(n-1) values: init_state, solver.py:256-262 */
+int rumgpu_bm_set_state(rumgpu_bm* bm, const double* x2, const double* s, const double* y, const double* z);
+int rumgpu_bm_get_state(rumgpu_bm* bm, double* x2, double* s, double* y, double* z);
+/* device-side state copy between two objects on the same rows (dtype conversion allowed): the
+ * float32 -> float64 handover of solve_kadmm_hybrid, solver.py:380-425 */
+int rumgpu_bm_copy_state(rumgpu_bm* dst, rumgpu_bm* src);
+/* keep the current state on the device / restart from it: Session keep= / warm=,
+ * rum_solver.py:922-924 (_run_bm_admm keep) and :914-925 (_bm_session warm) */
+int rumgpu_bm_keep(rumgpu_bm* bm);
+int rumgpu_bm_load_kept(rumgpu_bm* bm);
+/* run iterations (one_iter, solver.py:240-254); the state afterwards may hold a deferred s, y
+ * update, settled by rumgpu_bm_finalize or any state read */
+int rumgpu_bm_run(rumgpu_bm* bm, int64_t iters);
+/* capture `len` steady-state iterations as one HIP graph, replayed by rumgpu_bm_run when the
+ * count is a multiple of len (solver.py:269-292); len 0 drops it */
+int rumgpu_bm_set_graph(rumgpu_bm* bm, int64_t len);
+int rumgpu_bm_finalize(rumgpu_bm* bm);
+/* phi = ||x2[obs] - p|| (solver.py:321), compensated float64 sum */
+int rumgpu_bm_phi(rumgpu_bm* bm, double* phi);
+/* float64 push to a feasible RUM point: ub and its fitted probabilities (input row order, may
+ * be NULL). push_to_ub64, common/transforms.py:94-98 + arcflow_kernels.py:415-434 */
+int rumgpu_bm_push_ub64(rumgpu_bm* bm, double* ub, double* p_fit);
+/* the fitted probabilities of the last push */
+int rumgpu_bm_get_fit(rumgpu_bm* bm, double* p_fit);
+/* float64 Frank-Wolfe lower bound at x2: fw_lb_rows_gpu, common/transforms.py:101-114 */
+int rumgpu_bm_lb64(rumgpu_bm* bm, double* lb);
+/* x2 at the observed rows (input order), float64 */
+int rumgpu_bm_x_obs(rumgpu_bm* bm, double* out);
+int rumgpu_bm_info(rumgpu_bm* bm, int64_t* iters, int* nwin, int* win_bits);
+/* device time per fused pass kind over `iters` iterations (HIP events between launches on the
+ * context stream): ms_by_kind[7], launches_by_kind[7] for SY, RHS, D, XK, prox, middle-window
+ * transform, one-window pass. No reference counterpart: the measurement behind the roofline. */
+int rumgpu_bm_profile(rumgpu_bm* bm, int64_t iters, double* ms_by_kind, int64_t* launches_by_kind);
+/* building blocks exposed for tests: one full (a,b,c,d) butterfly of a host slab
+ * (common/butterfly.py:99-140 butterfly_fast), and the prox of a host slab (solver.py:66-139) */
+int rumgpu_butterfly64(rumgpu_ctx* ctx, int n, double* v, double a, double b, double c, double d);
+int rumgpu_butterfly32(rumgpu_ctx* ctx, int n, float* v, float a, float b, float c, float d);
+int rumgpu_prox_slab(rumgpu_bm* bm, const double* xk, double* out);
+
+/* float64 Frank-Wolfe lower bound at any point y on the rows, with its own scratch:
+ * fw_lb_rows_gpu, common/transforms.py:101-114 (flow_cg's final bound, rum_solver.py:928-938) */
+int rumgpu_lb64(rumgpu_ctx* ctx, int n, const uint64_t* menu, const int32_t* alt, int64_t M, const double* y,
+                const double* p, double* lb);
+
+/* ---- Flow-CG pricing (replaces bf_update, flow_cg/bf_pricing.py:8-47 with the sweep and
+ *      backtrace of flow_cg/solver.py:168-200, and StreamPricer, flow_cg/pricing_stream.py:267-412)
+ * Rows are given x-major (stable by alternative, cg_core.py:184-195): offsets[x]..offsets[x+1]
+ * are alternative x's rows, masks their menus.
+ * layout 0 = mask (one float32 per subset, the reference's row-order sums: exact for any scores)
+ * layout 1 = slab (scores scattered on the slab and summed over subsets by the butterfly, then
+ *            the layered max: the same values as mask when every partial sum is exact)
+ * layout 2 = stream (two live layers by colex rank), with dtype 0 f32, 1 u16, 2 u24 storage;
+ * k_pred: first layer whose predecessors are stored (stream), <= 0 = n - 10. */
+int rumgpu_cg_create(rumgpu_ctx* ctx, int n, const uint64_t* masks_x, const uint64_t* offsets, int64_t M, int layout,
+                     int dtype, int k_pred, rumgpu_cg** out);
+int rumgpu_cg_destroy(rumgpu_cg* cg);
+/* one pricing sweep: scores (float32, x-major) -> vmax (float64 decode), rank_pos[n] (position
+ * of each alternative, 0 = best), margin (u16/u24 quantization bound, 0 otherwise) */
+int rumgpu_cg_price(rumgpu_cg* cg, const float* scores_x, double* vmax, int32_t* rank_pos, double* margin);
+int rumgpu_cg_info(rumgpu_cg* cg, size_t* device_bytes, double* last_kernel_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,190 @@
+// Window passes of the generalized 2x2 butterfly over the subset lattice.
+//
+// A slab holds n item blocks of Nloc = 2^nb values (nb = n - 1 rest bits). A Kronecker
+// transform applies the same 2x2 matrix [[a, b], [c, d]] on every rest bit. Its bit stages
+// commute, so the nb bits are cut into windows and each window is one pass over the slab:
+//
+//   window [s, s+B)   the bits the pass transforms
+//   columns [0, C)    for a window above bit 0 (s >= C): 2^C contiguous values per row, so a
+//                     wavefront reads 2^C-element runs of memory (128 B) instead of one value
+//   outer bits        the rest: one tile per (item, outer bits), tile = 2^(B+C) values
+//
+// Inside a tile each thread holds R = 2^RL values that differ only in RL window bits, so RL
+// stages run in registers; the next RL bits are reached by one exchange through LDS. A window
+// of B bits costs ceil(B/RL) register rounds and ceil(B/RL) - 1 exchanges. Several transforms
+// and the pointwise ADMM updates between them run in one pass while the tile is resident
+// (the fused passes are in bm_admm.hip).
+#pragma once
+#include "common.hpp"
+
+struct WinGeom {
+  int nb;        // rest bits
+  int s;         // first window bit
+  int B;         // window bits
+  int C;         // column bits (low contiguous bits, 0 for the window at s = 0)
+  int TT;        // threads per tile = 2^(B + C - RL)
+  int TPB;       // tiles per block
+  i64 Nloc;      // 2^nb
+  i64 ntiles;    // n * 2^(nb - B - C)
+};
+
+// Host-side window plan for a slab of itemsize bytes: list of (s, B, C, RL).
+struct WinPlan {
+  int s, B, C, RL;
+};
+
+inline std::vector<WinPlan> plan_windows(int nb, int itemsize) {
+  std::vector<WinPlan> out;
+  if (nb <= 0) return out;
+  // bit 0 window up to 10 bits (a 1024-value contiguous tile); windows above it up to 8 bits
+  // with 16-32 columns, so every tile fits 36 KiB of LDS.
+  int k = nb <= 10 ? 1 : 1 + (nb - 10 + 7) / 8;
+  int lo = (nb + k - 1) / k;
+  if (lo < nb - 8 * (k - 1)) lo = nb - 8 * (k - 1);
+  if (lo > 10) lo = 10;
+  int rem = nb - lo;
+  out.push_back({0, lo, 0, lo < 3 ? lo : 3});
+  int s = lo;
+  for (int i = 1; i < k; ++i) {
+    int left = k - i;
+    int B = (rem + left - 1) / left;
+    int Cw = itemsize == 8 ? 4 : 5;
+    if (Cw > s) Cw = s;
+    int RL = B < 4 ? B : 4;
+    out.push_back({s, B, Cw, RL});
+    s += B;
+    rem -= B;
+  }
+  return out;
+}
+
+inline WinGeom make_geom(int n, const WinPlan& w) {
+  WinGeom g;
+  g.nb = n - 1;
+  g.s = w.s;
+  g.B = w.B;
+  g.C = w.C;
+  g.TT = 1 << (w.B + w.C - w.RL);
+  g.TPB = g.TT >= 256 ? 1 : 256 / g.TT;
+  g.Nloc = 1ll << g.nb;
+  g.ntiles = (i64)n << (g.nb - w.B - w.C);
+  if (g.ntiles < g.TPB) g.TPB = (int)g.ntiles;
+  if (g.TPB < 1) g.TPB = 1;
+  return g;
+}
+
+__host__ __device__ inline size_t tile_lds_elems(const WinGeom& g) {
+  size_t e = (size_t)1 << (g.B + g.C);
+  return e + (e >> 4) + 1;
+}
+
+template <typename T>
+struct Coef {
+  T a, b, c, d;
+};
+
+// One tile of a window pass held in registers. The layout index `cur` names the register
+// round whose RL window bits the thread currently holds.
+template <typename T, int RL>
+struct Tile {
+  static constexpr int R = 1 << RL;
+  T v[R];
+  int s, B, C, nr, cur;
+  int ot, col;        // thread's window bits outside the round, and its column
+  i64 base;           // slab index of the tile's (w = 0, col = 0) element
+  i64 restbase;       // base minus the item offset (the rest index of that element)
+  int item;           // the tile's item (alternative)
+  T* lds;
+  bool active;
+
+  __device__ __forceinline__ void init(const WinGeom& g, T* lds_all) {
+    s = g.s;
+    B = g.B;
+    C = g.C;
+    nr = (B + RL - 1) / RL;
+    cur = 0;
+    const int tl = threadIdx.x / g.TT;
+    const int t = threadIdx.x - tl * g.TT;
+    const i64 tile = (i64)blockIdx.x * g.TPB + tl;
+    active = tile < g.ntiles;
+    const int outer_bits = g.nb - B - C;
+    const i64 x = tile >> outer_bits;
+    const i64 o = tile & ((1ll << outer_bits) - 1);
+    const int lowo = s - C;
+    const i64 lowpart = o & ((1ll << lowo) - 1);
+    const i64 highpart = o >> lowo;
+    restbase = (lowpart << C) | (highpart << (s + B));
+    base = x * g.Nloc + restbase;
+    item = (int)x;
+    col = t & ((1 << C) - 1);
+    ot = t >> C;
+    lds = lds_all + (size_t)tl * tile_lds_elems(g);
+  }
+  __device__ __forceinline__ int lo_of(int r) const {
+    int l = r * RL;
+    return l < B - RL ? l : B - RL;
+  }
+  __device__ __forceinline__ int w_of(int r, int j) const {
+    const int lo = lo_of(r);
+    return ((ot >> lo) << (lo + RL)) | (j << lo) | (ot & ((1 << lo) - 1));
+  }
+  // slab index / rest index of register j in the current layout
+  __device__ __forceinline__ i64 gidx(int j) const { return base + ((i64)w_of(cur, j) << s) + col; }
+  __device__ __forceinline__ i64 rest(int j) const { return restbase + ((i64)w_of(cur, j) << s) + col; }
+  __device__ __forceinline__ int lidx(int r, int j) const {
+    const int L = (w_of(r, j) << C) + col;
+    return L + (L >> 4);
+  }
+  __device__ __forceinline__ void exchange(int r) {
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < R; ++j) lds[lidx(cur, j)] = v[j];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < R; ++j) v[j] = lds[lidx(r, j)];
+    cur = r;
+  }
+  __device__ __forceinline__ void stages(int r, const Coef<T>& k) {
+    const int lo = lo_of(r);
+    const int q0 = r * RL - lo;
+    const int q1e = (r + 1) * RL < B ? (r + 1) * RL : B;
+    const int q1 = q1e - lo;
+#pragma unroll
+    for (int q = 0; q < RL; ++q) {
+      if (q >= q0 && q < q1) {
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+          if (!(j & (1 << q))) {
+            const T X = v[j], Y = v[j | (1 << q)];
+            v[j] = k.a * X + k.b * Y;
+            v[j | (1 << q)] = k.c * X + k.d * Y;
+          }
+        }
+      }
+    }
+  }
+  // The whole window transform; starts from the current layout and runs the rounds in the
+  // direction that needs no exchange before the first one.
+  __device__ __forceinline__ void transform(const Coef<T>& k) {
+    const bool fwd = (cur == 0);
+    for (int i = 0; i < nr; ++i) {
+      const int r = fwd ? i : nr - 1 - i;
+      if (r != cur) exchange(r);
+      stages(r, k);
+    }
+  }
+  template <typename S>
+  __device__ __forceinline__ void load(const S* __restrict__ p) {
+    if (active) {
+#pragma unroll
+      for (int j = 0; j < R; ++j) v[j] = (T)p[gidx(j)];
+    }
+  }
+  template <typename S>
+  __device__ __forceinline__ void store(S* __restrict__ p) const {
+    if (active) {
+#pragma unroll
+      for (int j = 0; j < R; ++j) p[gidx(j)] = (S)v[j];
+    }
+  }
+};

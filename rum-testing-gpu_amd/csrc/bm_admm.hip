@@ -1,0 +1,1318 @@
+// BM-ADMM on the Block-Marschak slab, MI355X (gfx950).
+//
+// One ADMM iteration of the reference (solver/bm_admm/solver.py:240-254) is
+//   w = y - b1 s;  AT = A'w;  rhs = (z + b2 x2 - AT)/b1;  x = Q diag(inv_gl) Q^T rhs;  Ax = A x
+//   xk = a x + (1-a) x2 - z/b2;  x2n = prox(xk);  z -= b2 (a x + (1-a) x2 - x2n)
+//   s, y from Ax (fuse_sy);  x2 = x2n
+// with A', Q^T, Q, A Kronecker transforms over the n-1 rest bits. Here the transforms are cut
+// into windows (bfly.hpp) and the pointwise updates ride inside the window passes, so with two
+// windows an iteration is five passes over the slab:
+//
+//   SY  (window 1)  A on window 1 -> Ax; s, y update; w; A' on window 1      r tb s y  w tb s y
+//   RHS (window k)  A' on window k; rhs; Q^T on window k                     r tb z x2 w tb
+//   D   (window 1)  Q^T on window 1; * inv_gl; Q on window 1                  r tb      w tb
+//   XK  (window k)  Q on window k -> x; xk; A on window k                     r tb x2 z w xk tb
+//   prox            per menu: x2 = prox(xk), z = b2 (x2 - xk)                 r xk      w x2 z
+//
+// 20 slab transfers (the reference's schedule moves about 45). The A update of s, y is
+// deferred to the next iteration's first pass ("pending"). z = b2 (x2n - xk) equals the
+// reference's z -= b2 (xo - x2n) in exact arithmetic (xk = xo - z/b2).
+//
+// The certificates run in float64 whatever the iterate's dtype: the push to a feasible point
+// (common/transforms.py:94-98, arcflow_kernels.py:415-434) and the Frank-Wolfe lower bound
+// (transforms.py:101-114, arcflow_kernels.py:437-448).
+#include "bfly.hpp"
+#include "rumgpu_internal.hpp"
+#include "transform.hpp"
+
+#include <cmath>
+#include <algorithm>
+
+enum PassKind { K_FIRST, K_SY, K_SYONLY, K_RHS, K_D, K_XK, K_FULL, K_FULL_FIRST, K_T };
+
+template <typename T>
+struct PassArgs {
+  T* tb;
+  T* s;
+  T* y;
+  const T* z;
+  const T* x2;
+  T* xk;
+  const T* inv_gl;
+  T b1, b2, ib1, ib2, a;
+  Coef<T> cA, cAp, cQ, cQt, cT;
+  WinGeom g;
+};
+
+template <typename T, int RL, int KIND>
+__global__ __launch_bounds__(512) void bm_pass(PassArgs<T> P) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  Tile<T, RL> t;
+  t.init(P.g, reinterpret_cast<T*>(smem));
+  const T one = (T)1;
+  constexpr int R = Tile<T, RL>::R;
+  if constexpr (KIND == K_FIRST || KIND == K_FULL_FIRST) {
+    if (t.active) {
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
+        const i64 i = t.gidx(j);
+        t.v[j] = P.y[i] - P.b1 * P.s[i];                                    // fuse_w
+      }
+    }
+  } else {
+    t.load(P.tb);
+  }
+  if constexpr (KIND == K_SY || KIND == K_SYONLY || KIND == K_FULL) {
+    t.transform(P.cA);                                                      // Ax (raw A)
+    if (t.active) {
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
+        const i64 i = t.gidx(j);
+        const T sj = P.s[i], yj = P.y[i];
+        const T axor = -P.a * t.v[j] - (one - P.a) * sj;                     // fuse_sy
+        T sn = P.ib1 * yj - axor;
+        sn = sn > (T)0 ? sn : (T)0;
+        const T yn = yj - P.b1 * (axor + sn);
+        P.s[i] = sn;
+        P.y[i] = yn;
+        t.v[j] = yn - P.b1 * sn;                                            // next fuse_w
+      }
+    }
+    if constexpr (KIND == K_SYONLY) return;
+  }
+  if constexpr (KIND == K_FIRST || KIND == K_SY || KIND == K_RHS || KIND == K_FULL || KIND == K_FULL_FIRST)
+    t.transform(P.cAp);                                                     // raw A'
+  if constexpr (KIND == K_RHS || KIND == K_FULL || KIND == K_FULL_FIRST) {
+    if (t.active) {
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
+        const i64 i = t.gidx(j);
+        t.v[j] = (P.z[i] + P.b2 * P.x2[i] - t.v[j]) * P.ib1;                // fuse_rhs
+      }
+    }
+    t.transform(P.cQt);
+  }
+  if constexpr (KIND == K_D) t.transform(P.cQt);
+  if constexpr (KIND == K_D || KIND == K_FULL || KIND == K_FULL_FIRST) {
+    if (t.active) {
+#pragma unroll
+      for (int j = 0; j < R; ++j) t.v[j] *= P.inv_gl[t.rest(j)];            // diag_mul
+    }
+    t.transform(P.cQ);
+  }
+  if constexpr (KIND == K_XK) t.transform(P.cQ);
+  if constexpr (KIND == K_XK || KIND == K_FULL || KIND == K_FULL_FIRST) {
+    if (t.active) {
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
+        const i64 i = t.gidx(j);
+        P.xk[i] = P.a * t.v[j] + (one - P.a) * P.x2[i] - P.ib2 * P.z[i];    // fuse_xk
+      }
+    }
+  }
+  if constexpr (KIND == K_XK) t.transform(P.cA);
+  if constexpr (KIND == K_T) t.transform(P.cT);
+  t.store(P.tb);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Per-menu prox (bm_admm/solver.py:66-139). Thread = menu S (natural mask order, so for item i
+// the lanes of a wavefront touch a contiguous run of item i's block). Observed members come from
+// obsmask[S] (bit i set = row (S, i) observed) and p at pc[obsoff[S] + rank of i in obsmask].
+// The sort is a Batcher network on N registers, descending, ties to the lower item (the stable
+// insertion sort's order); the threshold sums run in double in sorted order as in the reference.
+template <int N>
+__device__ __forceinline__ void ce_desc(double (&key)[N], int (&pay)[N], int a, int b) {
+  const bool sw = (key[b] > key[a]) || (key[b] == key[a] && (pay[b] & 255) < (pay[a] & 255));
+  const double ka = key[a], kb = key[b];
+  const int pa = pay[a], pb = pay[b];
+  key[a] = sw ? kb : ka;
+  key[b] = sw ? ka : kb;
+  pay[a] = sw ? pb : pa;
+  pay[b] = sw ? pa : pb;
+}
+
+template <int N>
+__device__ __forceinline__ void sort_desc(double (&key)[N], int (&pay)[N]) {
+#pragma unroll
+  for (int p = 1; p < N; p <<= 1) {
+#pragma unroll
+    for (int k = p; k >= 1; k >>= 1) {
+#pragma unroll
+      for (int j = k % p; j + k < N; j += 2 * k) {
+#pragma unroll
+        for (int i = 0; i < k; ++i) {
+          if (i + j + k < N && (i + j) / (2 * p) == (i + j + k) / (2 * p)) ce_desc<N>(key, pay, i + j, i + j + k);
+        }
+      }
+    }
+  }
+}
+
+// MODE 0: iteration prox, writes x2 and z = b2 (x2 - xk).  MODE 1: cold start, input = xhat,
+// writes x2 only.  MODE 2: plain (reference schedule), writes x2n only.
+template <typename T, int N, int MODE>
+__global__ __launch_bounds__(256) void bm_prox(const T* __restrict__ xk, T* __restrict__ out, T* __restrict__ z,
+                                               const u32* __restrict__ obsmask, const int* __restrict__ obsoff,
+                                               const double* __restrict__ pc, T b2) {
+  const u64 S = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (S == 0 || S >= (1ull << N)) return;
+  constexpr i64 Nloc = 1ll << (N - 1);
+  const int k = __popcll(S);
+  if (k == 1) {
+    const int i = __ffsll((long long)S) - 1;
+    const i64 pos = (i64)i * Nloc;
+    out[pos] = (T)1;
+    if constexpr (MODE == 0) z[pos] = b2 * ((T)1 - xk[pos]);
+    return;
+  }
+  const u32 om = obsmask[S];
+  const int off = om ? obsoff[S] : 0;
+  const bool mixed = (om != 0) && ((u64)om != S);
+  const T w0 = om ? (T)1 : (T)0;
+  T v[N], vals[N], xh[N];
+  double key[N];
+  int pay[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const bool in = (S >> i) & 1ull;
+    const bool ob = (om >> i) & 1u;
+    T xv = (T)0, xhv = (T)0;
+    if (ob) xhv = (T)pc[off + __popc(om & ((1u << i) - 1u))];
+    if (in) {
+      const i64 pos = (i64)i * Nloc + (i64)compress_bit(S & ~(1ull << i), i);
+      xv = (MODE == 1) ? xhv : xk[pos];
+    }
+    v[i] = xv;
+    xh[i] = xhv;
+    if (mixed) {
+      key[i] = in ? (double)b2 * (double)xv + 2.0 * (ob ? 1.0 : 0.0) * (double)xhv : -HUGE_VAL;
+    } else {
+      vals[i] = (b2 * xv + (T)2 * w0 * xhv) / (b2 + (T)2 * w0);
+      key[i] = in ? (double)vals[i] : -HUGE_VAL;
+    }
+    pay[i] = i | (ob ? 256 : 0);
+  }
+  sort_desc<N>(key, pay);
+  double thr = 0.0;
+  if (mixed) {
+    double stc = 0.0, sic = 0.0;
+    const double c_obs = (double)b2 + 2.0, c_un = (double)b2 + 0.0;
+#pragma unroll
+    for (int a2 = 0; a2 < N; ++a2) {
+      if (a2 < k) {
+        const double c = (pay[a2] & 256) ? c_obs : c_un;
+        stc += key[a2] / c;
+        sic += 1.0 / c;
+        const double th = (stc - 1.0) / sic;
+        if (key[a2] - th > 0.0) thr = th;
+      }
+    }
+  } else {
+    double css = 0.0;
+#pragma unroll
+    for (int a2 = 0; a2 < N; ++a2) {
+      if (a2 < k) {
+        css += key[a2];
+        const double th = (css - 1.0) / (double)(a2 + 1);
+        if (key[a2] - th > 0.0) thr = th;
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    if ((S >> i) & 1ull) {
+      const bool ob = (om >> i) & 1u;
+      double yv;
+      if (mixed) {
+        const double ti = (double)b2 * (double)v[i] + 2.0 * (ob ? 1.0 : 0.0) * (double)xh[i];
+        const double c = (double)b2 + 2.0 * (ob ? 1.0 : 0.0);
+        yv = (ti - thr) / c;
+      } else {
+        yv = (double)vals[i] - thr;
+      }
+      const T o = yv > 0.0 ? (T)yv : (T)0.0;
+      const i64 pos = (i64)i * Nloc + (i64)compress_bit(S & ~(1ull << i), i);
+      out[pos] = o;
+      if constexpr (MODE == 0) z[pos] = b2 * (o - v[i]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Elementwise kernels of the plain (reference-schedule) iteration, bm_admm/solver.py:33-53.
+template <typename T>
+__global__ void ew_w(const T* y, const T* s, T b1, T* w, i64 m2) {
+  for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < m2; i += (i64)gridDim.x * blockDim.x)
+    w[i] = y[i] - b1 * s[i];
+}
+template <typename T>
+__global__ void ew_rhs(const T* AT, const T* z, const T* x2, T b2, T ib1, T* rhs, i64 m2) {
+  for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < m2; i += (i64)gridDim.x * blockDim.x)
+    rhs[i] = (z[i] + b2 * x2[i] - AT[i]) * ib1;
+}
+template <typename T>
+__global__ void ew_diag(const T* g, i64 Nloc, T* v, i64 m2) {
+  for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < m2; i += (i64)gridDim.x * blockDim.x)
+    v[i] *= g[i & (Nloc - 1)];
+}
+template <typename T>
+__global__ void ew_xk(const T* x, const T* x2, const T* z, T a, T ib2, T* xk, i64 m2) {
+  for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < m2; i += (i64)gridDim.x * blockDim.x)
+    xk[i] = a * x[i] + ((T)1 - a) * x2[i] - ib2 * z[i];
+}
+template <typename T>
+__global__ void ew_z(const T* x, const T* x2, const T* x2n, T a, T b2, T* z, i64 m2) {
+  for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < m2; i += (i64)gridDim.x * blockDim.x) {
+    T xo = a * x[i] + ((T)1 - a) * x2[i];
+    z[i] -= b2 * (xo - x2n[i]);
+  }
+}
+template <typename T>
+__global__ void ew_sy(const T* Ax, T a, T b1, T ib1, T* s, T* y, i64 m2) {
+  for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < m2; i += (i64)gridDim.x * blockDim.x) {
+    T axor = -a * Ax[i] - ((T)1 - a) * s[i];
+    T sn = ib1 * y[i] - axor;
+    sn = sn > (T)0 ? sn : (T)0;
+    y[i] -= b1 * (axor + sn);
+    s[i] = sn;
+  }
+}
+template <typename TA, typename TB>
+__global__ void ew_copy(const TA* a, TB* b, i64 m) {
+  for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (i64)gridDim.x * blockDim.x) b[i] = (TB)a[i];
+}
+template <typename T>
+__global__ void ew_fill(T* a, T v, i64 m) {
+  for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (i64)gridDim.x * blockDim.x) a[i] = v;
+}
+
+// inv_gl[rest] = 1 / (gamma + prod over bits of evals[bit]) in bit order (solver.py:207-213).
+template <typename T>
+__global__ void k_inv_gl(T* out, int nb, double e0, double e1, double gamma) {
+  const i64 r = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= (1ll << nb)) return;
+  double lam = 1.0;
+  for (int bit = 0; bit < nb; ++bit) lam *= ((r >> bit) & 1) ? e1 : e0;
+  out[r] = (T)(1.0 / (gamma + lam));
+}
+
+// ---------------------------------------------------------------------------------------------
+// Certificates (float64).
+// outwt64 (arcflow_kernels.py:357-365): node out-weight, x ascending.
+__global__ void k_outw(const double* __restrict__ w, double* __restrict__ outw, int n, i64 Nloc) {
+  const i64 D = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (D >= (1ll << n)) return;
+  double s = 0.0;
+  for (int x = 0; x < n; ++x) {
+    if (!((D >> x) & 1)) s += w[(i64)x * Nloc + (i64)compress_bit((u64)D, x)];
+  }
+  outw[D] = s;
+}
+
+// massg64 (arcflow_kernels.py:367-376): one cardinality layer of the forward push.
+__global__ void k_mass(const double* __restrict__ w, const double* __restrict__ outw, double* mass,
+                       const int* __restrict__ nodes, i64 ns, int n, i64 Nloc) {
+  const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ns) return;
+  const i64 C = nodes[i];
+  double s = 0.0;
+  for (int x = 0; x < n; ++x) {
+    if ((C >> x) & 1) {
+      const i64 P = C ^ (1ll << x);
+      const double ow = outw[P];
+      if (ow > 0.0) s += mass[P] * w[(i64)x * Nloc + (i64)compress_bit((u64)P, x)] / ow;
+    }
+  }
+  mass[C] = s;
+}
+
+// lpath64 (arcflow_kernels.py:388-397): one layer of the longest path.
+__global__ void k_lpath(const double* __restrict__ R, double* dp, const int* __restrict__ nodes, i64 ns, int n,
+                        i64 Nloc) {
+  const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ns) return;
+  const i64 C = nodes[i];
+  double best = -1e300;
+  for (int x = 0; x < n; ++x) {
+    if ((C >> x) & 1) {
+      const i64 P = C ^ (1ll << x);
+      const double v = dp[P] + R[(i64)x * Nloc + (i64)compress_bit((u64)P, x)];
+      if (v > best) best = v;
+    }
+  }
+  dp[C] = best;
+}
+
+// Compensated (Neumaier) block sums. Each block writes (sum, comp) pairs per quantity.
+__device__ __forceinline__ void two_sum(double& s, double& c, double v) {
+  const double t = s + v;
+  if (fabs(s) >= fabs(v)) c += (s - t) + v; else c += (v - t) + s;
+  s = t;
+}
+
+template <int Q>
+__device__ __forceinline__ void block_reduce_store(double (&s)[Q], double (&c)[Q], double* part) {
+  __shared__ double sh[2 * Q][256];
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) { sh[2 * q][t] = s[q]; sh[2 * q + 1][t] = c[q]; }
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (t < h) {
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        double ss = sh[2 * q][t], cc = sh[2 * q + 1][t] + sh[2 * q + 1][t + h];
+        two_sum(ss, cc, sh[2 * q][t + h]);
+        sh[2 * q][t] = ss;
+        sh[2 * q + 1][t] = cc;
+      }
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) { part[(size_t)blockIdx.x * 2 * Q + 2 * q] = sh[2 * q][0]; part[(size_t)blockIdx.x * 2 * Q + 2 * q + 1] = sh[2 * q + 1][0]; }
+  }
+}
+
+// phi: sum (x[obs] - p)^2; optionally the gathered x[obs] (float64) into fit.
+template <typename T>
+__global__ __launch_bounds__(256) void k_phi(const T* __restrict__ x, const i64* __restrict__ obs,
+                                             const double* __restrict__ p, i64 M, double* fit, double* part) {
+  double s[1] = {0.0}, c[1] = {0.0};
+  for (i64 j = (i64)blockIdx.x * blockDim.x + threadIdx.x; j < M; j += (i64)gridDim.x * blockDim.x) {
+    const double v = (double)x[obs[j]];
+    if (fit) fit[j] = v;
+    const double d = v - p[j];
+    two_sum(s[0], c[0], d * d);
+  }
+  block_reduce_store<1>(s, c, part);
+}
+
+// LB preparation: r = p - y at the observed rows, g[obs] = r (g zero-filled before),
+// f = sum r^2, ry = sum r y.
+template <typename T>
+__global__ __launch_bounds__(256) void k_lb_prep(const T* __restrict__ x, const i64* __restrict__ obs,
+                                                 const double* __restrict__ p, i64 M, double* g, double* part) {
+  double s[2] = {0.0, 0.0}, c[2] = {0.0, 0.0};
+  for (i64 j = (i64)blockIdx.x * blockDim.x + threadIdx.x; j < M; j += (i64)gridDim.x * blockDim.x) {
+    const double y = (double)x[obs[j]];
+    const double r = p[j] - y;
+    g[obs[j]] = r;
+    two_sum(s[0], c[0], r * r);
+    two_sum(s[1], c[1], r * y);
+  }
+  block_reduce_store<2>(s, c, part);
+}
+
+// The same with y given per row (float64) instead of read from a slab.
+__global__ __launch_bounds__(256) void k_lb_rows(const double* __restrict__ yv, const i64* __restrict__ obs,
+                                                 const double* __restrict__ p, i64 M, double* g, double* part) {
+  double s[2] = {0.0, 0.0}, c[2] = {0.0, 0.0};
+  for (i64 j = (i64)blockIdx.x * blockDim.x + threadIdx.x; j < M; j += (i64)gridDim.x * blockDim.x) {
+    const double y = yv[j];
+    const double r = p[j] - y;
+    g[obs[j]] = r;
+    two_sum(s[0], c[0], r * r);
+    two_sum(s[1], c[1], r * y);
+  }
+  block_reduce_store<2>(s, c, part);
+}
+
+// Final reduction of block partials: out[q] = sum over blocks of (sum + comp).
+__global__ void k_final(const double* part, int nblk, int Q, double* out) {
+  const int q = threadIdx.x;
+  if (q >= Q) return;
+  double s = 0.0, c = 0.0;
+  for (int b = 0; b < nblk; ++b) {
+    two_sum(s, c, part[(size_t)b * 2 * Q + 2 * q]);
+    c += part[(size_t)b * 2 * Q + 2 * q + 1];
+  }
+  out[q] = s + c;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Host side.
+namespace {
+
+template <typename T, int KIND>
+int launch_pass_rl(int RL, const PassArgs<T>& P, hipStream_t st) {
+  const WinGeom& g = P.g;
+  const size_t shm = (size_t)g.TPB * tile_lds_elems(g) * sizeof(T);
+  const dim3 grid((unsigned)((g.ntiles + g.TPB - 1) / g.TPB)), block((unsigned)(g.TT * g.TPB));
+  switch (RL) {
+    case 1: hipLaunchKernelGGL((bm_pass<T, 1, KIND>), grid, block, shm, st, P); break;
+    case 2: hipLaunchKernelGGL((bm_pass<T, 2, KIND>), grid, block, shm, st, P); break;
+    case 3: hipLaunchKernelGGL((bm_pass<T, 3, KIND>), grid, block, shm, st, P); break;
+    case 4: hipLaunchKernelGGL((bm_pass<T, 4, KIND>), grid, block, shm, st, P); break;
+    default: rg_set_error("bad RL %d", RL); return RG_EINVAL;
+  }
+  RG_LAUNCH_CHECK();
+  return RG_OK;
+}
+
+template <typename T>
+int launch_pass(int kind, int RL, const PassArgs<T>& P, hipStream_t st) {
+  switch (kind) {
+    case K_FIRST: return launch_pass_rl<T, K_FIRST>(RL, P, st);
+    case K_SY: return launch_pass_rl<T, K_SY>(RL, P, st);
+    case K_SYONLY: return launch_pass_rl<T, K_SYONLY>(RL, P, st);
+    case K_RHS: return launch_pass_rl<T, K_RHS>(RL, P, st);
+    case K_D: return launch_pass_rl<T, K_D>(RL, P, st);
+    case K_XK: return launch_pass_rl<T, K_XK>(RL, P, st);
+    case K_FULL: return launch_pass_rl<T, K_FULL>(RL, P, st);
+    case K_FULL_FIRST: return launch_pass_rl<T, K_FULL_FIRST>(RL, P, st);
+    case K_T: return launch_pass_rl<T, K_T>(RL, P, st);
+  }
+  rg_set_error("bad pass kind %d", kind);
+  return RG_EINVAL;
+}
+
+template <typename T, int MODE>
+int launch_prox(int n, const T* xk, T* out, T* z, const u32* om, const int* oo, const double* pc, T b2,
+                hipStream_t st) {
+  const i64 NN = 1ll << n;
+  const dim3 grid(grid_for(NN, 256)), block(256);
+#define RG_PROX_CASE(NV) \
+  case NV: hipLaunchKernelGGL((bm_prox<T, NV, MODE>), grid, block, 0, st, xk, out, z, om, oo, pc, b2); break;
+  switch (n) {
+    RG_PROX_CASE(2) RG_PROX_CASE(3) RG_PROX_CASE(4) RG_PROX_CASE(5) RG_PROX_CASE(6) RG_PROX_CASE(7)
+    RG_PROX_CASE(8) RG_PROX_CASE(9) RG_PROX_CASE(10) RG_PROX_CASE(11) RG_PROX_CASE(12) RG_PROX_CASE(13)
+    RG_PROX_CASE(14) RG_PROX_CASE(15) RG_PROX_CASE(16) RG_PROX_CASE(17) RG_PROX_CASE(18) RG_PROX_CASE(19)
+    RG_PROX_CASE(20) RG_PROX_CASE(21) RG_PROX_CASE(22) RG_PROX_CASE(23) RG_PROX_CASE(24) RG_PROX_CASE(25)
+    RG_PROX_CASE(26)
+    default: rg_set_error("bm_admm prox: n = %d outside 2..26", n); return RG_EUNSUP;
+  }
+#undef RG_PROX_CASE
+  RG_LAUNCH_CHECK();
+  return RG_OK;
+}
+
+inline unsigned ew_grid(i64 m) {
+  i64 g = (m + 255) / 256;
+  return (unsigned)(g > 65536 ? 65536 : (g < 1 ? 1 : g));
+}
+
+}  // namespace
+
+// The BM-ADMM object behind the C-ABI handle.
+struct rumgpu_bm {
+  rumgpu_ctx* ctx = nullptr;
+  int n = 0, nb = 0, isz = 8, plain = 0;
+  i64 Nloc = 0, m2 = 0, M = 0, NN = 0;
+  double beta = 1, alpha = 1.95, pp = 1.4142135623730951;
+  double e0 = 0, e1 = 0, gamma = 1, Q[4] = {0, 0, 0, 0};
+  std::vector<WinPlan> wins;
+  std::vector<WinGeom> geoms;
+  // dtype slabs
+  void *x2 = nullptr, *s = nullptr, *y = nullptr, *z = nullptr, *tb = nullptr, *xk = nullptr, *inv_gl = nullptr;
+  // kept state (Session keep / warm)
+  void *kx2 = nullptr, *ks = nullptr, *ky = nullptr, *kz = nullptr;
+  int kept_valid = 0;
+  // plain-schedule work slabs
+  void *pw = nullptr, *pAT = nullptr, *prhs = nullptr, *px = nullptr, *pAx = nullptr, *px2n = nullptr;
+  // observation structures
+  u32* obsmask = nullptr;
+  int* obsoff = nullptr;
+  double* pc = nullptr;       // p in (menu, ascending alternative) order
+  i64* pcpos = nullptr;       // host-side permutation input row -> pc index (device copy unused)
+  std::vector<i64> h_pcidx;   // input row j -> index in pc
+  i64* obs = nullptr;         // slab position per input row
+  double* p = nullptr;        // p per input row
+  // certificates
+  double* cert = nullptr;     // float64 slab
+  double* outw = nullptr;
+  double* mass = nullptr;     // also the longest-path dp
+  int* order = nullptr;
+  std::vector<i64> bounds;
+  double* part = nullptr;     // block partials
+  double* res = nullptr;      // device scalars
+  double* fit = nullptr;      // gathered x at obs (float64, input order)
+  double* h_res = nullptr;    // pinned host scalars
+  int pending = 0;            // s, y update of the last iteration still owed (tb holds partial A x)
+  i64 iters = 0;
+  // per-pass event profiler (rumgpu_bm_profile): events recorded between the launches of the
+  // fused passes, pass kind of each interval
+  int prof_on = 0;
+  std::vector<hipEvent_t> prof_ev;
+  std::vector<int> prof_kind;
+  size_t prof_n = 0;
+  // graph of `graph_len` steady-state iterations
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+  i64 graph_len = 0;
+};
+
+static void bm_free_all(rumgpu_bm* b) {
+  void* bufs[] = {b->x2, b->s, b->y, b->z, b->tb, b->xk, b->inv_gl, b->kx2, b->ks, b->ky, b->kz,
+                  b->pw, b->pAT, b->prhs, b->px, b->pAx, b->px2n, b->obsmask, b->obsoff, b->pc, b->obs,
+                  b->p, b->cert, b->outw, b->mass, b->order, b->part, b->res, b->fit};
+  for (void* q : bufs) rg_free(q);
+  if (b->h_res) (void)hipHostFree(b->h_res);
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  for (auto e : b->prof_ev) (void)hipEventDestroy(e);
+}
+
+// profiler kinds: 0 SY/FIRST, 1 RHS, 2 D, 3 XK, 4 prox, 5 middle-window transform, 6 FULL (one window)
+static inline void prof_mark(rumgpu_bm* b, int kind) {
+  if (!b->prof_on || b->prof_n >= b->prof_ev.size()) return;
+  (void)hipEventRecord(b->prof_ev[b->prof_n], b->ctx->stream);
+  b->prof_kind[b->prof_n] = kind;
+  b->prof_n++;
+}
+
+template <typename T>
+static PassArgs<T> pass_args(rumgpu_bm* b, int w) {
+  PassArgs<T> P;
+  P.tb = (T*)b->tb;
+  P.s = (T*)b->s;
+  P.y = (T*)b->y;
+  P.z = (const T*)b->z;
+  P.x2 = (const T*)b->x2;
+  P.xk = (T*)b->xk;
+  P.inv_gl = (const T*)b->inv_gl;
+  P.b1 = (T)b->beta;
+  P.b2 = (T)b->beta;
+  P.ib1 = (T)(1.0 / b->beta);
+  P.ib2 = (T)(1.0 / b->beta);
+  P.a = (T)b->alpha;
+  P.cA = coef<T>(1.0, -1.0, 0.0, b->pp);
+  P.cAp = coef<T>(1.0, 0.0, -1.0, b->pp);
+  P.cQ = coef<T>(b->Q[0], b->Q[1], b->Q[2], b->Q[3]);
+  P.cQt = coef<T>(b->Q[0], b->Q[2], b->Q[1], b->Q[3]);
+  P.cT = P.cA;
+  P.g = b->geoms[w];
+  return P;
+}
+
+template <typename T>
+static int run_pass(rumgpu_bm* b, int kind, int w, const Coef<T>* ct = nullptr) {
+  PassArgs<T> P = pass_args<T>(b, w);
+  if (ct) P.cT = *ct;
+  return launch_pass<T>(kind, b->wins[w].RL, P, b->ctx->stream);
+}
+
+// One fused iteration (see the header comment). With the profiler on, an event is recorded after
+// every launch; the interval ending at an event is charged to that launch's pass kind.
+template <typename T>
+static int fused_iter(rumgpu_bm* b) {
+  const int k = (int)b->wins.size();
+  hipStream_t st = b->ctx->stream;
+  const PassArgs<T> P0 = pass_args<T>(b, 0);
+  if (b->prof_on) prof_mark(b, -1);
+  if (k == 1) {
+    RG_TRY(run_pass<T>(b, b->pending ? K_FULL : K_FULL_FIRST, 0));
+    prof_mark(b, 6);
+  } else {
+    RG_TRY(run_pass<T>(b, b->pending ? K_SY : K_FIRST, 0));
+    prof_mark(b, 0);
+    for (int w = 1; w < k - 1; ++w) { RG_TRY(run_pass<T>(b, K_T, w, &P0.cAp)); prof_mark(b, 5); }
+    RG_TRY(run_pass<T>(b, K_RHS, k - 1));
+    prof_mark(b, 1);
+    for (int w = k - 2; w >= 1; --w) { RG_TRY(run_pass<T>(b, K_T, w, &P0.cQt)); prof_mark(b, 5); }
+    RG_TRY(run_pass<T>(b, K_D, 0));
+    prof_mark(b, 2);
+    for (int w = 1; w < k - 1; ++w) { RG_TRY(run_pass<T>(b, K_T, w, &P0.cQ)); prof_mark(b, 5); }
+    RG_TRY(run_pass<T>(b, K_XK, k - 1));
+    prof_mark(b, 3);
+  }
+  RG_TRY((launch_prox<T, 0>(b->n, (const T*)b->xk, (T*)b->x2, (T*)b->z, b->obsmask, b->obsoff, b->pc, (T)b->beta, st)));
+  prof_mark(b, 4);
+  for (int w = k - 2; w >= 1; --w) { RG_TRY(run_pass<T>(b, K_T, w, &P0.cA)); prof_mark(b, 5); }
+  b->pending = 1;
+  b->iters += 1;
+  return RG_OK;
+}
+
+// Settle the deferred s, y update so (x2, s, y, z) is the reference's state.
+template <typename T>
+static int finalize(rumgpu_bm* b) {
+  if (!b->pending) return RG_OK;
+  RG_TRY(run_pass<T>(b, K_SYONLY, 0));
+  b->pending = 0;
+  return RG_OK;
+}
+
+// Full transform over every window: dst = T(src), first window from src, the rest in place.
+template <typename TIN, typename T>
+static int full_transform(rumgpu_bm* b, const TIN* src, T* dst, Coef<T> k, int epi_wmax, int pro_extract) {
+  hipStream_t st = b->ctx->stream;
+  std::vector<WinPlan> wins = plan_windows(b->nb, sizeof(T));
+  const int nw = (int)wins.size();
+  for (int w = 0; w < nw; ++w) {
+    const WinGeom g = make_geom(b->n, wins[w]);
+    const int RL = wins[w].RL;
+    const bool last = (w == nw - 1);
+    if (w == 0 && pro_extract) {
+      if (last && epi_wmax) RG_TRY((launch_tr_rl<double, T, 1, 1>(RL, (const double*)src, dst, k, g, b->outw, b->mass, st)));
+      else RG_TRY((launch_tr_rl<double, T, 1, 0>(RL, (const double*)src, dst, k, g, b->outw, b->mass, st)));
+    } else if (w == 0) {
+      if (last && epi_wmax) RG_TRY((launch_tr_rl<TIN, T, 0, 1>(RL, src, dst, k, g, nullptr, nullptr, st)));
+      else RG_TRY((launch_tr_rl<TIN, T, 0, 0>(RL, src, dst, k, g, nullptr, nullptr, st)));
+    } else {
+      if (last && epi_wmax) RG_TRY((launch_tr_rl<T, T, 0, 1>(RL, dst, dst, k, g, nullptr, nullptr, st)));
+      else RG_TRY((launch_tr_rl<T, T, 0, 0>(RL, dst, dst, k, g, nullptr, nullptr, st)));
+    }
+  }
+  return RG_OK;
+}
+
+// One iteration in the reference's own schedule (the plain baseline): separate elementwise
+// kernels, four full butterflies, the literal z update and a copy.
+template <typename T>
+static int plain_iter(rumgpu_bm* b) {
+  hipStream_t st = b->ctx->stream;
+  const i64 m2 = b->m2;
+  const unsigned gr = ew_grid(m2);
+  T *x2 = (T*)b->x2, *s = (T*)b->s, *y = (T*)b->y, *z = (T*)b->z, *xk = (T*)b->xk;
+  T *w = (T*)b->pw, *AT = (T*)b->pAT, *rhs = (T*)b->prhs, *x = (T*)b->px, *Ax = (T*)b->pAx, *x2n = (T*)b->px2n;
+  const T b1 = (T)b->beta, b2 = (T)b->beta, ib1 = (T)(1.0 / b->beta), ib2 = (T)(1.0 / b->beta), a = (T)b->alpha;
+  hipLaunchKernelGGL(ew_w<T>, dim3(gr), dim3(256), 0, st, y, s, b1, w, m2);
+  RG_TRY((full_transform<T, T>(b, w, AT, coef<T>(1.0, 0.0, -1.0, b->pp), 0, 0)));
+  hipLaunchKernelGGL(ew_rhs<T>, dim3(gr), dim3(256), 0, st, AT, z, x2, b2, ib1, rhs, m2);
+  RG_TRY((full_transform<T, T>(b, rhs, x, coef<T>(b->Q[0], b->Q[2], b->Q[1], b->Q[3]), 0, 0)));
+  hipLaunchKernelGGL(ew_diag<T>, dim3(gr), dim3(256), 0, st, (const T*)b->inv_gl, b->Nloc, x, m2);
+  RG_TRY((full_transform<T, T>(b, x, x, coef<T>(b->Q[0], b->Q[1], b->Q[2], b->Q[3]), 0, 0)));
+  RG_TRY((full_transform<T, T>(b, x, Ax, coef<T>(1.0, -1.0, 0.0, b->pp), 0, 0)));
+  hipLaunchKernelGGL(ew_xk<T>, dim3(gr), dim3(256), 0, st, x, x2, z, a, ib2, xk, m2);
+  RG_TRY((launch_prox<T, 2>(b->n, xk, x2n, nullptr, b->obsmask, b->obsoff, b->pc, b2, st)));
+  hipLaunchKernelGGL(ew_z<T>, dim3(gr), dim3(256), 0, st, x, x2, x2n, a, b2, z, m2);
+  hipLaunchKernelGGL(ew_sy<T>, dim3(gr), dim3(256), 0, st, Ax, a, b1, ib1, s, y, m2);
+  hipLaunchKernelGGL((ew_copy<T, T>), dim3(gr), dim3(256), 0, st, x2n, x2, m2);
+  RG_LAUNCH_CHECK();
+  b->iters += 1;
+  return RG_OK;
+}
+
+template <typename T>
+static int do_iters(rumgpu_bm* b, i64 iters) {
+  if (b->plain) {
+    for (i64 i = 0; i < iters; ++i) RG_TRY(plain_iter<T>(b));
+    return RG_OK;
+  }
+  i64 i = 0;
+  if (!b->pending && iters > 0) {
+    RG_TRY(fused_iter<T>(b));
+    ++i;
+  }
+  // steady-state iterations replay the captured graph of graph_len iterations
+  if (b->gexec && b->graph_len > 0 && !b->prof_on) {
+    const i64 reps = (iters - i) / b->graph_len;
+    for (i64 r = 0; r < reps; ++r) RG_HIP(hipGraphLaunch(b->gexec, b->ctx->stream));
+    i += reps * b->graph_len;
+    b->iters += reps * b->graph_len;
+  }
+  for (; i < iters; ++i) RG_TRY(fused_iter<T>(b));
+  return RG_OK;
+}
+
+template <typename T>
+static int build_graph(rumgpu_bm* b, i64 len) {
+  if (b->gexec) { (void)hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
+  if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+  b->graph_len = 0;
+  if (len <= 0 || b->plain) return RG_OK;
+  hipStream_t st = b->ctx->stream;
+  const int pend = b->pending;
+  const i64 it0 = b->iters;
+  b->pending = 1;                               // the graph holds steady-state iterations
+  RG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  int rc = RG_OK;
+  for (i64 i = 0; i < len && rc == RG_OK; ++i) rc = fused_iter<T>(b);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(st, &g);
+  b->pending = pend;
+  b->iters = it0;
+  if (rc != RG_OK) { if (g) (void)hipGraphDestroy(g); return rc; }
+  if (e != hipSuccess) { rg_set_error("graph capture failed: %s", hipGetErrorString(e)); return RG_EHIP; }
+  b->graph = g;
+  RG_HIP(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
+  b->graph_len = len;
+  return RG_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Certificates.
+template <typename T>
+static int phi_impl(rumgpu_bm* b, const void* x, double* out_sq, int want_fit) {
+  hipStream_t st = b->ctx->stream;
+  const int nblk = (int)std::min<i64>(256, std::max<i64>(1, (b->M + 255) / 256));
+  hipLaunchKernelGGL(k_phi<T>, dim3(nblk), dim3(256), 0, st, (const T*)x, b->obs, b->p, b->M,
+                     want_fit ? b->fit : nullptr, b->part);
+  hipLaunchKernelGGL(k_final, dim3(1), dim3(64), 0, st, b->part, nblk, 1, b->res);
+  RG_LAUNCH_CHECK();
+  RG_HIP(hipMemcpyAsync(b->h_res, b->res, sizeof(double), hipMemcpyDeviceToHost, st));
+  RG_HIP(hipStreamSynchronize(st));
+  *out_sq = b->h_res[0];
+  return RG_OK;
+}
+
+template <typename T>
+static int push_impl(rumgpu_bm* b, double* ub) {
+  hipStream_t st = b->ctx->stream;
+  const int n = b->n;
+  // q = Mobius(x2) in float64, then w = max(q, 0) + 1e-12
+  RG_TRY((full_transform<T, double>(b, (const T*)b->x2, b->cert, coef<double>(1.0, -1.0, 0.0, 1.0), 1, 0)));
+  hipLaunchKernelGGL(k_outw, dim3(grid_for(b->NN, 256)), dim3(256), 0, st, b->cert, b->outw, n, b->Nloc);
+  hipLaunchKernelGGL((ew_fill<double>), dim3(1), dim3(64), 0, st, b->mass, 1.0, (i64)1);
+  for (int c = 1; c <= n; ++c) {
+    const i64 lo = b->bounds[c], ns = b->bounds[c + 1] - lo;
+    if (ns <= 0) continue;
+    hipLaunchKernelGGL(k_mass, dim3(grid_for(ns, 256)), dim3(256), 0, st, b->cert, b->outw, b->mass, b->order + lo, ns,
+                       n, b->Nloc);
+  }
+  RG_LAUNCH_CHECK();
+  // rf = extract, x_ub = zeta(rf) (in place)
+  RG_TRY((full_transform<double, double>(b, b->cert, b->cert, coef<double>(1.0, 1.0, 0.0, 1.0), 0, 1)));
+  double sq = 0.0;
+  RG_TRY(phi_impl<double>(b, b->cert, &sq, 1));
+  *ub = std::sqrt(sq);
+  return RG_OK;
+}
+
+template <typename T>
+static int lb_impl(rumgpu_bm* b, double* lb) {
+  hipStream_t st = b->ctx->stream;
+  const int n = b->n;
+  hipLaunchKernelGGL((ew_fill<double>), dim3(ew_grid(b->m2)), dim3(256), 0, st, b->cert, 0.0, b->m2);
+  const int nblk = (int)std::min<i64>(256, std::max<i64>(1, (b->M + 255) / 256));
+  hipLaunchKernelGGL(k_lb_prep<T>, dim3(nblk), dim3(256), 0, st, (const T*)b->x2, b->obs, b->p, b->M, b->cert, b->part);
+  hipLaunchKernelGGL(k_final, dim3(1), dim3(64), 0, st, b->part, nblk, 2, b->res);
+  RG_LAUNCH_CHECK();
+  RG_TRY((full_transform<double, double>(b, b->cert, b->cert, coef<double>(1.0, 0.0, 1.0, 1.0), 0, 0)));
+  hipLaunchKernelGGL((ew_fill<double>), dim3(1), dim3(64), 0, st, b->mass, 0.0, (i64)1);
+  for (int c = 1; c <= n; ++c) {
+    const i64 lo = b->bounds[c], ns = b->bounds[c + 1] - lo;
+    if (ns <= 0) continue;
+    hipLaunchKernelGGL(k_lpath, dim3(grid_for(ns, 256)), dim3(256), 0, st, b->cert, b->mass, b->order + lo, ns, n,
+                       b->Nloc);
+  }
+  RG_LAUNCH_CHECK();
+  RG_HIP(hipMemcpyAsync(b->res + 2, b->mass + (b->NN - 1), sizeof(double), hipMemcpyDeviceToDevice, st));
+  RG_HIP(hipMemcpyAsync(b->h_res, b->res, 3 * sizeof(double), hipMemcpyDeviceToHost, st));
+  RG_HIP(hipStreamSynchronize(st));
+  const double f = b->h_res[0], ry = b->h_res[1], vmax = b->h_res[2];
+  const double rc = vmax - ry;
+  const double l2 = f - 2.0 * rc;
+  *lb = std::sqrt(l2 > 0.0 ? l2 : 0.0);
+  return RG_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// C-ABI.
+#define BM_DISPATCH(b, fn, ...) ((b)->isz == 8 ? fn<double>(__VA_ARGS__) : fn<float>(__VA_ARGS__))
+
+extern "C" int rumgpu_bm_create(rumgpu_ctx* ctx, int n, const uint64_t* menu, const int32_t* alt, const double* p,
+                                int64_t M, int dtype_bytes, double beta, double alpha, double precond_p, double e0,
+                                double e1, double gamma, const double* Q, int flags, rumgpu_bm** out) {
+  RG_CHECK(ctx && out && menu && alt && p && Q, RG_EINVAL, "bm_create: null argument");
+  RG_CHECK(n >= 2 && n <= 26, RG_EUNSUP, "bm_admm handles 2 <= n <= 26 here (n = %d)", n);
+  RG_CHECK(dtype_bytes == 4 || dtype_bytes == 8, RG_EINVAL, "dtype must be 4 (float32) or 8 (float64) bytes");
+  RG_CHECK(M >= 1, RG_EINVAL, "bm_create: no rows");
+  RG_CHECK(beta > 0, RG_EINVAL, "beta must be > 0");
+  RG_HIP(hipSetDevice(ctx->device));
+  rumgpu_bm* b = new rumgpu_bm();
+  b->ctx = ctx;
+  b->n = n;
+  b->nb = n - 1;
+  b->isz = dtype_bytes;
+  b->plain = (flags & 1) ? 1 : 0;
+  b->Nloc = 1ll << (n - 1);
+  b->m2 = (i64)n * b->Nloc;
+  b->NN = 1ll << n;
+  b->M = M;
+  b->beta = beta;
+  b->alpha = alpha;
+  b->pp = precond_p;
+  b->e0 = e0;
+  b->e1 = e1;
+  b->gamma = gamma;
+  for (int i = 0; i < 4; ++i) b->Q[i] = Q[i];
+  b->wins = plan_windows(b->nb, dtype_bytes);
+  for (auto& w : b->wins) b->geoms.push_back(make_geom(n, w));
+  int rc = RG_OK;
+  auto fail = [&](int code) {
+    bm_free_all(b);
+    delete b;
+    return code;
+  };
+  // host-side observation structures
+  std::vector<u32> hmask(b->NN, 0);
+  std::vector<int> hoff(b->NN, 0);
+  std::vector<i64> hobs(M);
+  for (i64 j = 0; j < M; ++j) {
+    const u64 m = menu[j];
+    const int a = alt[j];
+    if (a < 0 || a >= n || !((m >> a) & 1ull) || m >= (1ull << n) || (m & (m - 1)) == 0) {
+      rg_set_error("bm_create: row %lld (menu %llu, alternative %d) is not a non-singleton cell of n = %d",
+                   (long long)j, (unsigned long long)m, a, n);
+      return fail(RG_EINVAL);
+    }
+    if ((hmask[m] >> a) & 1u) {
+      rg_set_error("bm_create: row %lld repeats (menu %llu, alternative %d)", (long long)j, (unsigned long long)m, a);
+      return fail(RG_EINVAL);
+    }
+    hmask[m] |= 1u << a;
+    hobs[j] = (i64)a * b->Nloc + (i64)compress_bit(m & ~(1ull << a), a);
+  }
+  // compact p order: menus ascending, alternatives ascending
+  {
+    i64 acc = 0;
+    for (i64 S = 0; S < b->NN; ++S) {
+      hoff[S] = (int)acc;
+      acc += __builtin_popcount(hmask[S]);
+    }
+    b->h_pcidx.resize(M);
+    for (i64 j = 0; j < M; ++j) {
+      const u64 m = menu[j];
+      const int a = alt[j];
+      b->h_pcidx[j] = hoff[m] + __builtin_popcount(hmask[m] & ((1u << a) - 1u));
+    }
+  }
+  const size_t T = dtype_bytes;
+  const size_t slab = (size_t)b->m2 * T;
+  void** slabs[] = {&b->x2, &b->s, &b->y, &b->z, &b->tb, &b->xk};
+  for (void** q : slabs)
+    if ((rc = rg_malloc(q, slab, "bm_admm state slab")) != RG_OK) return fail(rc);
+  if (b->plain) {
+    void** ps[] = {&b->pw, &b->pAT, &b->prhs, &b->px, &b->pAx, &b->px2n};
+    for (void** q : ps)
+      if ((rc = rg_malloc(q, slab, "bm_admm plain-schedule slab")) != RG_OK) return fail(rc);
+  }
+  if ((rc = rg_malloc(&b->inv_gl, (size_t)b->Nloc * T, "inv_gl")) != RG_OK) return fail(rc);
+  if ((rc = rg_malloc((void**)&b->obsmask, (size_t)b->NN * 4, "obsmask")) != RG_OK) return fail(rc);
+  if ((rc = rg_malloc((void**)&b->obsoff, (size_t)b->NN * 4, "obsoff")) != RG_OK) return fail(rc);
+  if ((rc = rg_malloc((void**)&b->pc, (size_t)M * 8, "p (menu order)")) != RG_OK) return fail(rc);
+  if ((rc = rg_malloc((void**)&b->obs, (size_t)M * 8, "obs index")) != RG_OK) return fail(rc);
+  if ((rc = rg_malloc((void**)&b->p, (size_t)M * 8, "p")) != RG_OK) return fail(rc);
+  if ((rc = rg_malloc((void**)&b->fit, (size_t)M * 8, "p_fit")) != RG_OK) return fail(rc);
+  if ((rc = rg_malloc((void**)&b->cert, (size_t)b->m2 * 8, "float64 certificate slab")) != RG_OK) return fail(rc);
+  if ((rc = rg_malloc((void**)&b->outw, (size_t)b->NN * 8, "outw")) != RG_OK) return fail(rc);
+  if ((rc = rg_malloc((void**)&b->mass, (size_t)b->NN * 8, "mass")) != RG_OK) return fail(rc);
+  if ((rc = rg_malloc((void**)&b->order, (size_t)b->NN * 4, "layers")) != RG_OK) return fail(rc);
+  if ((rc = rg_malloc((void**)&b->part, (size_t)256 * 4 * 8, "partials")) != RG_OK) return fail(rc);
+  if ((rc = rg_malloc((void**)&b->res, 16 * 8, "scalars")) != RG_OK) return fail(rc);
+  if (hipHostMalloc((void**)&b->h_res, 16 * 8, hipHostMallocDefault) != hipSuccess) {
+    rg_set_error("hipHostMalloc of 128 bytes failed");
+    return fail(RG_ENOMEM);
+  }
+  hipStream_t st = ctx->stream;
+  // layers: nodes by popcount, ascending within a layer
+  {
+    std::vector<int> hord(b->NN);
+    b->bounds.assign(n + 2, 0);
+    std::vector<i64> cnt(n + 2, 0);
+    for (i64 v = 0; v < b->NN; ++v) cnt[__builtin_popcountll(v)]++;
+    for (int c = 0; c <= n; ++c) b->bounds[c + 1] = b->bounds[c] + cnt[c];
+    std::vector<i64> fill(b->bounds.begin(), b->bounds.end());
+    for (i64 v = 0; v < b->NN; ++v) hord[fill[__builtin_popcountll(v)]++] = (int)v;
+    if (hipMemcpy(b->order, hord.data(), (size_t)b->NN * 4, hipMemcpyHostToDevice) != hipSuccess) return fail(RG_EHIP);
+  }
+  if (hipMemcpy(b->obsmask, hmask.data(), (size_t)b->NN * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(b->obsoff, hoff.data(), (size_t)b->NN * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(b->obs, hobs.data(), (size_t)M * 8, hipMemcpyHostToDevice) != hipSuccess) {
+    rg_set_error("bm_create: upload failed");
+    return fail(RG_EHIP);
+  }
+  if (dtype_bytes == 8)
+    hipLaunchKernelGGL(k_inv_gl<double>, dim3(grid_for(b->Nloc, 256)), dim3(256), 0, st, (double*)b->inv_gl, b->nb, e0, e1, gamma);
+  else
+    hipLaunchKernelGGL(k_inv_gl<float>, dim3(grid_for(b->Nloc, 256)), dim3(256), 0, st, (float*)b->inv_gl, b->nb, e0, e1, gamma);
+  *out = b;
+  rc = rumgpu_bm_set_p(b, p);
+  if (rc != RG_OK) {
+    *out = nullptr;
+    return fail(rc);
+  }
+  rc = rumgpu_bm_init_cold(b);
+  if (rc != RG_OK) {
+    *out = nullptr;
+    return fail(rc);
+  }
+  return RG_OK;
+}
+
+extern "C" int rumgpu_bm_destroy(rumgpu_bm* b) {
+  if (!b) return RG_OK;
+  (void)hipStreamSynchronize(b->ctx->stream);
+  bm_free_all(b);
+  delete b;
+  return RG_OK;
+}
+
+extern "C" int rumgpu_bm_set_p(rumgpu_bm* b, const double* p) {
+  RG_CHECK(b && p, RG_EINVAL, "bm_set_p: null argument");
+  std::vector<double> hpc(b->M);
+  for (i64 j = 0; j < b->M; ++j) hpc[b->h_pcidx[j]] = p[j];
+  hipStream_t st = b->ctx->stream;
+  RG_HIP(hipMemcpyAsync(b->p, p, (size_t)b->M * 8, hipMemcpyHostToDevice, st));
+  RG_HIP(hipMemcpyAsync(b->pc, hpc.data(), (size_t)b->M * 8, hipMemcpyHostToDevice, st));
+  RG_HIP(hipStreamSynchronize(st));
+  return RG_OK;
+}
+
+template <typename T>
+static int init_cold_t(rumgpu_bm* b) {
+  hipStream_t st = b->ctx->stream;
+  const unsigned gr = ew_grid(b->m2);
+  hipLaunchKernelGGL((ew_fill<T>), dim3(gr), dim3(256), 0, st, (T*)b->s, (T)0, b->m2);
+  hipLaunchKernelGGL((ew_fill<T>), dim3(gr), dim3(256), 0, st, (T*)b->y, (T)0, b->m2);
+  hipLaunchKernelGGL((ew_fill<T>), dim3(gr), dim3(256), 0, st, (T*)b->z, (T)0, b->m2);
+  RG_LAUNCH_CHECK();
+  RG_TRY((launch_prox<T, 1>(b->n, (const T*)b->xk, (T*)b->x2, nullptr, b->obsmask, b->obsoff, b->pc, (T)b->beta, st)));
+  b->pending = 0;
+  b->iters = 0;
+  RG_HIP(hipStreamSynchronize(st));
+  return RG_OK;
+}
+
+extern "C" int rumgpu_bm_init_cold(rumgpu_bm* b) {
+  RG_CHECK(b, RG_EINVAL, "bm_init_cold: null handle");
+  return BM_DISPATCH(b, init_cold_t, b);
+}
+
+template <typename T>
+static int set_state_t(rumgpu_bm* b, const double* x2, const double* s, const double* y, const double* z) {
+  hipStream_t st = b->ctx->stream;
+  const double* src[4] = {x2, s, y, z};
+  void* dst[4] = {b->x2, b->s, b->y, b->z};
+  std::vector<T> tmp((size_t)b->m2);
+  for (int q = 0; q < 4; ++q) {
+    for (i64 i = 0; i < b->m2; ++i) tmp[i] = (T)src[q][i];
+    RG_HIP(hipMemcpyAsync(dst[q], tmp.data(), (size_t)b->m2 * sizeof(T), hipMemcpyHostToDevice, st));
+    RG_HIP(hipStreamSynchronize(st));
+  }
+  b->pending = 0;
+  return RG_OK;
+}
+
+extern "C" int rumgpu_bm_set_state(rumgpu_bm* b, const double* x2, const double* s, const double* y, const double* z) {
+  RG_CHECK(b && x2 && s && y && z, RG_EINVAL, "bm_set_state: null argument");
+  return BM_DISPATCH(b, set_state_t, b, x2, s, y, z);
+}
+
+template <typename T>
+static int get_state_t(rumgpu_bm* b, double* x2, double* s, double* y, double* z) {
+  RG_TRY(finalize<T>(b));
+  hipStream_t st = b->ctx->stream;
+  double* dst[4] = {x2, s, y, z};
+  void* src[4] = {b->x2, b->s, b->y, b->z};
+  std::vector<T> tmp((size_t)b->m2);
+  for (int q = 0; q < 4; ++q) {
+    if (!dst[q]) continue;
+    RG_HIP(hipMemcpyAsync(tmp.data(), src[q], (size_t)b->m2 * sizeof(T), hipMemcpyDeviceToHost, st));
+    RG_HIP(hipStreamSynchronize(st));
+    for (i64 i = 0; i < b->m2; ++i) dst[q][i] = (double)tmp[i];
+  }
+  return RG_OK;
+}
+
+extern "C" int rumgpu_bm_get_state(rumgpu_bm* b, double* x2, double* s, double* y, double* z) {
+  RG_CHECK(b, RG_EINVAL, "bm_get_state: null handle");
+  return BM_DISPATCH(b, get_state_t, b, x2, s, y, z);
+}
+
+// Copy the state of src into dst on the device (any dtypes, same n and rows).
+template <typename TA, typename TB>
+static int copy_state_t(rumgpu_bm* dst, rumgpu_bm* src) {
+  RG_TRY(finalize<TA>(src));
+  hipStream_t st = dst->ctx->stream;
+  const unsigned gr = ew_grid(dst->m2);
+  void* a[4] = {src->x2, src->s, src->y, src->z};
+  void* b[4] = {dst->x2, dst->s, dst->y, dst->z};
+  for (int q = 0; q < 4; ++q) hipLaunchKernelGGL((ew_copy<TA, TB>), dim3(gr), dim3(256), 0, st, (const TA*)a[q], (TB*)b[q], dst->m2);
+  RG_LAUNCH_CHECK();
+  dst->pending = 0;
+  RG_HIP(hipStreamSynchronize(st));
+  return RG_OK;
+}
+
+extern "C" int rumgpu_bm_copy_state(rumgpu_bm* dst, rumgpu_bm* src) {
+  RG_CHECK(dst && src && dst->n == src->n && dst->M == src->M, RG_EINVAL, "bm_copy_state: incompatible handles");
+  if (src->isz == 8 && dst->isz == 8) return copy_state_t<double, double>(dst, src);
+  if (src->isz == 8 && dst->isz == 4) return copy_state_t<double, float>(dst, src);
+  if (src->isz == 4 && dst->isz == 8) return copy_state_t<float, double>(dst, src);
+  return copy_state_t<float, float>(dst, src);
+}
+
+template <typename T>
+static int keep_t(rumgpu_bm* b, int load) {
+  RG_TRY(finalize<T>(b));
+  hipStream_t st = b->ctx->stream;
+  void** live[4] = {&b->x2, &b->s, &b->y, &b->z};
+  void** kept[4] = {&b->kx2, &b->ks, &b->ky, &b->kz};
+  if (!load) {
+    for (int q = 0; q < 4; ++q) {
+      if (!*kept[q]) RG_TRY(rg_malloc(kept[q], (size_t)b->m2 * sizeof(T), "kept state slab"));
+      RG_HIP(hipMemcpyAsync(*kept[q], *live[q], (size_t)b->m2 * sizeof(T), hipMemcpyDeviceToDevice, st));
+    }
+    b->kept_valid = 1;
+  } else {
+    RG_CHECK(b->kept_valid, RG_EINVAL, "bm_load_kept: no kept state");
+    for (int q = 0; q < 4; ++q)
+      RG_HIP(hipMemcpyAsync(*live[q], *kept[q], (size_t)b->m2 * sizeof(T), hipMemcpyDeviceToDevice, st));
+    b->pending = 0;
+    b->iters = 0;
+  }
+  RG_HIP(hipStreamSynchronize(st));
+  return RG_OK;
+}
+
+extern "C" int rumgpu_bm_keep(rumgpu_bm* b) {
+  RG_CHECK(b, RG_EINVAL, "bm_keep: null handle");
+  return BM_DISPATCH(b, keep_t, b, 0);
+}
+extern "C" int rumgpu_bm_load_kept(rumgpu_bm* b) {
+  RG_CHECK(b, RG_EINVAL, "bm_load_kept: null handle");
+  return BM_DISPATCH(b, keep_t, b, 1);
+}
+
+extern "C" int rumgpu_bm_run(rumgpu_bm* b, int64_t iters) {
+  RG_CHECK(b && iters >= 0, RG_EINVAL, "bm_run: bad argument");
+  RG_HIP(hipSetDevice(b->ctx->device));
+  int rc = BM_DISPATCH(b, do_iters, b, iters);
+  if (rc != RG_OK) return rc;
+  RG_HIP(hipStreamSynchronize(b->ctx->stream));
+  return RG_OK;
+}
+
+extern "C" int rumgpu_bm_set_graph(rumgpu_bm* b, int64_t len) {
+  RG_CHECK(b && len >= 0, RG_EINVAL, "bm_set_graph: bad argument");
+  return BM_DISPATCH(b, build_graph, b, len);
+}
+
+extern "C" int rumgpu_bm_finalize(rumgpu_bm* b) {
+  RG_CHECK(b, RG_EINVAL, "bm_finalize: null handle");
+  RG_TRY(BM_DISPATCH(b, finalize, b));
+  RG_HIP(hipStreamSynchronize(b->ctx->stream));
+  return RG_OK;
+}
+
+extern "C" int rumgpu_bm_phi(rumgpu_bm* b, double* phi) {
+  RG_CHECK(b && phi, RG_EINVAL, "bm_phi: null argument");
+  double sq = 0.0;
+  RG_TRY(b->isz == 8 ? phi_impl<double>(b, b->x2, &sq, 0) : phi_impl<float>(b, b->x2, &sq, 0));
+  *phi = std::sqrt(sq);
+  return RG_OK;
+}
+
+extern "C" int rumgpu_bm_push_ub64(rumgpu_bm* b, double* ub, double* p_fit) {
+  RG_CHECK(b && ub, RG_EINVAL, "bm_push_ub64: null argument");
+  RG_TRY(BM_DISPATCH(b, push_impl, b, ub));
+  if (p_fit) {
+    RG_HIP(hipMemcpy(p_fit, b->fit, (size_t)b->M * 8, hipMemcpyDeviceToHost));
+  }
+  return RG_OK;
+}
+
+extern "C" int rumgpu_bm_get_fit(rumgpu_bm* b, double* p_fit) {
+  RG_CHECK(b && p_fit, RG_EINVAL, "bm_get_fit: null argument");
+  RG_HIP(hipMemcpy(p_fit, b->fit, (size_t)b->M * 8, hipMemcpyDeviceToHost));
+  return RG_OK;
+}
+
+extern "C" int rumgpu_bm_lb64(rumgpu_bm* b, double* lb) {
+  RG_CHECK(b && lb, RG_EINVAL, "bm_lb64: null argument");
+  return BM_DISPATCH(b, lb_impl, b, lb);
+}
+
+extern "C" int rumgpu_bm_x_obs(rumgpu_bm* b, double* out) {
+  RG_CHECK(b && out, RG_EINVAL, "bm_x_obs: null argument");
+  double sq = 0.0;
+  RG_TRY(b->isz == 8 ? phi_impl<double>(b, b->x2, &sq, 1) : phi_impl<float>(b, b->x2, &sq, 1));
+  RG_HIP(hipMemcpy(out, b->fit, (size_t)b->M * 8, hipMemcpyDeviceToHost));
+  return RG_OK;
+}
+
+extern "C" int rumgpu_bm_info(rumgpu_bm* b, int64_t* iters, int* nwin, int* win_bits) {
+  RG_CHECK(b, RG_EINVAL, "bm_info: null handle");
+  if (iters) *iters = b->iters;
+  if (nwin) *nwin = (int)b->wins.size();
+  if (win_bits)
+    for (size_t i = 0; i < b->wins.size(); ++i) win_bits[i] = b->wins[i].B;
+  return RG_OK;
+}
+
+// Standalone transform of a float64 host slab (the butterfly as a building block and its test
+// surface): v = T(v) with (a, b, c, d) on every rest bit.
+extern "C" int rumgpu_butterfly64(rumgpu_ctx* ctx, int n, double* v, double a, double bcoef, double c, double d) {
+  RG_CHECK(ctx && v && n >= 2 && n <= 30, RG_EINVAL, "butterfly64: bad argument");
+  const i64 m2 = (i64)n << (n - 1);
+  double* dv = nullptr;
+  RG_TRY(rg_malloc((void**)&dv, (size_t)m2 * 8, "butterfly slab"));
+  hipStream_t st = ctx->stream;
+  int rc = RG_OK;
+  if (hipMemcpy(dv, v, (size_t)m2 * 8, hipMemcpyHostToDevice) != hipSuccess) rc = RG_EHIP;
+  std::vector<WinPlan> wins = plan_windows(n - 1, 8);
+  for (size_t w = 0; w < wins.size() && rc == RG_OK; ++w) {
+    const WinGeom g = make_geom(n, wins[w]);
+    rc = launch_tr_rl<double, double, 0, 0>(wins[w].RL, dv, dv, coef<double>(a, bcoef, c, d), g, nullptr, nullptr, st);
+  }
+  if (rc == RG_OK && hipMemcpy(v, dv, (size_t)m2 * 8, hipMemcpyDeviceToHost) != hipSuccess) rc = RG_EHIP;
+  rg_free(dv);
+  if (rc == RG_EHIP) rg_set_error("butterfly64: HIP error");
+  return rc;
+}
+
+extern "C" int rumgpu_butterfly32(rumgpu_ctx* ctx, int n, float* v, float a, float bcoef, float c, float d) {
+  RG_CHECK(ctx && v && n >= 2 && n <= 30, RG_EINVAL, "butterfly32: bad argument");
+  const i64 m2 = (i64)n << (n - 1);
+  float* dv = nullptr;
+  RG_TRY(rg_malloc((void**)&dv, (size_t)m2 * 4, "butterfly slab"));
+  hipStream_t st = ctx->stream;
+  int rc = RG_OK;
+  if (hipMemcpy(dv, v, (size_t)m2 * 4, hipMemcpyHostToDevice) != hipSuccess) rc = RG_EHIP;
+  std::vector<WinPlan> wins = plan_windows(n - 1, 4);
+  for (size_t w = 0; w < wins.size() && rc == RG_OK; ++w) {
+    const WinGeom g = make_geom(n, wins[w]);
+    rc = launch_tr_rl<float, float, 0, 0>(wins[w].RL, dv, dv, coef<float>(a, bcoef, c, d), g, nullptr, nullptr, st);
+  }
+  if (rc == RG_OK && hipMemcpy(v, dv, (size_t)m2 * 4, hipMemcpyDeviceToHost) != hipSuccess) rc = RG_EHIP;
+  rg_free(dv);
+  if (rc == RG_EHIP) rg_set_error("butterfly32: HIP error");
+  return rc;
+}
+
+// Prox of a host slab (test surface for the per-menu kernel): out = prox(xk) for rows (menu,
+// alt, p) with penalty beta2, dtype by dtype_bytes. The values are float64 on the host.
+extern "C" int rumgpu_prox_slab(rumgpu_bm* b, const double* xk, double* out) {
+  RG_CHECK(b && xk && out, RG_EINVAL, "prox_slab: null argument");
+  RG_TRY(BM_DISPATCH(b, finalize, b));       // tb is used as scratch below
+  hipStream_t st = b->ctx->stream;
+  const i64 m2 = b->m2;
+  int rc = RG_OK;
+  if (b->isz == 8) {
+    RG_HIP(hipMemcpy(b->xk, xk, (size_t)m2 * 8, hipMemcpyHostToDevice));
+    rc = launch_prox<double, 2>(b->n, (const double*)b->xk, (double*)b->tb, nullptr, b->obsmask, b->obsoff, b->pc, b->beta, st);
+    if (rc == RG_OK) RG_HIP(hipMemcpy(out, b->tb, (size_t)m2 * 8, hipMemcpyDeviceToHost));
+  } else {
+    std::vector<float> tmp(m2);
+    for (i64 i = 0; i < m2; ++i) tmp[i] = (float)xk[i];
+    RG_HIP(hipMemcpy(b->xk, tmp.data(), (size_t)m2 * 4, hipMemcpyHostToDevice));
+    rc = launch_prox<float, 2>(b->n, (const float*)b->xk, (float*)b->tb, nullptr, b->obsmask, b->obsoff, b->pc, (float)b->beta, st);
+    if (rc == RG_OK) {
+      RG_HIP(hipMemcpy(tmp.data(), b->tb, (size_t)m2 * 4, hipMemcpyDeviceToHost));
+      for (i64 i = 0; i < m2; ++i) out[i] = tmp[i];
+    }
+  }
+  return rc;
+}
+
+// Standalone float64 lower bound at a point y on the rows (its own scratch): the same kernels
+// as rumgpu_bm_lb64.
+extern "C" int rumgpu_lb64(rumgpu_ctx* ctx, int n, const uint64_t* menu, const int32_t* alt, int64_t M,
+                           const double* y, const double* p, double* lb) {
+  RG_CHECK(ctx && menu && alt && y && p && lb && M >= 1, RG_EINVAL, "lb64: bad argument");
+  RG_CHECK(n >= 2 && n <= 30, RG_EUNSUP, "lb64: n = %d outside 2..30", n);
+  RG_HIP(hipSetDevice(ctx->device));
+  const i64 Nloc = 1ll << (n - 1), m2 = (i64)n * Nloc, NN = 1ll << n;
+  std::vector<i64> hobs(M);
+  for (i64 j = 0; j < M; ++j) {
+    const int a = alt[j];
+    RG_CHECK(a >= 0 && a < n && ((menu[j] >> a) & 1ull), RG_EINVAL, "lb64: row %lld alternative not in menu", (long long)j);
+    hobs[j] = (i64)a * Nloc + (i64)compress_bit(menu[j] & ~(1ull << a), a);
+  }
+  double *g = nullptr, *dp = nullptr, *dy = nullptr, *dpv = nullptr, *part = nullptr, *res = nullptr;
+  i64* obs = nullptr;
+  int* order = nullptr;
+  int rc = RG_OK;
+  if ((rc = rg_malloc((void**)&g, (size_t)m2 * 8, "lb64 slab")) == RG_OK &&
+      (rc = rg_malloc((void**)&dp, (size_t)NN * 8, "lb64 dp")) == RG_OK &&
+      (rc = rg_malloc((void**)&dy, (size_t)M * 8, "lb64 y")) == RG_OK &&
+      (rc = rg_malloc((void**)&dpv, (size_t)M * 8, "lb64 p")) == RG_OK &&
+      (rc = rg_malloc((void**)&obs, (size_t)M * 8, "lb64 obs")) == RG_OK &&
+      (rc = rg_malloc((void**)&order, (size_t)NN * 4, "lb64 layers")) == RG_OK &&
+      (rc = rg_malloc((void**)&part, (size_t)256 * 4 * 8, "lb64 partials")) == RG_OK &&
+      (rc = rg_malloc((void**)&res, 16 * 8, "lb64 scalars")) == RG_OK) {
+    hipStream_t st = ctx->stream;
+    std::vector<int> hord(NN);
+    std::vector<i64> bounds(n + 2, 0), cnt(n + 2, 0);
+    for (i64 v = 0; v < NN; ++v) cnt[__builtin_popcountll(v)]++;
+    for (int c = 0; c <= n; ++c) bounds[c + 1] = bounds[c] + cnt[c];
+    std::vector<i64> fill(bounds.begin(), bounds.end());
+    for (i64 v = 0; v < NN; ++v) hord[fill[__builtin_popcountll(v)]++] = (int)v;
+    // y is a float64 "slab" of M values read through obs: gather index = identity
+    std::vector<i64> ident(M);
+    for (i64 j = 0; j < M; ++j) ident[j] = j;
+    if (hipMemcpy(order, hord.data(), (size_t)NN * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(obs, hobs.data(), (size_t)M * 8, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dy, y, (size_t)M * 8, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dpv, p, (size_t)M * 8, hipMemcpyHostToDevice) != hipSuccess) {
+      rc = RG_EHIP;
+      rg_set_error("lb64: upload failed");
+    }
+    if (rc == RG_OK) {
+      hipLaunchKernelGGL((ew_fill<double>), dim3(ew_grid(m2)), dim3(256), 0, st, g, 0.0, m2);
+      const int nblk = (int)std::min<i64>(256, std::max<i64>(1, (M + 255) / 256));
+      // y per row is scattered into g directly by k_lb_scatter below
+      hipLaunchKernelGGL(k_lb_rows, dim3(nblk), dim3(256), 0, st, dy, obs, dpv, (i64)M, g, part);
+      hipLaunchKernelGGL(k_final, dim3(1), dim3(64), 0, st, part, nblk, 2, res);
+      std::vector<WinPlan> wins = plan_windows(n - 1, 8);
+      for (size_t w = 0; w < wins.size() && rc == RG_OK; ++w)
+        rc = launch_tr_rl<double, double, 0, 0>(wins[w].RL, g, g, coef<double>(1.0, 0.0, 1.0, 1.0),
+                                                make_geom(n, wins[w]), nullptr, nullptr, st);
+      if (rc == RG_OK) {
+        hipLaunchKernelGGL((ew_fill<double>), dim3(1), dim3(64), 0, st, dp, 0.0, (i64)1);
+        for (int c = 1; c <= n; ++c) {
+          const i64 lo = bounds[c], ns = bounds[c + 1] - lo;
+          if (ns <= 0) continue;
+          hipLaunchKernelGGL(k_lpath, dim3(grid_for(ns, 256)), dim3(256), 0, st, g, dp, order + lo, ns, n, Nloc);
+        }
+        double h[3];
+        if (hipMemcpyAsync(h, res, 2 * 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipMemcpyAsync(h + 2, dp + (NN - 1), 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess) {
+          rc = RG_EHIP;
+          rg_set_error("lb64: download failed");
+        } else {
+          const double l2 = h[0] - 2.0 * (h[2] - h[1]);
+          *lb = std::sqrt(l2 > 0.0 ? l2 : 0.0);
+        }
+      }
+    }
+  }
+  rg_free(g); rg_free(dp); rg_free(dy); rg_free(dpv); rg_free(obs); rg_free(order); rg_free(part); rg_free(res);
+  return rc;
+}
+
+// Per-pass device time of `iters` fused iterations, measured with HIP events on the context stream
+// between consecutive launches (the graph is bypassed). ms_by_kind[7] and launches_by_kind[7]:
+// 0 SY, 1 RHS, 2 D, 3 XK, 4 prox, 5 middle-window transform, 6 one-window FULL pass.
+extern "C" int rumgpu_bm_profile(rumgpu_bm* b, int64_t iters, double* ms_by_kind, int64_t* launches_by_kind) {
+  RG_CHECK(b && ms_by_kind && launches_by_kind && iters > 0, RG_EINVAL, "bm_profile: bad argument");
+  RG_CHECK(!b->plain, RG_EINVAL, "bm_profile: the plain schedule has no fused passes");
+  RG_HIP(hipSetDevice(b->ctx->device));
+  for (int q = 0; q < 7; ++q) { ms_by_kind[q] = 0.0; launches_by_kind[q] = 0; }
+  const int per_iter = 2 + 2 * (int)b->wins.size() + 4;
+  const i64 batch = 16;
+  if (b->prof_ev.size() < (size_t)(per_iter * batch)) {
+    for (auto e : b->prof_ev) (void)hipEventDestroy(e);
+    b->prof_ev.assign((size_t)per_iter * batch, nullptr);
+    for (auto& e : b->prof_ev) RG_HIP(hipEventCreate(&e));
+    b->prof_kind.assign(b->prof_ev.size(), -1);
+  }
+  i64 done = 0;
+  int rc = RG_OK;
+  while (done < iters && rc == RG_OK) {
+    const i64 now = std::min<i64>(batch, iters - done);
+    b->prof_on = 1;
+    b->prof_n = 0;
+    for (i64 i = 0; i < now && rc == RG_OK; ++i) rc = BM_DISPATCH(b, fused_iter, b);
+    b->prof_on = 0;
+    if (rc != RG_OK) break;
+    RG_HIP(hipStreamSynchronize(b->ctx->stream));
+    for (size_t e = 1; e < b->prof_n; ++e) {
+      const int kind = b->prof_kind[e];
+      if (kind < 0) continue;
+      float ms = 0.0f;
+      RG_HIP(hipEventElapsedTime(&ms, b->prof_ev[e - 1], b->prof_ev[e]));
+      ms_by_kind[kind] += ms;
+      launches_by_kind[kind] += 1;
+    }
+    done += now;
+  }
+  b->prof_on = 0;
+  return rc;
+}

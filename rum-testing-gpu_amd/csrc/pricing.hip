@@ -1,0 +1,625 @@
+// Flow-CG pricing on MI355X: the longest path through the subset lattice,
+//   V[D] = max over x in D of V[D minus x] + w(x, D),   w(x, D) = sum of the scores of x's rows
+//   whose menu is a subset of D,   pred[D] = the maximizing x (lowest x on ties),
+// and the ranking read back along pred from the full set (flow_cg/bf_pricing.py:8-47,
+// flow_cg/solver.py:168-200, flow_cg/pricing_stream.py:61-412).
+//
+// Three layouts:
+//   mask    one float32 V and one pred byte per subset; a thread per subset of a cardinality
+//           layer scans its members' rows in row order, exactly the reference's float32
+//           additions, so V and the ranking are bit-identical to bf_update for any scores.
+//           The row loop is uniform across a wavefront (scalar loads of the rows).
+//   slab    the scores are scattered on the Block-Marschak slab and summed over subsets by the
+//           float32 butterfly (1,0,1,1), which gives w(x, D) for every arc at once
+//           (common/transforms.py:108-113 uses the same identity for the lower bound); the
+//           layered max then reads one arc value per member. HBM traffic is O(n 2^n) instead
+//           of the mask scan's O(mp 2^(n-1)) tests; the float32 sums are in butterfly order,
+//           so V equals the mask layout's wherever every partial sum is exact (quantized duals).
+//   stream  two live layers addressed by colex rank, with float32, uint16 or uint24 block
+//           quantization (BLOCK = 1024 ranks per (min, step) pair) and predecessors stored
+//           for the top layers only; the rest of the path comes from an exact float32 mask DP on
+//           the sub-lattice of the node reached (pricing_stream.py:348-405).
+#include "rumgpu_internal.hpp"
+#include "transform.hpp"
+
+#include <cmath>
+#include <algorithm>
+#include <chrono>
+
+// ---------------------------------------------------------------------------------------------
+// mask layout: one cardinality layer
+__global__ __launch_bounds__(256) void cg_mask_layer(float* __restrict__ V, uint8_t* __restrict__ pred,
+                                                     const u32* __restrict__ masks, const float* __restrict__ scores,
+                                                     const u64* __restrict__ offsets, const u32* __restrict__ nodes,
+                                                     i64 ns, int n) {
+  const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool act = i < ns;
+  const u32 D = act ? nodes[i] : 0u;
+  float best = 0.0f;
+  bool seen = false;
+  int bx = 0;
+  for (int x = 0; x < n; ++x) {
+    const bool in = act && ((D >> x) & 1u);
+    float w = 0.0f;
+    const u64 lo = offsets[x], hi = offsets[x + 1];
+    for (u64 j = lo; j < hi; ++j) {
+      const u32 Dp = masks[j];
+      if ((Dp & D) == Dp) w += scores[j];
+    }
+    if (in) {
+      const float cand = V[D ^ (1u << x)] + w;
+      if (!seen || cand > best) {
+        best = cand;
+        seen = true;
+        bx = x;
+      }
+    }
+  }
+  if (act) {
+    V[D] = best;
+    pred[D] = (uint8_t)bx;
+  }
+}
+
+// slab layout: scatter of the scores, then one layer of the max over members
+__global__ void cg_scatter(float* __restrict__ g, const i64* __restrict__ pos, const float* __restrict__ scores, i64 M) {
+  const i64 j = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < M) g[pos[j]] = scores[j];
+}
+
+__global__ __launch_bounds__(256) void cg_slab_layer(float* __restrict__ V, uint8_t* __restrict__ pred,
+                                                     const float* __restrict__ R, const u32* __restrict__ nodes, i64 ns,
+                                                     int n, i64 Nloc) {
+  const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ns) return;
+  const u32 D = nodes[i];
+  float best = 0.0f;
+  bool seen = false;
+  int bx = 0;
+  for (int x = 0; x < n; ++x) {
+    if ((D >> x) & 1u) {
+      const u32 P = D ^ (1u << x);
+      const float cand = V[P] + R[(i64)x * Nloc + (i64)compress_bit(P, x)];
+      if (!seen || cand > best) {
+        best = cand;
+        seen = true;
+        bx = x;
+      }
+    }
+  }
+  V[D] = best;
+  pred[D] = (uint8_t)bx;
+}
+
+// walk pred from the full set (one thread): rank_pos[x] = position of x, 0 = best
+__global__ void cg_backtrace(const uint8_t* __restrict__ pred, int n, int32_t* rank_pos, u64 full) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  u64 D = full;
+  for (int i = 0; i < n; ++i) {
+    const int cur = pred[D];
+    rank_pos[cur] = i;
+    D ^= 1ull << cur;
+  }
+}
+
+__global__ void cg_fill_f32(float* a, float v, i64 m) {
+  for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (i64)gridDim.x * blockDim.x) a[i] = v;
+}
+__global__ void cg_fill_u8(uint8_t* a, uint8_t v, i64 m) {
+  for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (i64)gridDim.x * blockDim.x) a[i] = v;
+}
+
+// ---------------------------------------------------------------------------------------------
+// stream layout: one layer k of colex ranks, 1024 ranks per block (256 threads x 4).
+#define CG_BLOCK 1024
+#define CG_THREADS 256
+#define CG_VPT 4
+
+template <int MODE>
+__device__ __forceinline__ float cg_load_prev(const void* vprev, const uint8_t* lprev, const float* mprev,
+                                              const float* eprev, u64 rp) {
+  if constexpr (MODE == 0) {
+    return ((const float*)vprev)[rp];
+  } else if constexpr (MODE == 1) {
+    const float q = (float)((const uint16_t*)vprev)[rp];
+    return fmaf(q, eprev[rp / CG_BLOCK], mprev[rp / CG_BLOCK]);
+  } else {
+    const unsigned qi = ((unsigned)((const uint16_t*)vprev)[rp] << 8) | (unsigned)lprev[rp];
+    return fmaf((float)qi, eprev[rp / CG_BLOCK], mprev[rp / CG_BLOCK]);
+  }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(CG_THREADS) void cg_stream_layer(const u64* __restrict__ binom, int n, int k, u64 count,
+                                                            const void* vprev, const uint8_t* lprev, const float* mprev,
+                                                            const float* eprev, void* vcur, uint8_t* lcur, float* mcur,
+                                                            float* ecur, uint8_t* pred, int store_pred,
+                                                            const u64* __restrict__ masks,
+                                                            const float* __restrict__ scores,
+                                                            const u64* __restrict__ offsets) {
+  const u64 base = (u64)blockIdx.x * CG_BLOCK;
+  const int np1 = n + 1;
+  float vals[CG_VPT];
+  uint8_t preds[CG_VPT];
+  float tmin = 3.0e38f, tmax = -3.0e38f;
+#pragma unroll
+  for (int j = 0; j < CG_VPT; ++j) {
+    const u64 r = base + (u64)j * CG_THREADS + threadIdx.x;
+    vals[j] = 0.0f;
+    preds[j] = 0;
+    if (r < count) {
+      // colex unranking: the largest element is the largest b with C(b, t) <= r
+      u64 rr = r, D = 0;
+      int b = n - 1;
+      for (int t = k; t >= 1; --t) {
+        while (binom[b * np1 + t] > rr) --b;
+        D |= 1ull << b;
+        rr -= binom[b * np1 + t];
+        --b;
+      }
+      // members from the highest down; predecessor rank from prefix/suffix binomial sums
+      float best = 0.0f;
+      bool seen = false;
+      int bestx = 0;
+      u64 suf = 0, remain = r, left = D;
+      for (int t = k - 1; t >= 0; --t) {
+        const int x = 63 - __clzll((long long)left);
+        left ^= 1ull << x;
+        remain -= binom[x * np1 + (t + 1)];
+        const u64 rp = remain + suf;
+        suf += binom[x * np1 + t];
+        const float vp = cg_load_prev<MODE>(vprev, lprev, mprev, eprev, rp);
+        float w = 0.0f;
+        const u64 hi = offsets[x + 1];
+        for (u64 i = offsets[x]; i < hi; ++i) {
+          const u64 Dp = masks[i];
+          if ((Dp & D) == Dp) w += scores[i];
+        }
+        const float cand = vp + w;
+        if (!seen || cand >= best) {
+          best = cand;
+          seen = true;
+          bestx = x;
+        }
+      }
+      vals[j] = best;
+      preds[j] = (uint8_t)bestx;
+      tmin = best < tmin ? best : tmin;
+      tmax = best > tmax ? best : tmax;
+    }
+  }
+  if constexpr (MODE == 0) {
+#pragma unroll
+    for (int j = 0; j < CG_VPT; ++j) {
+      const u64 r = base + (u64)j * CG_THREADS + threadIdx.x;
+      if (r < count) {
+        ((float*)vcur)[r] = vals[j];
+        if (store_pred) pred[r] = preds[j];
+      }
+    }
+  } else {
+    __shared__ float smin[CG_THREADS / 64], smax[CG_THREADS / 64];
+    for (int o = 32; o > 0; o >>= 1) {
+      tmin = fminf(tmin, __shfl_xor(tmin, o, 64));
+      tmax = fmaxf(tmax, __shfl_xor(tmax, o, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+      smin[threadIdx.x >> 6] = tmin;
+      smax[threadIdx.x >> 6] = tmax;
+    }
+    __syncthreads();
+    float bmin = smin[0], bmax = smax[0];
+    for (int i = 1; i < CG_THREADS / 64; ++i) {
+      bmin = fminf(bmin, smin[i]);
+      bmax = fmaxf(bmax, smax[i]);
+    }
+    const float levels = MODE == 1 ? 65535.0f : 16777215.0f;
+    const float step = (bmax - bmin) / levels;     // 0 when the block is constant: q = 0
+    if (threadIdx.x == 0) {
+      mcur[blockIdx.x] = bmin;
+      ecur[blockIdx.x] = step;
+    }
+#pragma unroll
+    for (int j = 0; j < CG_VPT; ++j) {
+      const u64 r = base + (u64)j * CG_THREADS + threadIdx.x;
+      if (r < count) {
+        float q = (step > 0.0f) ? rintf((vals[j] - bmin) / step) : 0.0f;
+        q = fminf(fmaxf(q, 0.0f), levels);
+        const unsigned qi = (unsigned)q;
+        if constexpr (MODE == 1) {
+          ((uint16_t*)vcur)[r] = (uint16_t)qi;
+        } else {
+          ((uint16_t*)vcur)[r] = (uint16_t)(qi >> 8);
+          lcur[r] = (uint8_t)(qi & 255u);
+        }
+        if (store_pred) pred[r] = preds[j];
+      }
+    }
+  }
+}
+
+// largest quantization step of a layer: step[k] = max over blocks (one block of 256)
+__global__ void cg_layer_step(const float* __restrict__ e, u64 nblk, float* out) {
+  __shared__ float sh[256];
+  float m = 0.0f;
+  for (u64 i = threadIdx.x; i < nblk; i += 256) m = fmaxf(m, e[i]);
+  sh[threadIdx.x] = m;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) sh[threadIdx.x] = fmaxf(sh[threadIdx.x], sh[threadIdx.x + h]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = sh[0];
+}
+
+// ---------------------------------------------------------------------------------------------
+struct rumgpu_cg {
+  rumgpu_ctx* ctx = nullptr;
+  int n = 0, layout = 0, dtype = 0, k_pred = 1;
+  i64 M = 0, NN = 0, Nloc = 0;
+  std::vector<u64> h_masks, h_offsets;
+  std::vector<i64> bounds;
+  // device rows
+  u32* masks32 = nullptr;
+  u64* masks64 = nullptr;
+  u64* offsets = nullptr;
+  float* scores = nullptr;
+  i64* spos = nullptr;       // slab positions (slab layout)
+  // mask / slab
+  float* V = nullptr;
+  uint8_t* pred = nullptr;
+  u32* nodes = nullptr;
+  float* R = nullptr;        // slab
+  int32_t* rank = nullptr;
+  // stream
+  u64* binom = nullptr;
+  std::vector<u64> h_binom;
+  void* vbuf[2] = {nullptr, nullptr};
+  uint8_t* lbuf[2] = {nullptr, nullptr};
+  float* mbuf[2] = {nullptr, nullptr};
+  float* ebuf[2] = {nullptr, nullptr};
+  float* lstep = nullptr;
+  std::vector<uint8_t*> predk;   // per layer k (k >= k_pred)
+  // stream sub-lattice (mask DP on 2^k_pred nodes)
+  float* subV = nullptr;
+  uint8_t* subpred = nullptr;
+  u32* subnodes = nullptr;
+  std::vector<i64> subbounds;
+  u32* submasks = nullptr;
+  float* subscores = nullptr;
+  u64* suboffsets = nullptr;
+  size_t device_bytes = 0;
+  double last_ms = 0.0;
+};
+
+static void cg_free_all(rumgpu_cg* c) {
+  void* bufs[] = {c->masks32, c->masks64, c->offsets, c->scores, c->spos, c->V, c->pred, c->nodes, c->R, c->rank,
+                  c->binom, c->vbuf[0], c->vbuf[1], c->lbuf[0], c->lbuf[1], c->mbuf[0], c->mbuf[1], c->ebuf[0],
+                  c->ebuf[1], c->lstep, c->subV, c->subpred, c->subnodes, c->submasks, c->subscores, c->suboffsets};
+  for (void* q : bufs) rg_free(q);
+  for (auto* p : c->predk) rg_free(p);
+}
+
+static int cg_alloc(rumgpu_cg* c, void** p, size_t bytes, const char* what) {
+  int rc = rg_malloc(p, bytes, what);
+  if (rc == RG_OK) c->device_bytes += bytes;
+  return rc;
+}
+
+static void layer_order(int n, std::vector<u32>& ord, std::vector<i64>& bounds) {
+  const i64 NN = 1ll << n;
+  ord.resize(NN);
+  bounds.assign(n + 2, 0);
+  std::vector<i64> cnt(n + 2, 0);
+  for (i64 v = 0; v < NN; ++v) cnt[__builtin_popcountll(v)]++;
+  for (int k = 0; k <= n; ++k) bounds[k + 1] = bounds[k] + cnt[k];
+  std::vector<i64> fill(bounds.begin(), bounds.end());
+  for (i64 v = 0; v < NN; ++v) ord[fill[__builtin_popcountll(v)]++] = (u32)v;
+}
+
+static u64 binom_u64(int b, int t) {
+  if (t < 0 || t > b) return 0;
+  u64 c = 1;
+  for (int q = 0; q < t; ++q) c = c * (u64)(b - q) / (u64)(q + 1);
+  return c;
+}
+
+static u64 colex_rank(u64 mask) {
+  u64 r = 0;
+  int t = 0, b = 0;
+  while (mask) {
+    if (mask & 1) {
+      t += 1;
+      r += binom_u64(b, t);
+    }
+    mask >>= 1;
+    b += 1;
+  }
+  return r;
+}
+
+extern "C" int rumgpu_cg_create(rumgpu_ctx* ctx, int n, const uint64_t* masks_x, const uint64_t* offsets, int64_t M,
+                                int layout, int dtype, int k_pred, rumgpu_cg** out) {
+  RG_CHECK(ctx && masks_x && offsets && out, RG_EINVAL, "cg_create: null argument");
+  RG_CHECK(layout >= 0 && layout <= 2, RG_EINVAL, "cg_create: layout must be 0 (mask), 1 (slab) or 2 (stream)");
+  RG_CHECK(dtype >= 0 && dtype <= 2, RG_EINVAL, "cg_create: dtype must be 0 (f32), 1 (u16) or 2 (u24)");
+  RG_CHECK(layout == 2 || dtype == 0, RG_EINVAL, "cg_create: the mask and slab layouts store float32 only");
+  RG_CHECK(n >= 1 && n <= (layout == 2 ? 39 : 31), RG_EUNSUP, "cg_create: n = %d too large for this layout", n);
+  RG_CHECK(M >= 1 && (i64)offsets[n] == M, RG_EINVAL, "cg_create: offsets[n] must equal M");
+  RG_HIP(hipSetDevice(ctx->device));
+  rumgpu_cg* c = new rumgpu_cg();
+  c->ctx = ctx;
+  c->n = n;
+  c->layout = layout;
+  c->dtype = dtype;
+  c->M = M;
+  c->NN = 1ll << (layout == 2 ? 0 : n);
+  c->Nloc = 1ll << (n - 1);
+  c->h_masks.assign(masks_x, masks_x + M);
+  c->h_offsets.assign(offsets, offsets + n + 1);
+  auto fail = [&](int code) {
+    cg_free_all(c);
+    delete c;
+    return code;
+  };
+  int rc = RG_OK;
+  hipStream_t st = ctx->stream;
+  if ((rc = cg_alloc(c, (void**)&c->offsets, (size_t)(n + 1) * 8, "offsets")) != RG_OK) return fail(rc);
+  if ((rc = cg_alloc(c, (void**)&c->scores, (size_t)M * 4, "scores")) != RG_OK) return fail(rc);
+  if ((rc = cg_alloc(c, (void**)&c->rank, (size_t)64 * 4, "rank")) != RG_OK) return fail(rc);
+  RG_HIP(hipMemcpy(c->offsets, offsets, (size_t)(n + 1) * 8, hipMemcpyHostToDevice));
+  if (layout == 0 || layout == 1) {
+    c->NN = 1ll << n;
+    std::vector<u32> m32(M);
+    for (i64 j = 0; j < M; ++j) m32[j] = (u32)masks_x[j];
+    if ((rc = cg_alloc(c, (void**)&c->masks32, (size_t)M * 4, "menu masks")) != RG_OK) return fail(rc);
+    RG_HIP(hipMemcpy(c->masks32, m32.data(), (size_t)M * 4, hipMemcpyHostToDevice));
+    if ((rc = cg_alloc(c, (void**)&c->V, (size_t)c->NN * 4, "V")) != RG_OK) return fail(rc);
+    if ((rc = cg_alloc(c, (void**)&c->pred, (size_t)c->NN, "pred")) != RG_OK) return fail(rc);
+    std::vector<u32> ord;
+    layer_order(n, ord, c->bounds);
+    if ((rc = cg_alloc(c, (void**)&c->nodes, (size_t)c->NN * 4, "layers")) != RG_OK) return fail(rc);
+    RG_HIP(hipMemcpy(c->nodes, ord.data(), (size_t)c->NN * 4, hipMemcpyHostToDevice));
+    if (layout == 1) {
+      RG_CHECK(n >= 2, RG_EUNSUP, "slab pricing needs n >= 2");
+      std::vector<i64> pos(M);
+      for (int x = 0; x < n; ++x)
+        for (u64 j = offsets[x]; j < offsets[x + 1]; ++j)
+          pos[j] = (i64)x * c->Nloc + (i64)compress_bit(masks_x[j] & ~(1ull << x), x);
+      if ((rc = cg_alloc(c, (void**)&c->spos, (size_t)M * 8, "slab positions")) != RG_OK) return fail(rc);
+      RG_HIP(hipMemcpy(c->spos, pos.data(), (size_t)M * 8, hipMemcpyHostToDevice));
+      if ((rc = cg_alloc(c, (void**)&c->R, (size_t)n * c->Nloc * 4, "pricing slab")) != RG_OK) return fail(rc);
+    }
+  } else {
+    c->k_pred = k_pred <= 0 ? n - 10 : k_pred;
+    if (c->k_pred < 1) c->k_pred = 1;
+    if (c->k_pred > n) c->k_pred = n;
+    if ((rc = cg_alloc(c, (void**)&c->masks64, (size_t)M * 8, "menu masks")) != RG_OK) return fail(rc);
+    RG_HIP(hipMemcpy(c->masks64, masks_x, (size_t)M * 8, hipMemcpyHostToDevice));
+    c->h_binom.assign((size_t)(n + 1) * (n + 1), 0);
+    for (int b = 0; b <= n; ++b)
+      for (int t = 0; t <= b; ++t) c->h_binom[(size_t)b * (n + 1) + t] = binom_u64(b, t);
+    if ((rc = cg_alloc(c, (void**)&c->binom, c->h_binom.size() * 8, "binomials")) != RG_OK) return fail(rc);
+    RG_HIP(hipMemcpy(c->binom, c->h_binom.data(), c->h_binom.size() * 8, hipMemcpyHostToDevice));
+    u64 cap = 0;
+    for (int k = 0; k <= n; ++k) cap = std::max(cap, binom_u64(n, k));
+    const u64 nblk = (cap + CG_BLOCK - 1) / CG_BLOCK;
+    const size_t vsz = dtype == 0 ? 4 : 2;
+    for (int i = 0; i < 2; ++i) {
+      if ((rc = cg_alloc(c, &c->vbuf[i], (size_t)cap * vsz, "stream layer")) != RG_OK) return fail(rc);
+      if (dtype == 2 && (rc = cg_alloc(c, (void**)&c->lbuf[i], (size_t)cap, "stream layer low bytes")) != RG_OK)
+        return fail(rc);
+      if ((rc = cg_alloc(c, (void**)&c->mbuf[i], (size_t)nblk * 4, "block offsets")) != RG_OK) return fail(rc);
+      if ((rc = cg_alloc(c, (void**)&c->ebuf[i], (size_t)nblk * 4, "block steps")) != RG_OK) return fail(rc);
+    }
+    if ((rc = cg_alloc(c, (void**)&c->lstep, (size_t)(n + 1) * 4, "layer steps")) != RG_OK) return fail(rc);
+    c->predk.assign(n + 1, nullptr);
+    for (int k = c->k_pred; k <= n; ++k)
+      if ((rc = cg_alloc(c, (void**)&c->predk[k], (size_t)binom_u64(n, k), "stored predecessors")) != RG_OK)
+        return fail(rc);
+    // sub-lattice scaffolding
+    const int K = c->k_pred;
+    std::vector<u32> ord;
+    layer_order(K, ord, c->subbounds);
+    if ((rc = cg_alloc(c, (void**)&c->subV, (size_t)4 << K, "sub-lattice V")) != RG_OK) return fail(rc);
+    if ((rc = cg_alloc(c, (void**)&c->subpred, (size_t)1 << K, "sub-lattice pred")) != RG_OK) return fail(rc);
+    if ((rc = cg_alloc(c, (void**)&c->subnodes, (size_t)4 << K, "sub-lattice layers")) != RG_OK) return fail(rc);
+    RG_HIP(hipMemcpy(c->subnodes, ord.data(), (size_t)4 << K, hipMemcpyHostToDevice));
+    if ((rc = cg_alloc(c, (void**)&c->submasks, (size_t)M * 4, "sub-lattice masks")) != RG_OK) return fail(rc);
+    if ((rc = cg_alloc(c, (void**)&c->subscores, (size_t)M * 4, "sub-lattice scores")) != RG_OK) return fail(rc);
+    if ((rc = cg_alloc(c, (void**)&c->suboffsets, (size_t)(K + 1) * 8, "sub-lattice offsets")) != RG_OK)
+      return fail(rc);
+  }
+  RG_HIP(hipStreamSynchronize(st));
+  *out = c;
+  return RG_OK;
+}
+
+extern "C" int rumgpu_cg_destroy(rumgpu_cg* c) {
+  if (!c) return RG_OK;
+  (void)hipStreamSynchronize(c->ctx->stream);
+  cg_free_all(c);
+  delete c;
+  return RG_OK;
+}
+
+// mask DP over a stored-layer lattice of n nodes' bits (used for the mask layout and the stream
+// sub-lattice)
+static int mask_sweep(hipStream_t st, int n, float* V, uint8_t* pred, const u32* masks, const float* scores,
+                      const u64* offsets, const u32* nodes, const std::vector<i64>& bounds) {
+  const i64 NN = 1ll << n;
+  hipLaunchKernelGGL(cg_fill_f32, dim3(grid_for(NN, 256) > 4096 ? 4096 : grid_for(NN, 256)), dim3(256), 0, st, V, 0.0f, NN);
+  hipLaunchKernelGGL(cg_fill_u8, dim3(grid_for(NN, 256) > 4096 ? 4096 : grid_for(NN, 256)), dim3(256), 0, st, pred,
+                     (uint8_t)0, NN);
+  for (int k = 1; k <= n; ++k) {
+    const i64 lo = bounds[k], ns = bounds[k + 1] - lo;
+    if (ns <= 0) continue;
+    hipLaunchKernelGGL(cg_mask_layer, dim3(grid_for(ns, 256)), dim3(256), 0, st, V, pred, masks, scores, offsets,
+                       nodes + lo, ns, n);
+  }
+  RG_LAUNCH_CHECK();
+  return RG_OK;
+}
+
+extern "C" int rumgpu_cg_price(rumgpu_cg* c, const float* scores_x, double* vmax, int32_t* rank_pos, double* margin) {
+  RG_CHECK(c && scores_x && vmax && rank_pos, RG_EINVAL, "cg_price: null argument");
+  RG_HIP(hipSetDevice(c->ctx->device));
+  hipStream_t st = c->ctx->stream;
+  const int n = c->n;
+  auto t0 = std::chrono::steady_clock::now();
+  RG_HIP(hipMemcpyAsync(c->scores, scores_x, (size_t)c->M * 4, hipMemcpyHostToDevice, st));
+  if (margin) *margin = 0.0;
+  if (c->layout == 0 || c->layout == 1) {
+    if (c->layout == 0) {
+      RG_TRY(mask_sweep(st, n, c->V, c->pred, c->masks32, c->scores, c->offsets, c->nodes, c->bounds));
+    } else {
+      const i64 m2 = (i64)n * c->Nloc;
+      hipLaunchKernelGGL(cg_fill_f32, dim3(grid_for(m2, 256) > 8192 ? 8192 : grid_for(m2, 256)), dim3(256), 0, st, c->R, 0.0f, m2);
+      hipLaunchKernelGGL(cg_scatter, dim3(grid_for(c->M, 256)), dim3(256), 0, st, c->R, c->spos, c->scores, c->M);
+      RG_LAUNCH_CHECK();
+      std::vector<WinPlan> wins = plan_windows(n - 1, 4);
+      for (size_t w = 0; w < wins.size(); ++w)
+        RG_TRY((launch_tr_rl<float, float, 0, 0>(wins[w].RL, c->R, c->R, coef<float>(1.0, 0.0, 1.0, 1.0),
+                                                 make_geom(n, wins[w]), nullptr, nullptr, st)));
+      hipLaunchKernelGGL(cg_fill_f32, dim3(1), dim3(64), 0, st, c->V, 0.0f, (i64)1);
+      for (int k = 1; k <= n; ++k) {
+        const i64 lo = c->bounds[k], ns = c->bounds[k + 1] - lo;
+        if (ns <= 0) continue;
+        hipLaunchKernelGGL(cg_slab_layer, dim3(grid_for(ns, 256)), dim3(256), 0, st, c->V, c->pred, c->R,
+                           c->nodes + lo, ns, n, c->Nloc);
+      }
+      RG_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(cg_backtrace, dim3(1), dim3(64), 0, st, c->pred, n, c->rank, (u64)(c->NN - 1));
+    float vf = 0.0f;
+    RG_HIP(hipMemcpyAsync(&vf, c->V + (c->NN - 1), 4, hipMemcpyDeviceToHost, st));
+    RG_HIP(hipMemcpyAsync(rank_pos, c->rank, (size_t)n * 4, hipMemcpyDeviceToHost, st));
+    RG_HIP(hipStreamSynchronize(st));
+    *vmax = (double)vf;
+    c->last_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return RG_OK;
+  }
+  // stream layout
+  const int mode = c->dtype;
+  int cur = 0, prev = 1;
+  const size_t vsz = mode == 0 ? 4 : 2;
+  RG_HIP(hipMemsetAsync(c->vbuf[prev], 0, vsz, st));
+  if (c->lbuf[prev]) RG_HIP(hipMemsetAsync(c->lbuf[prev], 0, 1, st));
+  RG_HIP(hipMemsetAsync(c->mbuf[prev], 0, 4, st));
+  RG_HIP(hipMemsetAsync(c->ebuf[prev], 0, 4, st));
+  RG_HIP(hipMemsetAsync(c->lstep, 0, (size_t)(n + 1) * 4, st));
+  for (int k = 1; k <= n; ++k) {
+    const u64 count = binom_u64(n, k);
+    const u64 grid = (count + CG_BLOCK - 1) / CG_BLOCK;
+    const int store = k >= c->k_pred;
+    uint8_t* pr = store ? c->predk[k] : nullptr;
+    switch (mode) {
+      case 0:
+        hipLaunchKernelGGL(cg_stream_layer<0>, dim3((unsigned)grid), dim3(CG_THREADS), 0, st, c->binom, n, k, count,
+                           c->vbuf[prev], c->lbuf[prev], c->mbuf[prev], c->ebuf[prev], c->vbuf[cur], c->lbuf[cur],
+                           c->mbuf[cur], c->ebuf[cur], pr, store, c->masks64, c->scores, c->offsets);
+        break;
+      case 1:
+        hipLaunchKernelGGL(cg_stream_layer<1>, dim3((unsigned)grid), dim3(CG_THREADS), 0, st, c->binom, n, k, count,
+                           c->vbuf[prev], c->lbuf[prev], c->mbuf[prev], c->ebuf[prev], c->vbuf[cur], c->lbuf[cur],
+                           c->mbuf[cur], c->ebuf[cur], pr, store, c->masks64, c->scores, c->offsets);
+        break;
+      default:
+        hipLaunchKernelGGL(cg_stream_layer<2>, dim3((unsigned)grid), dim3(CG_THREADS), 0, st, c->binom, n, k, count,
+                           c->vbuf[prev], c->lbuf[prev], c->mbuf[prev], c->ebuf[prev], c->vbuf[cur], c->lbuf[cur],
+                           c->mbuf[cur], c->ebuf[cur], pr, store, c->masks64, c->scores, c->offsets);
+        break;
+    }
+    if (mode != 0) hipLaunchKernelGGL(cg_layer_step, dim3(1), dim3(256), 0, st, c->ebuf[cur], grid, c->lstep + k);
+    RG_LAUNCH_CHECK();
+    std::swap(cur, prev);
+  }
+  // top value and the per-layer steps
+  float top[3] = {0, 0, 0};
+  uint16_t q16 = 0;
+  uint8_t q8 = 0;
+  std::vector<float> steps(n + 1, 0.0f);
+  if (mode == 0) {
+    RG_HIP(hipMemcpyAsync(&top[0], c->vbuf[prev], 4, hipMemcpyDeviceToHost, st));
+  } else {
+    RG_HIP(hipMemcpyAsync(&q16, c->vbuf[prev], 2, hipMemcpyDeviceToHost, st));
+    if (mode == 2) RG_HIP(hipMemcpyAsync(&q8, c->lbuf[prev], 1, hipMemcpyDeviceToHost, st));
+    RG_HIP(hipMemcpyAsync(&top[1], c->mbuf[prev], 4, hipMemcpyDeviceToHost, st));
+    RG_HIP(hipMemcpyAsync(&top[2], c->ebuf[prev], 4, hipMemcpyDeviceToHost, st));
+    RG_HIP(hipMemcpyAsync(steps.data(), c->lstep, (size_t)(n + 1) * 4, hipMemcpyDeviceToHost, st));
+  }
+  RG_HIP(hipStreamSynchronize(st));
+  double mg = 0.0;
+  if (mode == 0) {
+    *vmax = (double)top[0];
+  } else if (mode == 1) {
+    *vmax = (double)top[1] + (double)q16 * (double)top[2];
+    for (int k = 1; k <= n; ++k) mg += (double)steps[k];
+    mg *= 0.5;
+  } else {
+    const unsigned q = ((unsigned)q16 << 8) | (unsigned)q8;
+    *vmax = (double)top[1] + (double)q * (double)top[2];
+    for (int k = 1; k <= n; ++k) mg += (double)steps[k];
+  }
+  if (margin) *margin = mg;
+  // backtrace: stored predecessors down to layer k_pred, then the exact sub-lattice
+  u64 D = (n >= 64) ? ~0ull : ((1ull << n) - 1ull);
+  int pos = 0;
+  for (int k = n; k > c->k_pred; --k) {
+    const u64 r = colex_rank(D);
+    uint8_t x = 0;
+    RG_HIP(hipMemcpy(&x, c->predk[k] + r, 1, hipMemcpyDeviceToHost));
+    rank_pos[x] = pos++;
+    D ^= 1ull << x;
+  }
+  const int K = c->k_pred;
+  std::vector<int> elems;
+  for (int b = 0; b < n; ++b)
+    if ((D >> b) & 1ull) elems.push_back(b);
+  RG_CHECK((int)elems.size() == K, RG_EHIP, "stream backtrace reached a node of the wrong size");
+  std::vector<int> posof(n, -1);
+  for (int j = 0; j < K; ++j) posof[elems[j]] = j;
+  // rows whose menu lies inside D, relabelled to 0..K-1, x-major (stable by new label)
+  std::vector<std::vector<std::pair<u32, float>>> rows(K);
+  for (int x = 0; x < n; ++x) {
+    if (posof[x] < 0) continue;
+    for (u64 j = c->h_offsets[x]; j < c->h_offsets[x + 1]; ++j) {
+      const u64 m = c->h_masks[j];
+      if ((m & D) != m) continue;
+      u32 sm = 0;
+      for (int e = 0; e < K; ++e)
+        if ((m >> elems[e]) & 1ull) sm |= 1u << e;
+      rows[posof[x]].push_back({sm, scores_x[j]});
+    }
+  }
+  std::vector<u32> sm;
+  std::vector<float> ss;
+  std::vector<u64> so(K + 1, 0);
+  for (int x = 0; x < K; ++x) {
+    for (auto& pr : rows[x]) {
+      sm.push_back(pr.first);
+      ss.push_back(pr.second);
+    }
+    so[x + 1] = sm.size();
+  }
+  if (!sm.empty()) {
+    RG_HIP(hipMemcpyAsync(c->submasks, sm.data(), sm.size() * 4, hipMemcpyHostToDevice, st));
+    RG_HIP(hipMemcpyAsync(c->subscores, ss.data(), ss.size() * 4, hipMemcpyHostToDevice, st));
+  }
+  RG_HIP(hipMemcpyAsync(c->suboffsets, so.data(), (size_t)(K + 1) * 8, hipMemcpyHostToDevice, st));
+  RG_TRY(mask_sweep(st, K, c->subV, c->subpred, c->submasks, c->subscores, c->suboffsets, c->subnodes, c->subbounds));
+  hipLaunchKernelGGL(cg_backtrace, dim3(1), dim3(64), 0, st, c->subpred, K, c->rank, (u64)((1ull << K) - 1ull));
+  int32_t order[64];
+  RG_HIP(hipMemcpyAsync(order, c->rank, (size_t)K * 4, hipMemcpyDeviceToHost, st));
+  RG_HIP(hipStreamSynchronize(st));
+  for (int j = 0; j < K; ++j) rank_pos[elems[j]] = pos + order[j];
+  c->last_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return RG_OK;
+}
+
+extern "C" int rumgpu_cg_info(rumgpu_cg* c, size_t* device_bytes, double* last_kernel_ms) {
+  RG_CHECK(c, RG_EINVAL, "cg_info: null handle");
+  if (device_bytes) *device_bytes = c->device_bytes;
+  if (last_kernel_ms) *last_kernel_ms = c->last_ms;
+  return RG_OK;
+}

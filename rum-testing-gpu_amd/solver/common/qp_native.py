@@ -1,0 +1,62 @@
+"""Restricted-master QP of Flow-CG: min 1/2 lam'Q lam - h'lam subject to lam >= 0, sum lam = 1.
+
+Same method as the reference (solver/common/qp_native.py:37-100): an active set whose equality
+problem is solved by Cholesky,
+    lam_A = Q_AA^-1 (h_A - mu 1),   mu = (1'Q_AA^-1 h_A - 1) / (1'Q_AA^-1 1),
+dropping the most negative weight, else adding the most violated inactive column (gradient below
+-tol), else stopping; regularization reg on the diagonal; a final Euclidean simplex projection;
+support = weights above 1e-9. The working set warm-starts from the previous call's support.
+"""
+import numpy as np
+from scipy.linalg import cho_factor, cho_solve
+
+
+def project_simplex(v):
+    """Euclidean projection onto the probability simplex (sort-based threshold)."""
+    k = v.size
+    u = -np.sort(-v)
+    cs = np.cumsum(u) - 1.0
+    ok = u - cs / np.arange(1, k + 1) > 0
+    if not ok.any():
+        return np.full(k, 1.0 / k)
+    r = int(np.flatnonzero(ok)[-1])
+    return np.maximum(v - cs[r] / (r + 1), 0.0)
+
+
+def solve_qp_native(Q, h, A_init=None, max_outer=200, tol=1e-10, reg=1e-12):
+    """Active-set solve of the simplex QP; returns (lam, support list)."""
+    K = int(h.size)
+    work = [int(i) for i in (A_init if A_init is not None else []) if 0 <= int(i) < K]
+    if not work:
+        work = [int(np.argmax(h))]
+    lam_w, idx = None, None
+    for _ in range(max_outer):
+        idx = np.asarray(work, dtype=np.int64)
+        k = idx.size
+        Qw = Q[np.ix_(idx, idx)] + reg * np.eye(k)
+        b = np.column_stack([h[idx], np.ones(k)])
+        try:
+            sol = cho_solve(cho_factor(Qw, lower=True, check_finite=False), b, check_finite=False)
+        except np.linalg.LinAlgError:
+            sol = np.linalg.lstsq(Qw, b, rcond=None)[0]
+        qh, qe = sol[:, 0], sol[:, 1]
+        se = float(qe.sum())
+        mu = (float(qh.sum()) - 1.0) / se if abs(se) > 1e-30 else 0.0
+        lam_w = qh - mu * qe
+        if k and float(lam_w.min()) < -tol:
+            work.pop(int(np.argmin(lam_w)))
+            continue
+        if K > k:
+            free = np.ones(K, dtype=bool)
+            free[idx] = False
+            cand = np.flatnonzero(free)
+            grad = Q[np.ix_(cand, idx)] @ lam_w + mu - h[cand]
+            j = int(np.argmin(grad))
+            if grad[j] < -tol:
+                work.append(int(cand[j]))
+                continue
+        break
+    lam = np.zeros(K)
+    lam[idx] = lam_w
+    lam = project_simplex(lam)
+    return lam, [int(i) for i in work if lam[i] > 1e-9]

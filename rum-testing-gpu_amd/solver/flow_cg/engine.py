@@ -1,0 +1,219 @@
+"""Flow-CG host engine: the column-generation loop of the reference's solve_cgbfqp_cholbitset
+(solver/flow_cg/solver.py:28-279) over the GPU pricing of librumgpu (rumgpu_cg_*).
+
+Per column: the master QP on (Q, h) (common/qp_native.py), the mixture p_out updated by the
+change in lam, the scalars err^2 = lam'Q lam - 2 h'lam + p'p and inner = h'lam - lam'Q lam, one
+pricing sweep on the GPU with scores float32(p - p_out), the certified gap
+LB = sqrt(max(err^2 - 2 (Vmax + margin - inner), 0)) (a running max), the stop tests, and the
+priced ranking's 0/1 column. Q = V'V and h = V'p are grown from packed column bitsets
+(qp_bitset_incremental.py:36-141 of the reference): Q[k, j] = popcount(bits_k & bits_j).
+"""
+import ctypes
+import time
+
+import numpy as np
+
+from .. import _lib
+from ..common.qp_native import solve_qp_native
+
+LAYOUTS = {"mask": 0, "slab": 1, "stream": 2}
+DTYPES = {"float32": 0, "uint16": 1, "uint24": 2}
+
+
+def xmajor(menu, alt, n):
+    """(perm, masks, alts, offsets): rows stably sorted by alternative (cg_core.py:184-195)."""
+    alt = np.asarray(alt, dtype=np.int64)
+    perm = np.argsort(alt, kind="stable")
+    counts = np.bincount(alt, minlength=n)
+    offsets = np.concatenate(([0], np.cumsum(counts))).astype(np.uint64)
+    return perm, np.ascontiguousarray(np.asarray(menu, dtype=np.uint64)[perm]), alt[perm], offsets
+
+
+class Pricer:
+    """A rumgpu_cg object: price(scores) -> (vmax, rank_pos, margin)."""
+
+    def __init__(self, n, masks_x, offsets, layout="mask", dtype="float32", k_pred=None, device=0):
+        self.ctx = _lib.context(device)
+        self.lib = self.ctx.lib
+        self.n = int(n)
+        self.masks = _lib.c_array(masks_x, np.uint64)
+        self.offsets = _lib.c_array(offsets, np.uint64)
+        self.M = int(self.masks.size)
+        self.layout, self.dtype = layout, dtype
+        h = ctypes.c_void_p()
+        _lib.check(self.lib.rumgpu_cg_create(self.ctx.h, self.n, _lib.ptr(self.masks), _lib.ptr(self.offsets), self.M,
+                                             LAYOUTS[layout], DTYPES[dtype], 0 if k_pred is None else int(k_pred),
+                                             ctypes.byref(h)), "cg_create")
+        self.h = h
+        self.rank = np.empty(self.n, dtype=np.int32)
+
+    def price(self, scores_x):
+        s = _lib.c_array(scores_x, np.float32)
+        vmax, margin = ctypes.c_double(), ctypes.c_double()
+        _lib.check(self.lib.rumgpu_cg_price(self.h, _lib.ptr(s), ctypes.byref(vmax), _lib.ptr(self.rank),
+                                            ctypes.byref(margin)), "cg_price")
+        return vmax.value, self.rank.copy(), margin.value
+
+    def info(self):
+        b, ms = ctypes.c_size_t(), ctypes.c_double()
+        _lib.check(self.lib.rumgpu_cg_info(self.h, ctypes.byref(b), ctypes.byref(ms)), "cg_info")
+        return b.value, ms.value
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.lib.rumgpu_cg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Columns:
+    """Ranking columns as packed bitsets with the Gram matrix Q, h = V'p, and p_out = V lam."""
+
+    def __init__(self, M, p, max_K):
+        self.M, self.W = int(M), (int(M) + 63) // 64
+        self.p = np.asarray(p, dtype=np.float64)
+        self.max_K = int(max_K)
+        self.bits = np.zeros((self.max_K, self.W), dtype=np.uint64)
+        self.Q = np.zeros((self.max_K, self.max_K))
+        self.h = np.zeros(self.max_K)
+        self.pos = []
+        self.keys = []
+        self.K = 0
+        self.p_out = np.zeros(self.M)
+
+    def add(self, col):
+        k = self.K
+        if k >= self.max_K:
+            raise RuntimeError("max_K = %d columns exceeded" % self.max_K)
+        padded = np.zeros(self.W * 64, dtype=np.uint8)
+        padded[:self.M] = col
+        b = np.packbits(padded, bitorder="little").view(np.uint64)
+        self.bits[k] = b
+        if k:
+            cnt = np.bitwise_count(self.bits[:k] & b).sum(axis=1, dtype=np.int64).astype(np.float64)
+            self.Q[k, :k] = cnt
+            self.Q[:k, k] = cnt
+        self.Q[k, k] = float(np.bitwise_count(b).sum(dtype=np.int64))
+        nz = np.flatnonzero(col)
+        self.h[k] = float(self.p[nz].sum())
+        self.pos.append(nz)
+        self.keys.append(bytes(np.asarray(col, dtype=np.uint8)))
+        self.K += 1
+
+    def apply_delta(self, dlam):
+        for a in np.flatnonzero(np.abs(dlam) > 1e-12):
+            self.p_out[self.pos[int(a)]] += float(dlam[a])
+
+    def view(self):
+        return self.Q[:self.K, :self.K], self.h[:self.K]
+
+
+def ranking_column(memb, alt_x, rank_pos, n):
+    """0/1 column of a ranking: row = 1 iff its alternative is the menu's top member
+    (cg_core.py:63-80 semantics). memb: (M, n) bool membership of the x-major rows."""
+    top = np.argmin(np.where(memb, np.asarray(rank_pos)[None, :], n + 1), axis=1)
+    return (top == alt_x).astype(np.uint8)
+
+
+def solve_cg(n, menu, alt, p, pricer=None, layout="mask", dtype="float32", rc_tol=0.0, max_iters=4000,
+             time_limit_s=0.0, seed=0, verbose=False, phi_ub_target=None, gap_rel_tol=None, gap_abs_tol=None,
+             init_columns=None, init_lambda=None, early_ub_check=False, lb_stop=None):
+    """Column generation to the first stop (solver.py:28-279 semantics). Rows in input order; the
+    engine works x-major and returns p_out x-major with `perm` (input row of each x-major row)."""
+    t_setup = time.perf_counter()
+    perm, masks_x, alt_x, offsets = xmajor(menu, alt, n)
+    p32 = np.asarray(p, dtype=np.float64)[perm].astype(np.float32)
+    p_obs = p32.astype(np.float64)
+    M = p_obs.size
+    own = pricer is None
+    if own:
+        pricer = Pricer(n, masks_x, offsets, layout=layout, dtype=dtype)
+    memb = ((masks_x[:, None] >> np.arange(n, dtype=np.uint64)[None, :]) & np.uint64(1)).astype(bool)
+    setup_s = time.perf_counter() - t_setup
+    p_sq = float(p_obs @ p_obs)
+    n_init = len(init_columns) if init_columns is not None else 0
+    cols = Columns(M, p_obs, max_iters + 2 + n_init)
+    if n_init:
+        for c in init_columns:
+            cols.add(np.asarray(c, dtype=np.uint8))
+        lam = (np.asarray(init_lambda, dtype=np.float64).copy() if init_lambda is not None
+               else np.full(n_init, 1.0 / n_init))
+        lam = np.maximum(lam, 0.0)
+        lam /= lam.sum()
+        cols.apply_delta(lam)
+        A_warm = [int(i) for i in np.flatnonzero(lam > 1e-12)] or [0]
+    else:
+        rank0 = np.random.RandomState(seed).permutation(n)
+        cols.add(ranking_column(memb, alt_x, rank0, n))
+        lam = np.array([1.0])
+        cols.apply_delta(lam)
+        A_warm = [0]
+    UB, lb_best, gap_rel, rc_final = float("inf"), 0.0, float("inf"), float("nan")
+    stop = "max_iters"
+    margins, trace = [], []
+    qp_s = price_s = 0.0
+    t0 = time.perf_counter()
+    it = 0
+    for it in range(1, max_iters + 1):
+        if time_limit_s > 0 and time.perf_counter() - t0 > time_limit_s:
+            stop = "time_limit"
+            break
+        tq = time.perf_counter()
+        Qv, hv = cols.view()
+        lam_new, A_warm = solve_qp_native(Qv, hv, A_init=A_warm)
+        cols.apply_delta(lam_new - lam)
+        lam = lam_new
+        lQl = float(lam @ (Qv @ lam))
+        hl = float(hv @ lam)
+        err2 = lQl - 2.0 * hl + p_sq
+        inner = hl - lQl
+        qp_s += time.perf_counter() - tq
+        if early_ub_check and phi_ub_target is not None:
+            UB = float(np.sqrt(max(0.0, err2)))
+            if UB <= phi_ub_target:
+                stop = "phi_ub"
+                break
+        resid = p_obs - cols.p_out
+        tpz = time.perf_counter()
+        vmax, rank_pos, margin = pricer.price(resid.astype(np.float32))
+        price_s += time.perf_counter() - tpz
+        if margin:
+            margins.append(margin)
+        fw_gap = (vmax + margin) - inner
+        rc_final = fw_gap
+        UB = float(np.sqrt(max(0.0, err2)))
+        lb_best = max(lb_best, float(np.sqrt(max(0.0, err2 - 2.0 * fw_gap))))
+        gap_rel = (UB - lb_best) / lb_best if lb_best > 0 else float("inf")
+        trace.append((it, time.perf_counter() - t0, UB, lb_best, cols.K))
+        if verbose and (it <= 5 or it % 25 == 0):
+            print("[flow_cg %4d] K=%4d dist=%.6f rc=%.3g gap=%.2e" % (it, cols.K, UB, fw_gap, gap_rel), flush=True)
+        if phi_ub_target is not None and UB <= phi_ub_target:
+            stop = "phi_ub"
+            break
+        if lb_stop is not None and lb_best >= lb_stop:
+            stop = "lb_stop"
+            break
+        if gap_rel_tol is not None and gap_rel <= gap_rel_tol:
+            stop = "gap_rel"
+            break
+        if gap_abs_tol is not None and UB - lb_best <= gap_abs_tol:
+            stop = "gap_abs"
+            break
+        if fw_gap < rc_tol:
+            stop = "rc"
+            break
+        cols.add(ranking_column(memb, alt_x, rank_pos, n))
+        lam = np.concatenate([lam, [0.0]])
+    wall = time.perf_counter() - t0
+    dev_bytes = pricer.info()[0]
+    if own:
+        pricer.close()
+    return dict(wall_s=wall, setup_s=setup_s, trace=trace, margins=margins, iters=it, n_cols=cols.K, lam=lam,
+                p_out=cols.p_out.copy(), perm=perm, columns=[np.frombuffer(k, dtype=np.uint8).copy() for k in cols.keys],
+                rc_final=rc_final, distance=UB, lb=lb_best, gap_rel=gap_rel, stop_reason=stop, qp_s=qp_s,
+                price_s=price_s, device_bytes=dev_bytes, layout=pricer.layout, dtype=pricer.dtype)

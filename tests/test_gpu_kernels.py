@@ -1,0 +1,227 @@
+"""GPU parity of the HIP kernels against the CPU oracle, through the C ABI.
+
+Bit-exact where the operations and their order are the same (every kernel is built with
+-ffp-contract=off, the oracle too): the butterfly (window passes apply the bit stages in ascending
+order), the per-menu prox, the plain (reference-schedule) iteration, the mask pricing sweep and the
+stream sweeps. Tolerance-based where the order differs by design: the fused iteration (window
+rounds in alternating direction, z = b2 (x2 - xk)) within 1e-12 relative after 1 and 20
+iterations in float64; the certificates (compensated device reductions) within 1e-12 relative.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from solver import _lib, instances
+from solver.bm_admm.engine import Handle
+from solver.flow_cg.engine import Pricer, xmajor
+
+pytestmark = pytest.mark.gpu
+
+
+def _rows(n, frac, rng, kmin=2):
+    menus = [m for m in range(1 << n) if bin(m).count("1") >= kmin and rng.random() < frac]
+    om, ox = [], []
+    for m in menus:
+        for a in range(n):
+            if (m >> a) & 1:
+                om.append(m)
+                ox.append(a)
+    menu, alt = np.array(om, dtype=np.uint64), np.array(ox, dtype=np.int32)
+    p = rng.random(menu.size)
+    _, inv = np.unique(menu, return_inverse=True)
+    p = p / np.bincount(inv, weights=p)[inv]
+    return menu, alt, p
+
+
+def _medium(n, cells, rng):
+    """random menus of size >= 2 until about `cells` cells, all members observed"""
+    seen, om, ox = set(), [], []
+    while len(om) < cells:
+        m = int(rng.integers(0, 1 << n))
+        if bin(m).count("1") < 2 or m in seen:
+            continue
+        seen.add(m)
+        for a in range(n):
+            if (m >> a) & 1:
+                om.append(m)
+                ox.append(a)
+    menu, alt = np.array(om, dtype=np.uint64), np.array(ox, dtype=np.int32)
+    p = rng.random(menu.size)
+    _, inv = np.unique(menu, return_inverse=True)
+    return menu, alt, p / np.bincount(inv, weights=p)[inv]
+
+
+@pytest.mark.parametrize("n", [2, 3, 6, 11, 12, 15, 18, 19])
+def test_butterfly64_bit_exact(n):
+    ctx = _lib.context(0)
+    rng = np.random.default_rng(n)
+    for coefs in [(1.0, -1.0, 0.0, 1.0), (1.0, 0.0, -1.0, math.sqrt(2.0)), (0.3, -1.7, 2.1, 0.9)]:
+        v = rng.standard_normal(n << (n - 1))
+        want = O.butterfly(v, n, *coefs)
+        got = v.copy()
+        _lib.check(ctx.lib.rumgpu_butterfly64(ctx.h, n, _lib.ptr(got), *coefs))
+        assert np.array_equal(got, want), (n, coefs, np.abs(got - want).max())
+
+
+@pytest.mark.parametrize("n", [4, 12, 18])
+def test_butterfly32_bit_exact(n):
+    ctx = _lib.context(0)
+    v = np.random.default_rng(n).standard_normal(n << (n - 1)).astype(np.float32)
+    want = O.butterfly(v, n, 1.0, 1.0, 0.0, 1.0)
+    got = v.copy()
+    _lib.check(ctx.lib.rumgpu_butterfly32(ctx.h, n, _lib.ptr(got), 1.0, 1.0, 0.0, 1.0))
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("dtype", ["float64", "float32"])
+@pytest.mark.parametrize("n", [3, 7, 12])
+def test_prox_bit_exact(n, dtype):
+    rng = np.random.default_rng(5 + n)
+    menu, alt, p = _rows(n, 0.5, rng)
+    keep = rng.random(menu.size) < 0.75                   # partly observed menus: mixed curvature
+    menu, alt, p = menu[keep], alt[keep], p[keep]
+    h = Handle(n, menu, alt, p, dtype=dtype, beta=1.3)
+    dt = np.dtype(dtype)
+    xk = rng.standard_normal(n << (n - 1)).astype(dt).astype(np.float64)
+    got = h.prox_slab(xk)
+    obs = O.obs_index(n, menu, alt)
+    want = O.prox(n, xk.astype(dt), O.scatter(n, obs, p.astype(dt), dt), O.scatter(n, obs, np.ones(obs.size, dt), dt),
+                  dt.type(1.3))
+    assert np.array_equal(got.astype(dt), want)
+    h.close()
+
+
+def _oracle_state(n, menu, alt, p, beta, dtype):
+    return O.AdmmState(n, menu, alt, p, beta, dtype=dtype)
+
+
+@pytest.mark.parametrize("n", [5, 9, 12, 14])
+def test_plain_iterations_bit_exact(n):
+    rng = np.random.default_rng(50 + n)
+    menu, alt, p = _medium(n, min(4000, n << (n - 2)), rng)
+    beta = 0.9
+    h = Handle(n, menu, alt, p, dtype="float64", beta=beta, plain=True)
+    st = _oracle_state(n, menu, alt, p, beta, np.float64)
+    cold = h.get_state()
+    assert np.array_equal(cold["x2"], st.x2)
+    h.run(7)
+    st.run(7)
+    got = h.get_state()
+    for k in ("x2", "s", "y", "z"):
+        assert np.array_equal(got[k], getattr(st, k)), (k, np.abs(got[k] - getattr(st, k)).max())
+    h.close()
+
+
+@pytest.mark.parametrize("n", [4, 8, 11, 13, 18])
+def test_fused_iterations_match_oracle(n):
+    rng = np.random.default_rng(70 + n)
+    menu, alt, p = _medium(n, min(6000, n << (n - 2)), rng)
+    beta = 1.1
+    h = Handle(n, menu, alt, p, dtype="float64", beta=beta)
+    st = _oracle_state(n, menu, alt, p, beta, np.float64)
+    for k_it in (1, 19):
+        h.run(k_it)
+        st.run(k_it)
+        got = h.get_state()
+        for k in ("x2", "s", "y", "z"):
+            ref = getattr(st, k)
+            scale = max(1.0, float(np.abs(ref).max()))
+            err = float(np.abs(got[k] - ref).max()) / scale
+            assert err < 1e-11, (n, k_it, k, err)
+    h.close()
+
+
+def test_fused_graph_replay_matches_eager():
+    n = 14
+    rng = np.random.default_rng(3)
+    menu, alt, p = _medium(n, 3000, rng)
+    a = Handle(n, menu, alt, p, dtype="float64", beta=1.0)
+    b = Handle(n, menu, alt, p, dtype="float64", beta=1.0)
+    b.set_graph(10)
+    for _ in range(3):                          # replays with host work in between
+        a.run(10)
+        b.run(10)
+        sa, sb = a.get_state(), b.get_state()
+        for k in ("x2", "s", "y", "z"):
+            assert np.array_equal(sa[k], sb[k]), k
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("n,dtype", [(7, "float64"), (12, "float32"), (16, "float64")])
+def test_certificates_match_oracle(n, dtype):
+    rng = np.random.default_rng(90 + n)
+    menu, alt, p = _medium(n, min(5000, n << (n - 2)), rng)
+    h = Handle(n, menu, alt, p, dtype=dtype, beta=1.0)
+    h.run(40)
+    x2 = h.get_state()["x2"]
+    obs = O.obs_index(n, menu, alt)
+    ub, fit = h.push(want_fit=True)
+    ub_o, fit_o = O.push_ub64(n, x2, obs, p)
+    assert abs(ub - ub_o) <= 1e-12 * max(1.0, ub_o)
+    assert np.allclose(fit, fit_o, rtol=0, atol=1e-12)
+    lb = h.lb()
+    lb_o, _ = O.fw_lb64(n, obs, x2[obs], p)
+    assert abs(lb - lb_o) <= 1e-10 * max(1.0, lb_o)
+    phi = h.phi()
+    assert abs(phi - math.sqrt(math.fsum((x2[obs] - p) ** 2))) <= 1e-12 * max(1.0, phi)
+    ctx = _lib.context(0)
+    import ctypes
+    lbv = ctypes.c_double()
+    _lib.check(ctx.lib.rumgpu_lb64(ctx.h, n, _lib.ptr(menu), _lib.ptr(alt), menu.size, _lib.ptr(x2[obs].copy()),
+                                   _lib.ptr(p), ctypes.byref(lbv)))
+    assert abs(lbv.value - lb_o) <= 1e-10 * max(1.0, lb_o)
+    h.close()
+
+
+# ------------------------------------------------------------------------------------- pricing
+@pytest.mark.parametrize("n", [6, 10, 14, 18])
+def test_mask_pricing_bit_exact(n):
+    rng = np.random.default_rng(n)
+    menu, alt, p = _medium(n, min(8000, n << (n - 3)), rng)
+    perm, masks_x, _alts, offsets = xmajor(menu, alt, n)
+    pr = Pricer(n, masks_x, offsets, layout="mask")
+    for trial in range(3):
+        scores = rng.standard_normal(menu.size).astype(np.float32)
+        vmax, rank, margin = pr.price(scores)
+        vo, rank_o, V, pred = O.price_mask(n, masks_x, scores, offsets)
+        assert vmax == vo and np.array_equal(rank, rank_o) and margin == 0.0
+    pr.close()
+
+
+@pytest.mark.parametrize("n", [8, 13, 18])
+def test_slab_pricing_equals_mask_on_quantized_duals(n):
+    """scores = k 2^-12 with |k| < 2^7 and mp 2^7 < 2^24: every float32 partial sum is exact, so the
+    butterfly-order sums equal the row-order sums and the columns are identical."""
+    rng = np.random.default_rng(100 + n)
+    menu, alt, p = _medium(n, min(20000, n << (n - 3)), rng)
+    perm, masks_x, _alts, offsets = xmajor(menu, alt, n)
+    pm = Pricer(n, masks_x, offsets, layout="mask")
+    ps = Pricer(n, masks_x, offsets, layout="slab")
+    for trial in range(3):
+        scores = (rng.integers(-127, 128, size=menu.size) / 4096.0).astype(np.float32)
+        a = pm.price(scores)
+        b = ps.price(scores)
+        assert a[0] == b[0] and np.array_equal(a[1], b[1])
+    pm.close()
+    ps.close()
+
+
+@pytest.mark.parametrize("mode", ["float32", "uint16", "uint24"])
+@pytest.mark.parametrize("n", [9, 14, 17])
+def test_stream_pricing_bit_exact(n, mode):
+    rng = np.random.default_rng(200 + n)
+    menu, alt, p = _medium(n, min(3000, n << (n - 3)), rng)
+    perm, masks_x, _alts, offsets = xmajor(menu, alt, n)
+    pr = Pricer(n, masks_x, offsets, layout="stream", dtype=mode)
+    omode = {"float32": "f32", "uint16": "u16", "uint24": "u24"}[mode]
+    scores = rng.standard_normal(menu.size).astype(np.float32)
+    vmax, rank, margin = pr.price(scores)
+    vo, mo, _pred = O.price_stream(n, omode, masks_x, scores, offsets)
+    assert vmax == vo and abs(margin - mo) <= 1e-12 * max(1.0, mo)
+    if mode == "float32":
+        vmask, rank_mask, _V, _p = O.price_mask(n, masks_x, scores, offsets)
+        assert vmax == vmask and np.array_equal(rank, rank_mask)
+    pr.close()

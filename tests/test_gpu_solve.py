@@ -1,0 +1,106 @@
+"""End-to-end GPU solves through the unchanged surface (RUM.calculate_distance, Session,
+rum.bootstrap), checked against the oracle's exact distance (n = 8, every menu: the benchmark's
+small configuration) and the reference's certified optimum brackets (data/opt_refs.json)."""
+import json
+import math
+import os
+
+import numpy as np
+import pandas as pd
+import pytest
+
+from conftest import GOLDEN
+from oracle import oracle as O
+from solver import instances
+from solver.inference import cells_from_panel
+from solver.rum_solver import RUM
+
+pytestmark = pytest.mark.gpu
+
+_EXACT = {}
+
+
+def _config1_exact():
+    """Exact distance of the n = 8 all-menus instance (oracle: enumeration of the 8! rankings)."""
+    if "d" not in _EXACT:
+        cd, n = instances.config1()
+        d, lb, x = O.exact_projection(n, cd[:, 0].astype(np.int64), cd[:, 1].astype(np.int64), cd[:, 2], tol=1e-14)
+        _EXACT["d"] = (d, lb)
+    return _EXACT["d"]
+
+
+@pytest.mark.parametrize("method,args", [("bm_admm", {"dtype": "float64"}), ("bm_admm", {"dtype": "mixed"}),
+                                         ("flow_cg", {}), ("flow_cg", {"pricing": "slab"})])
+def test_config1_exact_distance(method, args):
+    cd, n = instances.config1()
+    d, lb_exact = _config1_exact()
+    assert d - lb_exact < 1e-10
+    rum = RUM.from_rows(cd, n)
+    r = rum.calculate_distance(abs_tol=1e-8, rel_tol=0.0, method=method, method_args=args, max_time=300)
+    print(r)
+    assert r.exit_flag in (1, 3), r.message
+    assert r.lb <= d + 1e-9 and r.ub >= d - 1e-9
+    assert r.ub - r.lb <= 1e-8
+    assert abs(r.distance - d) <= 1e-8
+    assert np.sqrt(((r.p_fit - rum.choice_probability) ** 2).sum()) == pytest.approx(r.ub, abs=1e-12)
+
+
+@pytest.mark.parametrize("key,method,args", [("n18_d1pct", "bm_admm", {}), ("n18_d1pct", "flow_cg", {}),
+                                             ("n18_sparse", "flow_cg", {}), ("n18_m39", "bm_admm", {"dtype": "mixed"})])
+def test_reference_opt_brackets(key, method, args):
+    refs = json.load(open(os.path.join(GOLDEN, "opt_refs_subset.json")))
+    ref = refs[key]
+    n, cat = int(key.split("_")[0][1:]), key.split("_", 1)[1]
+    cd, mp = instances.get_instance(n, cat)
+    rum = RUM.from_rows(cd, n)
+    r = rum.calculate_distance(abs_tol=0.0, rel_tol=1e-3, method=method, method_args=args, max_time=300)
+    print(r)
+    assert r.exit_flag in (2, 3), r.message
+    # both brackets hold the true distance, so they overlap
+    assert r.lb <= ref["opt"] + 1e-9 and r.ub >= ref["lb"] - 1e-9
+
+
+def test_session_thresholds_and_warm_start():
+    cd, n = instances.get_instance(14, "m8")
+    rum = RUM.from_rows(cd, n)
+    with rum.session(method="bm_admm") as s:
+        r0 = s.solve(rum.choice_probability, abs_tol=1e-6, rel_tol=0, keep=True)
+        assert r0.exit_flag in (1, 3)
+        assert s.fit is not None and s.fit.shape == (rum.M,)
+        p2 = np.clip(rum.choice_probability + 0.01 * np.random.default_rng(0).standard_normal(rum.M), 0, 1)
+        rw = s.solve(p2, abs_tol=1e-6, rel_tol=0, warm=True)
+        assert rw.exit_flag in (1, 3)
+        # a threshold above the distance stops at ub < below
+        rb = s.solve(p2, abs_tol=1e-9, rel_tol=0, warm=True, below=rw.ub * 1.5)
+        assert rb.exit_flag == 5 and rb.ub < rw.ub * 1.5
+        ra = s.solve(p2, abs_tol=1e-9, rel_tol=0, warm=True, above=rw.lb * 0.5)
+        assert ra.exit_flag == 5 and ra.lb >= rw.lb * 0.5
+    with rum.session(method="flow_cg") as s:
+        c0 = s.solve(rum.choice_probability, abs_tol=1e-6, rel_tol=0, keep=True)
+        cw = s.solve(p2, abs_tol=1e-6, rel_tol=0, warm=True)
+        assert c0.exit_flag in (1, 3) and cw.exit_flag in (1, 3)
+        assert abs(cw.distance - rw.distance) <= 2e-6
+
+
+def test_bootstrap_small_panel_end_to_end(tmp_path):
+    """DSR alt0 data (n = 25, 301 cells), B = 60: p-value, interval, record, resume."""
+    panel = pd.read_csv(os.path.join(GOLDEN, "dsr_overload_data_25_long.csv"))
+    menu, alt, p = cells_from_panel(panel, 25, alternatives=[0])
+    rum = RUM(menu, alt, p, num_alternatives=25)
+    rec = json.load(open(os.path.join(GOLDEN, "dsr_alt0_record.json")))
+    ck = str(tmp_path / "ck.json")
+    with rum.bootstrap(panel=panel, B=60, checkpoint=ck) as boot:
+        pv = boot.p_value()
+        print(pv)
+        # p_hat's certified bracket overlaps the stored one (both at abs 5e-4)
+        assert boot.L0 <= rec["p_hat"]["ub"] + 1e-9 and boot.U0 >= rec["p_hat"]["lb"] - 1e-9
+        assert boot.U0 - boot.L0 <= 5e-4
+        assert boot.fingerprints["data"] == rec["fingerprints"]["data"]
+        solved = int((boot.flag != 127).sum())
+        # every decided draw agrees with the stored run's decision for the same draw index (the
+        # first 60 draws of B = 60 are not the first 60 of B = 10,000: weights are drawn per B)
+        assert pv.n_below + pv.n_above + pv.n_undecided == 60
+    with rum.bootstrap(panel=panel, B=60, checkpoint=ck) as b2:
+        assert int((b2.flag != 127).sum()) == solved
+        pv2 = b2.p_value()
+        assert pv2.p == pv.p

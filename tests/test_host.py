@@ -1,0 +1,161 @@
+"""CPU tests of the host side: the C ABI library loads and exports every declared symbol, the
+instance generators reproduce the reference's fingerprints, the draws reproduce the stored DSR
+record's fingerprints, Hong-Li's decisions reproduce the stored p-value, and the RUM surface
+validates its input with the reference's messages."""
+import json
+import os
+import re
+import ctypes
+
+import numpy as np
+import pandas as pd
+import pytest
+
+from conftest import GOLDEN, ROOT
+from solver import _lib, defaults, instances, params
+from solver.inference import Draws, MenuDraws, cells_from_panel, hong_li
+from solver.inference.bootstrap import _sha
+from solver.rum_solver import RUM, choose_method, _plan_flow_cg, _plan_bm_admm
+
+
+def test_library_exports_every_declared_symbol():
+    header = open(os.path.join(ROOT, "include", "rumgpu.h")).read()
+    declared = set(re.findall(r"\b(rumgpu_\w+)\s*\(", header))
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    missing = [s for s in sorted(declared) if not hasattr(lib, s)]
+    assert not missing, missing
+    assert declared == set(_lib.SIGNATURES), declared ^ set(_lib.SIGNATURES)
+    assert _lib.load().rumgpu_version() == 1
+
+
+def test_generators_reproduce_reference_fingerprints():
+    refs = json.load(open(os.path.join(GOLDEN, "opt_refs_subset.json")))
+    for key in ("n18_sparse", "n18_m2", "n18_m39", "n18_d1pct", "n18_medium", "n19_sparse", "n20_sparse",
+                "n20_d1pct", "n25_sparse"):
+        n, cat = int(key.split("_")[0][1:]), key.split("_", 1)[1]
+        cd, mp = instances.get_instance(n, cat)
+        assert instances.fingerprint(cd) == refs[key]["cd_hash"] and mp == refs[key]["mp"], key
+
+
+def test_benchmark_configs_are_deterministic():
+    cd, n = instances.config1()
+    assert n == 8 and cd.shape == (1016, 3)
+    sums = {}
+    for m, p in zip(cd[:, 0], cd[:, 2]):
+        sums[int(m)] = sums.get(int(m), 0.0) + p
+    assert len(sums) == 247 and all(abs(v - 1) < 1e-12 for v in sums.values())
+    assert instances.fingerprint(cd) == instances.fingerprint(instances.config1()[0])
+    cd2, n2 = instances.config2()
+    assert n2 == 18 and abs(cd2.shape[0] - 23593) < 40
+
+
+def _dsr():
+    panel = pd.read_csv(os.path.join(GOLDEN, "dsr_overload_data_25_long.csv"))
+    menu, alt, p = cells_from_panel(panel, 25, alternatives=[0])
+    return panel, menu, alt, p
+
+
+def test_dsr_draws_reproduce_stored_fingerprints():
+    rec = json.load(open(os.path.join(GOLDEN, "dsr_alt0_record.json")))
+    panel, menu, alt, p = _dsr()
+    assert menu.size == 301
+    assert _sha(np.column_stack([menu.astype(np.float64), alt.astype(np.float64), p])) == rec["fingerprints"]["data"]
+    D = Draws(panel, menu, alt, p, B=10000, seed=0)
+    assert D.input_fingerprint == rec["fingerprints"]["panel"]
+    for _ in D.blocks():
+        pass
+    assert D.fingerprint == rec["fingerprints"]["draws"]
+    assert D.n_redraws == rec["panel"]["redraws"]
+
+
+def test_hong_li_reproduces_stored_decisions():
+    rec = json.load(open(os.path.join(GOLDEN, "dsr_alt0_record.json")))
+    s = rec["settings"]
+    r_n, eps_n, sc = hong_li.scaling(s["n"])
+    assert abs(r_n - s["r_n"]) < 1e-12 and abs(eps_n - s["eps_n"]) < 1e-15
+    L0, U0 = rec["p_hat"]["lb"], rec["p_hat"]["ub"]
+    tau = (sc * L0, sc * U0)
+    assert np.allclose(tau, rec["tau"], rtol=0, atol=1e-12)
+    dec = hong_li.decide(rec["draws"]["lb"], rec["draws"]["ub"], *tau)
+    p_lo, p_hi = hong_li.p_bracket(dec)
+    pv = rec["results"]["p_value"]["0.05"]
+    assert (int((dec == 0).sum()), int((dec == 1).sum()), int((dec == 2).sum())) == (pv["n_below"], pv["n_above"],
+                                                                                    pv["n_undecided"])
+    assert p_hi == pv["p"]
+
+
+def test_menu_draws_statistics():
+    rng = np.random.default_rng(7)
+    rows = []
+    for m in range(1, 32):
+        items = [a for a in range(5) if (m >> a) & 1]
+        if len(items) < 2:
+            continue
+        for r in range(60):
+            rows.append((m, int(rng.choice(items))))
+    rep = pd.DataFrame(rows, columns=["menu", "chosen"])
+    menu, alt, p = cells_from_panel(rep, 5)
+    D = MenuDraws(rep, menu, alt, p, B=2000, seed=0)
+    X = np.vstack([b for _, _, b in D.blocks()])
+    inv = np.unique(menu, return_inverse=True)[1]
+    sums = np.zeros((X.shape[0], inv.max() + 1))
+    np.add.at(sums.T, inv, X.T)
+    assert np.abs(sums - 1).max() < 1e-12
+    N = D.N[inv]
+    assert np.abs(X * N - np.round(X * N)).max() < 1e-9
+    D2 = MenuDraws(rep, menu, alt, p, B=2000, seed=0)
+    for _ in D2.blocks():
+        pass
+    assert D2.fingerprint == D.fingerprint
+
+
+def test_rum_input_errors_match_reference_messages():
+    with pytest.raises(ValueError, match="no observations"):
+        RUM([], [], [])
+    with pytest.raises(ValueError, match="row 0: empty menu"):
+        RUM([0], [0], [1.0])
+    with pytest.raises(ValueError, match="row 0: alternative 2 is not in its menu"):
+        RUM([3], [2], [1.0])
+    with pytest.raises(ValueError, match=r"row 0: \(menu 3, alternative 0\) appears more than once"):
+        RUM([3, 3], [0, 0], [0.5, 0.5])
+    with pytest.raises(ValueError, match="outside"):
+        RUM([3, 3], [0, 1], [1.5, 0.5])
+    rum = RUM([3, 3, 1], [0, 1, 0], [0.5, 0.5, 1.0])
+    assert rum.n == 2 and rum._single.size == 1
+    with pytest.raises(ValueError, match="unknown method"):
+        rum.calculate_distance(method="nope")
+
+
+def test_planner_rules_and_beta():
+    prof = dict(use_gpu=True, gpu=dict(supported=True, name="stub"), ram_bytes=64 << 30)
+    import solver.machine as machine
+    saved = machine.gpu_free_bytes
+    machine.gpu_free_bytes = lambda device=0: 250 << 30
+    try:
+        assert choose_method(18, 23568, False, prof)[0] == "bm_admm"
+        assert choose_method(18, 306, False, prof)[0] == "flow_cg"
+        assert choose_method(10, 5000, False, prof)[0] == "flow_cg"
+        a = _plan_bm_admm(18, 23568, False, prof, dict(defaults.METHOD_ARGS["bm_admm"]))
+        assert a["dtype"] == "float64" and a["use_graph"] is True
+        assert abs(a["beta"] - params.bm_beta_default(23568)) < 1e-15
+        a = _plan_flow_cg(25, 301, False, prof, dict(defaults.METHOD_ARGS["flow_cg"]))
+        assert (a["pricing"], a["dtype"]) == ("mask", "float32")
+        a = _plan_flow_cg(25, 301, False, prof, dict(defaults.METHOD_ARGS["flow_cg"], pricing="stream_u16"))
+        assert (a["pricing"], a["dtype"]) == ("stream", "uint16")
+        with pytest.raises(ValueError, match="contradicts"):
+            _plan_flow_cg(20, 10, False, prof, dict(defaults.METHOD_ARGS["flow_cg"], pricing="stream_u16",
+                                                    dtype="float32"))
+        machine.gpu_free_bytes = lambda device=0: 1 << 20
+        with pytest.raises(RuntimeError, match="needs"):
+            _plan_bm_admm(22, 10000, False, prof, dict(defaults.METHOD_ARGS["bm_admm"]))
+    finally:
+        machine.gpu_free_bytes = saved
+    # the reference's beta anchors (params.py of the reference: mp = 306 -> 0.6, 24k -> 5)
+    assert abs(params.bm_beta_default(306) - 0.6) < 0.05 and abs(params.bm_beta_default(24000) - 5.0) < 1.0
+
+
+def test_no_gpu_means_no_method():
+    rum = RUM.from_rows(instances.warmup_instance(6))
+    prof = dict(use_gpu=False, gpu=None, ram_bytes=1 << 34)
+    with pytest.raises(RuntimeError, match="no method can run"):
+        rum.calculate_distance(machine=prof)
