@@ -42,13 +42,17 @@ def _parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--dtype", default="mixed", choices=["mixed", "float64", "float32"])
+    ap.add_argument("--dtype", default="float64", choices=["mixed", "float64", "float32"])
+    ap.add_argument("--beta", default="cert", help="'cert' (certificate rule), 'ref' (race rule) or a number")
     ap.add_argument("--plain-steps", type=int, default=1)
     ap.add_argument("--abs-tol", type=float, default=1e-5)
     ap.add_argument("--max-time", type=float, default=120.0)
     ap.add_argument("--profile-iters", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=20)
+    ap.add_argument("--traffic-run", action="store_true",
+                    help="only the workload for HBM counters: --profile-iters fused iterations on the n=18 "
+                         "instance, then a device copy of the 4 state slabs (a copy kernel of known size)")
     return ap.parse_args()
 
 
@@ -83,6 +87,19 @@ def main():
 
     cd, n = instances.config2()
     rum = RUM.from_rows(cd, n)
+    if a.traffic_run:
+        from solver import params
+        g = rum._geom
+        h = Handle(n, g.menu, g.alternative.astype(np.int32), rum.choice_probability[rum._rest], dtype=a.dtype,
+                   beta=params.bm_beta_cert(g.M))
+        h2 = Handle(n, g.menu, g.alternative.astype(np.int32), rum.choice_probability[rum._rest], dtype=a.dtype,
+                    beta=params.bm_beta_cert(g.M))
+        h.run(a.profile_iters)
+        h2.copy_state_from(h)           # 4 x ew_copy: each reads and writes n 2^(n-1) values
+        print("[bench] traffic run: %d iterations, state copy of 4 x %d bytes" % (a.profile_iters,
+                                                                                 (n << (n - 1)) * h.dtype.itemsize),
+              file=sys.stderr, flush=True)
+        return
     p = rum.choice_probability
     mp = rum._geom.M
     m2 = n << (n - 1)
@@ -90,7 +107,10 @@ def main():
                                                                              instances.fingerprint(cd)),
           file=sys.stderr, flush=True)
 
-    sess = rum.session(method="bm_admm", method_args={"dtype": a.dtype})
+    margs = {"dtype": a.dtype}
+    if a.beta != "ref":
+        margs["beta"] = a.beta if a.beta == "cert" else float(a.beta)
+    sess = rum.session(method="bm_admm", method_args=margs)
     results = []
     for i in range(a.warmup):
         r = sess.solve(p, abs_tol=a.abs_tol, rel_tol=0.0, max_time=a.max_time)
@@ -150,8 +170,9 @@ def main():
     # plain: the reference schedule in float64, same hooks and stops
     plain = None
     if a.plain_steps > 0:
+        from solver import params
         h = Handle(n, rum._geom.menu, rum._geom.alternative.astype(np.int32), p[rum._rest], dtype="float64",
-                   beta=r_last.method_args["beta"], plain=True)
+                   beta=params.bm_beta_default(mp), plain=True)
         st = dict(ub=math.inf, lb=0.0)
 
         def push(hh):
@@ -178,7 +199,8 @@ def main():
                   file=sys.stderr, flush=True)
         h.close()
         plain = dict(value=round(len(times) / sum(times), 5), unit="certified solves/s", dtype="f64",
-                     schedule="reference (plain kernels)", iterations=iters, seconds=[round(t, 3) for t in times],
+                     schedule="reference (plain kernels)", beta=params.bm_beta_default(mp),
+                     iterations=iters, seconds=[round(t, 3) for t in times],
                      certified=bool(rp["stop_reason"] == "gap_abs"))
 
     cpu = None
@@ -189,7 +211,7 @@ def main():
                 warmup=a.warmup, ms_per_step=round(1e3 * elapsed / a.steps, 2), higher_is_better=True,
                 scaling="weak", vs_baseline=None, dtype={"mixed": "f32+f64", "float64": "f64", "float32": "f32"}[a.dtype],
                 data="synthetic",
-                config=dict(workload="certified BM-ADMM solve (abs gap 1e-5, rel 0), n=18, 1% density "
+                config=dict(workload="certified BM-ADMM solve (abs gap 1e-5, rel 0), n=18, 1%% density "
                                      "(config2 seed 18001, %d cells)" % mp, n=n, cells=mp, abs_tol=a.abs_tol,
                             beta=r_last.method_args["beta"], check=50,
                             iterations=[r.iterations for r in results],

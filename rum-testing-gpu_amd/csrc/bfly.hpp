@@ -97,12 +97,15 @@ struct Tile {
   T* lds;
   bool active;
 
-  __device__ __forceinline__ void init(const WinGeom& g, T* lds_all) {
+  // start_last: begin in the last round's layout, where a window at bit 0 reads contiguous memory
+  // across the lanes (the first round's layout gives each thread R consecutive values instead);
+  // the first transform then runs its rounds in descending order.
+  __device__ __forceinline__ void init(const WinGeom& g, T* lds_all, bool start_last = false) {
     s = g.s;
     B = g.B;
     C = g.C;
     nr = (B + RL - 1) / RL;
-    cur = 0;
+    cur = (start_last && C == 0) ? nr - 1 : 0;
     const int tl = threadIdx.x / g.TT;
     const int t = threadIdx.x - tl * g.TT;
     const i64 tile = (i64)blockIdx.x * g.TPB + tl;
@@ -162,6 +165,12 @@ struct Tile {
         }
       }
     }
+  }
+  // For a window at bit 0, move to the last round's layout, where a wavefront's lanes hold
+  // consecutive slab positions for every register (one contiguous 512 B run per access for
+  // float64); windows with columns are contiguous along the lanes in every layout already.
+  __device__ __forceinline__ void coalesce() {
+    if (C == 0 && cur != nr - 1) exchange(nr - 1);
   }
   // The whole window transform; starts from the current layout and runs the rounds in the
   // direction that needs no exchange before the first one.

@@ -48,9 +48,10 @@ template <typename T, int RL, int KIND>
 __global__ __launch_bounds__(512) void bm_pass(PassArgs<T> P) {
   extern __shared__ __align__(16) unsigned char smem[];
   Tile<T, RL> t;
-  t.init(P.g, reinterpret_cast<T*>(smem));
+  t.init(P.g, reinterpret_cast<T*>(smem), true);
   const T one = (T)1;
   constexpr int R = Tile<T, RL>::R;
+  // every global access happens in a lane-contiguous layout (t.coalesce() before it)
   if constexpr (KIND == K_FIRST || KIND == K_FULL_FIRST) {
     if (t.active) {
 #pragma unroll
@@ -64,6 +65,7 @@ __global__ __launch_bounds__(512) void bm_pass(PassArgs<T> P) {
   }
   if constexpr (KIND == K_SY || KIND == K_SYONLY || KIND == K_FULL) {
     t.transform(P.cA);                                                      // Ax (raw A)
+    t.coalesce();
     if (t.active) {
 #pragma unroll
       for (int j = 0; j < R; ++j) {
@@ -83,6 +85,7 @@ __global__ __launch_bounds__(512) void bm_pass(PassArgs<T> P) {
   if constexpr (KIND == K_FIRST || KIND == K_SY || KIND == K_RHS || KIND == K_FULL || KIND == K_FULL_FIRST)
     t.transform(P.cAp);                                                     // raw A'
   if constexpr (KIND == K_RHS || KIND == K_FULL || KIND == K_FULL_FIRST) {
+    t.coalesce();
     if (t.active) {
 #pragma unroll
       for (int j = 0; j < R; ++j) {
@@ -94,6 +97,7 @@ __global__ __launch_bounds__(512) void bm_pass(PassArgs<T> P) {
   }
   if constexpr (KIND == K_D) t.transform(P.cQt);
   if constexpr (KIND == K_D || KIND == K_FULL || KIND == K_FULL_FIRST) {
+    // inv_gl is one item block (2^(n-1) values, L2-resident): read it in any layout
     if (t.active) {
 #pragma unroll
       for (int j = 0; j < R; ++j) t.v[j] *= P.inv_gl[t.rest(j)];            // diag_mul
@@ -102,6 +106,7 @@ __global__ __launch_bounds__(512) void bm_pass(PassArgs<T> P) {
   }
   if constexpr (KIND == K_XK) t.transform(P.cQ);
   if constexpr (KIND == K_XK || KIND == K_FULL || KIND == K_FULL_FIRST) {
+    t.coalesce();
     if (t.active) {
 #pragma unroll
       for (int j = 0; j < R; ++j) {
@@ -112,6 +117,7 @@ __global__ __launch_bounds__(512) void bm_pass(PassArgs<T> P) {
   }
   if constexpr (KIND == K_XK) t.transform(P.cA);
   if constexpr (KIND == K_T) t.transform(P.cT);
+  t.coalesce();
   t.store(P.tb);
 }
 
@@ -145,6 +151,83 @@ __device__ __forceinline__ void sort_desc(double (&key)[N], int (&pay)[N]) {
           if (i + j + k < N && (i + j) / (2 * p) == (i + j + k) / (2 * p)) ce_desc<N>(key, pay, i + j, i + j + k);
         }
       }
+    }
+  }
+}
+
+// Iteration prox (the hot one): x2 = prox(xk) and z = b2 (x2 - xk) for every menu. The
+// threshold comes from Michelot's active-set iteration: start from every member, drop those at or
+// below the threshold, recompute, until the set is stable. It reaches the same support and
+// threshold as the reference's sorted scan without sorting, so only the members' values (T) and
+// keys (double) stay in registers.
+template <typename T, int N>
+__global__ __launch_bounds__(256, 4) void bm_prox_iter(const T* __restrict__ xk, T* __restrict__ out, T* __restrict__ z,
+                                                    const u32* __restrict__ obsmask, const int* __restrict__ obsoff,
+                                                    const double* __restrict__ pc, T b2) {
+  const u32 S = blockIdx.x * blockDim.x + threadIdx.x;
+  if (S == 0 || S >= (1u << N)) return;
+  constexpr u32 Nloc = 1u << (N - 1);     // slab positions fit 32 bits for n <= 26
+  const int k = __popc(S);
+  if (k == 1) {
+    const u32 pos = (u32)(__ffs((int)S) - 1) * Nloc;
+    out[pos] = (T)1;
+    z[pos] = b2 * ((T)1 - xk[pos]);
+    return;
+  }
+  const u32 om = obsmask[S];
+  const int off = om ? obsoff[S] : 0;
+  const bool mixed = (om != 0) && (om != S);
+  const double c_obs = (double)b2 + 2.0, c_un = (double)b2;
+  double key[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    key[i] = 0.0;
+    if ((S >> i) & 1u) {
+      const u32 pos = (u32)i * Nloc + (u32)compress_bit(S & ~(1u << i), i);
+      const T xv = xk[pos];
+      if (om == 0) {
+        key[i] = (double)xv;                       // (b2 v) / b2 = v: no observation in the menu
+      } else {
+        const bool ob = (om >> i) & 1u;
+        const T xhv = ob ? (T)pc[off + __popc(om & ((1u << i) - 1u))] : (T)0;
+        key[i] = mixed ? (double)b2 * (double)xv + 2.0 * (ob ? 1.0 : 0.0) * (double)xhv
+                       : (double)((b2 * xv + (T)2 * xhv) / (b2 + (T)2));
+      }
+    }
+  }
+  u32 act = S;
+  double thr = 0.0;
+  // curvature c_i = b2 + 2 w_i enters through its reciprocal (two constants per call)
+  const double r_obs = 1.0 / c_obs, r_un = 1.0 / c_un;
+  const u32 wobs = mixed ? om : 0u;
+#pragma unroll 1
+  for (int it = 0; it <= N; ++it) {
+    double s1 = 0.0, s2 = 0.0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      if ((act >> i) & 1u) {
+        const double r = mixed ? (((wobs >> i) & 1u) ? r_obs : r_un) : 1.0;
+        s1 += key[i] * r;
+        s2 += r;
+      }
+    }
+    thr = (s1 - 1.0) / s2;
+    u32 nact = 0;
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+      if (((act >> i) & 1u) && key[i] > thr) nact |= 1u << i;
+    if (nact == act || nact == 0) break;
+    act = nact;
+  }
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    if ((S >> i) & 1u) {
+      const double r = mixed ? (((wobs >> i) & 1u) ? r_obs : r_un) : 1.0;
+      const double yv = (key[i] - thr) * r;
+      const T o = yv > 0.0 ? (T)yv : (T)0.0;
+      const u32 pos = (u32)i * Nloc + (u32)compress_bit(S & ~(1u << i), i);
+      out[pos] = o;
+      z[pos] = b2 * (o - xk[pos]);                 // re-read: the line was just loaded
     }
   }
 }
@@ -193,29 +276,31 @@ __global__ __launch_bounds__(256) void bm_prox(const T* __restrict__ xk, T* __re
     }
     pay[i] = i | (ob ? 256 : 0);
   }
-  sort_desc<N>(key, pay);
   double thr = 0.0;
-  if (mixed) {
-    double stc = 0.0, sic = 0.0;
-    const double c_obs = (double)b2 + 2.0, c_un = (double)b2 + 0.0;
+  {
+    sort_desc<N>(key, pay);
+    if (mixed) {
+      double stc = 0.0, sic = 0.0;
+      const double c_obs = (double)b2 + 2.0, c_un = (double)b2 + 0.0;
 #pragma unroll
-    for (int a2 = 0; a2 < N; ++a2) {
-      if (a2 < k) {
-        const double c = (pay[a2] & 256) ? c_obs : c_un;
-        stc += key[a2] / c;
-        sic += 1.0 / c;
-        const double th = (stc - 1.0) / sic;
-        if (key[a2] - th > 0.0) thr = th;
+      for (int a2 = 0; a2 < N; ++a2) {
+        if (a2 < k) {
+          const double c = (pay[a2] & 256) ? c_obs : c_un;
+          stc += key[a2] / c;
+          sic += 1.0 / c;
+          const double th = (stc - 1.0) / sic;
+          if (key[a2] - th > 0.0) thr = th;
+        }
       }
-    }
-  } else {
-    double css = 0.0;
+    } else {
+      double css = 0.0;
 #pragma unroll
-    for (int a2 = 0; a2 < N; ++a2) {
-      if (a2 < k) {
-        css += key[a2];
-        const double th = (css - 1.0) / (double)(a2 + 1);
-        if (key[a2] - th > 0.0) thr = th;
+      for (int a2 = 0; a2 < N; ++a2) {
+        if (a2 < k) {
+          css += key[a2];
+          const double th = (css - 1.0) / (double)(a2 + 1);
+          if (key[a2] - th > 0.0) thr = th;
+        }
       }
     }
   }
@@ -446,6 +531,7 @@ int launch_pass_rl(int RL, const PassArgs<T>& P, hipStream_t st) {
     case 2: hipLaunchKernelGGL((bm_pass<T, 2, KIND>), grid, block, shm, st, P); break;
     case 3: hipLaunchKernelGGL((bm_pass<T, 3, KIND>), grid, block, shm, st, P); break;
     case 4: hipLaunchKernelGGL((bm_pass<T, 4, KIND>), grid, block, shm, st, P); break;
+    case 5: hipLaunchKernelGGL((bm_pass<T, 5, KIND>), grid, block, shm, st, P); break;
     default: rg_set_error("bad RL %d", RL); return RG_EINVAL;
   }
   RG_LAUNCH_CHECK();
@@ -474,8 +560,13 @@ int launch_prox(int n, const T* xk, T* out, T* z, const u32* om, const int* oo, 
                 hipStream_t st) {
   const i64 NN = 1ll << n;
   const dim3 grid(grid_for(NN, 256)), block(256);
-#define RG_PROX_CASE(NV) \
-  case NV: hipLaunchKernelGGL((bm_prox<T, NV, MODE>), grid, block, 0, st, xk, out, z, om, oo, pc, b2); break;
+#define RG_PROX_CASE(NV)                                                                                   \
+  case NV:                                                                                                 \
+    if constexpr (MODE == 0)                                                                               \
+      hipLaunchKernelGGL((bm_prox_iter<T, NV>), grid, block, 0, st, xk, out, z, om, oo, pc, b2);           \
+    else                                                                                                   \
+      hipLaunchKernelGGL((bm_prox<T, NV, MODE>), grid, block, 0, st, xk, out, z, om, oo, pc, b2);          \
+    break;
   switch (n) {
     RG_PROX_CASE(2) RG_PROX_CASE(3) RG_PROX_CASE(4) RG_PROX_CASE(5) RG_PROX_CASE(6) RG_PROX_CASE(7)
     RG_PROX_CASE(8) RG_PROX_CASE(9) RG_PROX_CASE(10) RG_PROX_CASE(11) RG_PROX_CASE(12) RG_PROX_CASE(13)
@@ -908,11 +999,11 @@ extern "C" int rumgpu_bm_create(rumgpu_ctx* ctx, int n, const uint64_t* menu, co
     for (int c = 0; c <= n; ++c) b->bounds[c + 1] = b->bounds[c] + cnt[c];
     std::vector<i64> fill(b->bounds.begin(), b->bounds.end());
     for (i64 v = 0; v < b->NN; ++v) hord[fill[__builtin_popcountll(v)]++] = (int)v;
-    if (hipMemcpy(b->order, hord.data(), (size_t)b->NN * 4, hipMemcpyHostToDevice) != hipSuccess) return fail(RG_EHIP);
+    if (rg_copy(ctx->stream, b->order, hord.data(), (size_t)b->NN * 4, hipMemcpyHostToDevice) != hipSuccess) return fail(RG_EHIP);
   }
-  if (hipMemcpy(b->obsmask, hmask.data(), (size_t)b->NN * 4, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(b->obsoff, hoff.data(), (size_t)b->NN * 4, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(b->obs, hobs.data(), (size_t)M * 8, hipMemcpyHostToDevice) != hipSuccess) {
+  if (rg_copy(ctx->stream, b->obsmask, hmask.data(), (size_t)b->NN * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      rg_copy(ctx->stream, b->obsoff, hoff.data(), (size_t)b->NN * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      rg_copy(ctx->stream, b->obs, hobs.data(), (size_t)M * 8, hipMemcpyHostToDevice) != hipSuccess) {
     rg_set_error("bm_create: upload failed");
     return fail(RG_EHIP);
   }
@@ -1102,14 +1193,14 @@ extern "C" int rumgpu_bm_push_ub64(rumgpu_bm* b, double* ub, double* p_fit) {
   RG_CHECK(b && ub, RG_EINVAL, "bm_push_ub64: null argument");
   RG_TRY(BM_DISPATCH(b, push_impl, b, ub));
   if (p_fit) {
-    RG_HIP(hipMemcpy(p_fit, b->fit, (size_t)b->M * 8, hipMemcpyDeviceToHost));
+    RG_HIP(rg_copy(b->ctx->stream, p_fit, b->fit, (size_t)b->M * 8, hipMemcpyDeviceToHost));
   }
   return RG_OK;
 }
 
 extern "C" int rumgpu_bm_get_fit(rumgpu_bm* b, double* p_fit) {
   RG_CHECK(b && p_fit, RG_EINVAL, "bm_get_fit: null argument");
-  RG_HIP(hipMemcpy(p_fit, b->fit, (size_t)b->M * 8, hipMemcpyDeviceToHost));
+  RG_HIP(rg_copy(b->ctx->stream, p_fit, b->fit, (size_t)b->M * 8, hipMemcpyDeviceToHost));
   return RG_OK;
 }
 
@@ -1122,7 +1213,7 @@ extern "C" int rumgpu_bm_x_obs(rumgpu_bm* b, double* out) {
   RG_CHECK(b && out, RG_EINVAL, "bm_x_obs: null argument");
   double sq = 0.0;
   RG_TRY(b->isz == 8 ? phi_impl<double>(b, b->x2, &sq, 1) : phi_impl<float>(b, b->x2, &sq, 1));
-  RG_HIP(hipMemcpy(out, b->fit, (size_t)b->M * 8, hipMemcpyDeviceToHost));
+  RG_HIP(rg_copy(b->ctx->stream, out, b->fit, (size_t)b->M * 8, hipMemcpyDeviceToHost));
   return RG_OK;
 }
 
@@ -1144,13 +1235,13 @@ extern "C" int rumgpu_butterfly64(rumgpu_ctx* ctx, int n, double* v, double a, d
   RG_TRY(rg_malloc((void**)&dv, (size_t)m2 * 8, "butterfly slab"));
   hipStream_t st = ctx->stream;
   int rc = RG_OK;
-  if (hipMemcpy(dv, v, (size_t)m2 * 8, hipMemcpyHostToDevice) != hipSuccess) rc = RG_EHIP;
+  if (rg_copy(ctx->stream, dv, v, (size_t)m2 * 8, hipMemcpyHostToDevice) != hipSuccess) rc = RG_EHIP;
   std::vector<WinPlan> wins = plan_windows(n - 1, 8);
   for (size_t w = 0; w < wins.size() && rc == RG_OK; ++w) {
     const WinGeom g = make_geom(n, wins[w]);
     rc = launch_tr_rl<double, double, 0, 0>(wins[w].RL, dv, dv, coef<double>(a, bcoef, c, d), g, nullptr, nullptr, st);
   }
-  if (rc == RG_OK && hipMemcpy(v, dv, (size_t)m2 * 8, hipMemcpyDeviceToHost) != hipSuccess) rc = RG_EHIP;
+  if (rc == RG_OK && rg_copy(ctx->stream, v, dv, (size_t)m2 * 8, hipMemcpyDeviceToHost) != hipSuccess) rc = RG_EHIP;
   rg_free(dv);
   if (rc == RG_EHIP) rg_set_error("butterfly64: HIP error");
   return rc;
@@ -1163,13 +1254,13 @@ extern "C" int rumgpu_butterfly32(rumgpu_ctx* ctx, int n, float* v, float a, flo
   RG_TRY(rg_malloc((void**)&dv, (size_t)m2 * 4, "butterfly slab"));
   hipStream_t st = ctx->stream;
   int rc = RG_OK;
-  if (hipMemcpy(dv, v, (size_t)m2 * 4, hipMemcpyHostToDevice) != hipSuccess) rc = RG_EHIP;
+  if (rg_copy(ctx->stream, dv, v, (size_t)m2 * 4, hipMemcpyHostToDevice) != hipSuccess) rc = RG_EHIP;
   std::vector<WinPlan> wins = plan_windows(n - 1, 4);
   for (size_t w = 0; w < wins.size() && rc == RG_OK; ++w) {
     const WinGeom g = make_geom(n, wins[w]);
     rc = launch_tr_rl<float, float, 0, 0>(wins[w].RL, dv, dv, coef<float>(a, bcoef, c, d), g, nullptr, nullptr, st);
   }
-  if (rc == RG_OK && hipMemcpy(v, dv, (size_t)m2 * 4, hipMemcpyDeviceToHost) != hipSuccess) rc = RG_EHIP;
+  if (rc == RG_OK && rg_copy(ctx->stream, v, dv, (size_t)m2 * 4, hipMemcpyDeviceToHost) != hipSuccess) rc = RG_EHIP;
   rg_free(dv);
   if (rc == RG_EHIP) rg_set_error("butterfly32: HIP error");
   return rc;
@@ -1184,16 +1275,16 @@ extern "C" int rumgpu_prox_slab(rumgpu_bm* b, const double* xk, double* out) {
   const i64 m2 = b->m2;
   int rc = RG_OK;
   if (b->isz == 8) {
-    RG_HIP(hipMemcpy(b->xk, xk, (size_t)m2 * 8, hipMemcpyHostToDevice));
+    RG_HIP(rg_copy(b->ctx->stream, b->xk, xk, (size_t)m2 * 8, hipMemcpyHostToDevice));
     rc = launch_prox<double, 2>(b->n, (const double*)b->xk, (double*)b->tb, nullptr, b->obsmask, b->obsoff, b->pc, b->beta, st);
-    if (rc == RG_OK) RG_HIP(hipMemcpy(out, b->tb, (size_t)m2 * 8, hipMemcpyDeviceToHost));
+    if (rc == RG_OK) RG_HIP(rg_copy(b->ctx->stream, out, b->tb, (size_t)m2 * 8, hipMemcpyDeviceToHost));
   } else {
     std::vector<float> tmp(m2);
     for (i64 i = 0; i < m2; ++i) tmp[i] = (float)xk[i];
-    RG_HIP(hipMemcpy(b->xk, tmp.data(), (size_t)m2 * 4, hipMemcpyHostToDevice));
+    RG_HIP(rg_copy(b->ctx->stream, b->xk, tmp.data(), (size_t)m2 * 4, hipMemcpyHostToDevice));
     rc = launch_prox<float, 2>(b->n, (const float*)b->xk, (float*)b->tb, nullptr, b->obsmask, b->obsoff, b->pc, (float)b->beta, st);
     if (rc == RG_OK) {
-      RG_HIP(hipMemcpy(tmp.data(), b->tb, (size_t)m2 * 4, hipMemcpyDeviceToHost));
+      RG_HIP(rg_copy(b->ctx->stream, tmp.data(), b->tb, (size_t)m2 * 4, hipMemcpyDeviceToHost));
       for (i64 i = 0; i < m2; ++i) out[i] = tmp[i];
     }
   }
@@ -1236,10 +1327,10 @@ extern "C" int rumgpu_lb64(rumgpu_ctx* ctx, int n, const uint64_t* menu, const i
     // y is a float64 "slab" of M values read through obs: gather index = identity
     std::vector<i64> ident(M);
     for (i64 j = 0; j < M; ++j) ident[j] = j;
-    if (hipMemcpy(order, hord.data(), (size_t)NN * 4, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(obs, hobs.data(), (size_t)M * 8, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(dy, y, (size_t)M * 8, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(dpv, p, (size_t)M * 8, hipMemcpyHostToDevice) != hipSuccess) {
+    if (rg_copy(ctx->stream, order, hord.data(), (size_t)NN * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        rg_copy(ctx->stream, obs, hobs.data(), (size_t)M * 8, hipMemcpyHostToDevice) != hipSuccess ||
+        rg_copy(ctx->stream, dy, y, (size_t)M * 8, hipMemcpyHostToDevice) != hipSuccess ||
+        rg_copy(ctx->stream, dpv, p, (size_t)M * 8, hipMemcpyHostToDevice) != hipSuccess) {
       rc = RG_EHIP;
       rg_set_error("lb64: upload failed");
     }

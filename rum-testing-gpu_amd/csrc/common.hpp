@@ -75,4 +75,12 @@ __host__ __device__ inline u64 expand_bit(u64 rest, int i) {
   return (rest & ((1ull << i) - 1ull)) | ((rest >> i) << (i + 1));
 }
 
+// Stream-ordered copy that returns when the copy is done: the engines' streams are non-blocking,
+// so a plain hipMemcpy (null stream) would not wait for their kernels.
+inline hipError_t rg_copy(hipStream_t st, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, st);
+  if (e != hipSuccess) return e;
+  return hipStreamSynchronize(st);
+}
+
 inline unsigned grid_for(i64 nthreads, int block) { return (unsigned)((nthreads + block - 1) / block); }

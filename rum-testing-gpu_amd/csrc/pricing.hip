@@ -367,19 +367,19 @@ extern "C" int rumgpu_cg_create(rumgpu_ctx* ctx, int n, const uint64_t* masks_x,
   if ((rc = cg_alloc(c, (void**)&c->offsets, (size_t)(n + 1) * 8, "offsets")) != RG_OK) return fail(rc);
   if ((rc = cg_alloc(c, (void**)&c->scores, (size_t)M * 4, "scores")) != RG_OK) return fail(rc);
   if ((rc = cg_alloc(c, (void**)&c->rank, (size_t)64 * 4, "rank")) != RG_OK) return fail(rc);
-  RG_HIP(hipMemcpy(c->offsets, offsets, (size_t)(n + 1) * 8, hipMemcpyHostToDevice));
+  RG_HIP(rg_copy(ctx->stream, c->offsets, offsets, (size_t)(n + 1) * 8, hipMemcpyHostToDevice));
   if (layout == 0 || layout == 1) {
     c->NN = 1ll << n;
     std::vector<u32> m32(M);
     for (i64 j = 0; j < M; ++j) m32[j] = (u32)masks_x[j];
     if ((rc = cg_alloc(c, (void**)&c->masks32, (size_t)M * 4, "menu masks")) != RG_OK) return fail(rc);
-    RG_HIP(hipMemcpy(c->masks32, m32.data(), (size_t)M * 4, hipMemcpyHostToDevice));
+    RG_HIP(rg_copy(ctx->stream, c->masks32, m32.data(), (size_t)M * 4, hipMemcpyHostToDevice));
     if ((rc = cg_alloc(c, (void**)&c->V, (size_t)c->NN * 4, "V")) != RG_OK) return fail(rc);
     if ((rc = cg_alloc(c, (void**)&c->pred, (size_t)c->NN, "pred")) != RG_OK) return fail(rc);
     std::vector<u32> ord;
     layer_order(n, ord, c->bounds);
     if ((rc = cg_alloc(c, (void**)&c->nodes, (size_t)c->NN * 4, "layers")) != RG_OK) return fail(rc);
-    RG_HIP(hipMemcpy(c->nodes, ord.data(), (size_t)c->NN * 4, hipMemcpyHostToDevice));
+    RG_HIP(rg_copy(ctx->stream, c->nodes, ord.data(), (size_t)c->NN * 4, hipMemcpyHostToDevice));
     if (layout == 1) {
       RG_CHECK(n >= 2, RG_EUNSUP, "slab pricing needs n >= 2");
       std::vector<i64> pos(M);
@@ -387,7 +387,7 @@ extern "C" int rumgpu_cg_create(rumgpu_ctx* ctx, int n, const uint64_t* masks_x,
         for (u64 j = offsets[x]; j < offsets[x + 1]; ++j)
           pos[j] = (i64)x * c->Nloc + (i64)compress_bit(masks_x[j] & ~(1ull << x), x);
       if ((rc = cg_alloc(c, (void**)&c->spos, (size_t)M * 8, "slab positions")) != RG_OK) return fail(rc);
-      RG_HIP(hipMemcpy(c->spos, pos.data(), (size_t)M * 8, hipMemcpyHostToDevice));
+      RG_HIP(rg_copy(ctx->stream, c->spos, pos.data(), (size_t)M * 8, hipMemcpyHostToDevice));
       if ((rc = cg_alloc(c, (void**)&c->R, (size_t)n * c->Nloc * 4, "pricing slab")) != RG_OK) return fail(rc);
     }
   } else {
@@ -395,12 +395,12 @@ extern "C" int rumgpu_cg_create(rumgpu_ctx* ctx, int n, const uint64_t* masks_x,
     if (c->k_pred < 1) c->k_pred = 1;
     if (c->k_pred > n) c->k_pred = n;
     if ((rc = cg_alloc(c, (void**)&c->masks64, (size_t)M * 8, "menu masks")) != RG_OK) return fail(rc);
-    RG_HIP(hipMemcpy(c->masks64, masks_x, (size_t)M * 8, hipMemcpyHostToDevice));
+    RG_HIP(rg_copy(ctx->stream, c->masks64, masks_x, (size_t)M * 8, hipMemcpyHostToDevice));
     c->h_binom.assign((size_t)(n + 1) * (n + 1), 0);
     for (int b = 0; b <= n; ++b)
       for (int t = 0; t <= b; ++t) c->h_binom[(size_t)b * (n + 1) + t] = binom_u64(b, t);
     if ((rc = cg_alloc(c, (void**)&c->binom, c->h_binom.size() * 8, "binomials")) != RG_OK) return fail(rc);
-    RG_HIP(hipMemcpy(c->binom, c->h_binom.data(), c->h_binom.size() * 8, hipMemcpyHostToDevice));
+    RG_HIP(rg_copy(ctx->stream, c->binom, c->h_binom.data(), c->h_binom.size() * 8, hipMemcpyHostToDevice));
     u64 cap = 0;
     for (int k = 0; k <= n; ++k) cap = std::max(cap, binom_u64(n, k));
     const u64 nblk = (cap + CG_BLOCK - 1) / CG_BLOCK;
@@ -424,7 +424,7 @@ extern "C" int rumgpu_cg_create(rumgpu_ctx* ctx, int n, const uint64_t* masks_x,
     if ((rc = cg_alloc(c, (void**)&c->subV, (size_t)4 << K, "sub-lattice V")) != RG_OK) return fail(rc);
     if ((rc = cg_alloc(c, (void**)&c->subpred, (size_t)1 << K, "sub-lattice pred")) != RG_OK) return fail(rc);
     if ((rc = cg_alloc(c, (void**)&c->subnodes, (size_t)4 << K, "sub-lattice layers")) != RG_OK) return fail(rc);
-    RG_HIP(hipMemcpy(c->subnodes, ord.data(), (size_t)4 << K, hipMemcpyHostToDevice));
+    RG_HIP(rg_copy(ctx->stream, c->subnodes, ord.data(), (size_t)4 << K, hipMemcpyHostToDevice));
     if ((rc = cg_alloc(c, (void**)&c->submasks, (size_t)M * 4, "sub-lattice masks")) != RG_OK) return fail(rc);
     if ((rc = cg_alloc(c, (void**)&c->subscores, (size_t)M * 4, "sub-lattice scores")) != RG_OK) return fail(rc);
     if ((rc = cg_alloc(c, (void**)&c->suboffsets, (size_t)(K + 1) * 8, "sub-lattice offsets")) != RG_OK)
@@ -568,7 +568,7 @@ extern "C" int rumgpu_cg_price(rumgpu_cg* c, const float* scores_x, double* vmax
   for (int k = n; k > c->k_pred; --k) {
     const u64 r = colex_rank(D);
     uint8_t x = 0;
-    RG_HIP(hipMemcpy(&x, c->predk[k] + r, 1, hipMemcpyDeviceToHost));
+    RG_HIP(rg_copy(c->ctx->stream, &x, c->predk[k] + r, 1, hipMemcpyDeviceToHost));
     rank_pos[x] = pos++;
     D ^= 1ull << x;
   }

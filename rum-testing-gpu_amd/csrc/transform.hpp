@@ -51,6 +51,7 @@ int launch_tr_rl(int RL, const TIN* src, T* dst, Coef<T> k, const WinGeom& g, co
     case 2: hipLaunchKernelGGL((tr_pass<TIN, T, 2, PRO, EPI>), grid, block, shm, st, src, dst, k, g, outw, mass); break;
     case 3: hipLaunchKernelGGL((tr_pass<TIN, T, 3, PRO, EPI>), grid, block, shm, st, src, dst, k, g, outw, mass); break;
     case 4: hipLaunchKernelGGL((tr_pass<TIN, T, 4, PRO, EPI>), grid, block, shm, st, src, dst, k, g, outw, mass); break;
+    case 5: hipLaunchKernelGGL((tr_pass<TIN, T, 5, PRO, EPI>), grid, block, shm, st, src, dst, k, g, outw, mass); break;
     default: rg_set_error("bad RL %d", RL); return RG_EINVAL;
   }
   RG_LAUNCH_CHECK();

@@ -20,12 +20,14 @@ OPT_FLOOR = 0.0
 
 METHOD_ARGS = {
     "bm_admm": dict(
-        beta=None,              # penalty; None = 10^-1.48 mp^0.508 clipped to [0.3, 100] (params.bm_beta_default)
+        beta=None,              # penalty; None = 10^-1.48 mp^0.508 clipped to [0.3, 100] (params.bm_beta_default);
+                                # "cert" = 0.2 of it (params.bm_beta_cert), tuned for certificates
         alpha=1.95,             # over-relaxation
         precond_p=math.sqrt(2.0),
         check=50,               # iterations between upper bounds (feasibility push)
         lb_every=1,             # upper-bound checks between lower bounds
-        dtype="auto",           # "auto" = float64; "float32"; "mixed" = float32 sprint, float64 finish
+        dtype="auto",           # "auto" = float64; "float32"; "mixed" = float32 sprint, float64 finish (the
+                                # float32 leg helps only where float32's Mobius rounding is small, n <~ 14)
         use_graph="auto",       # replay `check` iterations as one HIP graph; auto = n <= 22
         max_iter=10 ** 7,
     ),

@@ -577,7 +577,14 @@ def _plan_bm_admm(n, mp, weighted, prof, a):
                            % (n, dt, need / 2 ** 30, budget / 2 ** 30))
     g = (n <= 22) if a["use_graph"] == "auto" else bool(a["use_graph"])
     a.update(dtype=dt, use_graph=g, _gpu_bytes=need)
-    a["beta"] = params.bm_beta_default(mp) if a["beta"] is None else float(a["beta"])
+    if a["beta"] is None:
+        a["beta"] = params.bm_beta_default(mp)
+    elif a["beta"] == "cert":
+        a["beta"] = params.bm_beta_cert(mp)
+    else:
+        a["beta"] = float(a["beta"])
+        if not a["beta"] > 0:
+            raise ValueError("bm_admm beta must be > 0, None (the race rule) or 'cert'")
     return a
 
 

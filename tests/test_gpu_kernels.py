@@ -170,7 +170,8 @@ def test_certificates_match_oracle(n, dtype):
     ctx = _lib.context(0)
     import ctypes
     lbv = ctypes.c_double()
-    _lib.check(ctx.lib.rumgpu_lb64(ctx.h, n, _lib.ptr(menu), _lib.ptr(alt), menu.size, _lib.ptr(x2[obs].copy()),
+    y_obs = np.ascontiguousarray(x2[obs])           # held: ctypes keeps only the address
+    _lib.check(ctx.lib.rumgpu_lb64(ctx.h, n, _lib.ptr(menu), _lib.ptr(alt), menu.size, _lib.ptr(y_obs),
                                    _lib.ptr(p), ctypes.byref(lbv)))
     assert abs(lbv.value - lb_o) <= 1e-10 * max(1.0, lb_o)
     h.close()
