@@ -50,6 +50,7 @@ def _parse():
     ap.add_argument("--profile-iters", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=20)
+    ap.add_argument("--no-graph", action="store_true", help="eager launches instead of HIP graph replays")
     ap.add_argument("--traffic-run", action="store_true",
                     help="only the workload for HBM counters: --profile-iters fused iterations on the n=18 "
                          "instance, then a device copy of the 4 state slabs (a copy kernel of known size)")
@@ -108,6 +109,8 @@ def main():
           file=sys.stderr, flush=True)
 
     margs = {"dtype": a.dtype}
+    if a.no_graph:
+        margs["use_graph"] = False
     if a.beta != "ref":
         margs["beta"] = a.beta if a.beta == "cert" else float(a.beta)
     sess = rum.session(method="bm_admm", method_args=margs)
