@@ -83,6 +83,34 @@ int rumgpu_bm_info(rumgpu_bm* bm, int64_t* iters, int* nwin, int* win_bits);
  * context stream): ms_by_kind[7], launches_by_kind[7] for SY, RHS, D, XK, prox, middle-window
  * transform, one-window pass. No reference counterpart: the measurement behind the roofline. */
 int rumgpu_bm_profile(rumgpu_bm* bm, int64_t iters, double* ms_by_kind, int64_t* launches_by_kind);
+
+/* ---- Draw slots: many bootstrap draws per launch (replaces the per-draw Session.solve loop of
+ * Bootstrap._solve, inference/bootstrap.py:308-335, warm from the kept p-hat state, :318).
+ * A batch object holds `slots` independent ADMM states on the same rows; rumgpu_bm_run advances
+ * every slot, each slot gets its own p, its own start and its own float64 certificates. The
+ * single-solve entry points above act on slot 0; set_state, copy_state and load_kept refuse a
+ * batch object (use slot_load). */
+int rumgpu_bm_create_batch(rumgpu_ctx* ctx, int n, const uint64_t* menu, const int32_t* alt, const double* p,
+                           int64_t M, int dtype_bytes, double beta, double alpha, double precond_p, double e0,
+                           double e1, double gamma, const double* Q, int slots, rumgpu_bm** out);
+/* p of one slot (input row order): Session.solve(P_b), bootstrap.py:318 */
+int rumgpu_bm_slot_set_p(rumgpu_bm* bm, int slot, const double* p);
+/* warm start of one slot from the state `src` kept (rumgpu_bm_keep): init_state=warm,
+ * rum_solver.py:914-925; any dtypes; the slot's next iteration starts from that state */
+int rumgpu_bm_slot_load(rumgpu_bm* bm, int slot, rumgpu_bm* src);
+/* cold start of one slot: solver.py:264-267 */
+int rumgpu_bm_slot_cold(rumgpu_bm* bm, int slot);
+/* corrections added at the observed cells of one slot's x2 and z (either may be null), and the
+ * observed cells of x2 and z read back (float64, input row order). No reference counterpart: the
+ * learned warm start's output and its training targets. */
+int rumgpu_bm_slot_add_obs(rumgpu_bm* bm, int slot, const double* dx2, const double* dz);
+int rumgpu_bm_slot_get_obs(rumgpu_bm* bm, int slot, double* x2_obs, double* z_obs);
+/* float64 certificates of every slot: phi[slots] (solver.py:321), ub[slots] (push_to_ub64,
+ * transforms.py:94-98, fitted point kept per slot), lb[slots] when want_lb (fw_lb_rows_gpu,
+ * transforms.py:101-114) */
+int rumgpu_bm_certify(rumgpu_bm* bm, int want_lb, double* ub, double* lb, double* phi);
+/* the fitted point of one slot's last certificate: p_fit, rum_solver.py:884-889 */
+int rumgpu_bm_slot_fit(rumgpu_bm* bm, int slot, double* p_fit);
 /* building blocks exposed for tests: one full (a,b,c,d) butterfly of a host slab
  * (common/butterfly.py:99-140 butterfly_fast), and the prox of a host slab (solver.py:66-139) */
 int rumgpu_butterfly64(rumgpu_ctx* ctx, int n, double* v, double a, double b, double c, double d);

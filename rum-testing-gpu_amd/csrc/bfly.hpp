@@ -25,7 +25,8 @@ struct WinGeom {
   int TT;        // threads per tile = 2^(B + C - RL)
   int TPB;       // tiles per block
   i64 Nloc;      // 2^nb
-  i64 ntiles;    // n * 2^(nb - B - C)
+  i64 ntiles;    // items * 2^(nb - B - C)
+  int nper;      // items per slot (n): a batch of D draws is n * D item blocks
 };
 
 // Host-side window plan for a slab of itemsize bytes: list of (s, B, C, RL).
@@ -58,8 +59,9 @@ inline std::vector<WinPlan> plan_windows(int nb, int itemsize) {
   return out;
 }
 
-inline WinGeom make_geom(int n, const WinPlan& w) {
+inline WinGeom make_geom(int n, const WinPlan& w, int slots = 1) {
   WinGeom g;
+  g.nper = n;
   g.nb = n - 1;
   g.s = w.s;
   g.B = w.B;
@@ -67,7 +69,7 @@ inline WinGeom make_geom(int n, const WinPlan& w) {
   g.TT = 1 << (w.B + w.C - w.RL);
   g.TPB = g.TT >= 256 ? 1 : 256 / g.TT;
   g.Nloc = 1ll << g.nb;
-  g.ntiles = (i64)n << (g.nb - w.B - w.C);
+  g.ntiles = ((i64)n * slots) << (g.nb - w.B - w.C);
   if (g.ntiles < g.TPB) g.TPB = (int)g.ntiles;
   if (g.TPB < 1) g.TPB = 1;
   return g;

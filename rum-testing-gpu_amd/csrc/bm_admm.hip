@@ -42,6 +42,7 @@ struct PassArgs {
   T b1, b2, ib1, ib2, a;
   Coef<T> cA, cAp, cQ, cQt, cT;
   WinGeom g;
+  const int* fresh;   // per slot: 1 = state just loaded, skip the deferred s, y update once
 };
 
 template <typename T, int RL, int KIND>
@@ -67,10 +68,15 @@ __global__ __launch_bounds__(512) void bm_pass(PassArgs<T> P) {
     t.transform(P.cA);                                                      // Ax (raw A)
     t.coalesce();
     if (t.active) {
+      const bool fresh = P.fresh && P.fresh[t.item / P.g.nper];
 #pragma unroll
       for (int j = 0; j < R; ++j) {
         const i64 i = t.gidx(j);
         const T sj = P.s[i], yj = P.y[i];
+        if (fresh) {                                                        // a refilled slot
+          t.v[j] = yj - P.b1 * sj;
+          continue;
+        }
         const T axor = -P.a * t.v[j] - (one - P.a) * sj;                     // fuse_sy
         T sn = P.ib1 * yj - axor;
         sn = sn > (T)0 ? sn : (T)0;
@@ -163,9 +169,19 @@ __device__ __forceinline__ void sort_desc(double (&key)[N], int (&pay)[N]) {
 template <typename T, int N>
 __global__ __launch_bounds__(256, 4) void bm_prox_iter(const T* __restrict__ xk, T* __restrict__ out, T* __restrict__ z,
                                                     const u32* __restrict__ obsmask, const int* __restrict__ obsoff,
-                                                    const double* __restrict__ pc, T b2) {
+                                                    const double* __restrict__ pc, T b2, i64 m2slot, i64 Mslot,
+                                                    int* __restrict__ fresh) {
+  // slot = blockIdx.y: its slab and its p
+  xk += (i64)blockIdx.y * m2slot;
+  out += (i64)blockIdx.y * m2slot;
+  z += (i64)blockIdx.y * m2slot;
+  pc += (i64)blockIdx.y * Mslot;
   const u32 S = blockIdx.x * blockDim.x + threadIdx.x;
-  if (S == 0 || S >= (1u << N)) return;
+  if (S == 0) {
+    if (fresh) fresh[blockIdx.y] = 0;        // this iteration's SY pass has consumed the flag
+    return;
+  }
+  if (S >= (1u << N)) return;
   constexpr u32 Nloc = 1u << (N - 1);     // slab positions fit 32 bits for n <= 26
   const int k = __popc(S);
   if (k == 1) {
@@ -237,7 +253,11 @@ __global__ __launch_bounds__(256, 4) void bm_prox_iter(const T* __restrict__ xk,
 template <typename T, int N, int MODE>
 __global__ __launch_bounds__(256) void bm_prox(const T* __restrict__ xk, T* __restrict__ out, T* __restrict__ z,
                                                const u32* __restrict__ obsmask, const int* __restrict__ obsoff,
-                                               const double* __restrict__ pc, T b2) {
+                                               const double* __restrict__ pc, T b2, i64 m2slot, i64 Mslot) {
+  if (xk) xk += (i64)blockIdx.y * m2slot;
+  out += (i64)blockIdx.y * m2slot;
+  if (z) z += (i64)blockIdx.y * m2slot;
+  pc += (i64)blockIdx.y * Mslot;
   const u64 S = (u64)blockIdx.x * blockDim.x + threadIdx.x;
   if (S == 0 || S >= (1ull << N)) return;
   constexpr i64 Nloc = 1ll << (N - 1);
@@ -372,6 +392,36 @@ __global__ void ew_fill(T* a, T v, i64 m) {
   for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (i64)gridDim.x * blockDim.x) a[i] = v;
 }
 
+// a[d * stride] = v for d < D, and out[d] = a[d * stride]
+__global__ void k_set_strided(double* a, i64 stride, int D, double v) {
+  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d < D) a[(i64)d * stride] = v;
+}
+__global__ void k_get_strided(const double* a, i64 stride, int D, double* out) {
+  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d < D) out[d] = a[(i64)d * stride];
+}
+
+// Warm-start corrections at the observed cells of one slot: x2[obs] += dx, z[obs] += dz.
+template <typename T>
+__global__ void k_add_obs(T* x2, T* z, const i64* __restrict__ obs, const double* __restrict__ dx,
+                          const double* __restrict__ dz, i64 M) {
+  const i64 j = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= M) return;
+  const i64 q = obs[j];
+  if (dx) x2[q] = (T)((double)x2[q] + dx[j]);
+  if (dz) z[q] = (T)((double)z[q] + dz[j]);
+}
+
+// x2 and z at the observed cells of one slot, as float64.
+template <typename T>
+__global__ void k_get_obs(const T* x2, const T* z, const i64* __restrict__ obs, double* ox, double* oz, i64 M) {
+  const i64 j = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= M) return;
+  ox[j] = (double)x2[obs[j]];
+  oz[j] = (double)z[obs[j]];
+}
+
 // inv_gl[rest] = 1 / (gamma + prod over bits of evals[bit]) in bit order (solver.py:207-213).
 template <typename T>
 __global__ void k_inv_gl(T* out, int nb, double e0, double e1, double gamma) {
@@ -386,6 +436,8 @@ __global__ void k_inv_gl(T* out, int nb, double e0, double e1, double gamma) {
 // Certificates (float64).
 // outwt64 (arcflow_kernels.py:357-365): node out-weight, x ascending.
 __global__ void k_outw(const double* __restrict__ w, double* __restrict__ outw, int n, i64 Nloc) {
+  w += (i64)blockIdx.y * n * Nloc;
+  outw += (i64)blockIdx.y << n;
   const i64 D = (i64)blockIdx.x * blockDim.x + threadIdx.x;
   if (D >= (1ll << n)) return;
   double s = 0.0;
@@ -398,6 +450,9 @@ __global__ void k_outw(const double* __restrict__ w, double* __restrict__ outw, 
 // massg64 (arcflow_kernels.py:367-376): one cardinality layer of the forward push.
 __global__ void k_mass(const double* __restrict__ w, const double* __restrict__ outw, double* mass,
                        const int* __restrict__ nodes, i64 ns, int n, i64 Nloc) {
+  w += (i64)blockIdx.y * n * Nloc;
+  outw += (i64)blockIdx.y << n;
+  mass += (i64)blockIdx.y << n;
   const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ns) return;
   const i64 C = nodes[i];
@@ -415,6 +470,8 @@ __global__ void k_mass(const double* __restrict__ w, const double* __restrict__ 
 // lpath64 (arcflow_kernels.py:388-397): one layer of the longest path.
 __global__ void k_lpath(const double* __restrict__ R, double* dp, const int* __restrict__ nodes, i64 ns, int n,
                         i64 Nloc) {
+  R += (i64)blockIdx.y * n * Nloc;
+  dp += (i64)blockIdx.y << n;
   const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ns) return;
   const i64 C = nodes[i];
@@ -456,15 +513,20 @@ __device__ __forceinline__ void block_reduce_store(double (&s)[Q], double (&c)[Q
     __syncthreads();
   }
   if (t == 0) {
+    const size_t b = (size_t)blockIdx.y * gridDim.x + blockIdx.x;     // (slot, block)
 #pragma unroll
-    for (int q = 0; q < Q; ++q) { part[(size_t)blockIdx.x * 2 * Q + 2 * q] = sh[2 * q][0]; part[(size_t)blockIdx.x * 2 * Q + 2 * q + 1] = sh[2 * q + 1][0]; }
+    for (int q = 0; q < Q; ++q) { part[b * 2 * Q + 2 * q] = sh[2 * q][0]; part[b * 2 * Q + 2 * q + 1] = sh[2 * q + 1][0]; }
   }
 }
 
 // phi: sum (x[obs] - p)^2; optionally the gathered x[obs] (float64) into fit.
 template <typename T>
 __global__ __launch_bounds__(256) void k_phi(const T* __restrict__ x, const i64* __restrict__ obs,
-                                             const double* __restrict__ p, i64 M, double* fit, double* part) {
+                                             const double* __restrict__ p, i64 M, double* fit, double* part,
+                                             i64 m2slot) {
+  x += (i64)blockIdx.y * m2slot;
+  p += (i64)blockIdx.y * M;
+  if (fit) fit += (i64)blockIdx.y * M;
   double s[1] = {0.0}, c[1] = {0.0};
   for (i64 j = (i64)blockIdx.x * blockDim.x + threadIdx.x; j < M; j += (i64)gridDim.x * blockDim.x) {
     const double v = (double)x[obs[j]];
@@ -479,7 +541,11 @@ __global__ __launch_bounds__(256) void k_phi(const T* __restrict__ x, const i64*
 // f = sum r^2, ry = sum r y.
 template <typename T>
 __global__ __launch_bounds__(256) void k_lb_prep(const T* __restrict__ x, const i64* __restrict__ obs,
-                                                 const double* __restrict__ p, i64 M, double* g, double* part) {
+                                                 const double* __restrict__ p, i64 M, double* g, double* part,
+                                                 i64 m2slot) {
+  x += (i64)blockIdx.y * m2slot;
+  g += (i64)blockIdx.y * m2slot;
+  p += (i64)blockIdx.y * M;
   double s[2] = {0.0, 0.0}, c[2] = {0.0, 0.0};
   for (i64 j = (i64)blockIdx.x * blockDim.x + threadIdx.x; j < M; j += (i64)gridDim.x * blockDim.x) {
     const double y = (double)x[obs[j]];
@@ -509,6 +575,8 @@ __global__ __launch_bounds__(256) void k_lb_rows(const double* __restrict__ yv, 
 __global__ void k_final(const double* part, int nblk, int Q, double* out) {
   const int q = threadIdx.x;
   if (q >= Q) return;
+  part += (size_t)blockIdx.x * nblk * 2 * Q;        // slot = blockIdx.x
+  out += (size_t)blockIdx.x * Q;
   double s = 0.0, c = 0.0;
   for (int b = 0; b < nblk; ++b) {
     two_sum(s, c, part[(size_t)b * 2 * Q + 2 * q]);
@@ -557,15 +625,15 @@ int launch_pass(int kind, int RL, const PassArgs<T>& P, hipStream_t st) {
 
 template <typename T, int MODE>
 int launch_prox(int n, const T* xk, T* out, T* z, const u32* om, const int* oo, const double* pc, T b2,
-                hipStream_t st) {
+                hipStream_t st, int D = 1, i64 m2 = 0, i64 M = 0, int* fresh = nullptr) {
   const i64 NN = 1ll << n;
-  const dim3 grid(grid_for(NN, 256)), block(256);
+  const dim3 grid(grid_for(NN, 256), (unsigned)D), block(256);
 #define RG_PROX_CASE(NV)                                                                                   \
   case NV:                                                                                                 \
     if constexpr (MODE == 0)                                                                               \
-      hipLaunchKernelGGL((bm_prox_iter<T, NV>), grid, block, 0, st, xk, out, z, om, oo, pc, b2);           \
+      hipLaunchKernelGGL((bm_prox_iter<T, NV>), grid, block, 0, st, xk, out, z, om, oo, pc, b2, m2, M, fresh); \
     else                                                                                                   \
-      hipLaunchKernelGGL((bm_prox<T, NV, MODE>), grid, block, 0, st, xk, out, z, om, oo, pc, b2);          \
+      hipLaunchKernelGGL((bm_prox<T, NV, MODE>), grid, block, 0, st, xk, out, z, om, oo, pc, b2, m2, M);  \
     break;
   switch (n) {
     RG_PROX_CASE(2) RG_PROX_CASE(3) RG_PROX_CASE(4) RG_PROX_CASE(5) RG_PROX_CASE(6) RG_PROX_CASE(7)
@@ -591,6 +659,9 @@ inline unsigned ew_grid(i64 m) {
 struct rumgpu_bm {
   rumgpu_ctx* ctx = nullptr;
   int n = 0, nb = 0, isz = 8, plain = 0;
+  int D = 1;                  // draw slots: every slab, p, fit and node array holds D blocks
+  int* fresh = nullptr;       // per slot: state loaded since the last iteration
+  double* dwarm = nullptr;    // 2 M float64: warm-start corrections at the observed cells
   i64 Nloc = 0, m2 = 0, M = 0, NN = 0;
   double beta = 1, alpha = 1.95, pp = 1.4142135623730951;
   double e0 = 0, e1 = 0, gamma = 1, Q[4] = {0, 0, 0, 0};
@@ -638,7 +709,7 @@ struct rumgpu_bm {
 static void bm_free_all(rumgpu_bm* b) {
   void* bufs[] = {b->x2, b->s, b->y, b->z, b->tb, b->xk, b->inv_gl, b->kx2, b->ks, b->ky, b->kz,
                   b->pw, b->pAT, b->prhs, b->px, b->pAx, b->px2n, b->obsmask, b->obsoff, b->pc, b->obs,
-                  b->p, b->cert, b->outw, b->mass, b->order, b->part, b->res, b->fit};
+                  b->p, b->cert, b->outw, b->mass, b->order, b->part, b->res, b->fit, b->fresh, b->dwarm};
   for (void* q : bufs) rg_free(q);
   if (b->h_res) (void)hipHostFree(b->h_res);
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
@@ -675,6 +746,7 @@ static PassArgs<T> pass_args(rumgpu_bm* b, int w) {
   P.cQt = coef<T>(b->Q[0], b->Q[2], b->Q[1], b->Q[3]);
   P.cT = P.cA;
   P.g = b->geoms[w];
+  P.fresh = b->fresh;
   return P;
 }
 
@@ -709,7 +781,8 @@ static int fused_iter(rumgpu_bm* b) {
     RG_TRY(run_pass<T>(b, K_XK, k - 1));
     prof_mark(b, 3);
   }
-  RG_TRY((launch_prox<T, 0>(b->n, (const T*)b->xk, (T*)b->x2, (T*)b->z, b->obsmask, b->obsoff, b->pc, (T)b->beta, st)));
+  RG_TRY((launch_prox<T, 0>(b->n, (const T*)b->xk, (T*)b->x2, (T*)b->z, b->obsmask, b->obsoff, b->pc, (T)b->beta, st,
+                            b->D, b->m2, b->M, b->fresh)));
   prof_mark(b, 4);
   for (int w = k - 2; w >= 1; --w) { RG_TRY(run_pass<T>(b, K_T, w, &P0.cA)); prof_mark(b, 5); }
   b->pending = 1;
@@ -728,12 +801,13 @@ static int finalize(rumgpu_bm* b) {
 
 // Full transform over every window: dst = T(src), first window from src, the rest in place.
 template <typename TIN, typename T>
-static int full_transform(rumgpu_bm* b, const TIN* src, T* dst, Coef<T> k, int epi_wmax, int pro_extract) {
+static int full_transform(rumgpu_bm* b, const TIN* src, T* dst, Coef<T> k, int epi_wmax, int pro_extract,
+                          int slots = 1) {
   hipStream_t st = b->ctx->stream;
   std::vector<WinPlan> wins = plan_windows(b->nb, sizeof(T));
   const int nw = (int)wins.size();
   for (int w = 0; w < nw; ++w) {
-    const WinGeom g = make_geom(b->n, wins[w]);
+    const WinGeom g = make_geom(b->n, wins[w], slots);
     const int RL = wins[w].RL;
     const bool last = (w == nw - 1);
     if (w == 0 && pro_extract) {
@@ -826,68 +900,73 @@ static int build_graph(rumgpu_bm* b, i64 len) {
 
 // ---------------------------------------------------------------------------------------------
 // Certificates.
+// The certificates of slots 0..D-1 (D = 1: the single-solve entry points, which act on slot 0).
 template <typename T>
-static int phi_impl(rumgpu_bm* b, const void* x, double* out_sq, int want_fit) {
+static int phi_impl(rumgpu_bm* b, const void* x, double* out_sq, int want_fit, int D = 1) {
   hipStream_t st = b->ctx->stream;
   const int nblk = (int)std::min<i64>(256, std::max<i64>(1, (b->M + 255) / 256));
-  hipLaunchKernelGGL(k_phi<T>, dim3(nblk), dim3(256), 0, st, (const T*)x, b->obs, b->p, b->M,
-                     want_fit ? b->fit : nullptr, b->part);
-  hipLaunchKernelGGL(k_final, dim3(1), dim3(64), 0, st, b->part, nblk, 1, b->res);
+  hipLaunchKernelGGL(k_phi<T>, dim3(nblk, D), dim3(256), 0, st, (const T*)x, b->obs, b->p, b->M,
+                     want_fit ? b->fit : nullptr, b->part, b->m2);
+  hipLaunchKernelGGL(k_final, dim3(D), dim3(64), 0, st, b->part, nblk, 1, b->res);
   RG_LAUNCH_CHECK();
-  RG_HIP(hipMemcpyAsync(b->h_res, b->res, sizeof(double), hipMemcpyDeviceToHost, st));
+  RG_HIP(hipMemcpyAsync(b->h_res, b->res, (size_t)D * sizeof(double), hipMemcpyDeviceToHost, st));
   RG_HIP(hipStreamSynchronize(st));
-  *out_sq = b->h_res[0];
+  for (int d = 0; d < D; ++d) out_sq[d] = b->h_res[d];
   return RG_OK;
 }
 
 template <typename T>
-static int push_impl(rumgpu_bm* b, double* ub) {
+static int push_impl(rumgpu_bm* b, double* ub, int D = 1) {
   hipStream_t st = b->ctx->stream;
   const int n = b->n;
   // q = Mobius(x2) in float64, then w = max(q, 0) + 1e-12
-  RG_TRY((full_transform<T, double>(b, (const T*)b->x2, b->cert, coef<double>(1.0, -1.0, 0.0, 1.0), 1, 0)));
-  hipLaunchKernelGGL(k_outw, dim3(grid_for(b->NN, 256)), dim3(256), 0, st, b->cert, b->outw, n, b->Nloc);
-  hipLaunchKernelGGL((ew_fill<double>), dim3(1), dim3(64), 0, st, b->mass, 1.0, (i64)1);
+  RG_TRY((full_transform<T, double>(b, (const T*)b->x2, b->cert, coef<double>(1.0, -1.0, 0.0, 1.0), 1, 0, D)));
+  hipLaunchKernelGGL(k_outw, dim3(grid_for(b->NN, 256), D), dim3(256), 0, st, b->cert, b->outw, n, b->Nloc);
+  hipLaunchKernelGGL(k_set_strided, dim3(grid_for(D, 64)), dim3(64), 0, st, b->mass, b->NN, D, 1.0);
   for (int c = 1; c <= n; ++c) {
     const i64 lo = b->bounds[c], ns = b->bounds[c + 1] - lo;
     if (ns <= 0) continue;
-    hipLaunchKernelGGL(k_mass, dim3(grid_for(ns, 256)), dim3(256), 0, st, b->cert, b->outw, b->mass, b->order + lo, ns,
-                       n, b->Nloc);
+    hipLaunchKernelGGL(k_mass, dim3(grid_for(ns, 256), D), dim3(256), 0, st, b->cert, b->outw, b->mass, b->order + lo,
+                       ns, n, b->Nloc);
   }
   RG_LAUNCH_CHECK();
   // rf = extract, x_ub = zeta(rf) (in place)
-  RG_TRY((full_transform<double, double>(b, b->cert, b->cert, coef<double>(1.0, 1.0, 0.0, 1.0), 0, 1)));
-  double sq = 0.0;
-  RG_TRY(phi_impl<double>(b, b->cert, &sq, 1));
-  *ub = std::sqrt(sq);
+  RG_TRY((full_transform<double, double>(b, b->cert, b->cert, coef<double>(1.0, 1.0, 0.0, 1.0), 0, 1, D)));
+  RG_TRY(phi_impl<double>(b, b->cert, ub, 1, D));
+  for (int d = 0; d < D; ++d) ub[d] = std::sqrt(ub[d]);
   return RG_OK;
 }
 
 template <typename T>
-static int lb_impl(rumgpu_bm* b, double* lb) {
+static int lb_impl(rumgpu_bm* b, double* lb, int D = 1) {
   hipStream_t st = b->ctx->stream;
   const int n = b->n;
-  hipLaunchKernelGGL((ew_fill<double>), dim3(ew_grid(b->m2)), dim3(256), 0, st, b->cert, 0.0, b->m2);
+  hipLaunchKernelGGL((ew_fill<double>), dim3(ew_grid(b->m2 * D)), dim3(256), 0, st, b->cert, 0.0, b->m2 * D);
   const int nblk = (int)std::min<i64>(256, std::max<i64>(1, (b->M + 255) / 256));
-  hipLaunchKernelGGL(k_lb_prep<T>, dim3(nblk), dim3(256), 0, st, (const T*)b->x2, b->obs, b->p, b->M, b->cert, b->part);
-  hipLaunchKernelGGL(k_final, dim3(1), dim3(64), 0, st, b->part, nblk, 2, b->res);
+  hipLaunchKernelGGL(k_lb_prep<T>, dim3(nblk, D), dim3(256), 0, st, (const T*)b->x2, b->obs, b->p, b->M, b->cert,
+                     b->part, b->m2);
+  hipLaunchKernelGGL(k_final, dim3(D), dim3(64), 0, st, b->part, nblk, 2, b->res);
   RG_LAUNCH_CHECK();
-  RG_TRY((full_transform<double, double>(b, b->cert, b->cert, coef<double>(1.0, 0.0, 1.0, 1.0), 0, 0)));
-  hipLaunchKernelGGL((ew_fill<double>), dim3(1), dim3(64), 0, st, b->mass, 0.0, (i64)1);
+  RG_TRY((full_transform<double, double>(b, b->cert, b->cert, coef<double>(1.0, 0.0, 1.0, 1.0), 0, 0, D)));
+  hipLaunchKernelGGL(k_set_strided, dim3(grid_for(D, 64)), dim3(64), 0, st, b->mass, b->NN, D, 0.0);
   for (int c = 1; c <= n; ++c) {
     const i64 lo = b->bounds[c], ns = b->bounds[c + 1] - lo;
     if (ns <= 0) continue;
-    hipLaunchKernelGGL(k_lpath, dim3(grid_for(ns, 256)), dim3(256), 0, st, b->cert, b->mass, b->order + lo, ns, n,
+    hipLaunchKernelGGL(k_lpath, dim3(grid_for(ns, 256), D), dim3(256), 0, st, b->cert, b->mass, b->order + lo, ns, n,
                        b->Nloc);
   }
+  // longest path to the full set of each slot: res[2 D + d]
+  hipLaunchKernelGGL(k_get_strided, dim3(grid_for(D, 64)), dim3(64), 0, st, b->mass + (b->NN - 1), b->NN, D,
+                     b->res + 2 * D);
   RG_LAUNCH_CHECK();
-  RG_HIP(hipMemcpyAsync(b->res + 2, b->mass + (b->NN - 1), sizeof(double), hipMemcpyDeviceToDevice, st));
-  RG_HIP(hipMemcpyAsync(b->h_res, b->res, 3 * sizeof(double), hipMemcpyDeviceToHost, st));
+  RG_HIP(hipMemcpyAsync(b->h_res, b->res, (size_t)3 * D * sizeof(double), hipMemcpyDeviceToHost, st));
   RG_HIP(hipStreamSynchronize(st));
-  const double f = b->h_res[0], ry = b->h_res[1], vmax = b->h_res[2];
-  const double rc = vmax - ry;
-  const double l2 = f - 2.0 * rc;
-  *lb = std::sqrt(l2 > 0.0 ? l2 : 0.0);
+  for (int d = 0; d < D; ++d) {
+    const double f = b->h_res[2 * d], ry = b->h_res[2 * d + 1], vmax = b->h_res[2 * D + d];
+    const double rc = vmax - ry;
+    const double l2 = f - 2.0 * rc;
+    lb[d] = std::sqrt(l2 > 0.0 ? l2 : 0.0);
+  }
   return RG_OK;
 }
 
@@ -895,10 +974,12 @@ static int lb_impl(rumgpu_bm* b, double* lb) {
 // C-ABI.
 #define BM_DISPATCH(b, fn, ...) ((b)->isz == 8 ? fn<double>(__VA_ARGS__) : fn<float>(__VA_ARGS__))
 
-extern "C" int rumgpu_bm_create(rumgpu_ctx* ctx, int n, const uint64_t* menu, const int32_t* alt, const double* p,
-                                int64_t M, int dtype_bytes, double beta, double alpha, double precond_p, double e0,
-                                double e1, double gamma, const double* Q, int flags, rumgpu_bm** out) {
+static int bm_create_impl(rumgpu_ctx* ctx, int n, const uint64_t* menu, const int32_t* alt, const double* p, int64_t M,
+                          int dtype_bytes, double beta, double alpha, double precond_p, double e0, double e1,
+                          double gamma, const double* Q, int flags, int slots, rumgpu_bm** out) {
   RG_CHECK(ctx && out && menu && alt && p && Q, RG_EINVAL, "bm_create: null argument");
+  RG_CHECK(slots >= 1 && slots <= 65535, RG_EINVAL, "bm_create: slots = %d outside 1..65535", slots);
+  RG_CHECK(slots == 1 || !(flags & 1), RG_EINVAL, "bm_create: the plain schedule runs one slot");
   RG_CHECK(n >= 2 && n <= 26, RG_EUNSUP, "bm_admm handles 2 <= n <= 26 here (n = %d)", n);
   RG_CHECK(dtype_bytes == 4 || dtype_bytes == 8, RG_EINVAL, "dtype must be 4 (float32) or 8 (float64) bytes");
   RG_CHECK(M >= 1, RG_EINVAL, "bm_create: no rows");
@@ -910,6 +991,7 @@ extern "C" int rumgpu_bm_create(rumgpu_ctx* ctx, int n, const uint64_t* menu, co
   b->nb = n - 1;
   b->isz = dtype_bytes;
   b->plain = (flags & 1) ? 1 : 0;
+  b->D = slots;
   b->Nloc = 1ll << (n - 1);
   b->m2 = (i64)n * b->Nloc;
   b->NN = 1ll << n;
@@ -922,7 +1004,7 @@ extern "C" int rumgpu_bm_create(rumgpu_ctx* ctx, int n, const uint64_t* menu, co
   b->gamma = gamma;
   for (int i = 0; i < 4; ++i) b->Q[i] = Q[i];
   b->wins = plan_windows(b->nb, dtype_bytes);
-  for (auto& w : b->wins) b->geoms.push_back(make_geom(n, w));
+  for (auto& w : b->wins) b->geoms.push_back(make_geom(n, w, slots));
   int rc = RG_OK;
   auto fail = [&](int code) {
     bm_free_all(b);
@@ -963,7 +1045,8 @@ extern "C" int rumgpu_bm_create(rumgpu_ctx* ctx, int n, const uint64_t* menu, co
     }
   }
   const size_t T = dtype_bytes;
-  const size_t slab = (size_t)b->m2 * T;
+  const size_t D = (size_t)slots;
+  const size_t slab = (size_t)b->m2 * T * D;
   void** slabs[] = {&b->x2, &b->s, &b->y, &b->z, &b->tb, &b->xk};
   for (void** q : slabs)
     if ((rc = rg_malloc(q, slab, "bm_admm state slab")) != RG_OK) return fail(rc);
@@ -975,18 +1058,20 @@ extern "C" int rumgpu_bm_create(rumgpu_ctx* ctx, int n, const uint64_t* menu, co
   if ((rc = rg_malloc(&b->inv_gl, (size_t)b->Nloc * T, "inv_gl")) != RG_OK) return fail(rc);
   if ((rc = rg_malloc((void**)&b->obsmask, (size_t)b->NN * 4, "obsmask")) != RG_OK) return fail(rc);
   if ((rc = rg_malloc((void**)&b->obsoff, (size_t)b->NN * 4, "obsoff")) != RG_OK) return fail(rc);
-  if ((rc = rg_malloc((void**)&b->pc, (size_t)M * 8, "p (menu order)")) != RG_OK) return fail(rc);
+  if ((rc = rg_malloc((void**)&b->pc, (size_t)M * 8 * D, "p (menu order)")) != RG_OK) return fail(rc);
   if ((rc = rg_malloc((void**)&b->obs, (size_t)M * 8, "obs index")) != RG_OK) return fail(rc);
-  if ((rc = rg_malloc((void**)&b->p, (size_t)M * 8, "p")) != RG_OK) return fail(rc);
-  if ((rc = rg_malloc((void**)&b->fit, (size_t)M * 8, "p_fit")) != RG_OK) return fail(rc);
-  if ((rc = rg_malloc((void**)&b->cert, (size_t)b->m2 * 8, "float64 certificate slab")) != RG_OK) return fail(rc);
-  if ((rc = rg_malloc((void**)&b->outw, (size_t)b->NN * 8, "outw")) != RG_OK) return fail(rc);
-  if ((rc = rg_malloc((void**)&b->mass, (size_t)b->NN * 8, "mass")) != RG_OK) return fail(rc);
+  if ((rc = rg_malloc((void**)&b->p, (size_t)M * 8 * D, "p")) != RG_OK) return fail(rc);
+  if ((rc = rg_malloc((void**)&b->fit, (size_t)M * 8 * D, "p_fit")) != RG_OK) return fail(rc);
+  if ((rc = rg_malloc((void**)&b->dwarm, (size_t)M * 8 * 2, "warm-start corrections")) != RG_OK) return fail(rc);
+  if ((rc = rg_malloc((void**)&b->cert, (size_t)b->m2 * 8 * D, "float64 certificate slab")) != RG_OK) return fail(rc);
+  if ((rc = rg_malloc((void**)&b->outw, (size_t)b->NN * 8 * D, "outw")) != RG_OK) return fail(rc);
+  if ((rc = rg_malloc((void**)&b->mass, (size_t)b->NN * 8 * D, "mass")) != RG_OK) return fail(rc);
   if ((rc = rg_malloc((void**)&b->order, (size_t)b->NN * 4, "layers")) != RG_OK) return fail(rc);
-  if ((rc = rg_malloc((void**)&b->part, (size_t)256 * 4 * 8, "partials")) != RG_OK) return fail(rc);
-  if ((rc = rg_malloc((void**)&b->res, 16 * 8, "scalars")) != RG_OK) return fail(rc);
-  if (hipHostMalloc((void**)&b->h_res, 16 * 8, hipHostMallocDefault) != hipSuccess) {
-    rg_set_error("hipHostMalloc of 128 bytes failed");
+  if ((rc = rg_malloc((void**)&b->part, (size_t)256 * 4 * 8 * D, "partials")) != RG_OK) return fail(rc);
+  if ((rc = rg_malloc((void**)&b->res, (16 + 4 * D) * 8, "scalars")) != RG_OK) return fail(rc);
+  if ((rc = rg_malloc((void**)&b->fresh, D * 4, "slot flags")) != RG_OK) return fail(rc);
+  if (hipHostMalloc((void**)&b->h_res, (16 + 4 * D) * 8, hipHostMallocDefault) != hipSuccess) {
+    rg_set_error("hipHostMalloc of %zu bytes failed", (size_t)(16 + 4 * D) * 8);
     return fail(RG_ENOMEM);
   }
   hipStream_t st = ctx->stream;
@@ -1007,6 +1092,7 @@ extern "C" int rumgpu_bm_create(rumgpu_ctx* ctx, int n, const uint64_t* menu, co
     rg_set_error("bm_create: upload failed");
     return fail(RG_EHIP);
   }
+  hipLaunchKernelGGL((ew_fill<int>), dim3(grid_for((i64)D, 256)), dim3(256), 0, st, b->fresh, 0, (i64)D);
   if (dtype_bytes == 8)
     hipLaunchKernelGGL(k_inv_gl<double>, dim3(grid_for(b->Nloc, 256)), dim3(256), 0, st, (double*)b->inv_gl, b->nb, e0, e1, gamma);
   else
@@ -1025,6 +1111,19 @@ extern "C" int rumgpu_bm_create(rumgpu_ctx* ctx, int n, const uint64_t* menu, co
   return RG_OK;
 }
 
+extern "C" int rumgpu_bm_create(rumgpu_ctx* ctx, int n, const uint64_t* menu, const int32_t* alt, const double* p,
+                                int64_t M, int dtype_bytes, double beta, double alpha, double precond_p, double e0,
+                                double e1, double gamma, const double* Q, int flags, rumgpu_bm** out) {
+  return bm_create_impl(ctx, n, menu, alt, p, M, dtype_bytes, beta, alpha, precond_p, e0, e1, gamma, Q, flags, 1, out);
+}
+
+extern "C" int rumgpu_bm_create_batch(rumgpu_ctx* ctx, int n, const uint64_t* menu, const int32_t* alt,
+                                      const double* p, int64_t M, int dtype_bytes, double beta, double alpha,
+                                      double precond_p, double e0, double e1, double gamma, const double* Q,
+                                      int slots, rumgpu_bm** out) {
+  return bm_create_impl(ctx, n, menu, alt, p, M, dtype_bytes, beta, alpha, precond_p, e0, e1, gamma, Q, 0, slots, out);
+}
+
 extern "C" int rumgpu_bm_destroy(rumgpu_bm* b) {
   if (!b) return RG_OK;
   (void)hipStreamSynchronize(b->ctx->stream);
@@ -1038,8 +1137,10 @@ extern "C" int rumgpu_bm_set_p(rumgpu_bm* b, const double* p) {
   std::vector<double> hpc(b->M);
   for (i64 j = 0; j < b->M; ++j) hpc[b->h_pcidx[j]] = p[j];
   hipStream_t st = b->ctx->stream;
-  RG_HIP(hipMemcpyAsync(b->p, p, (size_t)b->M * 8, hipMemcpyHostToDevice, st));
-  RG_HIP(hipMemcpyAsync(b->pc, hpc.data(), (size_t)b->M * 8, hipMemcpyHostToDevice, st));
+  for (int d = 0; d < b->D; ++d) {
+    RG_HIP(hipMemcpyAsync(b->p + (i64)d * b->M, p, (size_t)b->M * 8, hipMemcpyHostToDevice, st));
+    RG_HIP(hipMemcpyAsync(b->pc + (i64)d * b->M, hpc.data(), (size_t)b->M * 8, hipMemcpyHostToDevice, st));
+  }
   RG_HIP(hipStreamSynchronize(st));
   return RG_OK;
 }
@@ -1047,12 +1148,15 @@ extern "C" int rumgpu_bm_set_p(rumgpu_bm* b, const double* p) {
 template <typename T>
 static int init_cold_t(rumgpu_bm* b) {
   hipStream_t st = b->ctx->stream;
-  const unsigned gr = ew_grid(b->m2);
-  hipLaunchKernelGGL((ew_fill<T>), dim3(gr), dim3(256), 0, st, (T*)b->s, (T)0, b->m2);
-  hipLaunchKernelGGL((ew_fill<T>), dim3(gr), dim3(256), 0, st, (T*)b->y, (T)0, b->m2);
-  hipLaunchKernelGGL((ew_fill<T>), dim3(gr), dim3(256), 0, st, (T*)b->z, (T)0, b->m2);
+  const i64 m = b->m2 * b->D;
+  const unsigned gr = ew_grid(m);
+  hipLaunchKernelGGL((ew_fill<T>), dim3(gr), dim3(256), 0, st, (T*)b->s, (T)0, m);
+  hipLaunchKernelGGL((ew_fill<T>), dim3(gr), dim3(256), 0, st, (T*)b->y, (T)0, m);
+  hipLaunchKernelGGL((ew_fill<T>), dim3(gr), dim3(256), 0, st, (T*)b->z, (T)0, m);
+  hipLaunchKernelGGL((ew_fill<int>), dim3(grid_for((i64)b->D, 256)), dim3(256), 0, st, b->fresh, 0, (i64)b->D);
   RG_LAUNCH_CHECK();
-  RG_TRY((launch_prox<T, 1>(b->n, (const T*)b->xk, (T*)b->x2, nullptr, b->obsmask, b->obsoff, b->pc, (T)b->beta, st)));
+  RG_TRY((launch_prox<T, 1>(b->n, (const T*)b->xk, (T*)b->x2, nullptr, b->obsmask, b->obsoff, b->pc, (T)b->beta, st,
+                            b->D, b->m2, b->M)));
   b->pending = 0;
   b->iters = 0;
   RG_HIP(hipStreamSynchronize(st));
@@ -1081,6 +1185,7 @@ static int set_state_t(rumgpu_bm* b, const double* x2, const double* s, const do
 
 extern "C" int rumgpu_bm_set_state(rumgpu_bm* b, const double* x2, const double* s, const double* y, const double* z) {
   RG_CHECK(b && x2 && s && y && z, RG_EINVAL, "bm_set_state: null argument");
+  RG_CHECK(b->D == 1, RG_EINVAL, "bm_set_state: a batch handle loads slots with bm_slot_load");
   return BM_DISPATCH(b, set_state_t, b, x2, s, y, z);
 }
 
@@ -1122,6 +1227,7 @@ static int copy_state_t(rumgpu_bm* dst, rumgpu_bm* src) {
 
 extern "C" int rumgpu_bm_copy_state(rumgpu_bm* dst, rumgpu_bm* src) {
   RG_CHECK(dst && src && dst->n == src->n && dst->M == src->M, RG_EINVAL, "bm_copy_state: incompatible handles");
+  RG_CHECK(dst->D == 1, RG_EINVAL, "bm_copy_state: a batch handle loads slots with bm_slot_load");
   if (src->isz == 8 && dst->isz == 8) return copy_state_t<double, double>(dst, src);
   if (src->isz == 8 && dst->isz == 4) return copy_state_t<double, float>(dst, src);
   if (src->isz == 4 && dst->isz == 8) return copy_state_t<float, double>(dst, src);
@@ -1157,6 +1263,7 @@ extern "C" int rumgpu_bm_keep(rumgpu_bm* b) {
 }
 extern "C" int rumgpu_bm_load_kept(rumgpu_bm* b) {
   RG_CHECK(b, RG_EINVAL, "bm_load_kept: null handle");
+  RG_CHECK(b->D == 1, RG_EINVAL, "bm_load_kept: a batch handle loads slots with bm_slot_load");
   return BM_DISPATCH(b, keep_t, b, 1);
 }
 
@@ -1223,6 +1330,135 @@ extern "C" int rumgpu_bm_info(rumgpu_bm* b, int64_t* iters, int* nwin, int* win_
   if (nwin) *nwin = (int)b->wins.size();
   if (win_bits)
     for (size_t i = 0; i < b->wins.size(); ++i) win_bits[i] = b->wins[i].B;
+  return RG_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Draw slots of a batch handle. Slot d owns block d of every slab; the iterations run every slot
+// at once (rumgpu_bm_run), each slot is loaded, corrected and certified on its own.
+#define RG_SLOT_CHECK(b, slot, what)                                                                    \
+  RG_CHECK((b) && (slot) >= 0 && (slot) < (b)->D, RG_EINVAL, what ": slot %d outside 0..%d", (int)(slot), \
+           (b) ? (b)->D - 1 : -1)
+
+extern "C" int rumgpu_bm_slot_set_p(rumgpu_bm* b, int slot, const double* p) {
+  RG_SLOT_CHECK(b, slot, "bm_slot_set_p");
+  RG_CHECK(p, RG_EINVAL, "bm_slot_set_p: null p");
+  std::vector<double> hpc(b->M);
+  for (i64 j = 0; j < b->M; ++j) hpc[b->h_pcidx[j]] = p[j];
+  hipStream_t st = b->ctx->stream;
+  RG_HIP(hipMemcpyAsync(b->p + (i64)slot * b->M, p, (size_t)b->M * 8, hipMemcpyHostToDevice, st));
+  RG_HIP(hipMemcpyAsync(b->pc + (i64)slot * b->M, hpc.data(), (size_t)b->M * 8, hipMemcpyHostToDevice, st));
+  RG_HIP(hipStreamSynchronize(st));
+  return RG_OK;
+}
+
+static int mark_fresh(rumgpu_bm* b, int slot) {
+  hipLaunchKernelGGL((ew_fill<int>), dim3(1), dim3(64), 0, b->ctx->stream, b->fresh + slot, 1, (i64)1);
+  RG_LAUNCH_CHECK();
+  return RG_OK;
+}
+
+template <typename TA, typename TB>
+static int slot_load_t(rumgpu_bm* dst, int slot, rumgpu_bm* src) {
+  hipStream_t st = dst->ctx->stream;
+  const unsigned gr = ew_grid(dst->m2);
+  const i64 off = (i64)slot * dst->m2 * (i64)sizeof(TB);
+  void* a[4] = {src->kx2, src->ks, src->ky, src->kz};
+  void* b[4] = {dst->x2, dst->s, dst->y, dst->z};
+  for (int q = 0; q < 4; ++q)
+    hipLaunchKernelGGL((ew_copy<TA, TB>), dim3(gr), dim3(256), 0, st, (const TA*)a[q], (TB*)((char*)b[q] + off), dst->m2);
+  RG_LAUNCH_CHECK();
+  RG_TRY(mark_fresh(dst, slot));
+  RG_HIP(hipStreamSynchronize(st));
+  return RG_OK;
+}
+
+extern "C" int rumgpu_bm_slot_load(rumgpu_bm* b, int slot, rumgpu_bm* src) {
+  RG_SLOT_CHECK(b, slot, "bm_slot_load");
+  RG_CHECK(src && src->n == b->n && src->M == b->M, RG_EINVAL, "bm_slot_load: incompatible source handle");
+  RG_CHECK(src->kept_valid, RG_EINVAL, "bm_slot_load: the source handle keeps no state (bm_keep)");
+  RG_CHECK(src->ctx->stream == b->ctx->stream, RG_EINVAL, "bm_slot_load: handles of different contexts");
+  if (src->isz == 8 && b->isz == 8) return slot_load_t<double, double>(b, slot, src);
+  if (src->isz == 8 && b->isz == 4) return slot_load_t<double, float>(b, slot, src);
+  if (src->isz == 4 && b->isz == 8) return slot_load_t<float, double>(b, slot, src);
+  return slot_load_t<float, float>(b, slot, src);
+}
+
+template <typename T>
+static int slot_cold_t(rumgpu_bm* b, int slot) {
+  hipStream_t st = b->ctx->stream;
+  const i64 o = (i64)slot * b->m2;
+  const unsigned gr = ew_grid(b->m2);
+  hipLaunchKernelGGL((ew_fill<T>), dim3(gr), dim3(256), 0, st, (T*)b->s + o, (T)0, b->m2);
+  hipLaunchKernelGGL((ew_fill<T>), dim3(gr), dim3(256), 0, st, (T*)b->y + o, (T)0, b->m2);
+  hipLaunchKernelGGL((ew_fill<T>), dim3(gr), dim3(256), 0, st, (T*)b->z + o, (T)0, b->m2);
+  RG_LAUNCH_CHECK();
+  RG_TRY((launch_prox<T, 1>(b->n, nullptr, (T*)b->x2 + o, nullptr, b->obsmask, b->obsoff, b->pc + (i64)slot * b->M,
+                            (T)b->beta, st)));
+  RG_TRY(mark_fresh(b, slot));
+  RG_HIP(hipStreamSynchronize(st));
+  return RG_OK;
+}
+
+extern "C" int rumgpu_bm_slot_cold(rumgpu_bm* b, int slot) {
+  RG_SLOT_CHECK(b, slot, "bm_slot_cold");
+  return BM_DISPATCH(b, slot_cold_t, b, slot);
+}
+
+template <typename T>
+static int slot_obs_t(rumgpu_bm* b, int slot, const double* dx, const double* dz, double* ox, double* oz) {
+  hipStream_t st = b->ctx->stream;
+  const i64 o = (i64)slot * b->m2, M = b->M;
+  double *ddx = b->dwarm, *ddz = b->dwarm + M;
+  if (ox || oz) {
+    hipLaunchKernelGGL(k_get_obs<T>, dim3(grid_for(M, 256)), dim3(256), 0, st, (const T*)b->x2 + o, (const T*)b->z + o,
+                       b->obs, ddx, ddz, M);
+    RG_LAUNCH_CHECK();
+    if (ox) RG_HIP(rg_copy(st, ox, ddx, (size_t)M * 8, hipMemcpyDeviceToHost));
+    if (oz) RG_HIP(rg_copy(st, oz, ddz, (size_t)M * 8, hipMemcpyDeviceToHost));
+    return RG_OK;
+  }
+  if (dx) RG_HIP(hipMemcpyAsync(ddx, dx, (size_t)M * 8, hipMemcpyHostToDevice, st));
+  if (dz) RG_HIP(hipMemcpyAsync(ddz, dz, (size_t)M * 8, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_add_obs<T>, dim3(grid_for(M, 256)), dim3(256), 0, st, (T*)b->x2 + o, (T*)b->z + o, b->obs,
+                     dx ? (const double*)ddx : nullptr, dz ? (const double*)ddz : nullptr, M);
+  RG_LAUNCH_CHECK();
+  RG_HIP(hipStreamSynchronize(st));
+  return RG_OK;
+}
+
+extern "C" int rumgpu_bm_slot_add_obs(rumgpu_bm* b, int slot, const double* dx2, const double* dz) {
+  RG_SLOT_CHECK(b, slot, "bm_slot_add_obs");
+  return BM_DISPATCH(b, slot_obs_t, b, slot, dx2, dz, nullptr, nullptr);
+}
+
+extern "C" int rumgpu_bm_slot_get_obs(rumgpu_bm* b, int slot, double* x2_obs, double* z_obs) {
+  RG_SLOT_CHECK(b, slot, "bm_slot_get_obs");
+  RG_CHECK(x2_obs || z_obs, RG_EINVAL, "bm_slot_get_obs: no output");
+  return BM_DISPATCH(b, slot_obs_t, b, slot, nullptr, nullptr, x2_obs, z_obs);
+}
+
+extern "C" int rumgpu_bm_certify(rumgpu_bm* b, int want_lb, double* ub, double* lb, double* phi) {
+  RG_CHECK(b && ub && phi && (lb || !want_lb), RG_EINVAL, "bm_certify: null argument");
+  RG_HIP(hipSetDevice(b->ctx->device));
+  const int D = b->D;
+  if (b->isz == 8) {
+    RG_TRY(phi_impl<double>(b, b->x2, phi, 0, D));
+    RG_TRY(push_impl<double>(b, ub, D));
+    if (want_lb) RG_TRY(lb_impl<double>(b, lb, D));
+  } else {
+    RG_TRY(phi_impl<float>(b, b->x2, phi, 0, D));
+    RG_TRY(push_impl<float>(b, ub, D));
+    if (want_lb) RG_TRY(lb_impl<float>(b, lb, D));
+  }
+  for (int d = 0; d < D; ++d) phi[d] = std::sqrt(phi[d]);
+  return RG_OK;
+}
+
+extern "C" int rumgpu_bm_slot_fit(rumgpu_bm* b, int slot, double* p_fit) {
+  RG_SLOT_CHECK(b, slot, "bm_slot_fit");
+  RG_CHECK(p_fit, RG_EINVAL, "bm_slot_fit: null p_fit");
+  RG_HIP(rg_copy(b->ctx->stream, p_fit, b->fit + (i64)slot * b->M, (size_t)b->M * 8, hipMemcpyDeviceToHost));
   return RG_OK;
 }
 

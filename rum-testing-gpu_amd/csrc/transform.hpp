@@ -19,7 +19,9 @@ __global__ __launch_bounds__(512) void tr_pass(const TIN* src, T* dst, Coef<T> k
 #pragma unroll
       for (int j = 0; j < R; ++j) {
         const i64 a = t.gidx(j);
-        const i64 tail = (i64)expand_bit((u64)t.rest(j), t.item);
+        // item blocks of slot d are items [d n, (d + 1) n); its node arrays start at d 2^n
+        const int d = t.item / g.nper, x = t.item - d * g.nper;
+        const i64 tail = ((i64)d << g.nper) + (i64)expand_bit((u64)t.rest(j), x);
         const double ow = outw[tail];
         t.v[j] = (T)((ow > 0.0) ? mass[tail] * (double)src[a] / ow : 0.0);
       }

@@ -45,6 +45,14 @@ SIGNATURES = {
     "rumgpu_bm_x_obs": [_P, _P],
     "rumgpu_bm_info": [_P, _P, _P, _P],
     "rumgpu_bm_profile": [_P, _I64, _P, _P],
+    "rumgpu_bm_create_batch": [_P, _I, _P, _P, _P, _I64, _I, _D, _D, _D, _D, _D, _D, _P, _I, _P],
+    "rumgpu_bm_slot_set_p": [_P, _I, _P],
+    "rumgpu_bm_slot_load": [_P, _I, _P],
+    "rumgpu_bm_slot_cold": [_P, _I],
+    "rumgpu_bm_slot_add_obs": [_P, _I, _P, _P],
+    "rumgpu_bm_slot_get_obs": [_P, _I, _P, _P],
+    "rumgpu_bm_certify": [_P, _I, _P, _P, _P],
+    "rumgpu_bm_slot_fit": [_P, _I, _P],
     "rumgpu_butterfly64": [_P, _I, _P, _D, _D, _D, _D],
     "rumgpu_butterfly32": [_P, _I, _P, _F, _F, _F, _F],
     "rumgpu_prox_slab": [_P, _P, _P],

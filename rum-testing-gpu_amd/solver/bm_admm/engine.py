@@ -34,7 +34,7 @@ class Handle:
     """A rumgpu_bm object (see include/rumgpu.h)."""
 
     def __init__(self, n, menu, alt, p, dtype="float64", beta=1.0, alpha=1.95, precond_p=math.sqrt(2.0),
-                 plain=False, device=0):
+                 plain=False, device=0, slots=1):
         self.ctx = _lib.context(device)
         self.lib = self.ctx.lib
         self.n = int(n)
@@ -48,10 +48,18 @@ class Handle:
         e0, e1, gamma, Q = eigen(precond_p, beta, beta)
         h = ctypes.c_void_p()
         pv = _lib.c_array(p, np.float64)
-        _lib.check(self.lib.rumgpu_bm_create(self.ctx.h, self.n, _lib.ptr(self.menu), _lib.ptr(self.alt), _lib.ptr(pv),
-                                             self.M, self.dtype.itemsize, self.beta, self.alpha, self.precond_p,
-                                             e0, e1, gamma, _lib.ptr(Q), 1 if plain else 0, ctypes.byref(h)),
-                   "bm_create")
+        self.slots = int(slots)
+        if self.slots == 1:
+            rc = self.lib.rumgpu_bm_create(self.ctx.h, self.n, _lib.ptr(self.menu), _lib.ptr(self.alt), _lib.ptr(pv),
+                                           self.M, self.dtype.itemsize, self.beta, self.alpha, self.precond_p,
+                                           e0, e1, gamma, _lib.ptr(Q), 1 if plain else 0, ctypes.byref(h))
+        else:
+            if plain:
+                raise ValueError("the plain schedule runs one slot")
+            rc = self.lib.rumgpu_bm_create_batch(self.ctx.h, self.n, _lib.ptr(self.menu), _lib.ptr(self.alt),
+                                                 _lib.ptr(pv), self.M, self.dtype.itemsize, self.beta, self.alpha,
+                                                 self.precond_p, e0, e1, gamma, _lib.ptr(Q), self.slots, ctypes.byref(h))
+        _lib.check(rc, "bm_create")
         self.h = h
         self.plain = bool(plain)
         self.graph_len = 0
@@ -125,6 +133,46 @@ class Handle:
         out = np.empty(self.M)
         _lib.check(self.lib.rumgpu_bm_x_obs(self.h, _lib.ptr(out)), "bm_x_obs")
         return out
+
+    # ---------------------------------------------------------------- draw slots (batch objects)
+    def slot_set_p(self, slot, p):
+        pv = _lib.c_array(p, np.float64)
+        if pv.size != self.M:
+            raise ValueError("p: expected %d values" % self.M)
+        _lib.check(self.lib.rumgpu_bm_slot_set_p(self.h, int(slot), _lib.ptr(pv)), "bm_slot_set_p")
+
+    def slot_load(self, slot, src):
+        """slot <- the state `src` kept (src.keep())"""
+        _lib.check(self.lib.rumgpu_bm_slot_load(self.h, int(slot), src.h), "bm_slot_load")
+
+    def slot_cold(self, slot):
+        _lib.check(self.lib.rumgpu_bm_slot_cold(self.h, int(slot)), "bm_slot_cold")
+
+    def slot_add_obs(self, slot, dx2=None, dz=None):
+        a = None if dx2 is None else _lib.c_array(dx2, np.float64)
+        b = None if dz is None else _lib.c_array(dz, np.float64)
+        for v in (a, b):
+            if v is not None and v.size != self.M:
+                raise ValueError("corrections: expected %d values" % self.M)
+        _lib.check(self.lib.rumgpu_bm_slot_add_obs(self.h, int(slot), _lib.ptr(a), _lib.ptr(b)), "bm_slot_add_obs")
+
+    def slot_get_obs(self, slot):
+        x, z = np.empty(self.M), np.empty(self.M)
+        _lib.check(self.lib.rumgpu_bm_slot_get_obs(self.h, int(slot), _lib.ptr(x), _lib.ptr(z)), "bm_slot_get_obs")
+        return x, z
+
+    def certify(self, want_lb=True):
+        """float64 (ub, lb, phi) of every slot (lb None unless want_lb)"""
+        ub, phi = np.empty(self.slots), np.empty(self.slots)
+        lb = np.empty(self.slots) if want_lb else None
+        _lib.check(self.lib.rumgpu_bm_certify(self.h, 1 if want_lb else 0, _lib.ptr(ub), _lib.ptr(lb), _lib.ptr(phi)),
+                   "bm_certify")
+        return ub, lb, phi
+
+    def slot_fit(self, slot):
+        fit = np.empty(self.M)
+        _lib.check(self.lib.rumgpu_bm_slot_fit(self.h, int(slot), _lib.ptr(fit)), "bm_slot_fit")
+        return fit
 
     def prox_slab(self, xk):
         xk = _lib.c_array(xk, np.float64)

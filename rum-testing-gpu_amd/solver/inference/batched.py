@@ -1,0 +1,266 @@
+"""Batched bootstrap solves: many draws per GPU launch, retired and refilled one by one.
+
+The reference solves the open draws one after another, each a Session.solve warm from p_hat with the
+band's early stops (inference/bootstrap.py:308-335, rum_solver.py:914-925). Here one batch object
+(rumgpu_bm_create_batch) holds `slots` draws side by side in every slab: each launch of the fused
+iteration advances all of them, `check` iterations are replayed as one HIP graph, and after every
+check the float64 certificates of all slots (pushed upper bound, Frank-Wolfe lower bound) come
+back in one call. A slot whose draw is decided (ub below tau_lo, lb at or above tau_hi) or whose
+bracket meets the tolerance is retired: its bracket goes to the bootstrap and the next open draw
+is loaded into that slot (its p uploaded, its state copied from the kept p_hat solve on the device,
+optionally corrected by the learned warm start). Every retired bracket is a float64 certificate
+of the same kind the per-draw solve gives, so decisions rest on certified bounds alone.
+
+The stop tests per slot are the per-draw solve's (bm_admm/solver.py:294-377 as the session runs
+it): running-min ub and running-max lb, ub target = below, lb target = above, absolute and
+relative gap on the non-singleton part, a time limit per draw (draw_time) and max_iter.
+"""
+import math
+import time
+
+import numpy as np
+
+from .. import _lib, defaults
+
+SLOT_CHOICES = (4, 8, 16, 32, 64, 128, 256)
+_TUNED = {}
+
+
+def slot_bytes(n, itemsize):
+    """device bytes per draw slot: 6 iterate slabs, the float64 certificate slab, 2 node arrays"""
+    m2 = n << (n - 1)
+    return 6 * itemsize * m2 + 8 * m2 + 16 * (1 << n)
+
+
+def autotune_slots(n, menu, alt, p, dtype, beta, alpha, precond_p, check, max_slots, budget_bytes, src=None):
+    """Draws per launch for this n: time `check` graph-replayed iterations plus one certificate at
+    each candidate slot count and keep the smallest count within 5 % of the best draw-iterations
+    per second (more slots only add tail waste and memory). Cached per (n, rows, dtype)."""
+    from ..bm_admm.engine import Handle
+    key = (n, len(menu), str(dtype), int(check))
+    if key in _TUNED:
+        return _TUNED[key]
+    per = slot_bytes(n, np.dtype(dtype).itemsize)
+    cands = [d for d in SLOT_CHOICES if d <= max(4, max_slots) and d * per <= budget_bytes]
+    if not cands:
+        cands = [1]
+    rates = {}
+    for d in cands:
+        h = Handle(n, menu, alt, p, dtype=dtype, beta=beta, alpha=alpha, precond_p=precond_p, slots=d)
+        try:
+            h.set_graph(check)
+            h.run(check)                           # first iteration eager, graph warm
+            t0 = time.perf_counter()
+            h.run(check)
+            h.certify(want_lb=True)
+            rates[d] = d * check / (time.perf_counter() - t0)
+        finally:
+            h.close()
+    best = max(rates.values())
+    pick = min(d for d, r in rates.items() if r >= 0.95 * best)
+    _TUNED[key] = (pick, rates)
+    return _TUNED[key]
+
+
+class BatchedSolver:
+    """Solves bootstrap draws in slots; used by Bootstrap(batch=dict(...)).
+
+    slots   draws per launch (None: the autotuner's pick for this n)
+    dtype   iterate dtype of the batch ("float64"; "float32" iterates still get float64 bounds)
+    model   learned warm start: an object with correction(dp) -> (dx2, dz) on the non-singleton
+            rows (dp = P_b - p_hat there), or None; the certificate judges every draw either way
+    collect keep (dp, x2_obs - x2_hat_obs, z_obs - z_hat_obs) of every draw retired with a met
+            tolerance, as training pairs for the warm-start model
+    """
+
+    def __init__(self, boot, slots=None, dtype="float64", model=None, collect=False, max_slots=256, verbose=False):
+        from ..bm_admm.engine import Handle
+        sess = boot.session
+        if sess.method != "bm_admm":
+            raise ValueError("the batched bootstrap runs bm_admm draws (the session's method is %s)" % sess.method)
+        rum = boot.rum
+        self.boot, self.rum, self.verbose = boot, rum, verbose
+        a = sess._args
+        self.check, self.lb_every, self.max_iter = int(a["check"]), int(a["lb_every"]), int(a["max_iter"])
+        self.beta, self.alpha, self.precond_p = float(a["beta"]), float(a["alpha"]), float(a["precond_p"])
+        g = rum._geom
+        self.n, self.menu, self.alt = rum.n, g.menu, g.alternative.astype(np.int32)
+        self.rest, self.single = rum._rest, rum._single
+        self.dtype = np.dtype(dtype)
+        self.model, self.collect, self.pairs = model, bool(collect), []
+        kept = sess._kept
+        self.src = kept["handle"] if kept is not None else None
+        p0 = boot.p_hat[self.rest]
+        self.p_hat_rest = p0
+        from .. import machine as _machine
+        budget = defaults.AUTO["gpu_memory_fraction"] * _machine.gpu_free_bytes()
+        t0 = time.perf_counter()
+        self.tune = None
+        if slots is None:
+            slots, rates = autotune_slots(self.n, self.menu, self.alt, p0, self.dtype, self.beta, self.alpha,
+                                          self.precond_p, self.check, max_slots, budget)
+            self.tune = rates
+        self.D = int(slots)
+        if self.D * slot_bytes(self.n, self.dtype.itemsize) > budget:
+            raise MemoryError("out of memory: %d slots at n = %d need %.1f GB" %
+                              (self.D, self.n, self.D * slot_bytes(self.n, self.dtype.itemsize) / 2 ** 30))
+        self.h = Handle(self.n, self.menu, self.alt, p0, dtype=self.dtype, beta=self.beta, alpha=self.alpha,
+                        precond_p=self.precond_p, slots=self.D)
+        self.h.set_graph(self.check)
+        self.x_hat_obs = self.z_hat_obs = None
+        self.stats = dict(slots=self.D, autotune=self.tune, setup_s=time.perf_counter() - t0, draws=0, launches=0,
+                          checks=0, iterations=0, slot_iterations=0, busy_slot_iterations=0, solve_s=0.0,
+                          certify_s=0.0, refill_s=0.0, model_s=0.0)
+
+    # ------------------------------------------------------------------ one draw into a slot
+    def _load(self, slot, P_rest):
+        t0 = time.perf_counter()
+        h = self.h
+        h.slot_set_p(slot, P_rest)
+        if self.src is not None:
+            h.slot_load(slot, self.src)
+        else:
+            h.slot_cold(slot)
+        if self.model is not None and self.src is not None:
+            tm = time.perf_counter()
+            dx, dz = self.model.correction(P_rest - self.p_hat_rest)
+            self.stats["model_s"] += time.perf_counter() - tm
+            if dx is not None or dz is not None:
+                h.slot_add_obs(slot, dx, dz)
+        self.stats["refill_s"] += time.perf_counter() - t0
+
+    def solve(self, idx, P_of, abs_tol, rel_tol, tau_lo, tau_hi, draw_time=None):
+        """Yields (b, L, U, exit_flag) per draw of idx (full-distance bracket, Result flags)."""
+        idx = [int(b) for b in idx]
+        if not idx:
+            return
+        D, h = self.D, self.h
+        kept = self.boot.session._kept               # a refinement round may have re-kept p_hat
+        self.src = kept["handle"] if kept is not None else None
+        if self.src is not None and (self.model is not None or self.collect):
+            h.slot_load(0, self.src)
+            self.x_hat_obs, self.z_hat_obs = h.slot_get_obs(0)
+        draw_time = float(defaults.BOOTSTRAP["draw_time"] if draw_time is None else draw_time)
+        chunk = max(4 * D, 64)
+        queue_pos, buf, buf_pos = 0, None, 0
+        # per-slot bookkeeping
+        b_of = np.full(D, -1, dtype=np.int64)
+        below = np.full(D, np.nan)
+        above = np.full(D, np.nan)
+        const = np.zeros(D)
+        ub = np.full(D, np.inf)
+        lb = np.zeros(D)
+        it = np.zeros(D, dtype=np.int64)
+        t_in = np.zeros(D)
+        dp_of = [None] * D
+
+        def next_draw():
+            nonlocal queue_pos, buf, buf_pos
+            if queue_pos >= len(idx):
+                return None
+            if buf is None or buf_pos >= len(buf):
+                sel = idx[queue_pos:queue_pos + chunk]
+                buf, buf_pos = np.asarray(P_of(sel), dtype=np.float64), 0
+            P = buf[buf_pos]
+            b = idx[queue_pos]
+            buf_pos += 1
+            queue_pos += 1
+            return b, P
+
+        def fill(slot):
+            nd = next_draw()
+            if nd is None:
+                b_of[slot] = -1
+                return
+            b, P = nd
+            r_s = 1.0 - P[self.single]
+            c = float(r_s @ r_s)
+            Pr = np.ascontiguousarray(P[self.rest])
+            self._load(slot, Pr)
+            b_of[slot], const[slot] = b, c
+            below[slot] = _on_rest(tau_lo, c, False)
+            above[slot] = _on_rest(tau_hi, c, True)
+            ub[slot], lb[slot], it[slot] = np.inf, 0.0, 0
+            t_in[slot] = time.perf_counter()
+            dp_of[slot] = (Pr - self.p_hat_rest) if self.collect else None
+
+        t_all = time.perf_counter()
+        for s in range(D):
+            fill(s)
+        checks = 0
+        while (b_of >= 0).any():
+            t0 = time.perf_counter()
+            h.run(self.check)
+            t1 = time.perf_counter()
+            want_lb = (checks % self.lb_every) == 0
+            u, l, _phi = h.certify(want_lb=want_lb)
+            t2 = time.perf_counter()
+            checks += 1
+            self.stats["solve_s"] += t1 - t0
+            self.stats["certify_s"] += t2 - t1
+            self.stats["checks"] += 1
+            self.stats["iterations"] += self.check
+            self.stats["slot_iterations"] += self.check * D
+            live = b_of >= 0
+            self.stats["busy_slot_iterations"] += self.check * int(live.sum())
+            it[live] += self.check
+            ub[live] = np.minimum(ub[live], u[live])
+            if want_lb:
+                lb[live] = np.maximum(lb[live], l[live])
+            now = time.perf_counter()
+            for s in np.flatnonzero(live):
+                stop = None
+                if not math.isnan(below[s]) and below[s] > 0 and ub[s] <= below[s]:
+                    stop = "threshold"
+                elif not math.isnan(above[s]) and lb[s] >= above[s]:
+                    stop = "threshold"
+                else:
+                    gap = ub[s] - lb[s]
+                    if gap <= abs_tol or (lb[s] > 0 and gap / lb[s] <= rel_tol):
+                        stop = "tol"
+                    elif now - t_in[s] > draw_time:
+                        stop = "time"
+                    elif it[s] >= self.max_iter:
+                        stop = "iters"
+                if stop is None:
+                    continue
+                L = math.sqrt(max(lb[s], 0.0) ** 2 + const[s])
+                U = math.sqrt(ub[s] ** 2 + const[s]) if math.isfinite(ub[s]) else math.inf
+                flag = _flag(L, U, abs_tol, rel_tol, stop)
+                if self.collect and flag in (1, 2, 3) and self.x_hat_obs is not None:
+                    x, z = h.slot_get_obs(s)
+                    self.pairs.append((dp_of[s], x - self.x_hat_obs, z - self.z_hat_obs, int(it[s])))
+                self.stats["draws"] += 1
+                yield int(b_of[s]), L, U, flag
+                fill(s)
+        self.stats["wall_s"] = self.stats.get("wall_s", 0.0) + time.perf_counter() - t_all
+        sl = max(1, self.stats["slot_iterations"])
+        self.stats["slot_use"] = self.stats["busy_slot_iterations"] / sl
+
+    def close(self):
+        if getattr(self, "h", None) is not None:
+            self.h.close()
+            self.h = None
+
+
+def _on_rest(tau, const, above):
+    """the threshold on the non-singleton part (rum_solver._on_rest), NaN for none"""
+    if tau is None:
+        return math.nan
+    tau = float(tau)
+    if tau * tau <= const:
+        return 0.0 if above else math.nan
+    return math.sqrt(tau * tau - const)
+
+
+def _flag(L, U, abs_tol, rel_tol, stop):
+    """Result.exit_flag of a bracket (rum_solver.py:42-46 semantics): tolerances first"""
+    gap = U - L
+    met_abs, met_rel = gap <= abs_tol, L > 0 and gap / L <= rel_tol
+    if met_abs and met_rel:
+        return 3
+    if met_abs:
+        return 1
+    if met_rel:
+        return 2
+    return {"threshold": 5, "time": 0, "iters": -1}.get(stop, -2)

@@ -1,0 +1,159 @@
+"""Draw slots of the batch object, through the C ABI, against single-draw objects.
+
+Every kernel of an iteration works item by item (the butterflies within an item block, the prox
+within a menu of one slot), and every certificate reduction of a slot runs the same blocks in the
+same order, so slot d of a batch object must equal a single object on the same p and start bit for
+bit: states, pushed upper bounds, lower bounds, phi and fitted points. The batched bootstrap must
+reach the per-draw bootstrap's decision on every draw.
+"""
+import numpy as np
+import pytest
+
+from solver import instances
+from solver.bm_admm.engine import Handle
+from solver.inference import cells_from_panel
+from solver.rum_solver import RUM
+
+pytestmark = pytest.mark.gpu
+
+
+def _medium(n, cells, rng):
+    seen, om, ox = set(), [], []
+    while len(om) < cells:
+        m = int(rng.integers(0, 1 << n))
+        if bin(m).count("1") < 2 or m in seen:
+            continue
+        seen.add(m)
+        for a in range(n):
+            if (m >> a) & 1:
+                om.append(m)
+                ox.append(a)
+    menu, alt = np.array(om, dtype=np.uint64), np.array(ox, dtype=np.int32)
+    p = rng.random(menu.size)
+    _, inv = np.unique(menu, return_inverse=True)
+    return menu, alt, p / np.bincount(inv, weights=p)[inv]
+
+
+def _perturb(menu, p, rng, eps=0.05):
+    q = np.clip(p + eps * rng.standard_normal(p.size), 1e-3, None)
+    _, inv = np.unique(menu, return_inverse=True)
+    return q / np.bincount(inv, weights=q)[inv]
+
+
+def _same(a, b):
+    assert np.array_equal(a, b), np.abs(np.asarray(a) - np.asarray(b)).max()
+
+
+@pytest.mark.parametrize("n,dtype,graph", [(9, "float64", 0), (12, "float64", 5), (18, "float64", 5),
+                                           (11, "float32", 4)])
+def test_slots_equal_single_objects(n, dtype, graph):
+    rng = np.random.default_rng(300 + n)
+    menu, alt, p = _medium(n, min(6000, n << (n - 3)), rng)
+    D = 3
+    P = [_perturb(menu, p, rng) for _ in range(D)]
+    beta = 0.8
+    bat = Handle(n, menu, alt, p, dtype=dtype, beta=beta, slots=D)
+    if graph:
+        bat.set_graph(graph)
+    for d in range(D):
+        bat.slot_set_p(d, P[d])
+        bat.slot_cold(d)
+    singles = []
+    for d in range(D):
+        h = Handle(n, menu, alt, P[d], dtype=dtype, beta=beta)
+        if graph:
+            h.set_graph(graph)
+        singles.append(h)
+    # a warm source: a single object's state after 12 iterations on p, kept
+    src = Handle(n, menu, alt, p, dtype="float64", beta=beta)
+    src.run(12)
+    src.keep()
+    for rnd in range(2):
+        bat.run(11)
+        for h in singles:
+            h.run(11)
+        ub, lb, phi = bat.certify(want_lb=True)
+        for d, h in enumerate(singles):
+            u1, fit1 = h.push(want_fit=True)
+            assert ub[d] == u1, (rnd, d, ub[d], u1)
+            assert lb[d] == h.lb(), (rnd, d)
+            assert phi[d] == h.phi(), (rnd, d)
+            _same(bat.slot_fit(d), fit1)
+            x, z = bat.slot_get_obs(d)
+            _same(x, h.x_obs())
+        if rnd == 0:
+            # refill slot 1 with a new draw warm from the kept state; its single twin copies the
+            # same state (src's live state equals what it kept)
+            Pn = _perturb(menu, p, rng)
+            bat.slot_set_p(1, Pn)
+            bat.slot_load(1, src)
+            singles[1].close()
+            h = Handle(n, menu, alt, Pn, dtype=dtype, beta=beta)
+            if graph:
+                h.set_graph(graph)
+            h.copy_state_from(src)
+            singles[1] = h
+    for h in singles + [bat, src]:
+        h.close()
+
+
+def test_slot_obs_corrections():
+    n = 10
+    rng = np.random.default_rng(7)
+    menu, alt, p = _medium(n, 900, rng)
+    bat = Handle(n, menu, alt, p, dtype="float64", beta=1.0, slots=2)
+    bat.run(5)
+    x0, z0 = bat.slot_get_obs(1)
+    dx, dz = 1e-3 * rng.standard_normal(menu.size), 1e-3 * rng.standard_normal(menu.size)
+    bat.slot_add_obs(1, dx, dz)
+    x1, z1 = bat.slot_get_obs(1)
+    _same(x1, x0 + dx)
+    _same(z1, z0 + dz)
+    xa, _ = bat.slot_get_obs(0)
+    bat.slot_add_obs(0, None, dz)
+    xb, zb = bat.slot_get_obs(0)
+    _same(xa, xb)
+    with pytest.raises(ValueError):
+        bat.slot_set_p(2, p)
+    bat.close()
+
+
+def _small_panel(seed=1, n=8):
+    """40 menus, 600 respondents x 8 menus from a 6-order mixture: p_hat near the polytope, so the
+    draws' distances straddle tau (9 of the first 96 above it, by exact enumeration)"""
+    rng = np.random.default_rng(seed)
+    menus, seen = [], set()
+    while len(menus) < 40:
+        m = int(rng.integers(0, 1 << n))
+        if bin(m).count("1") < 2 or m in seen:
+            continue
+        seen.add(m)
+        menus.append(m)
+    orders = [rng.permutation(n) for _ in range(6)]
+    w = rng.dirichlet(np.ones(6))
+    return instances._panel(rng, n, menus, 600, 8, orders, w), n
+
+
+def test_batched_bootstrap_same_decisions():
+    import pandas as pd
+    panel, n = _small_panel()
+    panel = pd.DataFrame(panel)
+    menu, alt, p = cells_from_panel(panel, n)
+    rum = RUM(menu, alt, p, num_alternatives=n)
+    B = 96
+    args = dict(method="bm_admm", method_args={"beta": "cert"}, abs_tol=1e-4, refine=1)
+    with rum.bootstrap(panel=panel, B=B, **args) as one:
+        pv1 = one.p_value()
+    with rum.bootstrap(panel=panel, B=B, batch=dict(slots=16), **args) as bat:
+        pv2 = bat.p_value()
+        st = bat.batch_stats
+    print(pv1)
+    print(pv2)
+    print(st)
+    d1, d2 = one._decisions(), bat._decisions()
+    assert np.array_equal(d1, d2), np.flatnonzero(d1 != d2)
+    assert pv1.p == pv2.p and pv1.n_below == pv2.n_below and pv1.n_above == pv2.n_above
+    # both brackets of every solved draw hold the same distance
+    solved = (one.flag != 127) & (bat.flag != 127)
+    assert (bat.L[solved] <= one.U[solved] + 1e-9).all() and (one.L[solved] <= bat.U[solved] + 1e-9).all()
+    assert pv1.n_solved >= 20 and st["draws"] >= pv2.n_solved
