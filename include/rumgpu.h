@@ -137,6 +137,11 @@ int rumgpu_cg_destroy(rumgpu_cg* cg);
 /* one pricing sweep: scores (float32, x-major) -> vmax (float64 decode), rank_pos[n] (position
  * of each alternative, 0 = best), margin (u16/u24 quantization bound, 0 otherwise) */
 int rumgpu_cg_price(rumgpu_cg* cg, const float* scores_x, double* vmax, int32_t* rank_pos, double* margin);
+/* D pricing sweeps in one pass over the lattice (mask layout): scores is D x M (draw-major, x-major
+ * rows), vmax[D], rank_pos[D x n]. Draw d's result equals rumgpu_cg_price on row d of scores bit
+ * for bit. No reference counterpart: the batched bootstrap prices the open draws' columns
+ * together instead of one bf_update sweep per draw (flow_cg/solver.py:168-200). */
+int rumgpu_cg_price_batch(rumgpu_cg* cg, int D, const float* scores, double* vmax, int32_t* rank_pos);
 int rumgpu_cg_info(rumgpu_cg* cg, size_t* device_bytes, double* last_kernel_ms);
 
 #ifdef __cplusplus

@@ -61,6 +61,76 @@ __global__ __launch_bounds__(256) void cg_mask_layer(float* __restrict__ V, uint
   }
 }
 
+// mask layout, DB draws per sweep: the subset tests of a node are shared by the draws, each draw
+// keeps its own sums, values and choices. V and pred are node-major ([node][DB]), scores row-major
+// ([row][DB]), so the DB values of one node or row are one contiguous line. Per draw the arithmetic
+// and its order are cg_mask_layer's (row order within x, x ascending, strict > after the first
+// member), so every draw's V and pred equal a single-draw sweep on its scores bit for bit.
+template <int DB>
+__global__ __launch_bounds__(256) void cg_mask_layer_batch(float* __restrict__ V, uint8_t* __restrict__ pred,
+                                                           const u32* __restrict__ masks,
+                                                           const float* __restrict__ scores,
+                                                           const u64* __restrict__ offsets,
+                                                           const u32* __restrict__ nodes, i64 ns, int n) {
+  const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ns) return;
+  const u32 D = nodes[i];
+  float best[DB];
+  uint8_t bx[DB];
+#pragma unroll
+  for (int d = 0; d < DB; ++d) {
+    best[d] = 0.0f;
+    bx[d] = 0;
+  }
+  bool seen = false;
+  for (int x = 0; x < n; ++x) {
+    if (!((D >> x) & 1u)) continue;
+    float w[DB];
+#pragma unroll
+    for (int d = 0; d < DB; ++d) w[d] = 0.0f;
+    const u64 lo = offsets[x], hi = offsets[x + 1];
+    for (u64 j = lo; j < hi; ++j) {
+      const u32 Dp = masks[j];
+      if ((Dp & D) == Dp) {
+        const float* sj = scores + j * DB;
+#pragma unroll
+        for (int d = 0; d < DB; ++d) w[d] += sj[d];
+      }
+    }
+    const float* vp = V + (size_t)(D ^ (1u << x)) * DB;
+#pragma unroll
+    for (int d = 0; d < DB; ++d) {
+      const float cand = vp[d] + w[d];
+      if (!seen || cand > best[d]) {
+        best[d] = cand;
+        bx[d] = (uint8_t)x;
+      }
+    }
+    seen = true;
+  }
+  float* vo = V + (size_t)D * DB;
+  uint8_t* po = pred + (size_t)D * DB;
+#pragma unroll
+  for (int d = 0; d < DB; ++d) {
+    vo[d] = best[d];
+    po[d] = bx[d];
+  }
+}
+
+// one thread per draw: walk the choices down from the full set; vmax = V[full]
+__global__ void cg_backtrace_batch(const uint8_t* __restrict__ pred, const float* __restrict__ V, int n, int DB,
+                                   int nd, int32_t* rank_pos, float* vmax, u64 full) {
+  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= nd) return;
+  u64 S = full;
+  vmax[d] = V[full * DB + d];
+  for (int i = 0; i < n; ++i) {
+    const int cur = pred[S * DB + d];
+    rank_pos[d * n + cur] = i;
+    S ^= 1ull << cur;
+  }
+}
+
 // slab layout: scatter of the scores, then one layer of the max over members
 __global__ void cg_scatter(float* __restrict__ g, const i64* __restrict__ pos, const float* __restrict__ scores, i64 M) {
   const i64 j = (i64)blockIdx.x * blockDim.x + threadIdx.x;
@@ -269,6 +339,12 @@ struct rumgpu_cg {
   float* V = nullptr;
   uint8_t* pred = nullptr;
   u32* nodes = nullptr;
+  // batched mask sweeps (allocated at the first rumgpu_cg_price_batch)
+  float* Vb = nullptr;
+  uint8_t* predb = nullptr;
+  float* scoresb = nullptr;
+  float* vmaxb = nullptr;
+  int32_t* rankb = nullptr;
   float* R = nullptr;        // slab
   int32_t* rank = nullptr;
   // stream
@@ -295,7 +371,8 @@ struct rumgpu_cg {
 static void cg_free_all(rumgpu_cg* c) {
   void* bufs[] = {c->masks32, c->masks64, c->offsets, c->scores, c->spos, c->V, c->pred, c->nodes, c->R, c->rank,
                   c->binom, c->vbuf[0], c->vbuf[1], c->lbuf[0], c->lbuf[1], c->mbuf[0], c->mbuf[1], c->ebuf[0],
-                  c->ebuf[1], c->lstep, c->subV, c->subpred, c->subnodes, c->submasks, c->subscores, c->suboffsets};
+                  c->ebuf[1], c->lstep, c->subV, c->subpred, c->subnodes, c->submasks, c->subscores, c->suboffsets,
+                  c->Vb, c->predb, c->scoresb, c->vmaxb, c->rankb};
   for (void* q : bufs) rg_free(q);
   for (auto* p : c->predk) rg_free(p);
 }
@@ -613,6 +690,68 @@ extern "C" int rumgpu_cg_price(rumgpu_cg* c, const float* scores_x, double* vmax
   RG_HIP(hipMemcpyAsync(order, c->rank, (size_t)K * 4, hipMemcpyDeviceToHost, st));
   RG_HIP(hipStreamSynchronize(st));
   for (int j = 0; j < K; ++j) rank_pos[elems[j]] = pos + order[j];
+  c->last_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return RG_OK;
+}
+
+#define CG_BATCH 16
+
+template <int DB>
+static int mask_sweep_batch(rumgpu_cg* c, hipStream_t st) {
+  const int n = c->n;
+  hipLaunchKernelGGL(cg_fill_f32, dim3(1), dim3(64), 0, st, c->Vb, 0.0f, (i64)DB);     // V[empty set] = 0
+  for (int k = 1; k <= n; ++k) {
+    const i64 lo = c->bounds[k], ns = c->bounds[k + 1] - lo;
+    if (ns <= 0) continue;
+    hipLaunchKernelGGL(cg_mask_layer_batch<DB>, dim3(grid_for(ns, 256)), dim3(256), 0, st, c->Vb, c->predb,
+                       c->masks32, c->scoresb, c->offsets, c->nodes + lo, ns, n);
+  }
+  RG_LAUNCH_CHECK();
+  return RG_OK;
+}
+
+extern "C" int rumgpu_cg_price_batch(rumgpu_cg* c, int D, const float* scores, double* vmax, int32_t* rank_pos) {
+  RG_CHECK(c && scores && vmax && rank_pos && D >= 1, RG_EINVAL, "cg_price_batch: bad argument");
+  RG_CHECK(c->layout == 0, RG_EUNSUP, "cg_price_batch: batched sweeps use the mask layout");
+  RG_HIP(hipSetDevice(c->ctx->device));
+  hipStream_t st = c->ctx->stream;
+  const int n = c->n;
+  const i64 M = c->M;
+  auto t0 = std::chrono::steady_clock::now();
+  if (!c->Vb) {
+    int rc = RG_OK;
+    if ((rc = cg_alloc(c, (void**)&c->Vb, (size_t)c->NN * CG_BATCH * 4, "batched V")) != RG_OK ||
+        (rc = cg_alloc(c, (void**)&c->predb, (size_t)c->NN * CG_BATCH, "batched pred")) != RG_OK ||
+        (rc = cg_alloc(c, (void**)&c->scoresb, (size_t)M * CG_BATCH * 4, "batched scores")) != RG_OK ||
+        (rc = cg_alloc(c, (void**)&c->vmaxb, (size_t)CG_BATCH * 4, "batched vmax")) != RG_OK ||
+        (rc = cg_alloc(c, (void**)&c->rankb, (size_t)CG_BATCH * 64 * 4, "batched ranks")) != RG_OK) {
+      rg_free(c->Vb), rg_free(c->predb), rg_free(c->scoresb), rg_free(c->vmaxb), rg_free(c->rankb);
+      c->Vb = nullptr, c->predb = nullptr, c->scoresb = nullptr, c->vmaxb = nullptr, c->rankb = nullptr;
+      return rc;
+    }
+  }
+  std::vector<float> tr((size_t)M * CG_BATCH), vf(CG_BATCH);
+  for (int first = 0; first < D; first += CG_BATCH) {
+    const int nd = std::min(CG_BATCH, D - first);
+    const int DB = nd <= 1 ? 1 : nd <= 2 ? 2 : nd <= 4 ? 4 : nd <= 8 ? 8 : 16;
+    for (i64 j = 0; j < M; ++j)
+      for (int d = 0; d < DB; ++d) tr[(size_t)j * DB + d] = d < nd ? scores[(size_t)(first + d) * M + j] : 0.0f;
+    RG_HIP(hipMemcpyAsync(c->scoresb, tr.data(), (size_t)M * DB * 4, hipMemcpyHostToDevice, st));
+    switch (DB) {
+      case 1: RG_TRY(mask_sweep_batch<1>(c, st)); break;
+      case 2: RG_TRY(mask_sweep_batch<2>(c, st)); break;
+      case 4: RG_TRY(mask_sweep_batch<4>(c, st)); break;
+      case 8: RG_TRY(mask_sweep_batch<8>(c, st)); break;
+      default: RG_TRY(mask_sweep_batch<16>(c, st)); break;
+    }
+    hipLaunchKernelGGL(cg_backtrace_batch, dim3(1), dim3(64), 0, st, c->predb, c->Vb, n, DB, nd, c->rankb, c->vmaxb,
+                       (u64)(c->NN - 1));
+    RG_LAUNCH_CHECK();
+    RG_HIP(hipMemcpyAsync(vf.data(), c->vmaxb, (size_t)nd * 4, hipMemcpyDeviceToHost, st));
+    RG_HIP(hipMemcpyAsync(rank_pos + (size_t)first * n, c->rankb, (size_t)nd * n * 4, hipMemcpyDeviceToHost, st));
+    RG_HIP(hipStreamSynchronize(st));
+    for (int d = 0; d < nd; ++d) vmax[first + d] = (double)vf[d];
+  }
   c->last_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return RG_OK;
 }

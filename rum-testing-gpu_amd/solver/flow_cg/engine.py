@@ -54,6 +54,16 @@ class Pricer:
                                             ctypes.byref(margin)), "cg_price")
         return vmax.value, self.rank.copy(), margin.value
 
+    def price_batch(self, scores):
+        """D sweeps at once (mask layout): scores (D, M) x-major -> vmax[D], rank_pos (D, n)"""
+        s = np.ascontiguousarray(scores, dtype=np.float32)
+        D = s.shape[0]
+        vmax = np.empty(D)
+        rank = np.empty((D, self.n), dtype=np.int32)
+        _lib.check(self.lib.rumgpu_cg_price_batch(self.h, D, _lib.ptr(s), _lib.ptr(vmax), _lib.ptr(rank)),
+                   "cg_price_batch")
+        return vmax, rank
+
     def info(self):
         b, ms = ctypes.c_size_t(), ctypes.c_double()
         _lib.check(self.lib.rumgpu_cg_info(self.h, ctypes.byref(b), ctypes.byref(ms)), "cg_info")
@@ -120,19 +130,42 @@ def ranking_column(memb, alt_x, rank_pos, n):
     return (top == alt_x).astype(np.uint8)
 
 
-def solve_cg(n, menu, alt, p, pricer=None, layout="mask", dtype="float32", rc_tol=0.0, max_iters=4000,
-             time_limit_s=0.0, seed=0, verbose=False, phi_ub_target=None, gap_rel_tol=None, gap_abs_tol=None,
-             init_columns=None, init_lambda=None, early_ub_check=False, lb_stop=None):
+def solve_cg(n, menu, alt, p, pricer=None, layout="mask", dtype="float32", **kw):
     """Column generation to the first stop (solver.py:28-279 semantics). Rows in input order; the
-    engine works x-major and returns p_out x-major with `perm` (input row of each x-major row)."""
+    engine works x-major and returns p_out x-major with `perm` (input row of each x-major row).
+    The loop itself is cg_steps; here every pricing request it yields goes to `pricer`."""
+    own = pricer is None
+    if own:
+        perm, masks_x, _alt_x, offsets = xmajor(menu, alt, n)
+        pricer = Pricer(n, masks_x, offsets, layout=layout, dtype=dtype)
+    try:
+        return drive(cg_steps(n, menu, alt, p, pricer, **kw), pricer)
+    finally:
+        if own:
+            pricer.close()
+
+
+def drive(gen, pricer):
+    """Run a pricing-request generator (cg_steps, or a runner built on it) with one pricer."""
+    try:
+        req = next(gen)
+        while True:
+            req = gen.send(pricer.price(req))
+    except StopIteration as e:
+        return e.value
+
+
+def cg_steps(n, menu, alt, p, pricer, rc_tol=0.0, max_iters=4000, time_limit_s=0.0, seed=0, verbose=False,
+             phi_ub_target=None, gap_rel_tol=None, gap_abs_tol=None, init_columns=None, init_lambda=None,
+             early_ub_check=False, lb_stop=None):
+    """solve_cg's loop as a generator: yields the float32 scores of each pricing sweep, receives
+    (vmax, rank_pos, margin), returns the result dict. A driver may price many of these loops in
+    one batched sweep (inference/batched.py); each loop's arithmetic is the same either way."""
     t_setup = time.perf_counter()
     perm, masks_x, alt_x, offsets = xmajor(menu, alt, n)
     p32 = np.asarray(p, dtype=np.float64)[perm].astype(np.float32)
     p_obs = p32.astype(np.float64)
     M = p_obs.size
-    own = pricer is None
-    if own:
-        pricer = Pricer(n, masks_x, offsets, layout=layout, dtype=dtype)
     memb = ((masks_x[:, None] >> np.arange(n, dtype=np.uint64)[None, :]) & np.uint64(1)).astype(bool)
     setup_s = time.perf_counter() - t_setup
     p_sq = float(p_obs @ p_obs)
@@ -180,7 +213,7 @@ def solve_cg(n, menu, alt, p, pricer=None, layout="mask", dtype="float32", rc_to
                 break
         resid = p_obs - cols.p_out
         tpz = time.perf_counter()
-        vmax, rank_pos, margin = pricer.price(resid.astype(np.float32))
+        vmax, rank_pos, margin = yield resid.astype(np.float32)
         price_s += time.perf_counter() - tpz
         if margin:
             margins.append(margin)
@@ -211,8 +244,6 @@ def solve_cg(n, menu, alt, p, pricer=None, layout="mask", dtype="float32", rc_to
         lam = np.concatenate([lam, [0.0]])
     wall = time.perf_counter() - t0
     dev_bytes = pricer.info()[0]
-    if own:
-        pricer.close()
     return dict(wall_s=wall, setup_s=setup_s, trace=trace, margins=margins, iters=it, n_cols=cols.K, lam=lam,
                 p_out=cols.p_out.copy(), perm=perm, columns=[np.frombuffer(k, dtype=np.uint8).copy() for k in cols.keys],
                 rc_final=rc_final, distance=UB, lb=lb_best, gap_rel=gap_rel, stop_reason=stop, qp_s=qp_s,

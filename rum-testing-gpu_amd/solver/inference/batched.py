@@ -20,7 +20,7 @@ import time
 
 import numpy as np
 
-from .. import _lib, defaults
+from .. import defaults
 
 SLOT_CHOICES = (4, 8, 16, 32, 64, 128, 256)
 _TUNED = {}
@@ -32,7 +32,7 @@ def slot_bytes(n, itemsize):
     return 6 * itemsize * m2 + 8 * m2 + 16 * (1 << n)
 
 
-def autotune_slots(n, menu, alt, p, dtype, beta, alpha, precond_p, check, max_slots, budget_bytes, src=None):
+def autotune_slots(n, menu, alt, p, dtype, beta, alpha, precond_p, check, max_slots, budget_bytes):
     """Draws per launch for this n: time `check` graph-replayed iterations plus one certificate at
     each candidate slot count and keep the smallest count within 5 % of the best draw-iterations
     per second (more slots only add tail waste and memory). Cached per (n, rows, dtype)."""
@@ -264,3 +264,104 @@ def _flag(L, U, abs_tol, rel_tol, stop):
     if met_rel:
         return 2
     return {"threshold": 5, "time": 0, "iters": -1}.get(stop, -2)
+
+
+class BatchedCG:
+    """flow_cg draws priced together: each open draw runs the session's own flow_cg runner (the
+    per-draw Session.solve, warm from the kept p_hat columns, with the band's stops), and every
+    round the pricing requests of all draws in flight go to the GPU as one batched mask sweep
+    (rumgpu_cg_price_batch), whose result for each draw equals its own sweep bit for bit. So every
+    draw's columns, weights and bracket are those of its per-draw solve; only the sweeps are shared.
+
+    slots   draws in flight (a multiple of 16 fills the sweeps)
+    """
+
+    def __init__(self, boot, slots=32, verbose=False, **_ignored):
+        sess = boot.session
+        if sess.method != "flow_cg":
+            raise ValueError("BatchedCG runs flow_cg draws (the session's method is %s)" % sess.method)
+        if sess._args.get("pricing") != "mask":
+            raise ValueError("batched sweeps need the mask pricing layout (the plan uses %s)" % sess._args["pricing"])
+        self.boot, self.sess, self.D, self.verbose = boot, sess, int(slots), verbose
+        self.stats = dict(slots=self.D, draws=0, rounds=0, sweeps=0, price_s=0.0, host_s=0.0, wall_s=0.0,
+                          draw_rounds=0)
+
+    def solve(self, idx, P_of, abs_tol, rel_tol, tau_lo, tau_hi, draw_time=None):
+        """Yields (b, L, U, exit_flag) per draw of idx, as Bootstrap._solve records them."""
+        from ..rum_solver import Result, _flow_cg_steps, _cg_pricer, _TimeUp, _Decided, _is_oom
+        idx = [int(b) for b in idx]
+        if not idx:
+            return
+        sess, D = self.sess, self.D
+        draw_time = float(defaults.BOOTSTRAP["draw_time"] if draw_time is None else draw_time)
+        chunk = max(4 * D, 64)
+        pos, buf = 0, None
+        live = {}                       # slot -> [b, task, gen, request]
+        pricer = None
+        t_all = time.perf_counter()
+
+        def step(slot, reply):
+            """advance one draw's runner; returns a finished (b, L, U, flag) or None"""
+            b, task, gen = live[slot][:3]
+            msg, stop = "", None
+            try:
+                live[slot][3] = next(gen) if reply is None else gen.send(reply)
+                return None
+            except StopIteration as e:
+                stop = e.value
+            except _TimeUp:
+                stop = "time"
+            except _Decided:
+                stop = "threshold"
+            except Exception as e:        # noqa: BLE001 -- reported through the exit flag, as Session.solve
+                stop = "oom" if _is_oom(e) else "error"
+                msg = "%s: %s" % (type(e).__name__, e)
+            res = sess._finish(task, stop, msg)
+            del live[slot]
+            return b, res.lb, res.ub, res.exit_flag
+
+        def start(slot):
+            nonlocal pos, buf, pricer
+            while pos < len(idx):
+                k = pos % chunk
+                if k == 0:
+                    buf = np.asarray(P_of(idx[pos:pos + chunk]), dtype=np.float64)
+                b, P = idx[pos], buf[k]
+                pos += 1
+                task = sess._task(P, abs_tol, rel_tol, draw_time, tau_lo, tau_hi, True, False, False)
+                if isinstance(task, Result):
+                    yield b, task.lb, task.ub, task.exit_flag
+                    continue
+                if pricer is None:
+                    pricer = _cg_pricer(task)
+                task.t_run = time.perf_counter()
+                live[slot] = [b, task, _flow_cg_steps(task, pricer), None]
+                done = step(slot, None)
+                if done is None:
+                    return
+                self.stats["draws"] += 1
+                yield done
+
+        for s in range(D):
+            yield from start(s)
+        while live:
+            slots = sorted(live)
+            req = np.stack([live[s][3] for s in slots])
+            t0 = time.perf_counter()
+            vmax, rank = pricer.price_batch(req)
+            t1 = time.perf_counter()
+            self.stats["price_s"] += t1 - t0
+            self.stats["rounds"] += 1
+            self.stats["sweeps"] += -(-len(slots) // 16)
+            self.stats["draw_rounds"] += len(slots)
+            for i, s in enumerate(slots):
+                done = step(s, (float(vmax[i]), rank[i].copy(), 0.0))
+                if done is not None:
+                    self.stats["draws"] += 1
+                    yield done
+                    yield from start(s)
+            self.stats["host_s"] += time.perf_counter() - t1
+        self.stats["wall_s"] += time.perf_counter() - t_all
+
+    def close(self):
+        pass

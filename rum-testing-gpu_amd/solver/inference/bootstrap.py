@@ -317,10 +317,11 @@ class Bootstrap:
         self.time["solve"] += time.perf_counter() - t0
 
     def _solve_batched(self, idx, abs_tol, rel_tol, stops, until):
-        from .batched import BatchedSolver
+        from .batched import BatchedCG, BatchedSolver
         t0 = time.perf_counter()
         if self._batched is None:
-            self._batched = BatchedSolver(self, **(self.batch or {}))
+            kind = BatchedCG if self.session.method == "flow_cg" else BatchedSolver
+            self._batched = kind(self, **(self.batch or {}))
         tau_lo, tau_hi = self.tau
         P_of = lambda sel: hong_li.perturb(self.p_hat, self.draws.rows(sel), self.r_n, self.eps_n)  # noqa: E731
         for b, lb, ub, flag in self._batched.solve(idx, P_of, abs_tol, rel_tol, tau_lo if stops else None,

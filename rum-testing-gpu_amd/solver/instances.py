@@ -206,6 +206,11 @@ def _panel(rng, n, menus, R_resp, per_resp, orders, weights):
     return dict(respondent=np.asarray(resp), menu=np.asarray(men, dtype=np.int64), chosen=np.asarray(cho, dtype=np.int64))
 
 
+def config4_alt0(seed=16002):
+    """Batched-bootstrap workload: config5's generator at n = 16 (301 cells of alternative 0)."""
+    return config5(seed, 16)
+
+
 def config4(seed=16001):
     """Batched bootstrap workload: n = 16, menus at 2 % of the cells; a panel of 520 respondents
     each answering 12 menus drawn uniformly from them, choices from a 50-order mixture. Returns
@@ -218,12 +223,14 @@ def config4(seed=16001):
     return _panel(rng, n, menus, 520, 12, orders, w), n
 
 
-def config5(seed=25001):
+def config5(seed=25001, n=25):
     """Large-n bootstrap workload: n = 25; 301 menus containing alternative 0 with popcount uniform
     in 2..25 (other members uniform); a panel of 520 respondents x 15 menus from a 100-order
-    mixture. Rows: cells_from_panel(panel, 25, alternatives=[0])."""
+    mixture. Rows: cells_from_panel(panel, 25, alternatives=[0]). The same generator at n = 16
+    (seed 16002) is the batched-bootstrap workload whose draws straddle tau (config4's panel, with
+    about 5 answers per menu, puts p_hat so far outside the polytope that the screen decides every
+    draw)."""
     rng = np.random.default_rng(seed)
-    n = 25
     seen, menus = set(), []
     while len(menus) < 301:
         k = int(rng.integers(2, n + 1))

@@ -428,6 +428,26 @@ class Session:
 
     def solve(self, p, abs_tol=None, rel_tol=None, max_time=None, below=None, above=None, warm=False, keep=False,
               verbose=False):
+        task = self._task(p, abs_tol, rel_tol, max_time, below, above, warm, keep, verbose)
+        if isinstance(task, Result):
+            return task
+        task.t_run = time.perf_counter()
+        message = ""
+        try:
+            stop = _RUN[self.method](task)
+        except _TimeUp:
+            stop = "time"
+        except _Decided:
+            stop = "threshold"
+        except KeyboardInterrupt:
+            stop, message = "interrupted", "interrupted by the user"
+        except Exception as e:                # noqa: BLE001
+            stop = "oom" if _is_oom(e) else "error"
+            message = "%s: %s" % (type(e).__name__, e)
+        return self._finish(task, stop, message)
+
+    def _task(self, p, abs_tol, rel_tol, max_time, below, above, warm, keep, verbose):
+        """The runner's task for one solve (or the Result itself when only singleton menus remain)."""
         rum, t0 = self.rum, time.perf_counter()
         abs_tol = float(defaults.ABS_TOL if abs_tol is None else abs_tol)
         rel_tol = float(defaults.REL_TOL if rel_tol is None else rel_tol)
@@ -445,28 +465,19 @@ class Session:
                                dict(lb=d, ub=d, p_fit=np.zeros(0), iterations=0, trace=[], info={}, setup=0.0),
                                "tol", "", 0.0, self.prof)
         st = dict(lb=0.0, ub=math.inf, p_fit=None, iterations=0, trace=[], info={}, setup=0.0)
-        task = types.SimpleNamespace(geom=geom, p=pr, W=W, args=self._args, abs_tol=abs_tol, rel_tol=rel_tol,
+        return types.SimpleNamespace(geom=geom, p=pr, W=W, args=self._args, abs_tol=abs_tol, rel_tol=rel_tol,
                                      deadline=t0 + max_time, seed=self.seed, verbose=verbose, t0=t0, state=st,
                                      const=const, ram=self.prof.get("ram_bytes"),
                                      below=_on_rest(below, const, False), above=_on_rest(above, const, True),
-                                     warm=self._kept if warm else None, keep=bool(keep), engines=self._engines)
-        t_run = time.perf_counter()
-        message = ""
-        try:
-            stop = _RUN[self.method](task)
-        except _TimeUp:
-            stop = "time"
-        except _Decided:
-            stop = "threshold"
-        except KeyboardInterrupt:
-            stop, message = "interrupted", "interrupted by the user"
-        except Exception as e:                # noqa: BLE001
-            stop = "oom" if _is_oom(e) else "error"
-            message = "%s: %s" % (type(e).__name__, e)
+                                     warm=self._kept if warm else None, keep=bool(keep), engines=self._engines,
+                                     t_run=t0)
+
+    def _finish(self, task, stop, message):
+        st = task.state
         kept = st.pop("keep", None)
-        res = rum._result(t0, t_run, self.method, self.auto_reason, self._args, abs_tol, rel_tol, self.seed, st, stop,
-                          message, const, self.prof)
-        if keep:
+        res = self.rum._result(task.t0, task.t_run, self.method, self.auto_reason, self._args, task.abs_tol,
+                               task.rel_tol, self.seed, st, stop, message, task.const, self.prof)
+        if task.keep:
             self._kept, self.fit = kept, res.p_fit
         return res
 
@@ -793,21 +804,36 @@ def _cg_stops(below, above, margin):
     return kw
 
 
-def _run_flow_cg(t):
-    """rum_solver.py:941-1004 of the reference: the engine solves for the float32-rounded p; the
-    bracket is recomputed for the true p (exact ub of the fitted mixture, float64 lower bound), with
-    one resume from the engine's columns when the float64 bracket misses the tolerance."""
-    from .flow_cg.engine import Pricer, solve_cg, xmajor
-    g, a, st = t.geom, t.args, t.state
-    n = g.num_alternatives
+def _cg_pricer(t):
+    """The session's pricer for this plan, created once."""
+    from .flow_cg.engine import Pricer, xmajor
+    g, a = t.geom, t.args
     eng = _engines(t)
     key = (a["pricing"], a["dtype"])
     if "cg" not in eng or eng["cg"].key != key:
         if "cg" in eng:
             eng.pop("cg").close()
-        perm, masks_x, _alts, offsets = xmajor(g.menu, g.alternative, n)
-        eng["cg"] = _EngineSet(key, {"pricer": Pricer(n, masks_x, offsets, layout=a["pricing"], dtype=a["dtype"])})
-    pricer = eng["cg"].items["pricer"]
+        perm, masks_x, _alts, offsets = xmajor(g.menu, g.alternative, g.num_alternatives)
+        eng["cg"] = _EngineSet(key, {"pricer": Pricer(g.num_alternatives, masks_x, offsets, layout=a["pricing"],
+                                                      dtype=a["dtype"])})
+    return eng["cg"].items["pricer"]
+
+
+def _run_flow_cg(t):
+    """rum_solver.py:941-1004 of the reference: the engine solves for the float32-rounded p; the
+    bracket is recomputed for the true p (exact ub of the fitted mixture, float64 lower bound), with
+    one resume from the engine's columns when the float64 bracket misses the tolerance."""
+    from .flow_cg.engine import drive
+    pricer = _cg_pricer(t)
+    return drive(_flow_cg_steps(t, pricer), pricer)
+
+
+def _flow_cg_steps(t, pricer):
+    """_run_flow_cg's body as a generator of pricing requests (flow_cg.engine.cg_steps); returns the
+    stop reason. The batched bootstrap drives many of these with one batched sweep per round."""
+    from .flow_cg.engine import cg_steps
+    g, a, st = t.geom, t.args, t.state
+    n = g.num_alternatives
     delta = float(np.linalg.norm(t.p - t.p.astype(np.float32).astype(np.float64)))
     rel, abs_e = t.rel_tol, max(t.abs_tol - 2 * delta, 0.0)
     race = getattr(t, "race", False)
@@ -824,7 +850,7 @@ def _run_flow_cg(t):
                   **stops)
         if not race:
             kw.update(gap_rel_tol=rel, gap_abs_tol=abs_e if abs_e else None)
-        r = solve_cg(n, g.menu, g.alternative, t.p, pricer=pricer, **kw)
+        r = yield from cg_steps(n, g.menu, g.alternative, t.p, pricer, **kw)
         p_fit = np.empty(g.M)
         p_fit[r["perm"]] = r["p_out"]
         st["p_fit"] = p_fit

@@ -157,3 +157,40 @@ def test_batched_bootstrap_same_decisions():
     solved = (one.flag != 127) & (bat.flag != 127)
     assert (bat.L[solved] <= one.U[solved] + 1e-9).all() and (one.L[solved] <= bat.U[solved] + 1e-9).all()
     assert pv1.n_solved >= 20 and st["draws"] >= pv2.n_solved
+
+
+@pytest.mark.parametrize("n,D", [(9, 5), (14, 16), (18, 21)])
+def test_batched_pricing_equals_single(n, D):
+    from solver.flow_cg.engine import Pricer, xmajor
+    rng = np.random.default_rng(400 + n)
+    menu, alt, p = _medium(n, min(8000, n << (n - 3)), rng)
+    perm, masks_x, _alts, offsets = xmajor(menu, alt, n)
+    pr = Pricer(n, masks_x, offsets, layout="mask")
+    S = rng.standard_normal((D, menu.size)).astype(np.float32)
+    vb, rb = pr.price_batch(S)
+    for d in range(D):
+        v1, r1, _m = pr.price(S[d])
+        assert vb[d] == v1 and np.array_equal(rb[d], r1), d
+    pr.close()
+
+
+def test_batched_flow_cg_bootstrap_identical_brackets():
+    import pandas as pd
+    panel, n = _small_panel()
+    panel = pd.DataFrame(panel)
+    menu, alt, p = cells_from_panel(panel, n)
+    rum = RUM(menu, alt, p, num_alternatives=n)
+    B = 96
+    args = dict(method="flow_cg", method_args={"pricing": "mask"}, abs_tol=1e-4, refine=1)
+    with rum.bootstrap(panel=panel, B=B, **args) as one:
+        pv1 = one.p_value()
+    with rum.bootstrap(panel=panel, B=B, batch=dict(slots=24), **args) as bat:
+        pv2 = bat.p_value()
+        st = bat.batch_stats
+    print(pv1)
+    print(pv2)
+    print(st)
+    # the same runner per draw, the same columns: identical brackets and flags
+    assert np.array_equal(one.L, bat.L) and np.array_equal(one.U, bat.U)
+    assert np.array_equal(one.flag, bat.flag)
+    assert pv1.p == pv2.p and pv1.n_solved >= 20
