@@ -142,6 +142,11 @@ int rumgpu_cg_price(rumgpu_cg* cg, const float* scores_x, double* vmax, int32_t*
  * for bit. No reference counterpart: the batched bootstrap prices the open draws' columns
  * together instead of one bf_update sweep per draw (flow_cg/solver.py:168-200). */
 int rumgpu_cg_price_batch(rumgpu_cg* cg, int D, const float* scores, double* vmax, int32_t* rank_pos);
+/* float64 max over rankings of sum_j r_j [the ranking picks row j's alternative] for r given per
+ * row (x-major): the longest path behind the Frank-Wolfe lower bound of fw_lb_rows_gpu
+ * (common/transforms.py:101-114), flow_cg's final bound (rum_solver.py:928-938), computed on the
+ * mask lattice instead of a float64 slab (mask layout only; n 2^n float64 scratch, kept) */
+int rumgpu_cg_vmax64(rumgpu_cg* cg, const double* r_x, double* vmax);
 int rumgpu_cg_info(rumgpu_cg* cg, size_t* device_bytes, double* last_kernel_ms);
 
 #ifdef __cplusplus

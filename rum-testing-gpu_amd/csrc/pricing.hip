@@ -117,6 +117,33 @@ __global__ __launch_bounds__(256) void cg_mask_layer_batch(float* __restrict__ V
   }
 }
 
+// float64 value of the same longest path (no choices kept): the Frank-Wolfe lower bound's
+// max over rankings of sum_j r_j [the ranking picks row j's alternative in its menu]
+// (fw_lb_rows_gpu, common/transforms.py:101-114, which reaches it through a float64 subset-sum
+// slab and a layered longest path; here the arc weights are the rows' subset tests, in float64)
+__global__ __launch_bounds__(256) void cg_mask_layer64(double* __restrict__ V, const u32* __restrict__ masks,
+                                                       const double* __restrict__ r, const u64* __restrict__ offsets,
+                                                       const u32* __restrict__ nodes, i64 ns, int n) {
+  const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ns) return;
+  const u32 D = nodes[i];
+  double best = 0.0;
+  bool seen = false;
+  for (int x = 0; x < n; ++x) {
+    if (!((D >> x) & 1u)) continue;
+    double w = 0.0;
+    const u64 lo = offsets[x], hi = offsets[x + 1];
+    for (u64 j = lo; j < hi; ++j) {
+      const u32 Dp = masks[j];
+      if ((Dp & D) == Dp) w += r[j];
+    }
+    const double cand = V[D ^ (1u << x)] + w;
+    if (!seen || cand > best) best = cand;
+    seen = true;
+  }
+  V[D] = best;
+}
+
 // one thread per draw: walk the choices down from the full set; vmax = V[full]
 __global__ void cg_backtrace_batch(const uint8_t* __restrict__ pred, const float* __restrict__ V, int n, int DB,
                                    int nd, int32_t* rank_pos, float* vmax, u64 full) {
@@ -173,6 +200,9 @@ __global__ void cg_backtrace(const uint8_t* __restrict__ pred, int n, int32_t* r
 }
 
 __global__ void cg_fill_f32(float* a, float v, i64 m) {
+  for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (i64)gridDim.x * blockDim.x) a[i] = v;
+}
+__global__ void cg_fill_f64(double* a, double v, i64 m) {
   for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (i64)gridDim.x * blockDim.x) a[i] = v;
 }
 __global__ void cg_fill_u8(uint8_t* a, uint8_t v, i64 m) {
@@ -345,6 +375,9 @@ struct rumgpu_cg {
   float* scoresb = nullptr;
   float* vmaxb = nullptr;
   int32_t* rankb = nullptr;
+  // float64 lower-bound sweep (allocated at the first rumgpu_cg_lb64)
+  double* V64 = nullptr;
+  double* r64 = nullptr;
   float* R = nullptr;        // slab
   int32_t* rank = nullptr;
   // stream
@@ -372,7 +405,7 @@ static void cg_free_all(rumgpu_cg* c) {
   void* bufs[] = {c->masks32, c->masks64, c->offsets, c->scores, c->spos, c->V, c->pred, c->nodes, c->R, c->rank,
                   c->binom, c->vbuf[0], c->vbuf[1], c->lbuf[0], c->lbuf[1], c->mbuf[0], c->mbuf[1], c->ebuf[0],
                   c->ebuf[1], c->lstep, c->subV, c->subpred, c->subnodes, c->submasks, c->subscores, c->suboffsets,
-                  c->Vb, c->predb, c->scoresb, c->vmaxb, c->rankb};
+                  c->Vb, c->predb, c->scoresb, c->vmaxb, c->rankb, c->V64, c->r64};
   for (void* q : bufs) rg_free(q);
   for (auto* p : c->predk) rg_free(p);
 }
@@ -753,6 +786,33 @@ extern "C" int rumgpu_cg_price_batch(rumgpu_cg* c, int D, const float* scores, d
     for (int d = 0; d < nd; ++d) vmax[first + d] = (double)vf[d];
   }
   c->last_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return RG_OK;
+}
+
+extern "C" int rumgpu_cg_vmax64(rumgpu_cg* c, const double* r_x, double* vmax) {
+  RG_CHECK(c && r_x && vmax, RG_EINVAL, "cg_vmax64: null argument");
+  RG_CHECK(c->layout == 0, RG_EUNSUP, "cg_vmax64: the float64 sweep uses the mask layout");
+  RG_HIP(hipSetDevice(c->ctx->device));
+  hipStream_t st = c->ctx->stream;
+  if (!c->V64) {
+    int rc = RG_OK;
+    if ((rc = cg_alloc(c, (void**)&c->V64, (size_t)c->NN * 8, "float64 V")) != RG_OK ||
+        (rc = cg_alloc(c, (void**)&c->r64, (size_t)c->M * 8, "float64 residual")) != RG_OK) {
+      rg_free(c->V64), rg_free(c->r64);
+      c->V64 = nullptr, c->r64 = nullptr;
+      return rc;
+    }
+  }
+  RG_HIP(hipMemcpyAsync(c->r64, r_x, (size_t)c->M * 8, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(cg_fill_f64, dim3(1), dim3(64), 0, st, c->V64, 0.0, (i64)1);
+  for (int k = 1; k <= c->n; ++k) {
+    const i64 lo = c->bounds[k], ns = c->bounds[k + 1] - lo;
+    if (ns <= 0) continue;
+    hipLaunchKernelGGL(cg_mask_layer64, dim3(grid_for(ns, 256)), dim3(256), 0, st, c->V64, c->masks32, c->r64,
+                       c->offsets, c->nodes + lo, ns, c->n);
+  }
+  RG_LAUNCH_CHECK();
+  RG_HIP(rg_copy(st, vmax, c->V64 + (c->NN - 1), 8, hipMemcpyDeviceToHost));
   return RG_OK;
 }
 

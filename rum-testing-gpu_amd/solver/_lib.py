@@ -60,6 +60,7 @@ SIGNATURES = {
     "rumgpu_cg_create": [_P, _I, _P, _P, _I64, _I, _I, _I, _P],
     "rumgpu_cg_destroy": [_P],
     "rumgpu_cg_price_batch": [_P, _I, _P, _P, _P],
+    "rumgpu_cg_vmax64": [_P, _P, _P],
     "rumgpu_cg_price": [_P, _P, _P, _P, _P],
     "rumgpu_cg_info": [_P, _P, _P],
 }

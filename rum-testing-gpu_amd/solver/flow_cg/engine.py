@@ -9,6 +9,7 @@ priced ranking's 0/1 column. Q = V'V and h = V'p are grown from packed column bi
 (qp_bitset_incremental.py:36-141 of the reference): Q[k, j] = popcount(bits_k & bits_j).
 """
 import ctypes
+import math
 import time
 
 import numpy as np
@@ -63,6 +64,23 @@ class Pricer:
         _lib.check(self.lib.rumgpu_cg_price_batch(self.h, D, _lib.ptr(s), _lib.ptr(vmax), _lib.ptr(rank)),
                    "cg_price_batch")
         return vmax, rank
+
+    def vmax64(self, r_x):
+        """float64 longest path for residuals r per x-major row (mask layout)"""
+        r = _lib.c_array(r_x, np.float64)
+        v = ctypes.c_double()
+        _lib.check(self.lib.rumgpu_cg_vmax64(self.h, _lib.ptr(r), ctypes.byref(v)), "cg_vmax64")
+        return v.value
+
+    def lb64(self, perm, y, p):
+        """float64 Frank-Wolfe lower bound at y (input row order), fw_lb_rows_gpu's formula:
+        r = p - y, lb = sqrt(max(0, |r|^2 - 2 (vmax(r) - r.y))) with compensated sums"""
+        y = np.asarray(y, dtype=np.float64)
+        r = np.asarray(p, dtype=np.float64) - y
+        vmax = self.vmax64(r[perm])
+        f = math.fsum(r * r)
+        ry = math.fsum(r * y)
+        return math.sqrt(max(0.0, f - 2.0 * (vmax - ry)))
 
     def info(self):
         b, ms = ctypes.c_size_t(), ctypes.c_double()

@@ -776,11 +776,15 @@ def _run_bm_admm(t):
             "lb_target": "threshold"}.get(r["stop_reason"], "stalled")
 
 
-def _flow_cg_lb(n, g, p_fit, p, engine_lb, delta, dtype):
-    """flow_cg's final lower bound in the most precise form that fits: the float64 slab bound
-    (rum_solver.py:928-938 of the reference), else the engine's bound lowered by delta."""
+def _flow_cg_lb(n, g, p_fit, p, engine_lb, delta, dtype, pricer=None, perm=None):
+    """flow_cg's final lower bound in the most precise form that fits (rum_solver.py:928-938 of the
+    reference): the float64 Frank-Wolfe bound, its longest path taken on the mask pricer's lattice
+    when the pricer has one (no slab), else through a float64 slab, else the engine's bound
+    lowered by delta."""
     from . import _lib
     import ctypes
+    if pricer is not None and pricer.layout == "mask":
+        return pricer.lb64(perm, p_fit, p), "float64", "mask"
     need = 16 * (n << (n - 1)) + 16 * (1 << n)
     if need <= defaults.AUTO["gpu_memory_fraction"] * _machine.gpu_free_bytes():
         ctx = _lib.context(0)
@@ -859,7 +863,7 @@ def _flow_cg_steps(t, pricer):
         if race or (below is not None and st["ub"] < below):
             lb, lb_dt, lb_lay = lb_prev, None, "not computed (below the threshold)"
         else:
-            lb, lb_dt, lb_lay = _flow_cg_lb(n, g, p_fit, t.p, float(r["lb"]), delta, a["dtype"])
+            lb, lb_dt, lb_lay = _flow_cg_lb(n, g, p_fit, t.p, float(r["lb"]), delta, a["dtype"], pricer, r["perm"])
         st["lb"] = max(lb_prev, lb)
         st["iterations"] = int(r["n_cols"])
         st["setup"] += float(r["setup_s"])

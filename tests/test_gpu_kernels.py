@@ -226,3 +226,25 @@ def test_stream_pricing_bit_exact(n, mode):
         vmask, rank_mask, _V, _p = O.price_mask(n, masks_x, scores, offsets)
         assert vmax == vmask and np.array_equal(rank, rank_mask)
     pr.close()
+
+
+@pytest.mark.parametrize("n", [7, 12, 16])
+def test_mask_lb64_matches_slab_bound(n):
+    """flow_cg's float64 lower bound on the mask lattice equals the slab form (fw_lb_rows_gpu) and
+    the oracle's within float64 rounding"""
+    import ctypes
+    rng = np.random.default_rng(600 + n)
+    menu, alt, p = _medium(n, min(5000, n << (n - 3)), rng)
+    perm, masks_x, _alts, offsets = xmajor(menu, alt, n)
+    pr = Pricer(n, masks_x, offsets, layout="mask")
+    y = np.clip(p + 0.05 * rng.standard_normal(p.size), 0.0, 1.0)
+    lb_mask = pr.lb64(perm, y, p)
+    obs = O.obs_index(n, menu, alt)
+    lb_o, _ = O.fw_lb64(n, obs, y, p)
+    ctx = _lib.context(0)
+    lbv = ctypes.c_double()
+    _lib.check(ctx.lib.rumgpu_lb64(ctx.h, n, _lib.ptr(menu), _lib.ptr(alt), menu.size, _lib.ptr(y), _lib.ptr(p),
+                                   ctypes.byref(lbv)))
+    assert abs(lb_mask - lb_o) <= 1e-10 * max(1.0, lb_o)
+    assert abs(lb_mask - lbv.value) <= 1e-10 * max(1.0, lb_o)
+    pr.close()
