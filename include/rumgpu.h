@@ -23,6 +23,7 @@ extern "C" {
 typedef struct rumgpu_ctx rumgpu_ctx;
 typedef struct rumgpu_bm rumgpu_bm;
 typedef struct rumgpu_cg rumgpu_cg;
+typedef struct rumgpu_mlp rumgpu_mlp;
 
 /* ---- context, device, errors (replaces CuPy's device/pool handling and machine.py:25-61,
  *      machine.py:156 gpu_free_bytes) ---- */
@@ -148,6 +149,26 @@ int rumgpu_cg_price_batch(rumgpu_cg* cg, int D, const float* scores, double* vma
  * mask lattice instead of a float64 slab (mask layout only; n 2^n float64 scratch, kept) */
 int rumgpu_cg_vmax64(rumgpu_cg* cg, const double* r_x, double* vmax);
 int rumgpu_cg_info(rumgpu_cg* cg, size_t* device_bytes, double* last_kernel_ms);
+
+/* ---- Learned warm start: in -> hidden -> hidden -> out MLP (ReLU), bf16 matrix-core forward and
+ * backward, Adam on a float32 master copy. No reference counterpart: it maps a bootstrap draw's
+ * change of p on the observed cells to a correction of the ADMM state there (the warm start);
+ * the float64 certificates still judge every draw. Parameters are one float32 vector: W1 (in x
+ * hidden, row-major), b1, W2 (hidden x hidden), b2, W3 (hidden x out), b3. ---- */
+int rumgpu_mlp_create(rumgpu_ctx* ctx, int in_dim, int hidden, int out_dim, int max_batch, rumgpu_mlp** out);
+int rumgpu_mlp_destroy(rumgpu_mlp* mlp);
+int rumgpu_mlp_param_count(rumgpu_mlp* mlp, int64_t* count);
+int rumgpu_mlp_set_params(rumgpu_mlp* mlp, const float* theta);
+int rumgpu_mlp_get_params(rumgpu_mlp* mlp, float* theta);
+/* Y (N x out) = MLP(X (N x in)), row-major host arrays, N <= max_batch */
+int rumgpu_mlp_forward(rumgpu_mlp* mlp, int N, const float* X, float* Y);
+/* loss = (1/N) sum_i 1/2 ||Y_i - T_i||^2 and its gradient (kept for rumgpu_mlp_adam; copied to
+ * grad, param_count floats, when grad is not NULL) */
+int rumgpu_mlp_backward(rumgpu_mlp* mlp, int N, const float* X, const float* T, float* loss, float* grad);
+/* one Adam step with the kept gradient (bias-corrected moments, step count inside) */
+int rumgpu_mlp_adam(rumgpu_mlp* mlp, float lr, float beta1, float beta2, float eps);
+/* C (M x N) = A (M x K) B (K x N) through the same bf16 matrix-core tile (a test surface) */
+int rumgpu_mlp_gemm_test(rumgpu_ctx* ctx, int M, int N, int K, const float* A, const float* B, float* C);
 
 #ifdef __cplusplus
 }

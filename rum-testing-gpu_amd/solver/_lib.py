@@ -61,6 +61,15 @@ SIGNATURES = {
     "rumgpu_cg_destroy": [_P],
     "rumgpu_cg_price_batch": [_P, _I, _P, _P, _P],
     "rumgpu_cg_vmax64": [_P, _P, _P],
+    "rumgpu_mlp_create": [_P, _I, _I, _I, _I, _P],
+    "rumgpu_mlp_destroy": [_P],
+    "rumgpu_mlp_param_count": [_P, _P],
+    "rumgpu_mlp_set_params": [_P, _P],
+    "rumgpu_mlp_get_params": [_P, _P],
+    "rumgpu_mlp_forward": [_P, _I, _P, _P],
+    "rumgpu_mlp_backward": [_P, _I, _P, _P, _P, _P],
+    "rumgpu_mlp_adam": [_P, _F, _F, _F, _F],
+    "rumgpu_mlp_gemm_test": [_P, _I, _I, _I, _P, _P, _P],
     "rumgpu_cg_price": [_P, _P, _P, _P, _P],
     "rumgpu_cg_info": [_P, _P, _P],
 }

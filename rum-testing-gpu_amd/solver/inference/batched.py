@@ -69,8 +69,9 @@ class BatchedSolver:
     dtype   iterate dtype of the batch ("float64"; "float32" iterates still get float64 bounds)
     model   learned warm start: an object with correction(dp) -> (dx2, dz) on the non-singleton
             rows (dp = P_b - p_hat there), or None; the certificate judges every draw either way
-    collect keep (dp, x2_obs - x2_hat_obs, z_obs - z_hat_obs) of every draw retired with a met
-            tolerance, as training pairs for the warm-start model
+    collect keep (dp, x2_obs - x2_hat_obs, z_obs - z_hat_obs, iterations) of every draw retired
+            with a met tolerance (True) or of every retired draw ("all"), as training pairs for the
+            warm-start model
     """
 
     def __init__(self, boot, slots=None, dtype="float64", model=None, collect=False, max_slots=256, verbose=False):
@@ -87,7 +88,7 @@ class BatchedSolver:
         self.n, self.menu, self.alt = rum.n, g.menu, g.alternative.astype(np.int32)
         self.rest, self.single = rum._rest, rum._single
         self.dtype = np.dtype(dtype)
-        self.model, self.collect, self.pairs = model, bool(collect), []
+        self.model, self.collect, self.pairs = model, collect, []
         kept = sess._kept
         self.src = kept["handle"] if kept is not None else None
         p0 = boot.p_hat[self.rest]
@@ -227,7 +228,7 @@ class BatchedSolver:
                 L = math.sqrt(max(lb[s], 0.0) ** 2 + const[s])
                 U = math.sqrt(ub[s] ** 2 + const[s]) if math.isfinite(ub[s]) else math.inf
                 flag = _flag(L, U, abs_tol, rel_tol, stop)
-                if self.collect and flag in (1, 2, 3) and self.x_hat_obs is not None:
+                if self.collect and (flag in (1, 2, 3) or self.collect == "all") and self.x_hat_obs is not None:
                     x, z = h.slot_get_obs(s)
                     self.pairs.append((dp_of[s], x - self.x_hat_obs, z - self.z_hat_obs, int(it[s])))
                 self.stats["draws"] += 1
