@@ -51,6 +51,11 @@ def _parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=20)
     ap.add_argument("--no-graph", action="store_true", help="eager launches instead of HIP graph replays")
+    ap.add_argument("--boot-draws", type=int, default=512,
+                    help="open bootstrap draws of the n=25 panel solved by the batched (tuned) leg; 0 skips it")
+    ap.add_argument("--boot-plain-draws", type=int, default=128,
+                    help="of those, solved one by one as the reference does (the plain leg)")
+    ap.add_argument("--boot-slots", type=int, default=64)
     ap.add_argument("--traffic-run", action="store_true",
                     help="only the workload for HBM counters: --profile-iters fused iterations on the n=18 "
                          "instance, then a device copy of the 4 state slabs (a copy kernel of known size)")
@@ -209,6 +214,7 @@ def main():
     cpu = None
     if not a.no_cpu:
         cpu = _cpu_baseline(n, rum, p, r_last, a)
+    boot = _bootstrap_bench(a) if a.boot_draws > 0 else None
 
     line = dict(metric=METRIC, value=round(value, 5), unit="certified solves/s", n_gpus=1, steps=a.steps,
                 warmup=a.warmup, ms_per_step=round(1e3 * elapsed / a.steps, 2), higher_is_better=True,
@@ -221,9 +227,83 @@ def main():
                             iters32=[r.info.get("iters32") for r in results],
                             iters64=[r.info.get("iters64") for r in results],
                             bracket=[r_last.lb, r_last.ub], certified=certified),
-                roofline=roofline, cpu_baseline=cpu, plain=plain,
+                roofline=roofline, cpu_baseline=cpu, plain=plain, bootstrap=boot,
                 tuned_vs_plain=round(value / plain["value"], 3) if plain and plain["value"] > 0 else None)
     print(json.dumps(line), flush=True)
+
+
+def _bootstrap_bench(a):
+    """Bootstrap draws/s on the n = 25 panel (config5: 301 cells of alternative 0, 520 respondents x
+    15 menus, B = 10,000, multinomial_reject seed 0), flow_cg (the method the reference runs on
+    such panels). p_hat is certified and the draws screened first (untimed); then the open draws
+    are solved with the band's stops, warm from p_hat:
+      plain  one Session.solve per draw with the float64 lower bound through a slab, as the
+             reference's runner computes it (rum_solver.py:928-938, bootstrap.py:308-335)
+      tuned  the batched driver: every draw in flight priced by one mask sweep per round, the final
+             float64 bound on the pricer's lattice
+    Both legs run the same CG per draw; the decisions (below / above / undecided) must agree."""
+    import numpy as np
+    import pandas as pd
+    from solver import instances
+    from solver.inference import cells_from_panel, hong_li
+    from solver.inference.batched import BatchedCG
+    from solver.rum_solver import RUM
+    panel, n = instances.config5()
+    panel = pd.DataFrame(panel)
+    menu, alt, p = cells_from_panel(panel, n, alternatives=[0])
+    rum = RUM(menu, alt, p, num_alternatives=n)
+    boot = rum.bootstrap(panel=panel, B=10000, method="flow_cg", method_args={"pricing": "mask"})
+    t0 = time.perf_counter()
+    boot._start()
+    t_start = time.perf_counter() - t0
+    und = np.flatnonzero(boot._undecided())
+    K = min(a.boot_draws, und.size)
+    Kp = min(a.boot_plain_draws, K)
+    idx = und[:K]
+    tau = boot.tau
+    print("[bench] bootstrap n=25: p_hat [%.8f, %.8f] certified and %d draws screened in %.2f s; %d open, "
+          "timing %d (plain %d)" % (boot.L0, boot.U0, boot.n_screened, t_start, und.size, K, Kp),
+          file=sys.stderr, flush=True)
+    P = hong_li.perturb(boot.p_hat, boot.draws.rows(idx[:Kp]), boot.r_n, boot.eps_n)
+    sess = boot.session
+    sess._args["_lb_form"] = "slab"
+    plain = {}
+    t0 = time.perf_counter()
+    for j, b in enumerate(idx[:Kp]):
+        r = sess.solve(P[j], abs_tol=boot.abs_tol, rel_tol=boot.rel_tol, max_time=boot.draw_time, warm=True,
+                       below=tau[0], above=tau[1])
+        plain[int(b)] = (r.lb, r.ub)
+    t_plain = time.perf_counter() - t0
+    sess._args["_lb_form"] = "mask"
+    print("[bench] bootstrap plain: %d draws %.3f s" % (Kp, t_plain), file=sys.stderr, flush=True)
+    P_of = lambda sel: hong_li.perturb(boot.p_hat, boot.draws.rows(sel), boot.r_n, boot.eps_n)  # noqa: E731
+    bs = BatchedCG(boot, slots=a.boot_slots)
+    t0 = time.perf_counter()
+    tuned = {b: (L, U) for b, L, U, _f in bs.solve(idx, P_of, boot.abs_tol, boot.rel_tol, tau[0], tau[1],
+                                                   boot.draw_time)}
+    t_tuned = time.perf_counter() - t0
+    print("[bench] bootstrap tuned: %d draws %.3f s" % (K, t_tuned), file=sys.stderr, flush=True)
+    common = [int(b) for b in idx[:Kp]]
+    d_plain = hong_li.decide([plain[b][0] for b in common], [plain[b][1] for b in common], *tau)
+    d_tuned = hong_li.decide([tuned[b][0] for b in common], [tuned[b][1] for b in common], *tau)
+    dec_all = hong_li.decide([tuned[int(b)][0] for b in idx], [tuned[int(b)][1] for b in idx], *tau)
+    v_plain, v_tuned = Kp / t_plain, K / t_tuned
+    st = bs.stats
+    boot.close()
+    return dict(workload="bootstrap draws of the n=25 panel (config5 seed 25001: 301 cells of alternative 0, "
+                         "520 respondents x 15 menus), B=10,000, flow_cg mask pricing, abs 5e-4; open draws "
+                         "after the screen, band stops, warm from p_hat",
+                unit="draws/s", tuned=round(v_tuned, 3), plain=round(v_plain, 3),
+                tuned_vs_plain=round(v_tuned / v_plain, 3), draws_tuned=K, draws_plain=Kp, open_draws=int(und.size),
+                screened=int(boot.n_screened), decisions_identical=bool(np.array_equal(d_plain, d_tuned)),
+                max_bracket_diff=float(max(max(abs(plain[b][0] - tuned[b][0]), abs(plain[b][1] - tuned[b][1]))
+                                           for b in common)),
+                decided=dict(below=int((dec_all == hong_li.BELOW).sum()), above=int((dec_all == hong_li.ABOVE).sum()),
+                             undecided=int((dec_all == hong_li.UNDECIDED).sum())),
+                projected_open_draws_s=dict(plain=round(und.size / v_plain, 1), tuned=round(und.size / v_tuned, 1)),
+                slots=a.boot_slots, rounds=st["rounds"], price_s=round(st["price_s"], 3), host_s=round(st["host_s"], 3),
+                start_s=round(t_start, 3),
+                learned="not applied to flow_cg draws; the MLP warm-starts BM-ADMM slots (tests/test_gpu_learned.py)")
 
 
 def _cpu_baseline(n, rum, p, r_last, a):

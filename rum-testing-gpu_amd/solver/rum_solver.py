@@ -863,7 +863,10 @@ def _flow_cg_steps(t, pricer):
         if race or (below is not None and st["ub"] < below):
             lb, lb_dt, lb_lay = lb_prev, None, "not computed (below the threshold)"
         else:
-            lb, lb_dt, lb_lay = _flow_cg_lb(n, g, p_fit, t.p, float(r["lb"]), delta, a["dtype"], pricer, r["perm"])
+            # the float64 bound on the pricer's mask lattice, or (a["_lb_form"] == "slab") through a
+            # float64 slab as the reference computes it
+            lbp = pricer if a.get("_lb_form", "mask") == "mask" else None
+            lb, lb_dt, lb_lay = _flow_cg_lb(n, g, p_fit, t.p, float(r["lb"]), delta, a["dtype"], lbp, r["perm"])
         st["lb"] = max(lb_prev, lb)
         st["iterations"] = int(r["n_cols"])
         st["setup"] += float(r["setup_s"])

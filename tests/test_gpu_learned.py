@@ -1,6 +1,6 @@
 """What the learned warm start gains: the batched bootstrap on an alternative-0 panel (301 cells,
-n = 12), the MLP trained on the GPU on the first 96 open draws the method certified, then the next
-48 open draws solved with and without it. Every draw keeps its own float64 bracket, so the
+n = 12, B = 2,048), the MLP trained on the GPU on the first 96 open draws the method certified, then
+the next 64 open draws solved with and without it. Every draw keeps its own float64 bracket, so the
 decisions must agree; the gain is printed (draw-iterations and wall time)."""
 import time
 
@@ -22,11 +22,11 @@ def test_learned_warm_start_gain():
     panel = pd.DataFrame(panel)
     menu, alt, p = cells_from_panel(panel, n, alternatives=[0])
     rum = RUM(menu, alt, p, num_alternatives=n)
-    boot = rum.bootstrap(panel=panel, B=1024, weights="multinomial", method="bm_admm", method_args={"beta": "cert"})
+    boot = rum.bootstrap(panel=panel, B=2048, weights="multinomial", method="bm_admm", method_args={"beta": "cert"})
     boot._start()
     und = np.flatnonzero(boot._undecided())
-    assert und.size >= 144, und.size
-    tr, te = und[:96], und[96:144]
+    assert und.size >= 160, und.size
+    tr, te = und[:96], und[96:160]
     P_of = lambda sel: hong_li.perturb(boot.p_hat, boot.draws.rows(sel), boot.r_n, boot.eps_n)  # noqa: E731
     tau = boot.tau
     bs = BatchedSolver(boot, slots=32, collect="all")
