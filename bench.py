@@ -56,6 +56,8 @@ def _parse():
     ap.add_argument("--boot-plain-draws", type=int, default=128,
                     help="of those, solved one by one as the reference does (the plain leg)")
     ap.add_argument("--boot-slots", type=int, default=64)
+    ap.add_argument("--bootstrap-only", action="store_true",
+                    help="only the bootstrap leg (prints its JSON object): the workload of its rocprof summary")
     ap.add_argument("--traffic-run", action="store_true",
                     help="only the workload for HBM counters: --profile-iters fused iterations on the n=18 "
                          "instance, then a device copy of the 4 state slabs (a copy kernel of known size)")
@@ -90,6 +92,10 @@ def main():
     from solver import instances, _lib
     from solver.rum_solver import RUM
     from solver.bm_admm.engine import Handle, solve_kadmm
+
+    if a.bootstrap_only:
+        print(json.dumps(_bootstrap_bench(a)), flush=True)
+        return
 
     cd, n = instances.config2()
     rum = RUM.from_rows(cd, n)

@@ -88,11 +88,20 @@ __global__ __launch_bounds__(256) void cg_mask_layer_batch(float* __restrict__ V
     float w[DB];
 #pragma unroll
     for (int d = 0; d < DB; ++d) w[d] = 0.0f;
+    // subset tests 32 rows at a time (uniform loads), then each lane adds only its own matches
+    // (a lane-private loop, ascending rows: the same order as the single-draw sweep)
     const u64 lo = offsets[x], hi = offsets[x + 1];
-    for (u64 j = lo; j < hi; ++j) {
-      const u32 Dp = masks[j];
-      if ((Dp & D) == Dp) {
-        const float* sj = scores + j * DB;
+    for (u64 base = lo; base < hi; base += 32) {
+      const int cnt = (hi - base) < 32 ? (int)(hi - base) : 32;
+      u32 mm = 0;
+      for (int t = 0; t < cnt; ++t) {
+        const u32 Dp = masks[base + t];
+        mm |= ((Dp & D) == Dp ? 1u : 0u) << t;
+      }
+      while (mm) {
+        const int t = __ffs((int)mm) - 1;
+        mm &= mm - 1u;
+        const float* sj = scores + (base + (u64)t) * DB;
 #pragma unroll
         for (int d = 0; d < DB; ++d) w[d] += sj[d];
       }
@@ -727,7 +736,7 @@ extern "C" int rumgpu_cg_price(rumgpu_cg* c, const float* scores_x, double* vmax
   return RG_OK;
 }
 
-#define CG_BATCH 16
+#define CG_BATCH 32
 
 template <int DB>
 static int mask_sweep_batch(rumgpu_cg* c, hipStream_t st) {
@@ -766,7 +775,7 @@ extern "C" int rumgpu_cg_price_batch(rumgpu_cg* c, int D, const float* scores, d
   std::vector<float> tr((size_t)M * CG_BATCH), vf(CG_BATCH);
   for (int first = 0; first < D; first += CG_BATCH) {
     const int nd = std::min(CG_BATCH, D - first);
-    const int DB = nd <= 1 ? 1 : nd <= 2 ? 2 : nd <= 4 ? 4 : nd <= 8 ? 8 : 16;
+    const int DB = nd <= 1 ? 1 : nd <= 2 ? 2 : nd <= 4 ? 4 : nd <= 8 ? 8 : nd <= 16 ? 16 : 32;
     for (i64 j = 0; j < M; ++j)
       for (int d = 0; d < DB; ++d) tr[(size_t)j * DB + d] = d < nd ? scores[(size_t)(first + d) * M + j] : 0.0f;
     RG_HIP(hipMemcpyAsync(c->scoresb, tr.data(), (size_t)M * DB * 4, hipMemcpyHostToDevice, st));
@@ -775,7 +784,8 @@ extern "C" int rumgpu_cg_price_batch(rumgpu_cg* c, int D, const float* scores, d
       case 2: RG_TRY(mask_sweep_batch<2>(c, st)); break;
       case 4: RG_TRY(mask_sweep_batch<4>(c, st)); break;
       case 8: RG_TRY(mask_sweep_batch<8>(c, st)); break;
-      default: RG_TRY(mask_sweep_batch<16>(c, st)); break;
+      case 16: RG_TRY(mask_sweep_batch<16>(c, st)); break;
+      default: RG_TRY(mask_sweep_batch<32>(c, st)); break;
     }
     hipLaunchKernelGGL(cg_backtrace_batch, dim3(1), dim3(64), 0, st, c->predb, c->Vb, n, DB, nd, c->rankb, c->vmaxb,
                        (u64)(c->NN - 1));

@@ -274,7 +274,7 @@ class BatchedCG:
     (rumgpu_cg_price_batch), whose result for each draw equals its own sweep bit for bit. So every
     draw's columns, weights and bracket are those of its per-draw solve; only the sweeps are shared.
 
-    slots   draws in flight (a multiple of 16 fills the sweeps)
+    slots   draws in flight (a multiple of 32 fills the sweeps)
     """
 
     def __init__(self, boot, slots=32, verbose=False, **_ignored):
@@ -353,7 +353,7 @@ class BatchedCG:
             t1 = time.perf_counter()
             self.stats["price_s"] += t1 - t0
             self.stats["rounds"] += 1
-            self.stats["sweeps"] += -(-len(slots) // 16)
+            self.stats["sweeps"] += -(-len(slots) // 32)
             self.stats["draw_rounds"] += len(slots)
             for i, s in enumerate(slots):
                 done = step(s, (float(vmax[i]), rank[i].copy(), 0.0))
