@@ -243,11 +243,13 @@ def _bootstrap_bench(a):
     15 menus, B = 10,000, multinomial_reject seed 0), flow_cg (the method the reference runs on
     such panels). p_hat is certified and the draws screened first (untimed); then the open draws
     are solved with the band's stops, warm from p_hat:
-      plain  one Session.solve per draw with the float64 lower bound through a slab, as the
-             reference's runner computes it (rum_solver.py:928-938, bootstrap.py:308-335)
-      tuned  the batched driver: every draw in flight priced by one mask sweep per round, the final
-             float64 bound on the pricer's lattice
-    Both legs run the same CG per draw; the decisions (below / above / undecided) must agree."""
+      plain  one Session.solve per draw with the float64 lower bound through a slab and the
+             numpy/scipy master QP, as the reference's runner computes them (rum_solver.py:928-938,
+             common/qp_native.py, bootstrap.py:308-335)
+      tuned  the batched driver: every draw in flight priced by one mask sweep per round, the native
+             master QP, the final float64 bound on the pricer's lattice
+    Both legs run the same column generation per draw; the decisions (below / above / undecided)
+    must agree."""
     import numpy as np
     import pandas as pd
     from solver import instances
@@ -272,7 +274,8 @@ def _bootstrap_bench(a):
           file=sys.stderr, flush=True)
     P = hong_li.perturb(boot.p_hat, boot.draws.rows(idx[:Kp]), boot.r_n, boot.eps_n)
     sess = boot.session
-    sess._args["_lb_form"] = "slab"
+    sess._args["_lb_form"] = "slab"       # the reference's float64 slab bound ...
+    sess._args["_qp"] = "numpy"           # ... and its numpy/scipy master QP
     plain = {}
     t0 = time.perf_counter()
     for j, b in enumerate(idx[:Kp]):
@@ -281,6 +284,7 @@ def _bootstrap_bench(a):
         plain[int(b)] = (r.lb, r.ub)
     t_plain = time.perf_counter() - t0
     sess._args["_lb_form"] = "mask"
+    sess._args["_qp"] = "native"
     print("[bench] bootstrap plain: %d draws %.3f s" % (Kp, t_plain), file=sys.stderr, flush=True)
     P_of = lambda sel: hong_li.perturb(boot.p_hat, boot.draws.rows(sel), boot.r_n, boot.eps_n)  # noqa: E731
     bs = BatchedCG(boot, slots=a.boot_slots)

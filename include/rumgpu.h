@@ -150,6 +150,13 @@ int rumgpu_cg_price_batch(rumgpu_cg* cg, int D, const float* scores, double* vma
 int rumgpu_cg_vmax64(rumgpu_cg* cg, const double* r_x, double* vmax);
 int rumgpu_cg_info(rumgpu_cg* cg, size_t* device_bytes, double* last_kernel_ms);
 
+/* Flow-CG restricted-master QP on the host: min 1/2 lam'Q lam - h'lam, lam >= 0, sum lam = 1, by
+ * the active set of solve_qp_native (common/qp_native.py:37-100) with its warm working set
+ * A_init, then the simplex projection (:20-33). Q is K x K row-major with leading dimension ldq.
+ * Writes lam[K] and the support (weights > 1e-9) into support[K], its size into n_support. */
+int rumgpu_qp_simplex(int K, const double* Q, int64_t ldq, const double* h, const int32_t* A_init, int nA,
+                      int max_outer, double tol, double reg, double* lam, int32_t* support, int* n_support);
+
 /* ---- Learned warm start: in -> hidden -> hidden -> out MLP (ReLU), bf16 matrix-core forward and
  * backward, Adam on a float32 master copy. No reference counterpart: it maps a bootstrap draw's
  * change of p on the observed cells to a correction of the ADMM state there (the warm start);

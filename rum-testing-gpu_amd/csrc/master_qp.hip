@@ -1,0 +1,249 @@
+// Flow-CG's restricted-master QP on the host, native: min 1/2 lam'Q lam - h'lam over the simplex
+// (lam >= 0, sum lam = 1), by the reference's active set (solver/common/qp_native.py:37-100):
+//   on the working set W: lam_W = Q_WW^-1 (h_W - mu 1), mu = (1'Q_WW^-1 h_W - 1) / (1'Q_WW^-1 1)
+//   (Q_WW + reg I, Cholesky); drop the most negative weight, else add the inactive column with
+//   the most negative gradient (below -tol), else stop; then a Euclidean simplex projection and
+//   support = weights above 1e-9.
+// The Cholesky factor of the working set is extended by one row when a column enters (O(k^2)) and
+// rebuilt when one leaves. Where it breaks down (a non-positive pivot of a nearly dependent set),
+// that step takes the minimum-norm least-squares solution from a symmetric eigendecomposition
+// (cyclic Jacobi), as the reference falls back to numpy's lstsq.
+#include "rumgpu_internal.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+namespace {
+
+struct Chol {
+  int k = 0;
+  std::vector<double> L;   // row-major lower factor, row stride cap (grown by doubling)
+  int cap = 0;
+  bool ok = false;
+  void reserve(int need) {
+    if (need <= cap) return;
+    cap = std::max(need, 2 * cap);
+    L.assign((size_t)cap * cap, 0.0);
+    ok = false;
+  }
+};
+
+// L L' = A (k x k, row-major in a, leading dimension k); false on a non-positive pivot
+bool chol_full(Chol& c, const std::vector<double>& a, int k) {
+  c.reserve(k);
+  c.k = k;
+  for (int i = 0; i < k; ++i) {
+    for (int j = 0; j <= i; ++j) {
+      double s = a[(size_t)i * k + j];
+      for (int t = 0; t < j; ++t) s -= c.L[(size_t)i * c.cap + t] * c.L[(size_t)j * c.cap + t];
+      if (i == j) {
+        if (!(s > 0.0)) return c.ok = false;
+        c.L[(size_t)i * c.cap + i] = std::sqrt(s);
+      } else {
+        c.L[(size_t)i * c.cap + j] = s / c.L[(size_t)j * c.cap + j];
+      }
+    }
+  }
+  return c.ok = true;
+}
+
+// append one row: q = A(new, old) (k values), d0 = A(new, new)
+bool chol_append(Chol& c, const double* q, double d0) {
+  const int k = c.k;
+  double* row = &c.L[(size_t)k * c.cap];
+  double s = d0;
+  for (int j = 0; j < k; ++j) {
+    double v = q[j];
+    for (int t = 0; t < j; ++t) v -= row[t] * c.L[(size_t)j * c.cap + t];
+    v /= c.L[(size_t)j * c.cap + j];
+    row[j] = v;
+    s -= v * v;
+  }
+  if (!(s > 0.0)) return c.ok = false;
+  row[k] = std::sqrt(s);
+  c.k = k + 1;
+  return true;
+}
+
+// x = (L L')^-1 b
+void chol_solve(const Chol& c, const double* b, double* x) {
+  const int k = c.k;
+  for (int i = 0; i < k; ++i) {
+    double s = b[i];
+    for (int t = 0; t < i; ++t) s -= c.L[(size_t)i * c.cap + t] * x[t];
+    x[i] = s / c.L[(size_t)i * c.cap + i];
+  }
+  for (int i = k - 1; i >= 0; --i) {
+    double s = x[i];
+    for (int t = i + 1; t < k; ++t) s -= c.L[(size_t)t * c.cap + i] * x[t];
+    x[i] = s / c.L[(size_t)i * c.cap + i];
+  }
+}
+
+// minimum-norm least squares A x = b for symmetric A (k x k) through cyclic Jacobi eigenpairs;
+// singular values below eps * k * max |eigenvalue| count as zero (numpy lstsq's rcond=None)
+void pinv_solve(std::vector<double> a, int k, const double* b1, const double* b2, double* x1, double* x2) {
+  std::vector<double> v((size_t)k * k, 0.0);
+  for (int i = 0; i < k; ++i) v[(size_t)i * k + i] = 1.0;
+  for (int sweep = 0; sweep < 60; ++sweep) {
+    double off = 0.0;
+    for (int p = 0; p < k; ++p)
+      for (int q = p + 1; q < k; ++q) off += a[(size_t)p * k + q] * a[(size_t)p * k + q];
+    if (off < 1e-30) break;
+    for (int p = 0; p < k; ++p) {
+      for (int q = p + 1; q < k; ++q) {
+        const double apq = a[(size_t)p * k + q];
+        if (std::fabs(apq) < 1e-300) continue;
+        const double theta = (a[(size_t)q * k + q] - a[(size_t)p * k + p]) / (2.0 * apq);
+        const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+        const double cs = 1.0 / std::sqrt(t * t + 1.0), sn = t * cs;
+        for (int r = 0; r < k; ++r) {
+          const double arp = a[(size_t)r * k + p], arq = a[(size_t)r * k + q];
+          a[(size_t)r * k + p] = cs * arp - sn * arq;
+          a[(size_t)r * k + q] = sn * arp + cs * arq;
+        }
+        for (int r = 0; r < k; ++r) {
+          const double apr = a[(size_t)p * k + r], aqr = a[(size_t)q * k + r];
+          a[(size_t)p * k + r] = cs * apr - sn * aqr;
+          a[(size_t)q * k + r] = sn * apr + cs * aqr;
+        }
+        for (int r = 0; r < k; ++r) {
+          const double vrp = v[(size_t)r * k + p], vrq = v[(size_t)r * k + q];
+          v[(size_t)r * k + p] = cs * vrp - sn * vrq;
+          v[(size_t)r * k + q] = sn * vrp + cs * vrq;
+        }
+      }
+    }
+  }
+  double emax = 0.0;
+  for (int i = 0; i < k; ++i) emax = std::max(emax, std::fabs(a[(size_t)i * k + i]));
+  const double cut = 2.220446049250313e-16 * k * emax;
+  for (int i = 0; i < k; ++i) x1[i] = x2[i] = 0.0;
+  for (int e = 0; e < k; ++e) {
+    const double lam = a[(size_t)e * k + e];
+    if (std::fabs(lam) <= cut) continue;
+    double c1 = 0.0, c2 = 0.0;
+    for (int r = 0; r < k; ++r) {
+      c1 += v[(size_t)r * k + e] * b1[r];
+      c2 += v[(size_t)r * k + e] * b2[r];
+    }
+    c1 /= lam, c2 /= lam;
+    for (int r = 0; r < k; ++r) {
+      x1[r] += c1 * v[(size_t)r * k + e];
+      x2[r] += c2 * v[(size_t)r * k + e];
+    }
+  }
+}
+
+// Euclidean projection onto the simplex by sorting (qp_native.py:20-33)
+void project_simplex(std::vector<double>& lam) {
+  const int K = (int)lam.size();
+  std::vector<double> u(lam);
+  std::sort(u.begin(), u.end(), [](double a, double b) { return a > b; });
+  double cs = 0.0, thr = 0.0;
+  int r = -1;
+  double run = 0.0;
+  for (int i = 0; i < K; ++i) {
+    run += u[i];
+    const double c = run - 1.0;
+    if (u[i] - c / (double)(i + 1) > 0.0) r = i, cs = c;
+  }
+  if (r < 0) {
+    for (auto& x : lam) x = 1.0 / K;
+    return;
+  }
+  thr = cs / (double)(r + 1);
+  for (auto& x : lam) x = std::max(x - thr, 0.0);
+}
+
+}  // namespace
+
+extern "C" int rumgpu_qp_simplex(int K, const double* Q, int64_t ldq, const double* h, const int32_t* A_init, int nA,
+                                 int max_outer, double tol, double reg, double* lam_out, int32_t* support,
+                                 int* n_support) {
+  RG_CHECK(K >= 1 && Q && h && lam_out && support && n_support && ldq >= K, RG_EINVAL, "qp_simplex: bad argument");
+  std::vector<int> work;
+  std::vector<char> in(K, 0);
+  for (int i = 0; i < nA; ++i) {
+    const int a = A_init ? A_init[i] : -1;
+    if (a >= 0 && a < K && !in[a]) work.push_back(a), in[a] = 1;
+  }
+  if (work.empty()) {
+    int j = 0;
+    for (int i = 1; i < K; ++i)
+      if (h[i] > h[j]) j = i;
+    work.push_back(j), in[j] = 1;
+  }
+  Chol ch;
+  ch.reserve(std::min(K, std::max(64, (int)work.size() + 16)));
+  std::vector<double> a, qh(K), qe(K), bh(K), be(K, 1.0), lam_w;
+  std::vector<int> solved;     // the working set lam_w belongs to
+  auto rebuild = [&]() {
+    const int k = (int)work.size();
+    a.assign((size_t)k * k, 0.0);
+    for (int i = 0; i < k; ++i)
+      for (int j = 0; j < k; ++j) a[(size_t)i * k + j] = Q[(int64_t)work[i] * ldq + work[j]] + (i == j ? reg : 0.0);
+    return chol_full(ch, a, k);
+  };
+  rebuild();
+  double mu = 0.0;
+  for (int outer = 0; outer < max_outer; ++outer) {
+    const int k = (int)work.size();
+    for (int i = 0; i < k; ++i) bh[i] = h[work[i]];
+    if (ch.ok && ch.k == k) {
+      chol_solve(ch, bh.data(), qh.data());
+      chol_solve(ch, be.data(), qe.data());
+    } else {
+      a.assign((size_t)k * k, 0.0);
+      for (int i = 0; i < k; ++i)
+        for (int j = 0; j < k; ++j) a[(size_t)i * k + j] = Q[(int64_t)work[i] * ldq + work[j]] + (i == j ? reg : 0.0);
+      pinv_solve(a, k, bh.data(), be.data(), qh.data(), qe.data());
+    }
+    double se = 0.0, sh = 0.0;
+    for (int i = 0; i < k; ++i) se += qe[i], sh += qh[i];
+    mu = std::fabs(se) > 1e-30 ? (sh - 1.0) / se : 0.0;
+    lam_w.assign(k, 0.0);
+    solved = work;
+    int imin = 0;
+    for (int i = 0; i < k; ++i) {
+      lam_w[i] = qh[i] - mu * qe[i];
+      if (lam_w[i] < lam_w[imin]) imin = i;
+    }
+    if (k && lam_w[imin] < -tol) {
+      in[work[imin]] = 0;
+      work.erase(work.begin() + imin);
+      rebuild();
+      continue;
+    }
+    if (K > k) {
+      int jbest = -1;
+      double gbest = 0.0;
+      for (int j = 0; j < K; ++j) {
+        if (in[j]) continue;
+        double g = mu - h[j];
+        const double* qj = Q + (int64_t)j * ldq;
+        for (int i = 0; i < k; ++i) g += qj[work[i]] * lam_w[i];
+        if (jbest < 0 || g < gbest) jbest = j, gbest = g;
+      }
+      if (jbest >= 0 && gbest < -tol) {
+        std::vector<double> q(k);
+        for (int i = 0; i < k; ++i) q[i] = Q[(int64_t)jbest * ldq + work[i]];
+        work.push_back(jbest), in[jbest] = 1;
+        if (!(ch.ok && ch.k == k && k + 1 <= ch.cap && chol_append(ch, q.data(), Q[(int64_t)jbest * ldq + jbest] + reg)))
+          rebuild();
+        continue;
+      }
+    }
+    break;
+  }
+  std::vector<double> lam(K, 0.0);
+  for (size_t i = 0; i < solved.size(); ++i) lam[solved[i]] = lam_w[i];
+  project_simplex(lam);
+  int ns = 0;
+  for (int w : work)
+    if (lam[w] > 1e-9) support[ns++] = w;
+  *n_support = ns;
+  for (int i = 0; i < K; ++i) lam_out[i] = lam[i];
+  return RG_OK;
+}

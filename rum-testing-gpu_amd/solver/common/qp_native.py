@@ -6,9 +6,17 @@ problem is solved by Cholesky,
 dropping the most negative weight, else adding the most violated inactive column (gradient below
 -tol), else stopping; regularization reg on the diagonal; a final Euclidean simplex projection;
 support = weights above 1e-9. The working set warm-starts from the previous call's support.
+
+solve_qp_native runs it in librumgpu (rumgpu_qp_simplex: the working set's Cholesky factor grows
+by one row per entering column instead of being refactored); solve_qp_numpy is the same method in
+numpy/scipy, step for step as the reference writes it.
 """
+import ctypes
+
 import numpy as np
 from scipy.linalg import cho_factor, cho_solve
+
+from .. import _lib
 
 
 def project_simplex(v):
@@ -24,7 +32,26 @@ def project_simplex(v):
 
 
 def solve_qp_native(Q, h, A_init=None, max_outer=200, tol=1e-10, reg=1e-12):
-    """Active-set solve of the simplex QP; returns (lam, support list)."""
+    """Active-set solve of the simplex QP in librumgpu; returns (lam, support list)."""
+    K = int(h.size)
+    Q = np.asarray(Q, dtype=np.float64)
+    if Q.strides[1] != 8 or Q.shape != (K, K):
+        Q = np.ascontiguousarray(Q)
+    ldq = Q.strides[0] // 8
+    h = np.ascontiguousarray(h, dtype=np.float64)
+    A = np.ascontiguousarray(A_init if A_init is not None else [], dtype=np.int32)
+    lam = np.empty(K)
+    sup = np.empty(K, dtype=np.int32)
+    ns = ctypes.c_int()
+    lib = _lib.load()
+    _lib.check(lib.rumgpu_qp_simplex(K, ctypes.c_void_p(Q.ctypes.data), ldq, _lib.ptr(h), _lib.ptr(A), int(A.size),
+                                     int(max_outer), float(tol), float(reg), _lib.ptr(lam), _lib.ptr(sup),
+                                     ctypes.byref(ns)), "qp_simplex")
+    return lam, [int(i) for i in sup[:ns.value]]
+
+
+def solve_qp_numpy(Q, h, A_init=None, max_outer=200, tol=1e-10, reg=1e-12):
+    """Active-set solve of the simplex QP in numpy/scipy; returns (lam, support list)."""
     K = int(h.size)
     work = [int(i) for i in (A_init if A_init is not None else []) if 0 <= int(i) < K]
     if not work:

@@ -15,7 +15,7 @@ import time
 import numpy as np
 
 from .. import _lib
-from ..common.qp_native import solve_qp_native
+from ..common.qp_native import solve_qp_native, solve_qp_numpy
 
 LAYOUTS = {"mask": 0, "slab": 1, "stream": 2}
 DTYPES = {"float32": 0, "uint16": 1, "uint24": 2}
@@ -133,12 +133,43 @@ class Columns:
         self.keys.append(bytes(np.asarray(col, dtype=np.uint8)))
         self.K += 1
 
+    def load_template(self, tpl):
+        """the columns of a template (bits, Gram, positions, keys of the same rows) as the first
+        columns; h from this object's p, column by column as add() computes it"""
+        k = tpl.K
+        if k > self.max_K:
+            raise RuntimeError("max_K = %d columns exceeded" % self.max_K)
+        self.bits[:k] = tpl.bits[:k]
+        self.Q[:k, :k] = tpl.Q[:k, :k]
+        for i in range(k):
+            self.h[i] = float(self.p[tpl.pos[i]].sum())
+        self.pos = list(tpl.pos)
+        self.keys = list(tpl.keys)
+        self.K = k
+
     def apply_delta(self, dlam):
         for a in np.flatnonzero(np.abs(dlam) > 1e-12):
             self.p_out[self.pos[int(a)]] += float(dlam[a])
 
     def view(self):
         return self.Q[:self.K, :self.K], self.h[:self.K]
+
+
+_TEMPLATES = []
+
+
+def _warm_template(init_columns, M):
+    """Bitsets and Gram matrix of a warm-start column list, built once per list (a session's kept
+    columns start every bootstrap draw): the integer Gram is the same whatever the draw's p."""
+    for lst, tpl in _TEMPLATES:
+        if lst is init_columns and tpl.M == M:
+            return tpl
+    tpl = Columns(M, np.zeros(M), len(init_columns))
+    for c in init_columns:
+        tpl.add(np.asarray(c, dtype=np.uint8))
+    _TEMPLATES.insert(0, (init_columns, tpl))
+    del _TEMPLATES[4:]
+    return tpl
 
 
 def ranking_column(memb, alt_x, rank_pos, n):
@@ -175,11 +206,12 @@ def drive(gen, pricer):
 
 def cg_steps(n, menu, alt, p, pricer, rc_tol=0.0, max_iters=4000, time_limit_s=0.0, seed=0, verbose=False,
              phi_ub_target=None, gap_rel_tol=None, gap_abs_tol=None, init_columns=None, init_lambda=None,
-             early_ub_check=False, lb_stop=None):
+             early_ub_check=False, lb_stop=None, qp="native"):
     """solve_cg's loop as a generator: yields the float32 scores of each pricing sweep, receives
     (vmax, rank_pos, margin), returns the result dict. A driver may price many of these loops in
     one batched sweep (inference/batched.py); each loop's arithmetic is the same either way."""
     t_setup = time.perf_counter()
+    qp_solve = {"native": solve_qp_native, "numpy": solve_qp_numpy}[qp]
     perm, masks_x, alt_x, offsets = xmajor(menu, alt, n)
     p32 = np.asarray(p, dtype=np.float64)[perm].astype(np.float32)
     p_obs = p32.astype(np.float64)
@@ -190,8 +222,7 @@ def cg_steps(n, menu, alt, p, pricer, rc_tol=0.0, max_iters=4000, time_limit_s=0
     n_init = len(init_columns) if init_columns is not None else 0
     cols = Columns(M, p_obs, max_iters + 2 + n_init)
     if n_init:
-        for c in init_columns:
-            cols.add(np.asarray(c, dtype=np.uint8))
+        cols.load_template(_warm_template(init_columns, M))
         lam = (np.asarray(init_lambda, dtype=np.float64).copy() if init_lambda is not None
                else np.full(n_init, 1.0 / n_init))
         lam = np.maximum(lam, 0.0)
@@ -216,7 +247,7 @@ def cg_steps(n, menu, alt, p, pricer, rc_tol=0.0, max_iters=4000, time_limit_s=0
             break
         tq = time.perf_counter()
         Qv, hv = cols.view()
-        lam_new, A_warm = solve_qp_native(Qv, hv, A_init=A_warm)
+        lam_new, A_warm = qp_solve(Qv, hv, A_init=A_warm)
         cols.apply_delta(lam_new - lam)
         lam = lam_new
         lQl = float(lam @ (Qv @ lam))

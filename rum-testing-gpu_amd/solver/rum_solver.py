@@ -851,7 +851,7 @@ def _flow_cg_steps(t, pricer):
             return "time"
         t_call = time.perf_counter()
         kw = dict(max_iters=int(a["max_iters"]), time_limit_s=remaining, seed=t.seed, verbose=t.verbose, **warm,
-                  **stops)
+                  **stops, qp=a.get("_qp", "native"))
         if not race:
             kw.update(gap_rel_tol=rel, gap_abs_tol=abs_e if abs_e else None)
         r = yield from cg_steps(n, g.menu, g.alternative, t.p, pricer, **kw)

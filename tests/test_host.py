@@ -159,3 +159,37 @@ def test_no_gpu_means_no_method():
     prof = dict(use_gpu=False, gpu=None, ram_bytes=1 << 34)
     with pytest.raises(RuntimeError, match="no method can run"):
         rum.calculate_distance(machine=prof)
+
+
+def test_native_master_qp_matches_numpy_and_oracle():
+    """rumgpu_qp_simplex (host code in the library) against the numpy active set and the oracle's
+    restatement, on Gram matrices of random 0/1 columns (the master QP's form), including
+    duplicated and dependent columns"""
+    from oracle import oracle as O
+    from solver.common.qp_native import solve_qp_native, solve_qp_numpy
+    rng = np.random.default_rng(4)
+    for trial in range(40):
+        M, K = int(rng.integers(3, 40)), int(rng.integers(1, 60))
+        C = (rng.random((K, M)) < 0.4).astype(np.float64)
+        if trial % 5 == 0 and K > 3:
+            C[1] = C[0]                               # a duplicate column
+            C[2] = np.clip(C[0] + C[3], 0, 1)         # often dependent
+        p = rng.random(M)
+        Q, h = C @ C.T, C @ p
+        lam, sup = solve_qp_native(Q, h)
+        lam2, sup2 = solve_qp_numpy(Q, h)
+        lam3, _ = O.qp_simplex(Q, h)
+        assert abs(lam.sum() - 1.0) < 1e-12 and (lam >= 0).all()
+        f = lambda l: 0.5 * l @ Q @ l - h @ l      # noqa: E731
+        # the same optimum value (the weights may differ on degenerate masters)
+        assert abs(f(lam) - f(lam2)) <= 1e-9 * max(1.0, abs(f(lam2))), trial
+        assert abs(f(lam) - f(lam3)) <= 1e-9 * max(1.0, abs(f(lam3))), trial
+        # KKT on the support: gradient = mu there, >= mu elsewhere
+        g = Q @ lam - h
+        on = lam > 1e-9
+        mu = g[on].mean()
+        assert np.abs(g[on] - mu).max() <= 1e-7 * max(1.0, np.abs(g).max()), trial
+        assert (g[~on] >= mu - 1e-7 * max(1.0, np.abs(g).max())).all(), trial
+        # warm start from a support gives the same optimum
+        lam4, _ = solve_qp_native(Q, h, A_init=sup2)
+        assert abs(f(lam4) - f(lam2)) <= 1e-9 * max(1.0, abs(f(lam2)))
