@@ -194,3 +194,22 @@ def test_batched_flow_cg_bootstrap_identical_brackets():
     assert np.array_equal(one.L, bat.L) and np.array_equal(one.U, bat.U)
     assert np.array_equal(one.flag, bat.flag)
     assert pv1.p == pv2.p and pv1.n_solved >= 20
+
+
+def test_batched_confidence_interval_equals_per_draw():
+    """Bootstrap.confidence_interval solves upper-tail draws to tolerance with no band stops
+    (bootstrap.py:212-240); batched flow_cg gives the per-draw brackets, so the same interval"""
+    import pandas as pd
+    panel, n = _small_panel()
+    panel = pd.DataFrame(panel)
+    menu, alt, p = cells_from_panel(panel, n)
+    rum = RUM(menu, alt, p, num_alternatives=n)
+    args = dict(method="flow_cg", method_args={"pricing": "mask"}, abs_tol=1e-4, refine=1)
+    with rum.bootstrap(panel=panel, B=96, **args) as one:
+        ci1 = one.confidence_interval()
+    with rum.bootstrap(panel=panel, B=96, batch=dict(slots=16), **args) as bat:
+        ci2 = bat.confidence_interval()
+    print(ci1)
+    print(ci2)
+    assert ci1.n_solved >= 1
+    assert ci1.lower == ci2.lower and ci1.lower_bracket == ci2.lower_bracket and ci1.n_solved == ci2.n_solved
