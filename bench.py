@@ -43,7 +43,9 @@ def _parse():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--dtype", default="float64", choices=["mixed", "float64", "float32"])
-    ap.add_argument("--beta", default="cert", help="'cert' (certificate rule), 'ref' (race rule) or a number")
+    ap.add_argument("--beta", default="adaptive",
+                    help="'cert' (certificate rule), 'adaptive' (balanced on the certified gap), 'ref' (race rule) "
+                         "or a number")
     ap.add_argument("--plain-steps", type=int, default=1)
     ap.add_argument("--abs-tol", type=float, default=1e-5)
     ap.add_argument("--max-time", type=float, default=120.0)
@@ -123,7 +125,7 @@ def main():
     if a.no_graph:
         margs["use_graph"] = False
     if a.beta != "ref":
-        margs["beta"] = a.beta if a.beta == "cert" else float(a.beta)
+        margs["beta"] = a.beta if a.beta in ("cert", "adaptive") else float(a.beta)
     sess = rum.session(method="bm_admm", method_args=margs)
     results = []
     for i in range(a.warmup):
@@ -228,7 +230,8 @@ def main():
                 data="synthetic",
                 config=dict(workload="certified BM-ADMM solve (abs gap 1e-5, rel 0), n=18, 1%% density "
                                      "(config2 seed 18001, %d cells)" % mp, n=n, cells=mp, abs_tol=a.abs_tol,
-                            beta=r_last.method_args["beta"], check=50,
+                            beta_rule=a.beta, beta=r_last.method_args["beta"],
+                            beta_final=r_last.info.get("beta_final"), check=50,
                             iterations=[r.iterations for r in results],
                             iters32=[r.info.get("iters32") for r in results],
                             iters64=[r.info.get("iters64") for r in results],

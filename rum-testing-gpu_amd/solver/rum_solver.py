@@ -592,10 +592,12 @@ def _plan_bm_admm(n, mp, weighted, prof, a):
         a["beta"] = params.bm_beta_default(mp)
     elif a["beta"] == "cert":
         a["beta"] = params.bm_beta_cert(mp)
+    elif a["beta"] == "adaptive":
+        a["beta"], a["_beta_adapt"] = params.bm_beta_adaptive_start(mp), dict(params.BM_BETA_ADAPT)
     else:
         a["beta"] = float(a["beta"])
         if not a["beta"] > 0:
-            raise ValueError("bm_admm beta must be > 0, None (the race rule) or 'cert'")
+            raise ValueError("bm_admm beta must be > 0, None (the race rule), 'cert' or 'adaptive'")
     return a
 
 
@@ -741,6 +743,9 @@ def _run_bm_admm(t):
     below, above, warm = getattr(t, "below", None), getattr(t, "above", None), getattr(t, "warm", None)
     # start: warm from the kept state (held by the handle that kept it) or cold
     first = hs[0]
+    for h in hs:                               # an adaptive penalty restarts from its start value
+        if h.beta != float(a["beta"]):
+            h.set_beta(float(a["beta"]))
     if warm is not None:
         src = warm["handle"]
         if src is first:
@@ -753,7 +758,8 @@ def _run_bm_admm(t):
     kw = dict(max_iter=int(a["max_iter"]), check=int(a["check"]), push_fn=push, ub_arm_immediately=True,
               lb_fn=None if race else lower, ub_running_min=True,
               gap_abs_tol=None if race else t.abs_tol, gap_rel_tol=None if race else t.rel_tol,
-              time_limit=max(t.deadline - time.perf_counter(), 0.0), verbose=t.verbose)
+              time_limit=max(t.deadline - time.perf_counter(), 0.0), verbose=t.verbose,
+              beta_adapt=None if race else a.get("_beta_adapt"))
     if below is not None and below > 0:
         kw.update(ub_target=below, ub_eps=0.0)
     if above is not None:
@@ -768,7 +774,8 @@ def _run_bm_admm(t):
         r = solve_kadmm(first, **kw)
         final = first
     st["iterations"] = int(r["iters"])
-    st["info"].update(per_iter_ms=round(r["per_iter_ms"], 4), push_s=round(r["push_s"], 3))
+    st["info"].update(per_iter_ms=round(r["per_iter_ms"], 4), push_s=round(r["push_s"], 3), beta_final=r["beta_final"],
+                      beta_changes=len(r["beta_changes"]))
     if getattr(t, "keep", False):
         final.keep()
         st["keep"] = dict(handle=final)
