@@ -248,3 +248,29 @@ def test_mask_lb64_matches_slab_bound(n):
     assert abs(lb_mask - lb_o) <= 1e-10 * max(1.0, lb_o)
     assert abs(lb_mask - lbv.value) <= 1e-10 * max(1.0, lb_o)
     pr.close()
+
+
+@pytest.mark.parametrize("graph", [0, 5])
+def test_set_beta_carries_the_state(graph):
+    """rumgpu_bm_set_beta settles the owed s, y update with the old penalty, then iterates with the
+    new one: equal, bit for bit, to restarting a new object at the new penalty from the state"""
+    n = 12
+    rng = np.random.default_rng(21)
+    menu, alt, p = _medium(n, 2500, rng)
+    a = Handle(n, menu, alt, p, dtype="float64", beta=0.7)
+    b = Handle(n, menu, alt, p, dtype="float64", beta=0.7)
+    if graph:
+        a.set_graph(graph)
+    a.run(10)
+    b.run(10)
+    a.set_beta(1.9)
+    st = b.get_state()
+    c = Handle(n, menu, alt, p, dtype="float64", beta=1.9)
+    c.set_state(st)
+    a.run(10)
+    c.run(10)
+    sa, sc = a.get_state(), c.get_state()
+    for k in ("x2", "s", "y", "z"):
+        assert np.array_equal(sa[k], sc[k]), k
+    for h in (a, b, c):
+        h.close()

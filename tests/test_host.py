@@ -193,3 +193,43 @@ def test_native_master_qp_matches_numpy_and_oracle():
         # warm start from a support gives the same optimum
         lam4, _ = solve_qp_native(Q, h, A_init=sup2)
         assert abs(f(lam4) - f(lam2)) <= 1e-9 * max(1.0, abs(f(lam2)))
+
+
+def test_adaptive_penalty_rule():
+    """solve_kadmm's beta_adapt on a scripted handle: doubles while ub - phi dominates, halves while
+    phi - lb dominates, a reversal takes sqrt(2) and freezes, at most max_changes changes"""
+    from solver.bm_admm.engine import solve_kadmm
+
+    class Fake:
+        dtype, graph_len = np.dtype(np.float64), 0
+
+        def __init__(self, script):
+            self.beta, self.k, self.script, self.log = 1.0, 0, script, []
+
+        def run(self, k):
+            self.k += 1
+
+        def phi(self):
+            return self.script(self.k)[1]
+
+        def set_beta(self, b):
+            self.beta = b
+            self.log.append(b)
+
+    def drive(script, **kw):
+        h = Fake(script)
+        rule = dict(every=2, ratio=10.0, factor=2.0, freeze_on_flip=True, max_changes=3)
+        rule.update(kw)
+        solve_kadmm(h, max_iter=40, check=1, push_fn=lambda hh: script(hh.k)[0], ub_arm_immediately=True,
+                    lb_fn=lambda hh: script(hh.k)[2], gap_abs_tol=1e-12, beta_adapt=rule)
+        return h.log
+
+    infeasible = lambda k: (1.0 + 1.0 / k, 0.5, 0.5 - 1e-3 / k)     # noqa: E731  ub - phi >> phi - lb
+    assert drive(infeasible) == [2.0, 4.0, 8.0]                      # capped at 3 changes
+    optimal = lambda k: (0.5 + 1e-3 / k, 0.5, 0.5 - 1.0 / k)         # noqa: E731
+    assert drive(optimal) == [0.5, 0.25, 0.125]
+    flip = lambda k: (infeasible if k < 3 else optimal)(k)           # noqa: E731
+    log = drive(flip, max_changes=10)
+    assert log[0] == 2.0 and abs(log[1] - 2.0 / np.sqrt(2.0)) < 1e-15 and len(log) == 2
+    balanced = lambda k: (0.6 + 1.0 / k, 0.5, 0.4 - 1.0 / k)         # noqa: E731
+    assert drive(balanced) == []
