@@ -150,7 +150,9 @@ int rumgpu_cg_price_batch(rumgpu_cg* cg, int D, const float* scores, double* vma
 /* float64 max over rankings of sum_j r_j [the ranking picks row j's alternative] for r given per
  * row (x-major): the longest path behind the Frank-Wolfe lower bound of fw_lb_rows_gpu
  * (common/transforms.py:101-114), flow_cg's final bound (rum_solver.py:928-938), computed on the
- * mask lattice instead of a float64 slab (mask layout only; n 2^n float64 scratch, kept) */
+ * pricer's lattice instead of a float64 slab: mask layout (2^n float64 scratch, kept) or stream
+ * layout (two float64 colex layers, sweep_max64 / fw_lb_stream64, pricing_stream.py:190-248,
+ * transforms.py:117-129) */
 int rumgpu_cg_vmax64(rumgpu_cg* cg, const double* r_x, double* vmax);
 int rumgpu_cg_info(rumgpu_cg* cg, size_t* device_bytes, double* last_kernel_ms);
 

@@ -224,6 +224,47 @@ __global__ void cg_fill_u8(uint8_t* a, uint8_t v, i64 m) {
 #define CG_THREADS 256
 #define CG_VPT 4
 
+// float64 value of one colex layer (sweep_max64, pricing_stream.py:190-248): the same unranking
+// and predecessor ranks as cg_stream_layer, values and row sums in float64, no choices kept.
+__global__ __launch_bounds__(CG_THREADS) void cg_stream_layer64(const u64* __restrict__ binom, int n, int k,
+                                                              u64 count, const double* __restrict__ vprev,
+                                                              double* __restrict__ vcur,
+                                                              const u64* __restrict__ masks,
+                                                              const double* __restrict__ r,
+                                                              const u64* __restrict__ offsets) {
+  const u64 rk = (u64)blockIdx.x * CG_THREADS + threadIdx.x;
+  if (rk >= count) return;
+  const int np1 = n + 1;
+  u64 rr = rk, D = 0;
+  int b = n - 1;
+  for (int t = k; t >= 1; --t) {
+    while (binom[b * np1 + t] > rr) --b;
+    D |= 1ull << b;
+    rr -= binom[b * np1 + t];
+    --b;
+  }
+  double best = 0.0;
+  bool seen = false;
+  u64 suf = 0, remain = rk, left = D;
+  for (int t = k - 1; t >= 0; --t) {
+    const int x = 63 - __clzll((long long)left);
+    left ^= 1ull << x;
+    remain -= binom[x * np1 + (t + 1)];
+    const u64 rp = remain + suf;
+    suf += binom[x * np1 + t];
+    double w = 0.0;
+    const u64 hi = offsets[x + 1];
+    for (u64 i = offsets[x]; i < hi; ++i) {
+      const u64 Dp = masks[i];
+      if ((Dp & D) == Dp) w += r[i];
+    }
+    const double cand = vprev[rp] + w;
+    if (!seen || cand > best) best = cand;
+    seen = true;
+  }
+  vcur[rk] = best;
+}
+
 template <int MODE>
 __device__ __forceinline__ float cg_load_prev(const void* vprev, const uint8_t* lprev, const float* mprev,
                                               const float* eprev, u64 rp) {
@@ -385,8 +426,9 @@ struct rumgpu_cg {
   float* vmaxb = nullptr;
   int32_t* rankb = nullptr;
   // float64 lower-bound sweep (allocated at the first rumgpu_cg_lb64)
-  double* V64 = nullptr;
+  double* V64 = nullptr;       // mask: V per node; stream: two layers of cap64 values
   double* r64 = nullptr;
+  u64 cap64 = 0;
   float* R = nullptr;        // slab
   int32_t* rank = nullptr;
   // stream
@@ -799,10 +841,41 @@ extern "C" int rumgpu_cg_price_batch(rumgpu_cg* c, int D, const float* scores, d
   return RG_OK;
 }
 
+static int stream_vmax64(rumgpu_cg* c, const double* r_x, double* vmax) {
+  hipStream_t st = c->ctx->stream;
+  const int n = c->n;
+  if (!c->V64) {
+    u64 cap = 0;
+    for (int k = 0; k <= n; ++k) cap = std::max(cap, binom_u64(n, k));
+    int rc = RG_OK;
+    if ((rc = cg_alloc(c, (void**)&c->V64, (size_t)cap * 8 * 2, "float64 stream layers")) != RG_OK ||
+        (rc = cg_alloc(c, (void**)&c->r64, (size_t)c->M * 8, "float64 residual")) != RG_OK) {
+      rg_free(c->V64), rg_free(c->r64);
+      c->V64 = nullptr, c->r64 = nullptr;
+      return rc;
+    }
+    c->cap64 = cap;
+  }
+  RG_HIP(hipMemcpyAsync(c->r64, r_x, (size_t)c->M * 8, hipMemcpyHostToDevice, st));
+  double* lay[2] = {c->V64, c->V64 + c->cap64};
+  int cur = 0, prev = 1;
+  hipLaunchKernelGGL(cg_fill_f64, dim3(1), dim3(64), 0, st, lay[prev], 0.0, (i64)1);     // layer 0: V = 0
+  for (int k = 1; k <= n; ++k) {
+    const u64 count = binom_u64(n, k);
+    hipLaunchKernelGGL(cg_stream_layer64, dim3((unsigned)((count + CG_THREADS - 1) / CG_THREADS)), dim3(CG_THREADS),
+                       0, st, c->binom, n, k, count, lay[prev], lay[cur], c->masks64, c->r64, c->offsets);
+    std::swap(cur, prev);
+  }
+  RG_LAUNCH_CHECK();
+  RG_HIP(rg_copy(st, vmax, lay[prev], 8, hipMemcpyDeviceToHost));
+  return RG_OK;
+}
+
 extern "C" int rumgpu_cg_vmax64(rumgpu_cg* c, const double* r_x, double* vmax) {
   RG_CHECK(c && r_x && vmax, RG_EINVAL, "cg_vmax64: null argument");
-  RG_CHECK(c->layout == 0, RG_EUNSUP, "cg_vmax64: the float64 sweep uses the mask layout");
+  RG_CHECK(c->layout == 0 || c->layout == 2, RG_EUNSUP, "cg_vmax64: the float64 sweep uses the mask or stream layout");
   RG_HIP(hipSetDevice(c->ctx->device));
+  if (c->layout == 2) return stream_vmax64(c, r_x, vmax);
   hipStream_t st = c->ctx->stream;
   if (!c->V64) {
     int rc = RG_OK;

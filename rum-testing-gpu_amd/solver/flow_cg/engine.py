@@ -66,7 +66,7 @@ class Pricer:
         return vmax, rank
 
     def vmax64(self, r_x):
-        """float64 longest path for residuals r per x-major row (mask layout)"""
+        """float64 longest path for residuals r per x-major row (mask or stream layout)"""
         r = _lib.c_array(r_x, np.float64)
         v = ctypes.c_double()
         _lib.check(self.lib.rumgpu_cg_vmax64(self.h, _lib.ptr(r), ctypes.byref(v)), "cg_vmax64")

@@ -790,8 +790,8 @@ def _flow_cg_lb(n, g, p_fit, p, engine_lb, delta, dtype, pricer=None, perm=None)
     lowered by delta."""
     from . import _lib
     import ctypes
-    if pricer is not None and pricer.layout == "mask":
-        return pricer.lb64(perm, p_fit, p), "float64", "mask"
+    if pricer is not None and pricer.layout in ("mask", "stream"):
+        return pricer.lb64(perm, p_fit, p), "float64", pricer.layout
     need = 16 * (n << (n - 1)) + 16 * (1 << n)
     if need <= defaults.AUTO["gpu_memory_fraction"] * _machine.gpu_free_bytes():
         ctx = _lib.context(0)

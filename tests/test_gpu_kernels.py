@@ -230,8 +230,8 @@ def test_stream_pricing_bit_exact(n, mode):
 
 @pytest.mark.parametrize("n", [7, 12, 16])
 def test_mask_lb64_matches_slab_bound(n):
-    """flow_cg's float64 lower bound on the mask lattice equals the slab form (fw_lb_rows_gpu) and
-    the oracle's within float64 rounding"""
+    """flow_cg's float64 lower bound on the mask lattice and on the stream layout's two float64
+    layers equals the slab form (fw_lb_rows_gpu) and the oracle's within float64 rounding"""
     import ctypes
     rng = np.random.default_rng(600 + n)
     menu, alt, p = _medium(n, min(5000, n << (n - 3)), rng)
@@ -247,6 +247,11 @@ def test_mask_lb64_matches_slab_bound(n):
                                    ctypes.byref(lbv)))
     assert abs(lb_mask - lb_o) <= 1e-10 * max(1.0, lb_o)
     assert abs(lb_mask - lbv.value) <= 1e-10 * max(1.0, lb_o)
+    # the two-layer float64 sweep of the stream layout (sweep_max64): the same bound
+    ps = Pricer(n, masks_x, offsets, layout="stream", dtype="uint16")
+    lb_stream = ps.lb64(perm, y, p)
+    assert abs(lb_stream - lb_mask) <= 1e-12 * max(1.0, lb_o)
+    ps.close()
     pr.close()
 
 
