@@ -136,9 +136,11 @@ struct Tile {
   // slab index / rest index of register j in the current layout
   __device__ __forceinline__ i64 gidx(int j) const { return base + ((i64)w_of(cur, j) << s) + col; }
   __device__ __forceinline__ i64 rest(int j) const { return restbase + ((i64)w_of(cur, j) << s) + col; }
+  // LDS slot of register j in round r's layout: the tile index with bits 4-7 folded into bits 0-3
+  // (a bijection on the tile), which spreads the strided rounds of a window at bit 0 over the banks
   __device__ __forceinline__ int lidx(int r, int j) const {
     const int L = (w_of(r, j) << C) + col;
-    return L + (L >> 4);
+    return L ^ ((L >> 4) & 15);
   }
   __device__ __forceinline__ void exchange(int r) {
     __syncthreads();
