@@ -194,12 +194,20 @@ def solve_cg(n, menu, alt, p, pricer=None, layout="mask", dtype="float32", **kw)
             pricer.close()
 
 
+def answer(pricer, req):
+    """The reply to one request of a runner: a pricing sweep for float32 scores, the float64
+    longest path for ("vmax64", r)."""
+    if isinstance(req, tuple):
+        return pricer.vmax64(req[1])
+    return pricer.price(req)
+
+
 def drive(gen, pricer):
-    """Run a pricing-request generator (cg_steps, or a runner built on it) with one pricer."""
+    """Run a request generator (cg_steps, or a runner built on it) with one pricer."""
     try:
         req = next(gen)
         while True:
-            req = gen.send(pricer.price(req))
+            req = gen.send(answer(pricer, req))
     except StopIteration as e:
         return e.value
 

@@ -275,15 +275,18 @@ class BatchedCG:
     draw's columns, weights and bracket are those of its per-draw solve; only the sweeps are shared.
 
     slots   draws in flight (a multiple of 32 fills the sweeps)
+    overlap two halves of the slots alternate, so one half's host steps run while the GPU prices
+            the other half
     """
 
-    def __init__(self, boot, slots=32, verbose=False, **_ignored):
+    def __init__(self, boot, slots=64, overlap=True, verbose=False, **_ignored):
         sess = boot.session
         if sess.method != "flow_cg":
             raise ValueError("BatchedCG runs flow_cg draws (the session's method is %s)" % sess.method)
         if sess._args.get("pricing") != "mask":
             raise ValueError("batched sweeps need the mask pricing layout (the plan uses %s)" % sess._args["pricing"])
         self.boot, self.sess, self.D, self.verbose = boot, sess, int(slots), verbose
+        self.overlap = bool(overlap) and self.D >= 2
         self.stats = dict(slots=self.D, draws=0, rounds=0, sweeps=0, price_s=0.0, host_s=0.0, wall_s=0.0,
                           draw_rounds=0)
 
@@ -345,24 +348,78 @@ class BatchedCG:
 
         for s in range(D):
             yield from start(s)
-        while live:
-            slots = sorted(live)
-            req = np.stack([live[s][3] for s in slots])
-            t0 = time.perf_counter()
-            vmax, rank = pricer.price_batch(req)
-            t1 = time.perf_counter()
-            self.stats["price_s"] += t1 - t0
-            self.stats["rounds"] += 1
-            self.stats["sweeps"] += -(-len(slots) // 32)
-            self.stats["draw_rounds"] += len(slots)
-            for i, s in enumerate(slots):
-                done = step(s, (float(vmax[i]), rank[i].copy(), 0.0))
-                if done is not None:
-                    self.stats["draws"] += 1
-                    yield done
-                    yield from start(s)
-            self.stats["host_s"] += time.perf_counter() - t1
+        if not self.overlap:
+            while live:
+                slots = sorted(live)
+                t0 = time.perf_counter()
+                replies = self._serve(pricer, slots, {q: live[q][3] for q in slots})
+                t1 = time.perf_counter()
+                self.stats["price_s"] += t1 - t0
+                for s in slots:
+                    done = step(s, replies[s])
+                    if done is not None:
+                        self.stats["draws"] += 1
+                        yield done
+                        yield from start(s)
+                self.stats["host_s"] += time.perf_counter() - t1
+        else:
+            # two halves of the slots take turns: while the GPU prices one half (a worker thread;
+            # the library call releases the GIL), the host runs the other half's column
+            # generation steps. Each draw's own sequence of requests and replies is unchanged.
+            from concurrent.futures import ThreadPoolExecutor
+            halves = [set(range(0, D, 2)), set(range(1, D, 2))]
+            ex = ThreadPoolExecutor(max_workers=1)
+
+            def launch(g):
+                slots = sorted(q for q in live if q in halves[g])
+                if not slots:
+                    return None
+                reqs = {q: live[q][3] for q in slots}
+                return ex.submit(lambda: (slots, self._serve(pricer, slots, reqs)))
+
+            try:
+                futs = [launch(0), launch(1)]
+                g = 0
+                while futs[0] is not None or futs[1] is not None:
+                    if futs[g] is None:
+                        g ^= 1
+                        continue
+                    t0 = time.perf_counter()
+                    slots, replies = futs[g].result()
+                    futs[g] = None
+                    t1 = time.perf_counter()
+                    self.stats["price_s"] += t1 - t0          # time the host waited for the GPU
+                    for s in slots:
+                        done = step(s, replies[s])
+                        if done is not None:
+                            self.stats["draws"] += 1
+                            yield done
+                            yield from start(s)
+                    futs[g] = launch(g)
+                    self.stats["host_s"] += time.perf_counter() - t1
+                    g ^= 1
+            finally:
+                ex.shutdown(wait=True)
         self.stats["wall_s"] += time.perf_counter() - t_all
+
+    def _serve(self, pricer, slots, reqs):
+        """Replies to the requests of `slots` (reqs: slot -> request): one batched sweep for every
+        pricing request, the float64 longest paths of finishing draws one by one."""
+        get = reqs.__getitem__
+        price = [q for q in slots if not isinstance(get(q), tuple)]
+        out = {}
+        if price:
+            vmax, rank = pricer.price_batch(np.stack([get(q) for q in price]))
+            self.stats["rounds"] += 1
+            self.stats["sweeps"] += -(-len(price) // 32)
+            self.stats["draw_rounds"] += len(price)
+            for i, q in enumerate(price):
+                out[q] = (float(vmax[i]), rank[i].copy(), 0.0)
+        for q in slots:
+            if q not in out:
+                out[q] = pricer.vmax64(get(q)[1])
+                self.stats["lb_sweeps"] = self.stats.get("lb_sweeps", 0) + 1
+        return out
 
     def close(self):
         pass

@@ -872,8 +872,15 @@ def _flow_cg_steps(t, pricer):
         else:
             # the float64 bound on the pricer's mask lattice, or (a["_lb_form"] == "slab") through a
             # float64 slab as the reference computes it
-            lbp = pricer if a.get("_lb_form", "mask") == "mask" else None
-            lb, lb_dt, lb_lay = _flow_cg_lb(n, g, p_fit, t.p, float(r["lb"]), delta, a["dtype"], lbp, r["perm"])
+            if a.get("_lb_form", "mask") == "mask" and pricer.layout in ("mask", "stream"):
+                # the longest path of the float64 bound is a request like a pricing sweep, so a
+                # batched driver can run it beside other draws' sweeps
+                rr = t.p - p_fit
+                vmax = yield ("vmax64", np.ascontiguousarray(rr[r["perm"]]))
+                l2 = math.fsum(rr * rr) - 2.0 * (vmax - math.fsum(rr * p_fit))
+                lb, lb_dt, lb_lay = math.sqrt(max(0.0, l2)), "float64", pricer.layout
+            else:
+                lb, lb_dt, lb_lay = _flow_cg_lb(n, g, p_fit, t.p, float(r["lb"]), delta, a["dtype"], None, r["perm"])
         st["lb"] = max(lb_prev, lb)
         st["iterations"] = int(r["n_cols"])
         st["setup"] += float(r["setup_s"])
