@@ -71,10 +71,24 @@ __global__ __launch_bounds__(256) void cg_mask_layer_batch(float* __restrict__ V
                                                            const u32* __restrict__ masks,
                                                            const float* __restrict__ scores,
                                                            const u64* __restrict__ offsets,
-                                                           const u32* __restrict__ nodes, i64 ns, int n) {
+                                                           const u32* __restrict__ nodes, i64 ns, int n,
+                                                           u32 zmask) {
   const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ns) return;
   const u32 D = nodes[i];
+  if ((D & zmask) == 0u) {
+    // no member has rows: every arc weighs 0, so by induction V = 0 here and the first member
+    // (ascending, strict >) is the choice; the sweep's values without reading the predecessors
+    float* vo = V + (size_t)D * DB;
+    uint8_t* po = pred + (size_t)D * DB;
+    const uint8_t x0 = (uint8_t)(__ffs((int)D) - 1);
+#pragma unroll
+    for (int d = 0; d < DB; ++d) {
+      vo[d] = 0.0f;
+      po[d] = x0;
+    }
+    return;
+  }
   float best[DB];
   uint8_t bx[DB];
 #pragma unroll
@@ -106,10 +120,12 @@ __global__ __launch_bounds__(256) void cg_mask_layer_batch(float* __restrict__ V
         for (int d = 0; d < DB; ++d) w[d] += sj[d];
       }
     }
-    const float* vp = V + (size_t)(D ^ (1u << x)) * DB;
+    const u32 P = D ^ (1u << x);
+    const float* vp = V + (size_t)P * DB;
+    const bool zero = (P & zmask) == 0u;          // a row-free predecessor holds V = 0
 #pragma unroll
     for (int d = 0; d < DB; ++d) {
-      const float cand = vp[d] + w[d];
+      const float cand = (zero ? 0.0f : vp[d]) + w[d];
       if (!seen || cand > best[d]) {
         best[d] = cand;
         bx[d] = (uint8_t)x;
@@ -132,10 +148,14 @@ __global__ __launch_bounds__(256) void cg_mask_layer_batch(float* __restrict__ V
 // slab and a layered longest path; here the arc weights are the rows' subset tests, in float64)
 __global__ __launch_bounds__(256) void cg_mask_layer64(double* __restrict__ V, const u32* __restrict__ masks,
                                                        const double* __restrict__ r, const u64* __restrict__ offsets,
-                                                       const u32* __restrict__ nodes, i64 ns, int n) {
+                                                       const u32* __restrict__ nodes, i64 ns, int n, u32 zmask) {
   const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ns) return;
   const u32 D = nodes[i];
+  if ((D & zmask) == 0u) {             // no member has rows: V = 0 (as in cg_mask_layer_batch)
+    V[D] = 0.0;
+    return;
+  }
   double best = 0.0;
   bool seen = false;
   for (int x = 0; x < n; ++x) {
@@ -408,6 +428,7 @@ struct rumgpu_cg {
   int n = 0, layout = 0, dtype = 0, k_pred = 1;
   i64 M = 0, NN = 0, Nloc = 0;
   std::vector<u64> h_masks, h_offsets;
+  u32 zmask = 0;             // alternatives with at least one row (mask layout, n <= 31)
   std::vector<i64> bounds;
   // device rows
   u32* masks32 = nullptr;
@@ -518,6 +539,8 @@ extern "C" int rumgpu_cg_create(rumgpu_ctx* ctx, int n, const uint64_t* masks_x,
   c->Nloc = 1ll << (n - 1);
   c->h_masks.assign(masks_x, masks_x + M);
   c->h_offsets.assign(offsets, offsets + n + 1);
+  for (int x = 0; x < n && x < 32; ++x)
+    if (offsets[x + 1] > offsets[x]) c->zmask |= 1u << x;
   auto fail = [&](int code) {
     cg_free_all(c);
     delete c;
@@ -788,7 +811,7 @@ static int mask_sweep_batch(rumgpu_cg* c, hipStream_t st) {
     const i64 lo = c->bounds[k], ns = c->bounds[k + 1] - lo;
     if (ns <= 0) continue;
     hipLaunchKernelGGL(cg_mask_layer_batch<DB>, dim3(grid_for(ns, 256)), dim3(256), 0, st, c->Vb, c->predb,
-                       c->masks32, c->scoresb, c->offsets, c->nodes + lo, ns, n);
+                       c->masks32, c->scoresb, c->offsets, c->nodes + lo, ns, n, c->zmask);
   }
   RG_LAUNCH_CHECK();
   return RG_OK;
@@ -892,7 +915,7 @@ extern "C" int rumgpu_cg_vmax64(rumgpu_cg* c, const double* r_x, double* vmax) {
     const i64 lo = c->bounds[k], ns = c->bounds[k + 1] - lo;
     if (ns <= 0) continue;
     hipLaunchKernelGGL(cg_mask_layer64, dim3(grid_for(ns, 256)), dim3(256), 0, st, c->V64, c->masks32, c->r64,
-                       c->offsets, c->nodes + lo, ns, c->n);
+                       c->offsets, c->nodes + lo, ns, c->n, c->zmask);
   }
   RG_LAUNCH_CHECK();
   RG_HIP(rg_copy(st, vmax, c->V64 + (c->NN - 1), 8, hipMemcpyDeviceToHost));

@@ -159,11 +159,17 @@ def test_batched_bootstrap_same_decisions():
     assert pv1.n_solved >= 20 and st["draws"] >= pv2.n_solved
 
 
-@pytest.mark.parametrize("n,D", [(9, 5), (14, 16), (18, 21)])
-def test_batched_pricing_equals_single(n, D):
+@pytest.mark.parametrize("n,D,alt0", [(9, 5, False), (14, 16, False), (18, 21, False), (12, 40, True),
+                                       (17, 32, True)])
+def test_batched_pricing_equals_single(n, D, alt0):
+    """alt0: rows of alternative 0 only (the bootstrap panels' shape), where the batched sweep skips
+    the subsets without a row-carrying member"""
     from solver.flow_cg.engine import Pricer, xmajor
     rng = np.random.default_rng(400 + n)
     menu, alt, p = _medium(n, min(8000, n << (n - 3)), rng)
+    if alt0:
+        keep = alt == 0
+        menu, alt, p = menu[keep], alt[keep], p[keep]
     perm, masks_x, _alts, offsets = xmajor(menu, alt, n)
     pr = Pricer(n, masks_x, offsets, layout="mask")
     S = rng.standard_normal((D, menu.size)).astype(np.float32)

@@ -228,13 +228,16 @@ def test_stream_pricing_bit_exact(n, mode):
     pr.close()
 
 
-@pytest.mark.parametrize("n", [7, 12, 16])
-def test_mask_lb64_matches_slab_bound(n):
+@pytest.mark.parametrize("n,alt0", [(7, False), (12, False), (16, False), (13, True)])
+def test_mask_lb64_matches_slab_bound(n, alt0):
     """flow_cg's float64 lower bound on the mask lattice and on the stream layout's two float64
     layers equals the slab form (fw_lb_rows_gpu) and the oracle's within float64 rounding"""
     import ctypes
     rng = np.random.default_rng(600 + n)
     menu, alt, p = _medium(n, min(5000, n << (n - 3)), rng)
+    if alt0:                                           # rows of alternative 0 only
+        keep = alt == 0
+        menu, alt, p = menu[keep], alt[keep], p[keep]
     perm, masks_x, _alts, offsets = xmajor(menu, alt, n)
     pr = Pricer(n, masks_x, offsets, layout="mask")
     y = np.clip(p + 0.05 * rng.standard_normal(p.size), 0.0, 1.0)
