@@ -71,6 +71,11 @@ int rumgpu_bm_set_graph(rumgpu_bm* bm, int64_t len);
  * over. The reference fixes beta per solve (solver.py:148-154); this is the hook of an adaptive
  * penalty (residual balancing on the certified gap). */
 int rumgpu_bm_set_beta(rumgpu_bm* bm, double beta);
+/* window plan of the fused passes: lo_bits at bit 0 (lo_rl register bits), the rest in windows
+ * of <= 8 bits (hi_rl register bits). The reference plans its butterfly passes by a calibrated
+ * DP (common/butterfly.py:125-195); here the autotuner (solver/bm_admm/autotune.py) times plans
+ * per n. Results are the same for every plan up to rounding order. */
+int rumgpu_bm_set_plan(rumgpu_bm* bm, int lo_bits, int lo_rl, int hi_rl);
 int rumgpu_bm_finalize(rumgpu_bm* bm);
 /* phi = ||x2[obs] - p|| (solver.py:321), compensated float64 sum */
 int rumgpu_bm_phi(rumgpu_bm* bm, double* phi);

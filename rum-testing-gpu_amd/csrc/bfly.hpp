@@ -34,15 +34,38 @@ struct WinPlan {
   int s, B, C, RL;
 };
 
+// An explicit plan: the window at bit 0 takes lo_bits (C = 0, lo_rl register bits), the bits above
+// it go to windows of at most 8 bits (hi_rl register bits) with 16 (float64) or 32 (float32)
+// columns. The autotuner picks among these per n.
+inline std::vector<WinPlan> plan_windows_explicit(int nb, int itemsize, int lo_bits, int lo_rl, int hi_rl) {
+  std::vector<WinPlan> out;
+  if (nb <= 0) return out;
+  const int lo = lo_bits < nb ? lo_bits : nb;
+  out.push_back({0, lo, 0, lo_rl < lo ? lo_rl : lo});
+  int s = lo, rem = nb - lo;
+  const int k = (rem + 7) / 8;
+  for (int i = 0; i < k; ++i) {
+    const int left = k - i;
+    const int B = (rem + left - 1) / left;
+    int Cw = itemsize == 8 ? 4 : 5;
+    if (Cw > s) Cw = s;
+    out.push_back({s, B, Cw, hi_rl < B ? hi_rl : B});
+    s += B;
+    rem -= B;
+  }
+  return out;
+}
+
 inline std::vector<WinPlan> plan_windows(int nb, int itemsize) {
   std::vector<WinPlan> out;
   if (nb <= 0) return out;
-  // bit 0 window up to 10 bits (a 1024-value contiguous tile); windows above it up to 8 bits
-  // with 16-32 columns, so every tile fits 36 KiB of LDS.
-  int k = nb <= 10 ? 1 : 1 + (nb - 10 + 7) / 8;
+  // bit 0 window up to 12 bits (a 4096-value contiguous tile, 33 KiB of float64 LDS); windows
+  // above it up to 8 bits with 16-32 columns. As few windows as that allows: each window is one
+  // fused pass per transform.
+  int k = nb <= 12 ? 1 : 1 + (nb - 12 + 7) / 8;
   int lo = (nb + k - 1) / k;
   if (lo < nb - 8 * (k - 1)) lo = nb - 8 * (k - 1);
-  if (lo > 10) lo = 10;
+  if (lo > 12) lo = 12;
   int rem = nb - lo;
   out.push_back({0, lo, 0, lo < 3 ? lo : 3});
   int s = lo;

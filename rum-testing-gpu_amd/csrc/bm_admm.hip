@@ -1281,6 +1281,30 @@ extern "C" int rumgpu_bm_set_graph(rumgpu_bm* b, int64_t len) {
   return BM_DISPATCH(b, build_graph, b, len);
 }
 
+// A window plan for the fused passes (the certificate transforms keep the default plan): the
+// owed s, y update is settled first, since the pass buffer holds a transform split by the old plan.
+extern "C" int rumgpu_bm_set_plan(rumgpu_bm* b, int lo_bits, int lo_rl, int hi_rl) {
+  RG_CHECK(b, RG_EINVAL, "bm_set_plan: null handle");
+  RG_CHECK(!b->plain, RG_EINVAL, "bm_set_plan: the plain schedule has no window plan");
+  RG_CHECK(lo_bits >= 1 && lo_bits <= 12 && lo_rl >= 1 && lo_rl <= 5 && hi_rl >= 1 && hi_rl <= 5, RG_EINVAL,
+           "bm_set_plan: lo_bits 1..12, register bits 1..5");
+  std::vector<WinPlan> wins = plan_windows_explicit(b->nb, b->isz, lo_bits, lo_rl, hi_rl);
+  std::vector<WinGeom> geoms;
+  for (auto& w : wins) {
+    const WinGeom g = make_geom(b->n, w, b->D);
+    const size_t lds = (size_t)g.TPB * tile_lds_elems(g) * b->isz;
+    RG_CHECK(g.TT * g.TPB <= 512 && lds <= 65536, RG_EINVAL,
+             "bm_set_plan: a tile of %d bits with %d register bits needs %d threads and %zu bytes of LDS per block",
+             w.B + w.C, w.RL, g.TT * g.TPB, lds);
+    geoms.push_back(g);
+  }
+  RG_TRY(BM_DISPATCH(b, finalize, b));
+  b->wins = wins;
+  b->geoms = geoms;
+  if (b->gexec) return BM_DISPATCH(b, build_graph, b, b->graph_len);
+  return RG_OK;
+}
+
 // A new penalty for the next iterations (b1 = b2 = beta, so gamma = b2/b1 and inv_gl stay). The
 // multipliers y, z are unscaled, so the state carries over as it is once the deferred s, y update
 // is settled with the old penalty; a captured graph is rebuilt.

@@ -46,6 +46,7 @@ SIGNATURES = {
     "rumgpu_bm_info": [_P, _P, _P, _P],
     "rumgpu_bm_profile": [_P, _I64, _P, _P],
     "rumgpu_bm_set_beta": [_P, _D],
+    "rumgpu_bm_set_plan": [_P, _I, _I, _I],
     "rumgpu_bm_create_batch": [_P, _I, _P, _P, _P, _I64, _I, _D, _D, _D, _D, _D, _D, _P, _I, _P],
     "rumgpu_bm_slot_set_p": [_P, _I, _P],
     "rumgpu_bm_slot_load": [_P, _I, _P],

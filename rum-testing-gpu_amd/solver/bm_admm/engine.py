@@ -101,6 +101,9 @@ class Handle:
         _lib.check(self.lib.rumgpu_bm_set_graph(self.h, int(length)), "bm_set_graph")
         self.graph_len = int(length)
 
+    def set_plan(self, lo_bits, lo_rl, hi_rl):
+        _lib.check(self.lib.rumgpu_bm_set_plan(self.h, int(lo_bits), int(lo_rl), int(hi_rl)), "bm_set_plan")
+
     def set_beta(self, beta):
         _lib.check(self.lib.rumgpu_bm_set_beta(self.h, float(beta)), "bm_set_beta")
         self.beta = float(beta)

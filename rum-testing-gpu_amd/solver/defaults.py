@@ -47,6 +47,8 @@ AUTO = dict(
     bm_admm_min_density={11: 0.02, 14: 0.01, 18: 0.005},
     exact_qp_max_n=10,
     gpu_memory_fraction=0.9,    # share of free GPU memory a method may plan to use
+    bm_autotune=True,           # time the fused passes' window plans once per (device, n, dtype)
+    bm_autotune_min_n=12,       # (solver/bm_admm/autotune.py; cached in ~/.cache/rum-testing-gpu_amd)
 )
 
 # rum.bootstrap(...): Hong and Li's p-value and confidence interval (solver/inference).

@@ -71,3 +71,12 @@ def can_use_gurobi(prof):
 def gpu_free_bytes(device=0):
     """Free device memory now (hipMemGetInfo)."""
     return _lib.mem_info(device)[0]
+
+
+def gpu_name(device=0):
+    """'<name> <arch>' of the device (the autotuner's cache key), or 'gpu' when unknown"""
+    try:
+        pr = _lib.device_props(device)
+        return "%s %s" % (pr["name"], pr["arch"])
+    except Exception:                         # noqa: BLE001
+        return "gpu"

@@ -699,6 +699,11 @@ def _bm_handles(t):
         for d in dts:
             h = Handle(n, g.menu, g.alternative.astype(np.int32), t.p, dtype=d, beta=a["beta"], alpha=a["alpha"],
                        precond_p=a["precond_p"])
+            if defaults.AUTO["bm_autotune"] and n >= defaults.AUTO["bm_autotune_min_n"]:
+                from .bm_admm.autotune import tune
+                plan, _us = tune(h, device=_machine.gpu_name())
+                t.state["info"]["window_plan"] = list(plan)
+                h.init_cold()               # the timing runs moved the state
             if a["use_graph"]:
                 h.set_graph(int(a["check"]))
             items[d] = h

@@ -282,3 +282,41 @@ def test_set_beta_carries_the_state(graph):
         assert np.array_equal(sa[k], sc[k]), k
     for h in (a, b, c):
         h.close()
+
+
+@pytest.mark.parametrize("n,plan", [(13, (6, 2, 3)), (13, (12, 4, 4)), (18, (11, 3, 4)), (18, (8, 2, 4))])
+def test_window_plans_match_oracle(n, plan):
+    """any window plan (rumgpu_bm_set_plan) gives the reference iteration within rounding"""
+    rng = np.random.default_rng(80 + n)
+    menu, alt, p = _medium(n, min(6000, n << (n - 2)), rng)
+    h = Handle(n, menu, alt, p, dtype="float64", beta=1.1)
+    h.set_plan(*plan)
+    h.set_graph(4)
+    st = _oracle_state(n, menu, alt, p, 1.1, np.float64)
+    h.run(13)
+    st.run(13)
+    got = h.get_state()
+    for k in ("x2", "s", "y", "z"):
+        ref = getattr(st, k)
+        err = float(np.abs(got[k] - ref).max()) / max(1.0, float(np.abs(ref).max()))
+        assert err < 1e-11, (plan, k, err)
+    h.close()
+
+
+def test_autotuner_picks_a_valid_plan(tmp_path, monkeypatch):
+    from solver.bm_admm import autotune
+    monkeypatch.setattr(autotune, "CACHE", str(tmp_path / "plans.json"))
+    monkeypatch.setattr(autotune, "_MEM", {})
+    n = 14
+    rng = np.random.default_rng(5)
+    menu, alt, p = _medium(n, 3000, rng)
+    h = Handle(n, menu, alt, p, dtype="float64", beta=1.0)
+    plan, us = autotune.tune(h, device="test", iters=20, warm=5)
+    assert tuple(plan) in autotune.candidates(n - 1) and len(us) >= 2
+    assert (tmp_path / "plans.json").exists()
+    # a second object of the same n and dtype takes the cached plan without timing
+    h2 = Handle(n, menu, alt, p, dtype="float64", beta=1.0)
+    plan2, _ = autotune.tune(h2, device="test")
+    assert plan2 == plan
+    h.close()
+    h2.close()
