@@ -287,6 +287,13 @@ class BatchedCG:
             raise ValueError("batched sweeps need the mask pricing layout (the plan uses %s)" % sess._args["pricing"])
         self.boot, self.sess, self.D, self.verbose = boot, sess, int(slots), verbose
         self.overlap = bool(overlap) and self.D >= 2
+        from .. import machine as _machine
+        n = boot.rum.n
+        need = (32 * 5 + 8) * (1 << n) + 32 * 4 * boot.rum.M      # batched V, pred, float64 V, scores
+        budget = defaults.AUTO["gpu_memory_fraction"] * _machine.gpu_free_bytes()
+        if need > budget:
+            raise MemoryError("out of memory: batched pricing at n = %d needs %.1f GB, %.1f GB may be used"
+                              % (n, need / 2 ** 30, budget / 2 ** 30))
         self.stats = dict(slots=self.D, draws=0, rounds=0, sweeps=0, price_s=0.0, host_s=0.0, wall_s=0.0,
                           draw_rounds=0)
 

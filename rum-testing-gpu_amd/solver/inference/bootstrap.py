@@ -291,7 +291,11 @@ class Bootstrap:
         if len(idx) == 0:
             return
         if self.batch is not None:
-            return self._solve_batched(idx, abs_tol, rel_tol, stops, until)
+            try:
+                return self._solve_batched(idx, abs_tol, rel_tol, stops, until)
+            except MemoryError as e:           # the batch does not fit: one draw at a time
+                self._say("batched solves do not fit (%s); solving one draw at a time" % e)
+                self.batch = None
         t0, s = time.perf_counter(), self.session
         tau_lo, tau_hi = self.tau
         step = max(1, CHUNK_BYTES // (8 * self.draws.M))
