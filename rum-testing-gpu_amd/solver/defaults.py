@@ -35,7 +35,8 @@ METHOD_ARGS = {
     "flow_cg": dict(
         dtype="auto",           # pricing values: "float32", "uint24" or "uint16"; auto = the first that fits
         pricing="auto",         # "mask" (one value per subset), "slab" (butterfly arc weights) or "stream"
-                                # (two live layers); old names stream_f32/_u24/_u16 work; auto = mask where it fits
+                                # (two live layers); old names stream_f32/_u24/_u16 work; auto = slab for many
+                                # rows per alternative (AUTO["cg_slab_rows_per_alt"]), else mask where it fits
         layers="auto",          # accepted for compatibility: this build always stores the layer lists
         max_iters=4000,         # columns; also sizes the host Gram matrix
     ),
@@ -47,6 +48,8 @@ AUTO = dict(
     bm_admm_min_density={11: 0.02, 14: 0.01, 18: 0.005},
     exact_qp_max_n=10,
     gpu_memory_fraction=0.9,    # share of free GPU memory a method may plan to use
+    cg_slab_rows_per_alt=16,    # flow_cg pricing "auto": the slab form above this many rows per alternative
+                                # (n = 16: 0.19 vs 0.50 ms; n = 22 at 0.5 %: 0.78 ms vs 1.06 s for stream u24)
     bm_autotune=True,           # time the fused passes' window plans once per (device, n, dtype)
     bm_autotune_min_n=12,       # (solver/bm_admm/autotune.py; cached in ~/.cache/rum-testing-gpu_amd)
 )

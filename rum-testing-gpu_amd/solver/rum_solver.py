@@ -638,6 +638,10 @@ def _plan_flow_cg(n, mp, weighted, prof, a):
         raise ValueError("%s pricing stores float32 only; dtype=%r needs pricing='stream' or 'auto'" % (layout, dt))
     budget = defaults.AUTO["gpu_memory_fraction"] * _machine.gpu_free_bytes()
     engines = list(_CG_ENGINE) if layout != "slab" else [("slab", "float32")]
+    if layout == "auto" and dt in ("auto", "float32") and mp > defaults.AUTO["cg_slab_rows_per_alt"] * n:
+        # many rows: the row-order sweeps test every row at every subset (mp 2^(n-1) tests), the
+        # slab form reads each arc weight once from a subset-sum butterfly (16 n 2^(n-1) bytes)
+        engines.insert(0, ("slab", "float32"))
     tried = []
     for lay, d in engines:
         if layout not in ("auto", lay) or dt not in ("auto", d) or (lay in ("mask", "slab") and n > 31):

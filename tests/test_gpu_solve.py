@@ -30,7 +30,7 @@ def _config1_exact():
 
 
 @pytest.mark.parametrize("method,args", [("bm_admm", {"dtype": "float64"}), ("bm_admm", {"dtype": "mixed"}),
-                                         ("flow_cg", {}), ("flow_cg", {"pricing": "slab"})])
+                                         ("flow_cg", {"pricing": "mask"}), ("flow_cg", {})])
 def test_config1_exact_distance(method, args):
     cd, n = instances.config1()
     d, lb_exact = _config1_exact()

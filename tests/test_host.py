@@ -140,6 +140,13 @@ def test_planner_rules_and_beta():
         assert abs(a["beta"] - params.bm_beta_default(23568)) < 1e-15
         a = _plan_flow_cg(25, 301, False, prof, dict(defaults.METHOD_ARGS["flow_cg"]))
         assert (a["pricing"], a["dtype"]) == ("mask", "float32")
+        # many rows per alternative: the slab form (no per-subset row loop)
+        a = _plan_flow_cg(22, 230691, False, prof, dict(defaults.METHOD_ARGS["flow_cg"]))
+        assert (a["pricing"], a["dtype"]) == ("slab", "float32")
+        a = _plan_flow_cg(22, 230691, False, prof, dict(defaults.METHOD_ARGS["flow_cg"], dtype="uint24"))
+        assert (a["pricing"], a["dtype"]) == ("stream", "uint24")
+        a = _plan_flow_cg(16, 250, False, prof, dict(defaults.METHOD_ARGS["flow_cg"]))
+        assert a["pricing"] == "mask"
         a = _plan_flow_cg(25, 301, False, prof, dict(defaults.METHOD_ARGS["flow_cg"], pricing="stream_u16"))
         assert (a["pricing"], a["dtype"]) == ("stream", "uint16")
         with pytest.raises(ValueError, match="contradicts"):
