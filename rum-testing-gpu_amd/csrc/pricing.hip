@@ -120,12 +120,10 @@ __global__ __launch_bounds__(256) void cg_mask_layer_batch(float* __restrict__ V
         for (int d = 0; d < DB; ++d) w[d] += sj[d];
       }
     }
-    const u32 P = D ^ (1u << x);
-    const float* vp = V + (size_t)P * DB;
-    const bool zero = (P & zmask) == 0u;          // a row-free predecessor holds V = 0
+    const float* vp = V + (size_t)(D ^ (1u << x)) * DB;
 #pragma unroll
     for (int d = 0; d < DB; ++d) {
-      const float cand = (zero ? 0.0f : vp[d]) + w[d];
+      const float cand = vp[d] + w[d];
       if (!seen || cand > best[d]) {
         best[d] = cand;
         bx[d] = (uint8_t)x;
