@@ -87,7 +87,7 @@ KIND_NAMES = {0: "bm_pass SY (A window 1, s/y update, A' window 1)", 1: "bm_pass
 def main():
     a = _parse()
     if a.gpus != 1:
-        print("bench.py measures one GPU: this build runs on a single MI355X and has no multi-GPU path "
+        print("bench.py measures one GPU: the engines run on a single MI355X and have no multi-GPU path "
               "(--gpus %d requested)" % a.gpus, file=sys.stderr)
         sys.exit(2)
     import numpy as np

@@ -171,7 +171,7 @@ int rumgpu_qp_simplex(int K, const double* Q, int64_t ldq, const double* h, cons
 /* ---- Learned warm start: in -> hidden -> hidden -> out MLP (ReLU), bf16 matrix-core forward and
  * backward, Adam on a float32 master copy. No reference counterpart: it maps a bootstrap draw's
  * change of p on the observed cells to a correction of the ADMM state there (the warm start);
- * the float64 certificates still judge every draw. Parameters are one float32 vector: W1 (in x
+ * the float64 certificates still decide every draw. Parameters are one float32 vector: W1 (in x
  * hidden, row-major), b1, W2 (hidden x hidden), b2, W3 (hidden x out), b3. ---- */
 int rumgpu_mlp_create(rumgpu_ctx* ctx, int in_dim, int hidden, int out_dim, int max_batch, rumgpu_mlp** out);
 int rumgpu_mlp_destroy(rumgpu_mlp* mlp);

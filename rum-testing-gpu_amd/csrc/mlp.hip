@@ -1,7 +1,7 @@
 // Learned warm start: a three-layer MLP (in -> hidden -> hidden -> out, ReLU) trained and run on
 // the GPU with bf16 matrix cores (v_mfma_f32_16x16x16_bf16: bf16 operands, float32 accumulation).
 // No reference counterpart: it maps a bootstrap draw's change of p on the observed cells to a
-// correction of the ADMM state there; every draw it touches is still judged by its float64
+// correction of the ADMM state there; every draw it touches is still decided by its float64
 // certificates.
 //
 // Parameters live in one float32 vector (the master copy: W1 in x hidden row-major, b1, W2, b2,

@@ -37,7 +37,7 @@ METHOD_ARGS = {
         pricing="auto",         # "mask" (one value per subset), "slab" (butterfly arc weights) or "stream"
                                 # (two live layers); old names stream_f32/_u24/_u16 work; auto = slab for many
                                 # rows per alternative (AUTO["cg_slab_rows_per_alt"]), else mask where it fits
-        layers="auto",          # accepted for compatibility: this build always stores the layer lists
+        layers="auto",          # accepted for compatibility: the engine always stores the layer lists
         max_iters=4000,         # columns; also sizes the host Gram matrix
     ),
 }

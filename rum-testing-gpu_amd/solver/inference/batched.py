@@ -68,7 +68,7 @@ class BatchedSolver:
     slots   draws per launch (None: the autotuner's pick for this n)
     dtype   iterate dtype of the batch ("float64"; "float32" iterates still get float64 bounds)
     model   learned warm start: an object with correction(dp) -> (dx2, dz) on the non-singleton
-            rows (dp = P_b - p_hat there), or None; the certificate judges every draw either way
+            rows (dp = P_b - p_hat there), or None; the certificate decides every draw either way
     collect keep (dp, x2_obs - x2_hat_obs, z_obs - z_hat_obs, iterations) of every draw retired
             with a met tolerance (True) or of every retired draw ("all"), as training pairs for the
             warm-start model

@@ -3,7 +3,7 @@
 The reference detects an NVIDIA card with nvidia-smi and CuPy (solver/machine.py:25-61) and keeps
 a profile dict; this module keeps the same profile keys (use_gpu, gpu{name, memory_total,
 supported, reason}, cpu_count, ram_bytes, gurobi) and fills them from HIP through the C ABI.
-Gurobi is not part of this build (gurobi = None), so the CPU-only methods are unavailable.
+Gurobi is not part of this package (gurobi = None), so the CPU-only methods are unavailable.
 """
 import os
 import platform

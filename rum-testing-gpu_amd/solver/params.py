@@ -37,7 +37,7 @@ def bm_beta_cert(mp):
 # dual side). When one part exceeds `ratio` times the other, beta is multiplied (infeasibility) or
 # divided (optimality) by `factor`; the first reversal of direction takes a half step (sqrt of
 # factor) and freezes beta, and at most `max_changes` changes are made. Start: `start` times the
-# race rule. Measured on this build (abs gap 1e-5, float64, ten n = 18-20 instances of 0.1-3 %
+# race rule. Measured on MI355X (abs gap 1e-5, float64, ten n = 18-20 instances of 0.1-3 %
 # density): 37 % fewer iterations than beta="cert" in total, from -47 % (n18_dense, n18_m231,
 # n19_m102) to +2 % (n18_m8); without the cap on changes n18_sparse drifted to 4.5 times the race
 # rule and needed 12 times the iterations.

@@ -21,7 +21,7 @@ Every result brackets the distance: lb <= dist <= ub, distance = ub = the distan
                -4 interrupted   -5 solver error (message says which)
 
 Methods: bm_admm and flow_cg run here (GPU). The reference's Gurobi and Flow-ADMM methods
-(flow_admm, ip_cg, ip_cg_heur, bm_qp, flow_qp) are outside this build and raise when named.
+(flow_admm, ip_cg, ip_cg_heur, bm_qp, flow_qp) are outside this package and raise when named.
 """
 import math
 import time
@@ -355,7 +355,7 @@ class RUM:
         else:
             name, reason = _method_name(method), ""
         if name not in BUILT:
-            raise RuntimeError("%s is not part of this build (the MI355X engines are bm_admm and flow_cg; the "
+            raise RuntimeError("%s is not part of this package (the MI355X engines are bm_admm and flow_cg; the "
                                "reference's Gurobi and Flow-ADMM methods were left out)" % name)
         args = dict(defaults.METHOD_ARGS[name])
         bad = set(method_args or {}) - set(args)
@@ -520,7 +520,7 @@ def choose_method(n, mp, weighted, prof):
     density = mp / (n << (n - 1))
     why = ["n = %d, %d rows, density %.3g%%" % (n, mp, 100 * density)]
     if weighted:
-        why.append("weighted norm: only the Gurobi methods take vcov, and they are not part of this build")
+        why.append("weighted norm: only the Gurobi methods take vcov, and they are not part of this package")
     elif not _machine.can_use_gpu(prof):
         g = prof.get("gpu")
         why.append("no usable GPU (%s)" % ((g or {}).get("reason") or "none found"
