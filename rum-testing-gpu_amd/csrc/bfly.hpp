@@ -174,6 +174,9 @@ struct Tile {
     for (int j = 0; j < R; ++j) v[j] = lds[lidx(r, j)];
     cur = r;
   }
+  // FMA: fuse each output's second product and the sum (one rounding fewer); the plain passes
+  // keep separately rounded products, the oracle's arithmetic
+  template <bool FMA = false>
   __device__ __forceinline__ void stages(int r, const Coef<T>& k) {
     const int lo = lo_of(r);
     const int q0 = r * RL - lo;
@@ -186,8 +189,13 @@ struct Tile {
         for (int j = 0; j < R; ++j) {
           if (!(j & (1 << q))) {
             const T X = v[j], Y = v[j | (1 << q)];
-            v[j] = k.a * X + k.b * Y;
-            v[j | (1 << q)] = k.c * X + k.d * Y;
+            if constexpr (FMA) {
+              v[j] = __builtin_fma(k.a, X, k.b * Y);
+              v[j | (1 << q)] = __builtin_fma(k.c, X, k.d * Y);
+            } else {
+              v[j] = k.a * X + k.b * Y;
+              v[j | (1 << q)] = k.c * X + k.d * Y;
+            }
           }
         }
       }
@@ -201,12 +209,13 @@ struct Tile {
   }
   // The whole window transform; starts from the current layout and runs the rounds in the
   // direction that needs no exchange before the first one.
+  template <bool FMA = false>
   __device__ __forceinline__ void transform(const Coef<T>& k) {
     const bool fwd = (cur == 0);
     for (int i = 0; i < nr; ++i) {
       const int r = fwd ? i : nr - 1 - i;
       if (r != cur) exchange(r);
-      stages(r, k);
+      stages<FMA>(r, k);
     }
   }
   template <typename S>

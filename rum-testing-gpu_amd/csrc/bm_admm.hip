@@ -65,7 +65,7 @@ __global__ __launch_bounds__(512) void bm_pass(PassArgs<T> P) {
     t.load(P.tb);
   }
   if constexpr (KIND == K_SY || KIND == K_SYONLY || KIND == K_FULL) {
-    t.transform(P.cA);                                                      // Ax (raw A)
+    t.template transform<true>(P.cA);                                                      // Ax (raw A)
     t.coalesce();
     if (t.active) {
       const bool fresh = P.fresh && P.fresh[t.item / P.g.nper];
@@ -89,7 +89,7 @@ __global__ __launch_bounds__(512) void bm_pass(PassArgs<T> P) {
     if constexpr (KIND == K_SYONLY) return;
   }
   if constexpr (KIND == K_FIRST || KIND == K_SY || KIND == K_RHS || KIND == K_FULL || KIND == K_FULL_FIRST)
-    t.transform(P.cAp);                                                     // raw A'
+    t.template transform<true>(P.cAp);                                                     // raw A'
   if constexpr (KIND == K_RHS || KIND == K_FULL || KIND == K_FULL_FIRST) {
     t.coalesce();
     if (t.active) {
@@ -99,18 +99,18 @@ __global__ __launch_bounds__(512) void bm_pass(PassArgs<T> P) {
         t.v[j] = (P.z[i] + P.b2 * P.x2[i] - t.v[j]) * P.ib1;                // fuse_rhs
       }
     }
-    t.transform(P.cQt);
+    t.template transform<true>(P.cQt);
   }
-  if constexpr (KIND == K_D) t.transform(P.cQt);
+  if constexpr (KIND == K_D) t.template transform<true>(P.cQt);
   if constexpr (KIND == K_D || KIND == K_FULL || KIND == K_FULL_FIRST) {
     // inv_gl is one item block (2^(n-1) values, L2-resident): read it in any layout
     if (t.active) {
 #pragma unroll
       for (int j = 0; j < R; ++j) t.v[j] *= P.inv_gl[t.rest(j)];            // diag_mul
     }
-    t.transform(P.cQ);
+    t.template transform<true>(P.cQ);
   }
-  if constexpr (KIND == K_XK) t.transform(P.cQ);
+  if constexpr (KIND == K_XK) t.template transform<true>(P.cQ);
   if constexpr (KIND == K_XK || KIND == K_FULL || KIND == K_FULL_FIRST) {
     t.coalesce();
     if (t.active) {
@@ -121,8 +121,8 @@ __global__ __launch_bounds__(512) void bm_pass(PassArgs<T> P) {
       }
     }
   }
-  if constexpr (KIND == K_XK) t.transform(P.cA);
-  if constexpr (KIND == K_T) t.transform(P.cT);
+  if constexpr (KIND == K_XK) t.template transform<true>(P.cA);
+  if constexpr (KIND == K_T) t.template transform<true>(P.cT);
   t.coalesce();
   t.store(P.tb);
 }
