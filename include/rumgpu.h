@@ -167,6 +167,14 @@ int rumgpu_cg_info(rumgpu_cg* cg, size_t* device_bytes, double* last_kernel_ms);
  * Writes lam[K] and the support (weights > 1e-9) into support[K], its size into n_support. */
 int rumgpu_qp_simplex(int K, const double* Q, int64_t ldq, const double* h, const int32_t* A_init, int nA,
                       int max_outer, double tol, double reg, double* lam, int32_t* support, int* n_support);
+/* the same with a state that keeps the last working set and its Cholesky factor: a column
+ * generation whose next call starts from that working set (Q only grew) skips the refactoring */
+typedef struct rumgpu_qp rumgpu_qp;
+int rumgpu_qp_create(rumgpu_qp** out);
+int rumgpu_qp_destroy(rumgpu_qp* state);
+int rumgpu_qp_simplex_warm(rumgpu_qp* state, int K, const double* Q, int64_t ldq, const double* h,
+                           const int32_t* A_init, int nA, int max_outer, double tol, double reg, double* lam,
+                           int32_t* support, int* n_support);
 
 /* ---- Learned warm start: in -> hidden -> hidden -> out MLP (ReLU), bf16 matrix-core forward and
  * backward, Adam on a float32 master copy. No reference counterpart: it maps a bootstrap draw's

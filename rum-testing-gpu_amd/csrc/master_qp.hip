@@ -159,9 +159,45 @@ void project_simplex(std::vector<double>& lam) {
 
 }  // namespace
 
+// A column generation's master QPs in sequence: the last working set and its factor. Q only grows
+// (new columns append rows and columns; old entries stay), so a call that starts from that
+// working set reuses the factor instead of refactoring it.
+struct rumgpu_qp {
+  std::vector<int> work;
+  Chol ch;
+};
+
+extern "C" int rumgpu_qp_create(rumgpu_qp** out) {
+  RG_CHECK(out, RG_EINVAL, "qp_create: null argument");
+  *out = new rumgpu_qp();
+  return RG_OK;
+}
+
+extern "C" int rumgpu_qp_destroy(rumgpu_qp* q) {
+  delete q;
+  return RG_OK;
+}
+
+static int qp_simplex_impl(int K, const double* Q, int64_t ldq, const double* h, const int32_t* A_init, int nA,
+                           int max_outer, double tol, double reg, double* lam_out, int32_t* support, int* n_support,
+                           rumgpu_qp* state);
+
 extern "C" int rumgpu_qp_simplex(int K, const double* Q, int64_t ldq, const double* h, const int32_t* A_init, int nA,
                                  int max_outer, double tol, double reg, double* lam_out, int32_t* support,
                                  int* n_support) {
+  return qp_simplex_impl(K, Q, ldq, h, A_init, nA, max_outer, tol, reg, lam_out, support, n_support, nullptr);
+}
+
+extern "C" int rumgpu_qp_simplex_warm(rumgpu_qp* state, int K, const double* Q, int64_t ldq, const double* h,
+                                      const int32_t* A_init, int nA, int max_outer, double tol, double reg,
+                                      double* lam_out, int32_t* support, int* n_support) {
+  RG_CHECK(state, RG_EINVAL, "qp_simplex_warm: null state");
+  return qp_simplex_impl(K, Q, ldq, h, A_init, nA, max_outer, tol, reg, lam_out, support, n_support, state);
+}
+
+static int qp_simplex_impl(int K, const double* Q, int64_t ldq, const double* h, const int32_t* A_init, int nA,
+                           int max_outer, double tol, double reg, double* lam_out, int32_t* support, int* n_support,
+                           rumgpu_qp* state) {
   RG_CHECK(K >= 1 && Q && h && lam_out && support && n_support && ldq >= K, RG_EINVAL, "qp_simplex: bad argument");
   std::vector<int> work;
   std::vector<char> in(K, 0);
@@ -186,7 +222,11 @@ extern "C" int rumgpu_qp_simplex(int K, const double* Q, int64_t ldq, const doub
       for (int j = 0; j < k; ++j) a[(size_t)i * k + j] = Q[(int64_t)work[i] * ldq + work[j]] + (i == j ? reg : 0.0);
     return chol_full(ch, a, k);
   };
-  rebuild();
+  if (state && state->ch.ok && state->work == work && state->ch.k == (int)work.size()) {
+    ch = std::move(state->ch);                 // the factor of this working set, from the last call
+  } else {
+    rebuild();
+  }
   double mu = 0.0;
   for (int outer = 0; outer < max_outer; ++outer) {
     const int k = (int)work.size();
@@ -236,6 +276,11 @@ extern "C" int rumgpu_qp_simplex(int K, const double* Q, int64_t ldq, const doub
       }
     }
     break;
+  }
+  if (state) {
+    state->work = work;
+    if (ch.ok && ch.k == (int)work.size()) state->ch = std::move(ch);
+    else state->ch.ok = false;
   }
   std::vector<double> lam(K, 0.0);
   for (size_t i = 0; i < solved.size(); ++i) lam[solved[i]] = lam_w[i];

@@ -64,6 +64,9 @@ SIGNATURES = {
     "rumgpu_cg_price_batch": [_P, _I, _P, _P, _P],
     "rumgpu_cg_vmax64": [_P, _P, _P],
     "rumgpu_qp_simplex": [_I, _P, _I64, _P, _P, _I, _I, _D, _D, _P, _P, _P],
+    "rumgpu_qp_create": [_P],
+    "rumgpu_qp_destroy": [_P],
+    "rumgpu_qp_simplex_warm": [_P, _I, _P, _I64, _P, _P, _I, _I, _D, _D, _P, _P, _P],
     "rumgpu_mlp_create": [_P, _I, _I, _I, _I, _P],
     "rumgpu_mlp_destroy": [_P],
     "rumgpu_mlp_param_count": [_P, _P],

@@ -31,8 +31,31 @@ def project_simplex(v):
     return np.maximum(v - cs[r] / (r + 1), 0.0)
 
 
-def solve_qp_native(Q, h, A_init=None, max_outer=200, tol=1e-10, reg=1e-12):
-    """Active-set solve of the simplex QP in librumgpu; returns (lam, support list)."""
+class QPState:
+    """The last working set of a sequence of master QPs and its Cholesky factor (rumgpu_qp), for a
+    column generation whose master only grows."""
+
+    def __init__(self):
+        self.lib = _lib.load()
+        h = ctypes.c_void_p()
+        _lib.check(self.lib.rumgpu_qp_create(ctypes.byref(h)), "qp_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.lib.rumgpu_qp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def solve_qp_native(Q, h, A_init=None, max_outer=200, tol=1e-10, reg=1e-12, state=None):
+    """Active-set solve of the simplex QP in librumgpu; returns (lam, support list). With a
+    QPState the factor of a working set equal to the last call's is reused."""
     K = int(h.size)
     Q = np.asarray(Q, dtype=np.float64)
     if Q.strides[1] != 8 or Q.shape != (K, K):
@@ -44,9 +67,14 @@ def solve_qp_native(Q, h, A_init=None, max_outer=200, tol=1e-10, reg=1e-12):
     sup = np.empty(K, dtype=np.int32)
     ns = ctypes.c_int()
     lib = _lib.load()
-    _lib.check(lib.rumgpu_qp_simplex(K, ctypes.c_void_p(Q.ctypes.data), ldq, _lib.ptr(h), _lib.ptr(A), int(A.size),
-                                     int(max_outer), float(tol), float(reg), _lib.ptr(lam), _lib.ptr(sup),
-                                     ctypes.byref(ns)), "qp_simplex")
+    if state is None:
+        rc = lib.rumgpu_qp_simplex(K, ctypes.c_void_p(Q.ctypes.data), ldq, _lib.ptr(h), _lib.ptr(A), int(A.size),
+                                   int(max_outer), float(tol), float(reg), _lib.ptr(lam), _lib.ptr(sup), ctypes.byref(ns))
+    else:
+        rc = lib.rumgpu_qp_simplex_warm(state.h, K, ctypes.c_void_p(Q.ctypes.data), ldq, _lib.ptr(h), _lib.ptr(A),
+                                        int(A.size), int(max_outer), float(tol), float(reg), _lib.ptr(lam),
+                                        _lib.ptr(sup), ctypes.byref(ns))
+    _lib.check(rc, "qp_simplex")
     return lam, [int(i) for i in sup[:ns.value]]
 
 

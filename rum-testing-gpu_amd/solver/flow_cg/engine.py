@@ -15,7 +15,7 @@ import time
 import numpy as np
 
 from .. import _lib
-from ..common.qp_native import solve_qp_native, solve_qp_numpy
+from ..common.qp_native import QPState, solve_qp_native, solve_qp_numpy
 
 LAYOUTS = {"mask": 0, "slab": 1, "stream": 2}
 DTYPES = {"float32": 0, "uint16": 1, "uint24": 2}
@@ -100,24 +100,50 @@ class Pricer:
 
 
 class Columns:
-    """Ranking columns as packed bitsets with the Gram matrix Q, h = V'p, and p_out = V lam."""
+    """Ranking columns as packed bitsets with the Gram matrix Q, h = V'p, and p_out = V lam.
 
-    def __init__(self, M, p, max_K):
+    Storage grows by doubling up to max_K columns. With few rows (M <= DENSE_ROWS) the columns are
+    also kept as a dense float64 matrix, so p_out and h of a template are matrix products."""
+
+    DENSE_ROWS = 4096
+
+    def __init__(self, M, p, max_K, start=64):
         self.M, self.W = int(M), (int(M) + 63) // 64
         self.p = np.asarray(p, dtype=np.float64)
         self.max_K = int(max_K)
-        self.bits = np.zeros((self.max_K, self.W), dtype=np.uint64)
-        self.Q = np.zeros((self.max_K, self.max_K))
-        self.h = np.zeros(self.max_K)
+        self.cap = 0
+        self.bits = np.zeros((0, self.W), dtype=np.uint64)
+        self.Q = np.zeros((0, 0))
+        self.h = np.zeros(0)
+        self.C = None
+        self.dense = self.M <= self.DENSE_ROWS
+        self._grow(min(self.max_K, max(1, int(start))))
         self.pos = []
         self.keys = []
         self.K = 0
         self.p_out = np.zeros(self.M)
 
+    def _grow(self, cap):
+        cap = min(self.max_K, cap)
+        k = self.cap
+        bits = np.zeros((cap, self.W), dtype=np.uint64)
+        Q = np.zeros((cap, cap))
+        h = np.zeros(cap)
+        bits[:k], Q[:k, :k], h[:k] = self.bits[:k], self.Q[:k, :k], self.h[:k]
+        self.bits, self.Q, self.h = bits, Q, h
+        if self.dense:
+            C = np.zeros((cap, self.M))
+            if self.C is not None:
+                C[:k] = self.C[:k]
+            self.C = C
+        self.cap = cap
+
     def add(self, col):
         k = self.K
         if k >= self.max_K:
             raise RuntimeError("max_K = %d columns exceeded" % self.max_K)
+        if k >= self.cap:
+            self._grow(2 * self.cap)
         padded = np.zeros(self.W * 64, dtype=np.uint8)
         padded[:self.M] = col
         b = np.packbits(padded, bitorder="little").view(np.uint64)
@@ -129,25 +155,36 @@ class Columns:
         self.Q[k, k] = float(np.bitwise_count(b).sum(dtype=np.int64))
         nz = np.flatnonzero(col)
         self.h[k] = float(self.p[nz].sum())
+        if self.dense:
+            self.C[k] = col
         self.pos.append(nz)
         self.keys.append(bytes(np.asarray(col, dtype=np.uint8)))
         self.K += 1
 
     def load_template(self, tpl):
         """the columns of a template (bits, Gram, positions, keys of the same rows) as the first
-        columns; h from this object's p, column by column as add() computes it"""
+        columns; h from this object's p"""
         k = tpl.K
         if k > self.max_K:
             raise RuntimeError("max_K = %d columns exceeded" % self.max_K)
+        if k > self.cap:
+            self._grow(max(k + 64, 2 * self.cap))
         self.bits[:k] = tpl.bits[:k]
         self.Q[:k, :k] = tpl.Q[:k, :k]
-        for i in range(k):
-            self.h[i] = float(self.p[tpl.pos[i]].sum())
+        if self.dense and tpl.dense:
+            self.C[:k] = tpl.C[:k]
+            self.h[:k] = tpl.C[:k] @ self.p
+        else:
+            for i in range(k):
+                self.h[i] = float(self.p[tpl.pos[i]].sum())
         self.pos = list(tpl.pos)
         self.keys = list(tpl.keys)
         self.K = k
 
     def apply_delta(self, dlam):
+        if self.dense:
+            self.p_out += dlam[:self.K] @ self.C[:self.K]
+            return
         for a in np.flatnonzero(np.abs(dlam) > 1e-12):
             self.p_out[self.pos[int(a)]] += float(dlam[a])
 
@@ -219,7 +256,9 @@ def cg_steps(n, menu, alt, p, pricer, rc_tol=0.0, max_iters=4000, time_limit_s=0
     (vmax, rank_pos, margin), returns the result dict. A driver may price many of these loops in
     one batched sweep (inference/batched.py); each loop's arithmetic is the same either way."""
     t_setup = time.perf_counter()
-    qp_solve = {"native": solve_qp_native, "numpy": solve_qp_numpy}[qp]
+    qp_state = QPState() if qp == "native" else None
+    qp_solve = {"native": lambda Q, h, A_init: solve_qp_native(Q, h, A_init=A_init, state=qp_state),
+                "numpy": solve_qp_numpy}[qp]
     perm, masks_x, alt_x, offsets = xmajor(menu, alt, n)
     p32 = np.asarray(p, dtype=np.float64)[perm].astype(np.float32)
     p_obs = p32.astype(np.float64)

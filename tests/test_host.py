@@ -240,3 +240,24 @@ def test_adaptive_penalty_rule():
     assert log[0] == 2.0 and abs(log[1] - 2.0 / np.sqrt(2.0)) < 1e-15 and len(log) == 2
     balanced = lambda k: (0.6 + 1.0 / k, 0.5, 0.4 - 1.0 / k)         # noqa: E731
     assert drive(balanced) == []
+
+
+def test_master_qp_state_sequence():
+    """a growing master solved with a QPState (its last working set's factor reused) reaches the
+    optimum of the stateless solve at every step"""
+    from solver.common.qp_native import QPState, solve_qp_native
+    rng = np.random.default_rng(11)
+    M = 60
+    C = (rng.random((40, M)) < 0.35).astype(np.float64)
+    p = rng.random(M)
+    st = QPState()
+    A1 = A2 = None
+    for K in range(3, 41):
+        Q, h = C[:K] @ C[:K].T, C[:K] @ p
+        Qbig = np.zeros((64, 64))
+        Qbig[:K, :K] = Q                             # a view with a wider row stride, as the engine has
+        lam1, A1 = solve_qp_native(Qbig[:K, :K], h, A_init=A1, state=st)
+        lam2, A2 = solve_qp_native(Q, h, A_init=A2)
+        f = lambda l: 0.5 * l @ Q @ l - h @ l      # noqa: E731
+        assert abs(f(lam1) - f(lam2)) <= 1e-10 * max(1.0, abs(f(lam2))), K
+    st.close()
