@@ -4,8 +4,8 @@
 //   (Q_WW + reg I, Cholesky); drop the most negative weight, else add the inactive column with
 //   the most negative gradient (below -tol), else stop; then a Euclidean simplex projection and
 //   support = weights above 1e-9.
-// The Cholesky factor of the working set is extended by one row when a column enters (O(k^2)) and
-// rebuilt when one leaves. Where it breaks down (a non-positive pivot of a nearly dependent set),
+// The Cholesky factor of the working set is extended by one row when a column enters and downdated
+// when one leaves (both O(k^2)). Where it breaks down (a non-positive pivot of a nearly dependent set),
 // that step takes the minimum-norm least-squares solution from a symmetric eigendecomposition
 // (cyclic Jacobi), as the reference falls back to numpy's lstsq.
 #include "rumgpu_internal.hpp"
@@ -66,6 +66,33 @@ bool chol_append(Chol& c, const double* q, double d0) {
   return true;
 }
 
+// drop row and column i of the factored matrix: the trailing block takes the rank-1 update by
+// the dropped column (the standard stable rotation form, rotation p acting on column p of the rows
+// below it), then rows and columns after i move up. The rotations run row by row, so every access
+// walks a row; each element still takes rotations 0, 1, ... in order.
+void chol_delete(Chol& c, int i) {
+  const int k = c.k, m = k - i - 1, cap = c.cap;
+  std::vector<double> cs(m > 0 ? m : 0), sn(m > 0 ? m : 0);
+  for (int q = 0; q < m; ++q) {
+    double* row = &c.L[(size_t)(i + 1 + q) * cap + (i + 1)];
+    double xq = row[-1];
+    for (int p = 0; p < q; ++p) {
+      row[p] = (row[p] + sn[p] * xq) / cs[p];
+      xq = cs[p] * xq - sn[p] * row[p];
+    }
+    const double d = row[q], r = std::sqrt(d * d + xq * xq);
+    cs[q] = r / d, sn[q] = xq / d;
+    row[q] = r;
+  }
+  for (int r = i; r < k - 1; ++r) {
+    double* dst = &c.L[(size_t)r * cap];
+    const double* src = &c.L[(size_t)(r + 1) * cap];
+    for (int col = 0; col < i; ++col) dst[col] = src[col];
+    for (int col = i; col <= r; ++col) dst[col] = src[col + 1];
+  }
+  c.k = k - 1;
+}
+
 // x = (L L')^-1 b
 void chol_solve(const Chol& c, const double* b, double* x) {
   const int k = c.k;
@@ -74,10 +101,12 @@ void chol_solve(const Chol& c, const double* b, double* x) {
     for (int t = 0; t < i; ++t) s -= c.L[(size_t)i * c.cap + t] * x[t];
     x[i] = s / c.L[(size_t)i * c.cap + i];
   }
+  // L' x = y by rows of L: x_i is final once rows below have been applied
   for (int i = k - 1; i >= 0; --i) {
-    double s = x[i];
-    for (int t = i + 1; t < k; ++t) s -= c.L[(size_t)t * c.cap + i] * x[t];
-    x[i] = s / c.L[(size_t)i * c.cap + i];
+    const double* row = &c.L[(size_t)i * c.cap];
+    const double xi = x[i] / row[i];
+    x[i] = xi;
+    for (int t = 0; t < i; ++t) x[t] -= row[t] * xi;
   }
 }
 
@@ -222,7 +251,26 @@ static int qp_simplex_impl(int K, const double* Q, int64_t ldq, const double* h,
       for (int j = 0; j < k; ++j) a[(size_t)i * k + j] = Q[(int64_t)work[i] * ldq + work[j]] + (i == j ? reg : 0.0);
     return chol_full(ch, a, k);
   };
-  if (state && state->ch.ok && state->work == work && state->ch.k == (int)work.size()) {
+  // With a state whose last working set holds every warm-start column (the usual case: the
+  // warm start is that set's support), start from the whole last set and its factor; its other
+  // members leave again at the first negative weight.
+  bool reuse = false;
+  if (state && state->ch.ok && state->ch.k == (int)state->work.size() && !state->work.empty()) {
+    reuse = true;
+    std::vector<char> inw(K, 0);
+    for (int w : state->work) {
+      if (w < 0 || w >= K) { reuse = false; break; }
+      inw[w] = 1;
+    }
+    for (size_t i = 0; reuse && i < work.size(); ++i)
+      if (!inw[work[i]]) reuse = false;
+    if (reuse) {
+      work = state->work;
+      std::fill(in.begin(), in.end(), 0);
+      for (int w : work) in[w] = 1;
+    }
+  }
+  if (reuse) {
     ch = std::move(state->ch);                 // the factor of this working set, from the last call
   } else {
     rebuild();
@@ -253,7 +301,8 @@ static int qp_simplex_impl(int K, const double* Q, int64_t ldq, const double* h,
     if (k && lam_w[imin] < -tol) {
       in[work[imin]] = 0;
       work.erase(work.begin() + imin);
-      rebuild();
+      if (ch.ok && ch.k == k) chol_delete(ch, imin);
+      else rebuild();
       continue;
     }
     if (K > k) {
