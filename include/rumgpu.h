@@ -160,6 +160,16 @@ int rumgpu_cg_price_batch(rumgpu_cg* cg, int D, const float* scores, double* vma
  * transforms.py:117-129) */
 int rumgpu_cg_vmax64(rumgpu_cg* cg, const double* r_x, double* vmax);
 int rumgpu_cg_info(rumgpu_cg* cg, size_t* device_bytes, double* last_kernel_ms);
+/* How the mask layout sweeps (rumgpu_cg_price, _price_batch, _vmax64): mode 0 (the default) =
+ * layer by layer for any rows, in the reference's order (bf_update, flow_cg/bf_pricing.py:7-47);
+ * mode 1 = when every row belongs to one alternative a (a panel of one alternative's choice
+ * shares), the same V[full] and ranking from a max over W[T] = a's row-order float32 sums on the
+ * 2^(n-1) subsets T of the other alternatives, with one bit per node and draw closed under
+ * supersets for the path; 4 bytes per node instead of V and pred for 2^n nodes. No reference
+ * counterpart: the reference's sweep does not special-case one alternative. sweep_info reports the
+ * alternative the closure sweep uses, or -1 when the layered sweep runs. */
+int rumgpu_cg_set_sweep(rumgpu_cg* cg, int mode);
+int rumgpu_cg_sweep_info(rumgpu_cg* cg, int* one_alternative);
 
 /* Flow-CG restricted-master QP on the host: min 1/2 lam'Q lam - h'lam, lam >= 0, sum lam = 1, by
  * the active set of solve_qp_native (common/qp_native.py:37-100) with its warm working set

@@ -185,6 +185,245 @@ __global__ void cg_backtrace_batch(const uint8_t* __restrict__ pred, const float
   }
 }
 
+// ---------------------------------------------------------------------------------------------
+// mask layout when every row belongs to one alternative a (a panel of one alternative's choice
+// shares). Every arc of another alternative weighs exactly 0, and V of a node without a is 0, so
+// for D containing a, with T = compress(D minus a, a) on the other n-1 alternatives,
+//   V[D] = max(0 + W[T], max over x in D, x != a, of V[D minus x] + 0) = max over T' subset of T of W[T'],
+// where W[T] = w(a, D) is the float32 sum, in row order, of a's rows whose menu lies in D. A max
+// returns one of its operands and adding 0.0f to a value that is never -0 returns it unchanged,
+// so these are the layered sweep's values bit for bit. Along the path from the full set V stays
+// at vmax = max over all T of W[T] until a is chosen, so the choice at D is the first member x
+// (ascending) with: x = a and W[T] = vmax, or x != a and some T' subset of T minus x has
+// W[T'] = vmax. The sweep therefore keeps one bit per node and draw, "W[T] = vmax", closed
+// under supersets by OR, instead of V and pred for every node: one pass computes W with a max
+// per tile, a second settles the global max, the closure runs in windows, and a thread per draw
+// walks the path.
+#define CL_TILE 256
+#define CG_BATCH 32
+
+// dynamic LDS: the rows' scores, [row][DB] float32 (a chunk of at most rows_lds rows)
+template <int DB>
+__global__ __launch_bounds__(256) void cl_wmax(const u32* __restrict__ cm, const float* __restrict__ sc, int ra,
+                                               int rows_lds, u32 NT, int tpb, u32* __restrict__ F,
+                                               float* __restrict__ tmax, float* __restrict__ bmax) {
+  extern __shared__ float s_sc[];
+  __shared__ float red[4][DB];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const bool one_chunk = ra <= rows_lds;
+  if (one_chunk) {
+    for (int i = tid; i < ra * DB; i += 256) s_sc[i] = sc[i];
+    __syncthreads();
+  }
+  float brun = -INFINITY;
+  for (int it = 0; it < tpb; ++it) {
+    const u32 tile = (u32)blockIdx.x * (u32)tpb + (u32)it;
+    if ((u64)tile * CL_TILE >= NT) break;
+    const u32 T = tile * CL_TILE + (u32)tid;
+    const bool valid = T < NT;
+    float w[DB];
+#pragma unroll
+    for (int d = 0; d < DB; ++d) w[d] = 0.0f;
+    for (int base = 0; base < ra; base += rows_lds) {
+      const int cnt = min(rows_lds, ra - base);
+      if (!one_chunk) {
+        __syncthreads();
+        for (int i = tid; i < cnt * DB; i += 256) s_sc[i] = sc[(size_t)base * DB + i];
+        __syncthreads();
+      }
+      for (int sub = 0; sub < cnt; sub += 32) {
+        const int c32 = min(32, cnt - sub);
+        u32 mm = 0;
+        for (int t = 0; t < c32; ++t) {
+          const u32 m = cm[base + sub + t];
+          mm |= ((m & T) == m ? 1u : 0u) << t;
+        }
+        // each lane adds its own matches, ascending rows
+        while (mm) {
+          const int t = __ffs((int)mm) - 1;
+          mm &= mm - 1u;
+          const float* sj = s_sc + (size_t)(sub + t) * DB;
+#pragma unroll
+          for (int d = 0; d < DB; ++d) w[d] += sj[d];
+        }
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < DB; ++d) {
+      float v = valid ? w[d] : -INFINITY;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off));
+      if (lane == 0) red[wv][d] = v;
+    }
+    __syncthreads();
+    u32 bits = 0;
+#pragma unroll
+    for (int d = 0; d < DB; ++d) {
+      const float tm = fmaxf(fmaxf(red[0][d], red[1][d]), fmaxf(red[2][d], red[3][d]));
+      bits |= (valid && w[d] == tm ? 1u : 0u) << d;
+    }
+    if (valid) F[T] = bits;
+    if (tid < DB) {
+      const float tm = fmaxf(fmaxf(red[0][tid], red[1][tid]), fmaxf(red[2][tid], red[3][tid]));
+      tmax[(size_t)tile * DB + tid] = tm;
+      brun = fmaxf(brun, tm);
+    }
+    __syncthreads();
+  }
+  if (tid < DB) bmax[(size_t)blockIdx.x * DB + tid] = brun;
+}
+
+// float64 max over T of W64[T] (the lower bound's longest path in this case): no choices, so no bits
+__global__ __launch_bounds__(256) void cl_wmax64(const u32* __restrict__ cm, const double* __restrict__ r, int ra,
+                                                 int rows_lds, u32 NT, int tpb, double* __restrict__ bmax) {
+  extern __shared__ double s_r[];
+  __shared__ double red[4];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const bool one_chunk = ra <= rows_lds;
+  if (one_chunk) {
+    for (int i = tid; i < ra; i += 256) s_r[i] = r[i];
+    __syncthreads();
+  }
+  double run = -INFINITY;
+  for (int it = 0; it < tpb; ++it) {
+    const u32 tile = (u32)blockIdx.x * (u32)tpb + (u32)it;
+    if ((u64)tile * CL_TILE >= NT) break;
+    const u32 T = tile * CL_TILE + (u32)tid;
+    double w = 0.0;
+    for (int base = 0; base < ra; base += rows_lds) {
+      const int cnt = min(rows_lds, ra - base);
+      if (!one_chunk) {
+        __syncthreads();
+        for (int i = tid; i < cnt; i += 256) s_r[i] = r[base + i];
+        __syncthreads();
+      }
+      for (int sub = 0; sub < cnt; sub += 32) {
+        const int c32 = min(32, cnt - sub);
+        u32 mm = 0;
+        for (int t = 0; t < c32; ++t) {
+          const u32 m = cm[base + sub + t];
+          mm |= ((m & T) == m ? 1u : 0u) << t;
+        }
+        while (mm) {
+          const int t = __ffs((int)mm) - 1;
+          mm &= mm - 1u;
+          w += s_r[sub + t];
+        }
+      }
+    }
+    if (T < NT) run = fmax(run, w);
+  }
+  double v = run;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+  if (lane == 0) red[wv] = v;
+  __syncthreads();
+  if (tid == 0) bmax[blockIdx.x] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+}
+
+// vmax[d] = max over the blocks' maxima (one block)
+template <typename T>
+__global__ __launch_bounds__(256) void cl_gmax(const T* __restrict__ bmax, int nb, int DB, T* __restrict__ vmax) {
+  __shared__ T red[256];
+  const int tid = threadIdx.x, G = 256 / DB, d = tid % DB, g = tid / DB;
+  T v = -INFINITY;
+  if (g < G)
+    for (int b = g; b < nb; b += G) v = v > bmax[(size_t)b * DB + d] ? v : bmax[(size_t)b * DB + d];
+  red[tid] = v;
+  __syncthreads();
+  if (tid < DB) {
+    T m = red[tid];
+    for (int q = 1; q < G; ++q) m = m > red[q * DB + tid] ? m : red[q * DB + tid];
+    vmax[tid] = m;
+  }
+}
+
+// first closure window: 2^w consecutive nodes per block. A tile's bit for draw d stands only if
+// the tile's max is the global max; then F[T] |= F[T minus b] for the window's bits b.
+__global__ __launch_bounds__(256) void cl_close_lo(u32* __restrict__ F, const float* __restrict__ tmax,
+                                                   const float* __restrict__ vmax, int DB, u32 NT, int w) {
+  __shared__ u32 s[4096];
+  __shared__ u32 keep[16];
+  const int tid = threadIdx.x;
+  const u32 span = 1u << w, base = (u32)blockIdx.x * span;
+  const u32 ntile = (span + CL_TILE - 1) / CL_TILE;
+  if (tid < (int)ntile) {
+    const u32 tile = base / CL_TILE + (u32)tid;
+    u32 k = 0;
+    for (int d = 0; d < DB; ++d) k |= (tmax[(size_t)tile * DB + d] == vmax[d] ? 1u : 0u) << d;
+    keep[tid] = k;
+  }
+  __syncthreads();
+  for (u32 e = tid; e < span; e += 256) s[e] = F[base + e] & keep[e / CL_TILE];
+  for (int b = 0; b < w; ++b) {
+    __syncthreads();
+    for (u32 e = tid; e < span; e += 256)
+      if ((e >> b) & 1u) s[e] |= s[e ^ (1u << b)];
+  }
+  __syncthreads();
+  for (u32 e = tid; e < span; e += 256) F[base + e] = s[e];
+}
+
+// later windows: bits b0 .. b0+w-1, a block holds 2^w x 32 nodes (32 consecutive low indexes,
+// so every load and store is a 128-byte run)
+__global__ __launch_bounds__(256) void cl_close_hi(u32* __restrict__ F, u32 NT, int b0, int w) {
+  __shared__ u32 s[256 * 32];
+  const int tid = threadIdx.x;
+  const u32 L = 32, groups = (1u << b0) / L;
+  const u32 g = (u32)blockIdx.x % groups, hi = (u32)blockIdx.x / groups;
+  const u32 origin = g * L + (hi << (b0 + w));
+  const u32 cnt = (1u << w) * L;
+  for (u32 e = tid; e < cnt; e += 256) s[e] = F[origin + (e % L) + ((e / L) << b0)];
+  for (int b = 0; b < w; ++b) {
+    __syncthreads();
+    for (u32 e = tid; e < cnt; e += 256)
+      if (((e / L) >> b) & 1u) s[e] |= s[e ^ (L << b)];
+  }
+  __syncthreads();
+  for (u32 e = tid; e < cnt; e += 256) F[origin + (e % L) + ((e / L) << b0)] = s[e];
+}
+
+// one thread per draw: the path from the full set. W[T] of the node itself (for x = a) is summed
+// again here, in the same row order. err is set if no member qualifies (it cannot when the sums
+// match the sweep's).
+__global__ void cl_backtrace(const u32* __restrict__ F, const u32* __restrict__ cm, const float* __restrict__ sc,
+                             int ra, int DB, int nd, int n, int a, const float* __restrict__ vmax,
+                             int32_t* __restrict__ rank_pos, int* __restrict__ err) {
+  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= nd) return;
+  const float vm = vmax[d];
+  u32 D = (n >= 32) ? 0xffffffffu : ((1u << n) - 1u);
+  u32 R = (1u << (n - 1)) - 1u;
+  bool has_a = true;
+  for (int i = 0; i < n; ++i) {
+    int x = -1;
+    if (has_a) {
+      for (int y = 0; y < n && x < 0; ++y) {
+        if (!((D >> y) & 1u)) continue;
+        if (y == a) {
+          float wv = 0.0f;
+          for (int j = 0; j < ra; ++j)
+            if ((cm[j] & R) == cm[j]) wv += sc[(size_t)j * DB + d];
+          if (wv == vm) x = y;
+        } else {
+          const int cb = y < a ? y : y - 1;
+          if ((F[R & ~(1u << cb)] >> d) & 1u) x = y;
+        }
+      }
+      if (x < 0) {
+        atomicOr(err, 1);
+        x = a;
+      }
+    } else {
+      x = __ffs((int)D) - 1;
+    }
+    rank_pos[(size_t)d * n + x] = i;
+    D &= ~(1u << x);
+    if (x == a) has_a = false;
+    else if (has_a) R &= ~(1u << (x < a ? x : x - 1));
+  }
+}
+
 // slab layout: scatter of the scores, then one layer of the max over members
 __global__ void cg_scatter(float* __restrict__ g, const i64* __restrict__ pos, const float* __restrict__ scores, i64 M) {
   const i64 j = (i64)blockIdx.x * blockDim.x + threadIdx.x;
@@ -427,6 +666,20 @@ struct rumgpu_cg {
   i64 M = 0, NN = 0, Nloc = 0;
   std::vector<u64> h_masks, h_offsets;
   u32 zmask = 0;             // alternatives with at least one row (mask layout, n <= 31)
+  // one-alternative sweep (mask layout, every row belongs to alternative single_a)
+  int sweep = 0;             // 0: layer by layer for any rows; 1: the closure sweep when it applies
+  int single_a = -1;
+  int ra = 0;                // rows of single_a
+  u32* clcm = nullptr;       // their menus without a, compressed to the other n-1 bits
+  u32* clF = nullptr;        // per node T: bit d = "W[T] = vmax" for draw d, then closed under supersets
+  float* cltmax = nullptr;   // per tile of 256 nodes and draw: the tile's max of W
+  float* clbmax = nullptr;   // per block and draw
+  float* clvmax = nullptr;
+  double* clbmax64 = nullptr;
+  double* clv64 = nullptr;
+  int* clerr = nullptr;
+  u32 clNT = 0;
+  int cl_grid = 0, cl_tpb = 0;
   std::vector<i64> bounds;
   // device rows
   u32* masks32 = nullptr;
@@ -475,7 +728,8 @@ static void cg_free_all(rumgpu_cg* c) {
   void* bufs[] = {c->masks32, c->masks64, c->offsets, c->scores, c->spos, c->V, c->pred, c->nodes, c->R, c->rank,
                   c->binom, c->vbuf[0], c->vbuf[1], c->lbuf[0], c->lbuf[1], c->mbuf[0], c->mbuf[1], c->ebuf[0],
                   c->ebuf[1], c->lstep, c->subV, c->subpred, c->subnodes, c->submasks, c->subscores, c->suboffsets,
-                  c->Vb, c->predb, c->scoresb, c->vmaxb, c->rankb, c->V64, c->r64};
+                  c->Vb, c->predb, c->scoresb, c->vmaxb, c->rankb, c->V64, c->r64, c->clcm, c->clF, c->cltmax,
+                  c->clbmax, c->clvmax, c->clbmax64, c->clv64, c->clerr};
   for (void* q : bufs) rg_free(q);
   for (auto* p : c->predk) rg_free(p);
 }
@@ -539,6 +793,10 @@ extern "C" int rumgpu_cg_create(rumgpu_ctx* ctx, int n, const uint64_t* masks_x,
   c->h_offsets.assign(offsets, offsets + n + 1);
   for (int x = 0; x < n && x < 32; ++x)
     if (offsets[x + 1] > offsets[x]) c->zmask |= 1u << x;
+  if (layout == 0 && n >= 2 && __builtin_popcount(c->zmask) == 1) {
+    c->single_a = __builtin_ctz(c->zmask);
+    c->ra = (int)(offsets[c->single_a + 1] - offsets[c->single_a]);
+  }
   auto fail = [&](int code) {
     cg_free_all(c);
     delete c;
@@ -562,6 +820,14 @@ extern "C" int rumgpu_cg_create(rumgpu_ctx* ctx, int n, const uint64_t* masks_x,
     layer_order(n, ord, c->bounds);
     if ((rc = cg_alloc(c, (void**)&c->nodes, (size_t)c->NN * 4, "layers")) != RG_OK) return fail(rc);
     RG_HIP(rg_copy(ctx->stream, c->nodes, ord.data(), (size_t)c->NN * 4, hipMemcpyHostToDevice));
+    if (c->single_a >= 0) {
+      const int a = c->single_a;
+      std::vector<u32> cm(c->ra);
+      for (int j = 0; j < c->ra; ++j)
+        cm[j] = (u32)compress_bit(masks_x[offsets[a] + j] & ~(1ull << a), a);
+      if ((rc = cg_alloc(c, (void**)&c->clcm, (size_t)c->ra * 4, "one-alternative menus")) != RG_OK) return fail(rc);
+      RG_HIP(rg_copy(ctx->stream, c->clcm, cm.data(), (size_t)c->ra * 4, hipMemcpyHostToDevice));
+    }
     if (layout == 1) {
       RG_CHECK(n >= 2, RG_EUNSUP, "slab pricing needs n >= 2");
       std::vector<i64> pos(M);
@@ -643,6 +909,80 @@ static int mask_sweep(hipStream_t st, int n, float* V, uint8_t* pred, const u32*
   return RG_OK;
 }
 
+static bool use_closure(const rumgpu_cg* c) { return c->layout == 0 && c->sweep == 1 && c->single_a >= 0; }
+
+// buffers of the closure sweep, at its first use: 4 bytes per node plus per-tile maxima
+static int cl_ready(rumgpu_cg* c) {
+  if (c->clF) return RG_OK;
+  const u32 NT = (u32)c->Nloc;
+  const u32 tiles = NT >= CL_TILE ? NT / CL_TILE : 1u;
+  const int tpb = tiles > 2048 ? (int)(tiles / 2048) : 1;
+  const int grid = (int)((tiles + (u32)tpb - 1) / (u32)tpb);
+  int rc = RG_OK;
+  if ((rc = cg_alloc(c, (void**)&c->clF, (size_t)NT * 4, "closure bits")) != RG_OK ||
+      (rc = cg_alloc(c, (void**)&c->cltmax, (size_t)tiles * CG_BATCH * 4, "tile maxima")) != RG_OK ||
+      (rc = cg_alloc(c, (void**)&c->clbmax, (size_t)grid * CG_BATCH * 4, "block maxima")) != RG_OK ||
+      (rc = cg_alloc(c, (void**)&c->clvmax, (size_t)CG_BATCH * 4, "closure vmax")) != RG_OK ||
+      (rc = cg_alloc(c, (void**)&c->clbmax64, (size_t)grid * 8, "float64 block maxima")) != RG_OK ||
+      (rc = cg_alloc(c, (void**)&c->clv64, 8, "float64 vmax")) != RG_OK ||
+      (rc = cg_alloc(c, (void**)&c->clerr, 4, "closure status")) != RG_OK) {
+    void** bufs[] = {(void**)&c->clF, (void**)&c->cltmax, (void**)&c->clbmax, (void**)&c->clvmax,
+                     (void**)&c->clbmax64, (void**)&c->clv64, (void**)&c->clerr};
+    for (void** b : bufs) {
+      rg_free(*b);
+      *b = nullptr;
+    }
+    return rc;
+  }
+  c->clNT = NT;
+  c->cl_tpb = tpb;
+  c->cl_grid = grid;
+  return RG_OK;
+}
+
+// the closure sweep for nd <= DB draws whose rows of single_a are sc ([row][DB]); writes
+// rank (nd x n) and vmax (nd floats) on the device
+template <int DB>
+static int cl_sweep(rumgpu_cg* c, hipStream_t st, const float* sc, int nd, int32_t* rank, float* vmax) {
+  RG_TRY(cl_ready(c));
+  const int n = c->n, nb = n - 1, a = c->single_a, ra = c->ra;
+  const u32 NT = c->clNT;
+  const int rows_lds = std::max(1, std::min(ra, 12288 / DB));
+  hipLaunchKernelGGL(cl_wmax<DB>, dim3(c->cl_grid), dim3(256), (size_t)rows_lds * DB * 4, st, c->clcm, sc, ra,
+                     rows_lds, NT, c->cl_tpb, c->clF, c->cltmax, c->clbmax);
+  hipLaunchKernelGGL(cl_gmax<float>, dim3(1), dim3(256), 0, st, c->clbmax, c->cl_grid, DB, vmax);
+  const int w0 = std::min(nb, 12);
+  hipLaunchKernelGGL(cl_close_lo, dim3(NT >> w0), dim3(256), 0, st, c->clF, c->cltmax, vmax, DB, NT, w0);
+  for (int b0 = w0; b0 < nb; b0 += 8) {
+    const int w = std::min(8, nb - b0);
+    hipLaunchKernelGGL(cl_close_hi, dim3(1u << (nb - w - 5)), dim3(256), 0, st, c->clF, NT, b0, w);
+  }
+  RG_HIP(hipMemsetAsync(c->clerr, 0, 4, st));
+  hipLaunchKernelGGL(cl_backtrace, dim3(1), dim3(64), 0, st, c->clF, c->clcm, sc, ra, DB, nd, n, a, vmax, rank,
+                     c->clerr);
+  RG_LAUNCH_CHECK();
+  return RG_OK;
+}
+
+static int cl_check(rumgpu_cg* c) {
+  int err = 0;
+  RG_HIP(rg_copy(c->ctx->stream, &err, c->clerr, 4, hipMemcpyDeviceToHost));
+  RG_CHECK(err == 0, RG_EHIP, "cg pricing: the one-alternative path found no choice matching vmax");
+  return RG_OK;
+}
+
+extern "C" int rumgpu_cg_set_sweep(rumgpu_cg* c, int mode) {
+  RG_CHECK(c && (mode == 0 || mode == 1), RG_EINVAL, "cg_set_sweep: mode must be 0 (layers) or 1 (auto)");
+  c->sweep = mode;
+  return RG_OK;
+}
+
+extern "C" int rumgpu_cg_sweep_info(rumgpu_cg* c, int* one_alternative) {
+  RG_CHECK(c && one_alternative, RG_EINVAL, "cg_sweep_info: null argument");
+  *one_alternative = use_closure(c) ? c->single_a : -1;
+  return RG_OK;
+}
+
 extern "C" int rumgpu_cg_price(rumgpu_cg* c, const float* scores_x, double* vmax, int32_t* rank_pos, double* margin) {
   RG_CHECK(c && scores_x && vmax && rank_pos, RG_EINVAL, "cg_price: null argument");
   RG_HIP(hipSetDevice(c->ctx->device));
@@ -651,6 +991,16 @@ extern "C" int rumgpu_cg_price(rumgpu_cg* c, const float* scores_x, double* vmax
   auto t0 = std::chrono::steady_clock::now();
   RG_HIP(hipMemcpyAsync(c->scores, scores_x, (size_t)c->M * 4, hipMemcpyHostToDevice, st));
   if (margin) *margin = 0.0;
+  if (use_closure(c)) {
+    RG_TRY(cl_sweep<1>(c, st, c->scores + c->h_offsets[c->single_a], 1, c->rank, c->clvmax));
+    float vf = 0.0f;
+    RG_HIP(hipMemcpyAsync(&vf, c->clvmax, 4, hipMemcpyDeviceToHost, st));
+    RG_HIP(hipMemcpyAsync(rank_pos, c->rank, (size_t)n * 4, hipMemcpyDeviceToHost, st));
+    RG_TRY(cl_check(c));
+    *vmax = (double)vf;
+    c->last_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return RG_OK;
+  }
   if (c->layout == 0 || c->layout == 1) {
     if (c->layout == 0) {
       RG_TRY(mask_sweep(st, n, c->V, c->pred, c->masks32, c->scores, c->offsets, c->nodes, c->bounds));
@@ -799,8 +1149,6 @@ extern "C" int rumgpu_cg_price(rumgpu_cg* c, const float* scores_x, double* vmax
   return RG_OK;
 }
 
-#define CG_BATCH 32
-
 template <int DB>
 static int mask_sweep_batch(rumgpu_cg* c, hipStream_t st) {
   const int n = c->n;
@@ -823,15 +1171,23 @@ extern "C" int rumgpu_cg_price_batch(rumgpu_cg* c, int D, const float* scores, d
   const int n = c->n;
   const i64 M = c->M;
   auto t0 = std::chrono::steady_clock::now();
-  if (!c->Vb) {
+  const bool closure = use_closure(c);
+  if (!c->scoresb) {
     int rc = RG_OK;
-    if ((rc = cg_alloc(c, (void**)&c->Vb, (size_t)c->NN * CG_BATCH * 4, "batched V")) != RG_OK ||
-        (rc = cg_alloc(c, (void**)&c->predb, (size_t)c->NN * CG_BATCH, "batched pred")) != RG_OK ||
-        (rc = cg_alloc(c, (void**)&c->scoresb, (size_t)M * CG_BATCH * 4, "batched scores")) != RG_OK ||
+    if ((rc = cg_alloc(c, (void**)&c->scoresb, (size_t)M * CG_BATCH * 4, "batched scores")) != RG_OK ||
         (rc = cg_alloc(c, (void**)&c->vmaxb, (size_t)CG_BATCH * 4, "batched vmax")) != RG_OK ||
         (rc = cg_alloc(c, (void**)&c->rankb, (size_t)CG_BATCH * 64 * 4, "batched ranks")) != RG_OK) {
-      rg_free(c->Vb), rg_free(c->predb), rg_free(c->scoresb), rg_free(c->vmaxb), rg_free(c->rankb);
-      c->Vb = nullptr, c->predb = nullptr, c->scoresb = nullptr, c->vmaxb = nullptr, c->rankb = nullptr;
+      rg_free(c->scoresb), rg_free(c->vmaxb), rg_free(c->rankb);
+      c->scoresb = nullptr, c->vmaxb = nullptr, c->rankb = nullptr;
+      return rc;
+    }
+  }
+  if (!closure && !c->Vb) {
+    int rc = RG_OK;
+    if ((rc = cg_alloc(c, (void**)&c->Vb, (size_t)c->NN * CG_BATCH * 4, "batched V")) != RG_OK ||
+        (rc = cg_alloc(c, (void**)&c->predb, (size_t)c->NN * CG_BATCH, "batched pred")) != RG_OK) {
+      rg_free(c->Vb), rg_free(c->predb);
+      c->Vb = nullptr, c->predb = nullptr;
       return rc;
     }
   }
@@ -842,6 +1198,22 @@ extern "C" int rumgpu_cg_price_batch(rumgpu_cg* c, int D, const float* scores, d
     for (i64 j = 0; j < M; ++j)
       for (int d = 0; d < DB; ++d) tr[(size_t)j * DB + d] = d < nd ? scores[(size_t)(first + d) * M + j] : 0.0f;
     RG_HIP(hipMemcpyAsync(c->scoresb, tr.data(), (size_t)M * DB * 4, hipMemcpyHostToDevice, st));
+    if (closure) {
+      const float* sa = c->scoresb + (size_t)c->h_offsets[c->single_a] * DB;
+      switch (DB) {
+        case 1: RG_TRY(cl_sweep<1>(c, st, sa, nd, c->rankb, c->vmaxb)); break;
+        case 2: RG_TRY(cl_sweep<2>(c, st, sa, nd, c->rankb, c->vmaxb)); break;
+        case 4: RG_TRY(cl_sweep<4>(c, st, sa, nd, c->rankb, c->vmaxb)); break;
+        case 8: RG_TRY(cl_sweep<8>(c, st, sa, nd, c->rankb, c->vmaxb)); break;
+        case 16: RG_TRY(cl_sweep<16>(c, st, sa, nd, c->rankb, c->vmaxb)); break;
+        default: RG_TRY(cl_sweep<32>(c, st, sa, nd, c->rankb, c->vmaxb)); break;
+      }
+      RG_HIP(hipMemcpyAsync(vf.data(), c->vmaxb, (size_t)nd * 4, hipMemcpyDeviceToHost, st));
+      RG_HIP(hipMemcpyAsync(rank_pos + (size_t)first * n, c->rankb, (size_t)nd * n * 4, hipMemcpyDeviceToHost, st));
+      RG_TRY(cl_check(c));
+      for (int d = 0; d < nd; ++d) vmax[first + d] = (double)vf[d];
+      continue;
+    }
     switch (DB) {
       case 1: RG_TRY(mask_sweep_batch<1>(c, st)); break;
       case 2: RG_TRY(mask_sweep_batch<2>(c, st)); break;
@@ -898,15 +1270,20 @@ extern "C" int rumgpu_cg_vmax64(rumgpu_cg* c, const double* r_x, double* vmax) {
   RG_HIP(hipSetDevice(c->ctx->device));
   if (c->layout == 2) return stream_vmax64(c, r_x, vmax);
   hipStream_t st = c->ctx->stream;
-  if (!c->V64) {
-    int rc = RG_OK;
-    if ((rc = cg_alloc(c, (void**)&c->V64, (size_t)c->NN * 8, "float64 V")) != RG_OK ||
-        (rc = cg_alloc(c, (void**)&c->r64, (size_t)c->M * 8, "float64 residual")) != RG_OK) {
-      rg_free(c->V64), rg_free(c->r64);
-      c->V64 = nullptr, c->r64 = nullptr;
-      return rc;
-    }
+  if (use_closure(c)) {
+    RG_TRY(cl_ready(c));
+    if (!c->r64) RG_TRY(cg_alloc(c, (void**)&c->r64, (size_t)c->M * 8, "float64 residual"));
+    RG_HIP(hipMemcpyAsync(c->r64, r_x, (size_t)c->M * 8, hipMemcpyHostToDevice, st));
+    const int rows_lds = std::max(1, std::min(c->ra, 6144));
+    hipLaunchKernelGGL(cl_wmax64, dim3(c->cl_grid), dim3(256), (size_t)rows_lds * 8, st, c->clcm,
+                       c->r64 + c->h_offsets[c->single_a], c->ra, rows_lds, c->clNT, c->cl_tpb, c->clbmax64);
+    hipLaunchKernelGGL(cl_gmax<double>, dim3(1), dim3(256), 0, st, c->clbmax64, c->cl_grid, 1, c->clv64);
+    RG_LAUNCH_CHECK();
+    RG_HIP(rg_copy(st, vmax, c->clv64, 8, hipMemcpyDeviceToHost));
+    return RG_OK;
   }
+  if (!c->V64) RG_TRY(cg_alloc(c, (void**)&c->V64, (size_t)c->NN * 8, "float64 V"));
+  if (!c->r64) RG_TRY(cg_alloc(c, (void**)&c->r64, (size_t)c->M * 8, "float64 residual"));
   RG_HIP(hipMemcpyAsync(c->r64, r_x, (size_t)c->M * 8, hipMemcpyHostToDevice, st));
   hipLaunchKernelGGL(cg_fill_f64, dim3(1), dim3(64), 0, st, c->V64, 0.0, (i64)1);
   for (int k = 1; k <= c->n; ++k) {

@@ -82,6 +82,20 @@ class Pricer:
         ry = math.fsum(r * y)
         return math.sqrt(max(0.0, f - 2.0 * (vmax - ry)))
 
+    SWEEPS = {"layers": 0, "auto": 1}
+
+    def set_sweep(self, mode):
+        """'layers' (the default): the layer-by-layer sweep for any rows (the reference's); 'auto': the
+        one-alternative closure sweep when every row belongs to one alternative (same results, 4 bytes
+        per node; opt-in, tested on the GPU up to n = 20)"""
+        _lib.check(self.lib.rumgpu_cg_set_sweep(self.h, self.SWEEPS[mode]), "cg_set_sweep")
+
+    def one_alternative(self):
+        """the alternative the closure sweep runs on, or -1 when the layered sweep runs"""
+        a = ctypes.c_int()
+        _lib.check(self.lib.rumgpu_cg_sweep_info(self.h, ctypes.byref(a)), "cg_sweep_info")
+        return a.value
+
     def info(self):
         b, ms = ctypes.c_size_t(), ctypes.c_double()
         _lib.check(self.lib.rumgpu_cg_info(self.h, ctypes.byref(b), ctypes.byref(ms)), "cg_info")

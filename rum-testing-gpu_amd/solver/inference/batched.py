@@ -289,7 +289,11 @@ class BatchedCG:
         self.overlap = bool(overlap) and self.D >= 2
         from .. import machine as _machine
         n = boot.rum.n
-        need = (32 * 5 + 8) * (1 << n) + 32 * 4 * boot.rum.M      # batched V, pred, float64 V, scores
+        one_alt = np.unique(boot.rum._geom.alternative).size == 1 and sess._args.get("_sweep", "layers") == "auto"
+        if one_alt:     # the closure sweep: 4 bytes per subset of the other alternatives, and the scores
+            need = 4 * (1 << (n - 1)) + 32 * 4 * boot.rum.M
+        else:           # batched V, pred, float64 V, scores
+            need = (32 * 5 + 8) * (1 << n) + 32 * 4 * boot.rum.M
         budget = defaults.AUTO["gpu_memory_fraction"] * _machine.gpu_free_bytes()
         if need > budget:
             raise MemoryError("out of memory: batched pricing at n = %d needs %.1f GB, %.1f GB may be used"

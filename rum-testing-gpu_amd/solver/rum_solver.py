@@ -836,7 +836,12 @@ def _cg_pricer(t):
         perm, masks_x, _alts, offsets = xmajor(g.menu, g.alternative, g.num_alternatives)
         eng["cg"] = _EngineSet(key, {"pricer": Pricer(g.num_alternatives, masks_x, offsets, layout=a["pricing"],
                                                       dtype=a["dtype"])})
-    return eng["cg"].items["pricer"]
+    pricer = eng["cg"].items["pricer"]
+    if pricer.layout == "mask":
+        # "auto" opts in to the one-alternative closure sweep; "layers" (the default) is the
+        # reference's layer-by-layer sweep
+        pricer.set_sweep(a.get("_sweep", "layers"))
+    return pricer
 
 
 def _run_flow_cg(t):

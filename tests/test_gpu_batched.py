@@ -180,6 +180,49 @@ def test_batched_pricing_equals_single(n, D, alt0):
     pr.close()
 
 
+@pytest.mark.parametrize("n,a,D,quant", [(2, 1, 3, False), (5, 0, 7, True), (9, 4, 32, True), (9, 8, 40, False),
+                                          (13, 0, 32, True), (14, 6, 5, False), (16, 15, 33, True),
+                                          (20, 0, 32, False)])
+def test_one_alternative_sweep_equals_layers(n, a, D, quant):
+    """Rows of one alternative: the closure sweep (opt-in) returns the layered sweep's vmax and
+    ranking bit for bit, single and batched, and the same float64 longest path; quantized scores
+    (multiples of 1/4) make ties common, which exercises the lowest-member rule. Sizes stay at
+    n <= 20: the closure sweep's first run inside the n = 25 bootstrap test ended with a GPU fault
+    whose cause has not been shown, so it is not run above n = 20."""
+    from oracle import oracle as orc
+    from solver.flow_cg.engine import Pricer, xmajor
+    rng = np.random.default_rng(900 + n)
+    menu, alt, _p = _medium(n, min(6000, max(2, (n << n) >> 3)), rng)
+    keep = alt == a
+    menu, alt = menu[keep], alt[keep]
+    perm, masks_x, _alts, offsets = xmajor(menu, alt, n)
+    pr = Pricer(n, masks_x, offsets, layout="mask")
+    assert pr.one_alternative() == -1
+    pr.set_sweep("auto")
+    assert pr.one_alternative() == a
+    S = rng.standard_normal((D, menu.size)).astype(np.float32)
+    if quant:
+        S = (np.round(S * 4) / 4).astype(np.float32)
+    r64 = rng.standard_normal(menu.size)
+    vb, rb = pr.price_batch(S)
+    v1 = [pr.price(S[d])[:2] for d in range(D)]
+    l1 = pr.vmax64(r64)
+    pr.set_sweep("layers")
+    assert pr.one_alternative() == -1
+    vb0, rb0 = pr.price_batch(S)
+    l0 = pr.vmax64(r64)
+    for d in range(D):
+        v0, r0, _m = pr.price(S[d])
+        assert vb[d] == v0 and np.array_equal(rb[d], r0), d
+        assert v1[d][0] == v0 and np.array_equal(v1[d][1], r0), d
+        assert vb0[d] == v0 and np.array_equal(rb0[d], r0), d
+        if n <= 14 and d < 4:
+            vo, ro, _V, _pred = orc.price_mask(n, masks_x, S[d], offsets)
+            assert vo == v0 and np.array_equal(ro, r0), d
+    assert l1 == l0
+    pr.close()
+
+
 def test_batched_flow_cg_bootstrap_identical_brackets():
     import pandas as pd
     panel, n = _small_panel()
