@@ -76,19 +76,11 @@ __global__ __launch_bounds__(256) void cg_mask_layer_batch(float* __restrict__ V
   const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ns) return;
   const u32 D = nodes[i];
-  if ((D & zmask) == 0u) {
-    // no member has rows: every arc weighs 0, so by induction V = 0 here and the first member
-    // (ascending, strict >) is the choice; the sweep's values without reading the predecessors
-    float* vo = V + (size_t)D * DB;
-    uint8_t* po = pred + (size_t)D * DB;
-    const uint8_t x0 = (uint8_t)(__ffs((int)D) - 1);
-#pragma unroll
-    for (int d = 0; d < DB; ++d) {
-      vo[d] = 0.0f;
-      po[d] = x0;
-    }
-    return;
-  }
+  // no member has rows: every arc weighs 0, so by induction V = 0 here and the first member
+  // (ascending, strict >) is the choice. Such a node is not stored: its successors read the
+  // empty set's row (0, set before the sweep, cache-resident) in its place, and the backtrace
+  // takes its lowest member.
+  if ((D & zmask) == 0u) return;
   float best[DB];
   uint8_t bx[DB];
 #pragma unroll
@@ -120,7 +112,8 @@ __global__ __launch_bounds__(256) void cg_mask_layer_batch(float* __restrict__ V
         for (int d = 0; d < DB; ++d) w[d] += sj[d];
       }
     }
-    const float* vp = V + (size_t)(D ^ (1u << x)) * DB;
+    const u32 P = D ^ (1u << x);
+    const float* vp = V + (size_t)((P & zmask) ? P : 0u) * DB;     // a row-free P reads V[empty] = 0
 #pragma unroll
     for (int d = 0; d < DB; ++d) {
       const float cand = vp[d] + w[d];
@@ -150,10 +143,7 @@ __global__ __launch_bounds__(256) void cg_mask_layer64(double* __restrict__ V, c
   const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ns) return;
   const u32 D = nodes[i];
-  if ((D & zmask) == 0u) {             // no member has rows: V = 0 (as in cg_mask_layer_batch)
-    V[D] = 0.0;
-    return;
-  }
+  if ((D & zmask) == 0u) return;       // no member has rows: V = 0, not stored (as in cg_mask_layer_batch)
   double best = 0.0;
   bool seen = false;
   for (int x = 0; x < n; ++x) {
@@ -164,7 +154,8 @@ __global__ __launch_bounds__(256) void cg_mask_layer64(double* __restrict__ V, c
       const u32 Dp = masks[j];
       if ((Dp & D) == Dp) w += r[j];
     }
-    const double cand = V[D ^ (1u << x)] + w;
+    const u32 P = D ^ (1u << x);
+    const double cand = V[(P & zmask) ? P : 0u] + w;
     if (!seen || cand > best) best = cand;
     seen = true;
   }
@@ -172,14 +163,15 @@ __global__ __launch_bounds__(256) void cg_mask_layer64(double* __restrict__ V, c
 }
 
 // one thread per draw: walk the choices down from the full set; vmax = V[full]
+// (nodes without a row-carrying member are not stored: V = 0 and the lowest member is the choice)
 __global__ void cg_backtrace_batch(const uint8_t* __restrict__ pred, const float* __restrict__ V, int n, int DB,
-                                   int nd, int32_t* rank_pos, float* vmax, u64 full) {
+                                   int nd, int32_t* rank_pos, float* vmax, u64 full, u32 zmask) {
   const int d = blockIdx.x * blockDim.x + threadIdx.x;
   if (d >= nd) return;
   u64 S = full;
-  vmax[d] = V[full * DB + d];
+  vmax[d] = (full & zmask) ? V[full * DB + d] : 0.0f;
   for (int i = 0; i < n; ++i) {
-    const int cur = pred[S * DB + d];
+    const int cur = (S & zmask) ? (int)pred[S * DB + d] : __ffsll((long long)S) - 1;
     rank_pos[d * n + cur] = i;
     S ^= 1ull << cur;
   }
@@ -1223,7 +1215,7 @@ extern "C" int rumgpu_cg_price_batch(rumgpu_cg* c, int D, const float* scores, d
       default: RG_TRY(mask_sweep_batch<32>(c, st)); break;
     }
     hipLaunchKernelGGL(cg_backtrace_batch, dim3(1), dim3(64), 0, st, c->predb, c->Vb, n, DB, nd, c->rankb, c->vmaxb,
-                       (u64)(c->NN - 1));
+                       (u64)(c->NN - 1), c->zmask);
     RG_LAUNCH_CHECK();
     RG_HIP(hipMemcpyAsync(vf.data(), c->vmaxb, (size_t)nd * 4, hipMemcpyDeviceToHost, st));
     RG_HIP(hipMemcpyAsync(rank_pos + (size_t)first * n, c->rankb, (size_t)nd * n * 4, hipMemcpyDeviceToHost, st));
@@ -1293,6 +1285,10 @@ extern "C" int rumgpu_cg_vmax64(rumgpu_cg* c, const double* r_x, double* vmax) {
                        c->offsets, c->nodes + lo, ns, c->n, c->zmask);
   }
   RG_LAUNCH_CHECK();
+  if (((u64)(c->NN - 1) & c->zmask) == 0) {       // no rows at all: every node is 0
+    *vmax = 0.0;
+    return RG_OK;
+  }
   RG_HIP(rg_copy(st, vmax, c->V64 + (c->NN - 1), 8, hipMemcpyDeviceToHost));
   return RG_OK;
 }
