@@ -109,8 +109,11 @@ struct Coef {
 };
 
 // One tile of a window pass held in registers. The layout index `cur` names the register
-// round whose RL window bits the thread currently holds.
-template <typename T, int RL>
+// round whose RL window bits the thread currently holds. BT > 0 fixes the window bits and CT >= 0
+// the column bits at compile time (they must equal the geometry's): the rounds then unroll and
+// the register and LDS index arithmetic folds to constants; BT = 0, CT = -1 read both from the
+// geometry.
+template <typename T, int RL, int BT = 0, int CT = -1>
 struct Tile {
   static constexpr int R = 1 << RL;
   T v[R];
@@ -127,8 +130,8 @@ struct Tile {
   // the first transform then runs its rounds in descending order.
   __device__ __forceinline__ void init(const WinGeom& g, T* lds_all, bool start_last = false) {
     s = g.s;
-    B = g.B;
-    C = g.C;
+    B = BT > 0 ? BT : g.B;
+    C = CT >= 0 ? CT : g.C;
     nr = (B + RL - 1) / RL;
     cur = (start_last && C == 0) ? nr - 1 : 0;
     const int tl = threadIdx.x / g.TT;
@@ -212,6 +215,7 @@ struct Tile {
   template <bool FMA = false>
   __device__ __forceinline__ void transform(const Coef<T>& k) {
     const bool fwd = (cur == 0);
+#pragma unroll
     for (int i = 0; i < nr; ++i) {
       const int r = fwd ? i : nr - 1 - i;
       if (r != cur) exchange(r);

@@ -45,13 +45,13 @@ struct PassArgs {
   const int* fresh;   // per slot: 1 = state just loaded, skip the deferred s, y update once
 };
 
-template <typename T, int RL, int KIND>
+template <typename T, int RL, int KIND, int BT = 0, int CT = -1>
 __global__ __launch_bounds__(512) void bm_pass(PassArgs<T> P) {
   extern __shared__ __align__(16) unsigned char smem[];
-  Tile<T, RL> t;
+  Tile<T, RL, BT, CT> t;
   t.init(P.g, reinterpret_cast<T*>(smem), true);
   const T one = (T)1;
-  constexpr int R = Tile<T, RL>::R;
+  constexpr int R = Tile<T, RL, BT, CT>::R;
   // every global access happens in a lane-contiguous layout (t.coalesce() before it)
   if constexpr (KIND == K_FIRST || KIND == K_FULL_FIRST) {
     if (t.active) {
@@ -594,6 +594,21 @@ int launch_pass_rl(int RL, const PassArgs<T>& P, hipStream_t st) {
   const WinGeom& g = P.g;
   const size_t shm = (size_t)g.TPB * tile_lds_elems(g) * sizeof(T);
   const dim3 grid((unsigned)((g.ntiles + g.TPB - 1) / g.TPB)), block((unsigned)(g.TT * g.TPB));
+  // the window at bit 0 of the fused passes with 3 register bits: geometry fixed at compile time
+  if constexpr (KIND == K_FIRST || KIND == K_SY || KIND == K_SYONLY || KIND == K_D || KIND == K_FULL ||
+                KIND == K_FULL_FIRST) {
+    if (RL == 3 && g.s == 0 && g.C == 0 && g.B >= 8 && g.B <= 12) {
+      switch (g.B) {
+        case 8: hipLaunchKernelGGL((bm_pass<T, 3, KIND, 8, 0>), grid, block, shm, st, P); break;
+        case 9: hipLaunchKernelGGL((bm_pass<T, 3, KIND, 9, 0>), grid, block, shm, st, P); break;
+        case 10: hipLaunchKernelGGL((bm_pass<T, 3, KIND, 10, 0>), grid, block, shm, st, P); break;
+        case 11: hipLaunchKernelGGL((bm_pass<T, 3, KIND, 11, 0>), grid, block, shm, st, P); break;
+        default: hipLaunchKernelGGL((bm_pass<T, 3, KIND, 12, 0>), grid, block, shm, st, P); break;
+      }
+      RG_LAUNCH_CHECK();
+      return RG_OK;
+    }
+  }
   switch (RL) {
     case 1: hipLaunchKernelGGL((bm_pass<T, 1, KIND>), grid, block, shm, st, P); break;
     case 2: hipLaunchKernelGGL((bm_pass<T, 2, KIND>), grid, block, shm, st, P); break;
