@@ -194,18 +194,31 @@ __global__ __launch_bounds__(256, 4) void bm_prox_iter(const T* __restrict__ xk,
   const int off = om ? obsoff[S] : 0;
   const bool mixed = (om != 0) && (om != S);
   const double c_obs = (double)b2 + 2.0, c_un = (double)b2;
+  // every member's value first, with no use in between, so all the loads are in flight together;
+  // key = v when the menu has no observation ((b2 v) / b2 = v)
   double key[N];
 #pragma unroll
   for (int i = 0; i < N; ++i) {
     key[i] = 0.0;
-    if ((S >> i) & 1u) {
-      const u32 pos = (u32)i * Nloc + (u32)compress_bit(S & ~(1u << i), i);
-      const T xv = xk[pos];
-      if (om == 0) {
-        key[i] = (double)xv;                       // (b2 v) / b2 = v: no observation in the menu
-      } else {
+    if ((S >> i) & 1u) key[i] = (double)xk[(u32)i * Nloc + (u32)compress_bit(S & ~(1u << i), i)];
+  }
+  if (om != 0) {
+    // observed menus: the observed members' p, loaded without branches (a clamped index for the
+    // others, whose value is not used) so the loads are in flight together, then the keys
+    const int nob = __popc(om);
+    double xh[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      const int rk = __popc(om & ((1u << i) - 1u));
+      const double v = pc[off + (rk < nob ? rk : nob - 1)];
+      xh[i] = ((om >> i) & 1u) ? v : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      if ((S >> i) & 1u) {
+        const T xv = (T)key[i];
         const bool ob = (om >> i) & 1u;
-        const T xhv = ob ? (T)pc[off + __popc(om & ((1u << i) - 1u))] : (T)0;
+        const T xhv = ob ? (T)xh[i] : (T)0;
         key[i] = mixed ? (double)b2 * (double)xv + 2.0 * (ob ? 1.0 : 0.0) * (double)xhv
                        : (double)((b2 * xv + (T)2 * xhv) / (b2 + (T)2));
       }
@@ -235,15 +248,42 @@ __global__ __launch_bounds__(256, 4) void bm_prox_iter(const T* __restrict__ xk,
     if (nact == act || nact == 0) break;
     act = nact;
   }
+  if (om == 0) {
+    // no observation: r = 1 and xk at pos is the key itself
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      if ((S >> i) & 1u) {
+        const double yv = (key[i] - thr) * 1.0;
+        const T o = yv > 0.0 ? (T)yv : (T)0.0;
+        const u32 pos = (u32)i * Nloc + (u32)compress_bit(S & ~(1u << i), i);
+        out[pos] = o;
+        z[pos] = b2 * (o - (T)key[i]);
+      }
+    }
+    return;
+  }
+  // observed menus: the outputs replace the keys, xk is read again (all loads in flight together)
 #pragma unroll
   for (int i = 0; i < N; ++i) {
     if ((S >> i) & 1u) {
       const double r = mixed ? (((wobs >> i) & 1u) ? r_obs : r_un) : 1.0;
       const double yv = (key[i] - thr) * r;
-      const T o = yv > 0.0 ? (T)yv : (T)0.0;
+      key[i] = (double)(yv > 0.0 ? (T)yv : (T)0.0);
+    }
+  }
+  T xo[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    xo[i] = (T)0;
+    if ((S >> i) & 1u) xo[i] = xk[(u32)i * Nloc + (u32)compress_bit(S & ~(1u << i), i)];
+  }
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    if ((S >> i) & 1u) {
+      const T o = (T)key[i];
       const u32 pos = (u32)i * Nloc + (u32)compress_bit(S & ~(1u << i), i);
       out[pos] = o;
-      z[pos] = b2 * (o - xk[pos]);                 // re-read: the line was just loaded
+      z[pos] = b2 * (o - xo[i]);
     }
   }
 }
@@ -605,6 +645,24 @@ int launch_pass_rl(int RL, const PassArgs<T>& P, hipStream_t st) {
         case 11: hipLaunchKernelGGL((bm_pass<T, 3, KIND, 11, 0>), grid, block, shm, st, P); break;
         default: hipLaunchKernelGGL((bm_pass<T, 3, KIND, 12, 0>), grid, block, shm, st, P); break;
       }
+      RG_LAUNCH_CHECK();
+      return RG_OK;
+    }
+  }
+  // the window above it (16 float64 or 32 float32 columns) of the fused passes, likewise
+  if constexpr (KIND == K_RHS || KIND == K_XK) {
+    constexpr int CW = sizeof(T) == 8 ? 4 : 5;
+    if ((RL == 3 || RL == 4) && g.C == CW && g.B >= 4 && g.B <= 8) {
+#define RG_HI_CASE(BB)                                                                                   \
+  case BB:                                                                                               \
+    if (RL == 3) hipLaunchKernelGGL((bm_pass<T, 3, KIND, BB, CW>), grid, block, shm, st, P);             \
+    else hipLaunchKernelGGL((bm_pass<T, 4, KIND, BB, CW>), grid, block, shm, st, P);                     \
+    break;
+      switch (g.B) {
+        RG_HI_CASE(4) RG_HI_CASE(5) RG_HI_CASE(6) RG_HI_CASE(7)
+        default: RG_HI_CASE(8)
+      }
+#undef RG_HI_CASE
       RG_LAUNCH_CHECK();
       return RG_OK;
     }
