@@ -14,7 +14,7 @@ import json
 import sqlite3
 import sys
 
-# fused pass kernels of BM-ADMM by template arguments: bm_pass<T, RL, KIND>, KIND 1 SY, 3 RHS, 4 D, 5 XK
+# fused pass kernels of BM-ADMM by template arguments: bm_pass<T, RL, KIND[, B, C]>, KIND 1 SY, 3 RHS, 4 D, 5 XK
 PASS_KIND = {1: 0, 3: 1, 4: 2, 5: 3}
 
 
