@@ -526,6 +526,105 @@ __global__ void k_lpath(const double* __restrict__ R, double* dp, const int* __r
   dp[C] = best;
 }
 
+// The same three layer kernels with n fixed at compile time: the member loop unrolls and every
+// load is unconditional (a non-member x reads the values at the node itself, a valid index, and
+// is masked out afterwards), so all loads of a node are in flight before the first is used (the
+// runtime-n loops wait for each member's loads in turn, and k_mass waited for outw before loading
+// the rest). Same arithmetic, same order of the sums and of the max, so the results are bit for
+// bit those of the kernels above.
+template <int N>
+__global__ void k_outw_n(const double* __restrict__ w, double* __restrict__ outw, i64 Nloc) {
+  w += (i64)blockIdx.y * N * Nloc;
+  outw += (i64)blockIdx.y << N;
+  const i64 D = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (D >= (1ll << N)) return;
+  double v[N];
+#pragma unroll
+  for (int x = 0; x < N; ++x) v[x] = w[(i64)x * Nloc + (i64)compress_bit((u64)D, x)];
+  double s = 0.0;
+#pragma unroll
+  for (int x = 0; x < N; ++x)
+    if (!((D >> x) & 1)) s += v[x];
+  outw[D] = s;
+}
+
+template <int N>
+__global__ void k_mass_n(const double* __restrict__ w, const double* __restrict__ outw, double* mass,
+                         const int* __restrict__ nodes, i64 ns, i64 Nloc) {
+  w += (i64)blockIdx.y * N * Nloc;
+  outw += (i64)blockIdx.y << N;
+  mass += (i64)blockIdx.y << N;
+  const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ns) return;
+  const i64 C = nodes[i];
+  double ow[N], mv[N], wv[N];
+#pragma unroll
+  for (int x = 0; x < N; ++x) {
+    const i64 P = ((C >> x) & 1) ? C ^ (1ll << x) : C;
+    ow[x] = outw[P];
+    mv[x] = mass[P];
+    wv[x] = w[(i64)x * Nloc + (i64)compress_bit((u64)P, x)];
+  }
+  double s = 0.0;
+#pragma unroll
+  for (int x = 0; x < N; ++x)
+    if (((C >> x) & 1) && ow[x] > 0.0) s += mv[x] * wv[x] / ow[x];
+  mass[C] = s;
+}
+
+template <int N>
+__global__ void k_lpath_n(const double* __restrict__ R, double* dp, const int* __restrict__ nodes, i64 ns,
+                          i64 Nloc) {
+  R += (i64)blockIdx.y * N * Nloc;
+  dp += (i64)blockIdx.y << N;
+  const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ns) return;
+  const i64 C = nodes[i];
+  double v[N];
+#pragma unroll
+  for (int x = 0; x < N; ++x) {
+    const i64 P = ((C >> x) & 1) ? C ^ (1ll << x) : C;
+    v[x] = dp[P] + R[(i64)x * Nloc + (i64)compress_bit((u64)P, x)];
+  }
+  double best = -1e300;
+#pragma unroll
+  for (int x = 0; x < N; ++x)
+    if (((C >> x) & 1) && v[x] > best) best = v[x];
+  dp[C] = best;
+}
+
+// launchers: the compile-time-n kernels for n = 2..26, the runtime-n ones otherwise
+#define RG_N_CASES(M)                                                                                    \
+  M(2) M(3) M(4) M(5) M(6) M(7) M(8) M(9) M(10) M(11) M(12) M(13) M(14) M(15) M(16) M(17) M(18) M(19)  \
+  M(20) M(21) M(22) M(23) M(24) M(25) M(26)
+static void launch_outw(int n, dim3 grid, hipStream_t st, const double* w, double* outw, i64 Nloc) {
+  switch (n) {
+#define RG_C(NV) case NV: hipLaunchKernelGGL(k_outw_n<NV>, grid, dim3(256), 0, st, w, outw, Nloc); return;
+    RG_N_CASES(RG_C)
+#undef RG_C
+    default: hipLaunchKernelGGL(k_outw, grid, dim3(256), 0, st, w, outw, n, Nloc);
+  }
+}
+static void launch_mass(int n, dim3 grid, hipStream_t st, const double* w, const double* outw, double* mass,
+                        const int* nodes, i64 ns, i64 Nloc) {
+  switch (n) {
+#define RG_C(NV) case NV: hipLaunchKernelGGL(k_mass_n<NV>, grid, dim3(256), 0, st, w, outw, mass, nodes, ns, Nloc); return;
+    RG_N_CASES(RG_C)
+#undef RG_C
+    default: hipLaunchKernelGGL(k_mass, grid, dim3(256), 0, st, w, outw, mass, nodes, ns, n, Nloc);
+  }
+}
+static void launch_lpath(int n, dim3 grid, hipStream_t st, const double* R, double* dp, const int* nodes, i64 ns,
+                         i64 Nloc) {
+  switch (n) {
+#define RG_C(NV) case NV: hipLaunchKernelGGL(k_lpath_n<NV>, grid, dim3(256), 0, st, R, dp, nodes, ns, Nloc); return;
+    RG_N_CASES(RG_C)
+#undef RG_C
+    default: hipLaunchKernelGGL(k_lpath, grid, dim3(256), 0, st, R, dp, nodes, ns, n, Nloc);
+  }
+}
+#undef RG_N_CASES
+
 // Compensated (Neumaier) block sums. Each block writes (sum, comp) pairs per quantity.
 __device__ __forceinline__ void two_sum(double& s, double& c, double v) {
   const double t = s + v;
@@ -994,13 +1093,12 @@ static int push_impl(rumgpu_bm* b, double* ub, int D = 1) {
   const int n = b->n;
   // q = Mobius(x2) in float64, then w = max(q, 0) + 1e-12
   RG_TRY((full_transform<T, double>(b, (const T*)b->x2, b->cert, coef<double>(1.0, -1.0, 0.0, 1.0), 1, 0, D)));
-  hipLaunchKernelGGL(k_outw, dim3(grid_for(b->NN, 256), D), dim3(256), 0, st, b->cert, b->outw, n, b->Nloc);
+  launch_outw(n, dim3(grid_for(b->NN, 256), D), st, b->cert, b->outw, b->Nloc);
   hipLaunchKernelGGL(k_set_strided, dim3(grid_for(D, 64)), dim3(64), 0, st, b->mass, b->NN, D, 1.0);
   for (int c = 1; c <= n; ++c) {
     const i64 lo = b->bounds[c], ns = b->bounds[c + 1] - lo;
     if (ns <= 0) continue;
-    hipLaunchKernelGGL(k_mass, dim3(grid_for(ns, 256), D), dim3(256), 0, st, b->cert, b->outw, b->mass, b->order + lo,
-                       ns, n, b->Nloc);
+    launch_mass(n, dim3(grid_for(ns, 256), D), st, b->cert, b->outw, b->mass, b->order + lo, ns, b->Nloc);
   }
   RG_LAUNCH_CHECK();
   // rf = extract, x_ub = zeta(rf) (in place)
@@ -1025,8 +1123,7 @@ static int lb_impl(rumgpu_bm* b, double* lb, int D = 1) {
   for (int c = 1; c <= n; ++c) {
     const i64 lo = b->bounds[c], ns = b->bounds[c + 1] - lo;
     if (ns <= 0) continue;
-    hipLaunchKernelGGL(k_lpath, dim3(grid_for(ns, 256), D), dim3(256), 0, st, b->cert, b->mass, b->order + lo, ns, n,
-                       b->Nloc);
+    launch_lpath(n, dim3(grid_for(ns, 256), D), st, b->cert, b->mass, b->order + lo, ns, b->Nloc);
   }
   // longest path to the full set of each slot: res[2 D + d]
   hipLaunchKernelGGL(k_get_strided, dim3(grid_for(D, 64)), dim3(64), 0, st, b->mass + (b->NN - 1), b->NN, D,
@@ -1694,7 +1791,7 @@ extern "C" int rumgpu_lb64(rumgpu_ctx* ctx, int n, const uint64_t* menu, const i
         for (int c = 1; c <= n; ++c) {
           const i64 lo = bounds[c], ns = bounds[c + 1] - lo;
           if (ns <= 0) continue;
-          hipLaunchKernelGGL(k_lpath, dim3(grid_for(ns, 256)), dim3(256), 0, st, g, dp, order + lo, ns, n, Nloc);
+          launch_lpath(n, dim3(grid_for(ns, 256)), st, g, dp, order + lo, ns, Nloc);
         }
         double h[3];
         if (hipMemcpyAsync(h, res, 2 * 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
