@@ -159,6 +159,11 @@ int rumgpu_cg_price_batch(rumgpu_cg* cg, int D, const float* scores, double* vma
  * layout (two float64 colex layers, sweep_max64 / fw_lb_stream64, pricing_stream.py:190-248,
  * transforms.py:117-129) */
 int rumgpu_cg_vmax64(rumgpu_cg* cg, const double* r_x, double* vmax);
+/* the same float64 longest path for D residual vectors at once (mask layout): r is D x M (draw-major,
+ * x-major rows), vmax[D]; draw d's value equals rumgpu_cg_vmax64 on row d bit for bit. No reference
+ * counterpart: the batched bootstrap takes the final bounds of the draws that finish in one round
+ * together instead of one fw_lb_rows_gpu per draw (transforms.py:101-114). */
+int rumgpu_cg_vmax64_batch(rumgpu_cg* cg, int D, const double* r, double* vmax);
 int rumgpu_cg_info(rumgpu_cg* cg, size_t* device_bytes, double* last_kernel_ms);
 /* How the mask layout sweeps (rumgpu_cg_price, _price_batch, _vmax64): mode 0 (the default) =
  * layer by layer for any rows, in the reference's order (bf_update, flow_cg/bf_pricing.py:7-47);

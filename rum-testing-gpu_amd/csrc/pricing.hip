@@ -162,6 +162,58 @@ __global__ __launch_bounds__(256) void cg_mask_layer64(double* __restrict__ V, c
   V[D] = best;
 }
 
+// the float64 longest path for DB draws at once (the lower bounds of draws that finish in the same
+// round): node-major V ([node][DB], 8 B per draw), row-major residuals ([row][DB]); per draw the
+// sums and the max run in cg_mask_layer64's order, so each draw's value equals its own sweep's
+template <int DB>
+__global__ __launch_bounds__(256) void cg_mask_layer64_batch(double* __restrict__ V, const u32* __restrict__ masks,
+                                                             const double* __restrict__ r,
+                                                             const u64* __restrict__ offsets,
+                                                             const u32* __restrict__ nodes, i64 ns, int n,
+                                                             u32 zmask) {
+  const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ns) return;
+  const u32 D = nodes[i];
+  if ((D & zmask) == 0u) return;       // no member has rows: V = 0, not stored (as in cg_mask_layer_batch)
+  double best[DB];
+#pragma unroll
+  for (int d = 0; d < DB; ++d) best[d] = 0.0;
+  bool seen = false;
+  for (int x = 0; x < n; ++x) {
+    if (!((D >> x) & 1u)) continue;
+    double w[DB];
+#pragma unroll
+    for (int d = 0; d < DB; ++d) w[d] = 0.0;
+    const u64 lo = offsets[x], hi = offsets[x + 1];
+    for (u64 base = lo; base < hi; base += 32) {
+      const int cnt = (hi - base) < 32 ? (int)(hi - base) : 32;
+      u32 mm = 0;
+      for (int t = 0; t < cnt; ++t) {
+        const u32 Dp = masks[base + t];
+        mm |= ((Dp & D) == Dp ? 1u : 0u) << t;
+      }
+      while (mm) {
+        const int t = __ffs((int)mm) - 1;
+        mm &= mm - 1u;
+        const double* rj = r + (base + (u64)t) * DB;
+#pragma unroll
+        for (int d = 0; d < DB; ++d) w[d] += rj[d];
+      }
+    }
+    const u32 P = D ^ (1u << x);
+    const double* vp = V + (size_t)((P & zmask) ? P : 0u) * DB;     // a row-free P reads V[empty] = 0
+#pragma unroll
+    for (int d = 0; d < DB; ++d) {
+      const double cand = vp[d] + w[d];
+      if (!seen || cand > best[d]) best[d] = cand;
+    }
+    seen = true;
+  }
+  double* vo = V + (size_t)D * DB;
+#pragma unroll
+  for (int d = 0; d < DB; ++d) vo[d] = best[d];
+}
+
 // one thread per draw: walk the choices down from the full set; vmax = V[full]
 // (nodes without a row-carrying member are not stored: V = 0 and the lowest member is the choice)
 __global__ void cg_backtrace_batch(const uint8_t* __restrict__ pred, const float* __restrict__ V, int n, int DB,
@@ -692,6 +744,8 @@ struct rumgpu_cg {
   // float64 lower-bound sweep (allocated at the first rumgpu_cg_lb64)
   double* V64 = nullptr;       // mask: V per node; stream: two layers of cap64 values
   double* r64 = nullptr;
+  double* V64b = nullptr;      // batched float64 bound: [node][CG_LB_BATCH] (allocated at first use)
+  double* r64b = nullptr;
   u64 cap64 = 0;
   float* R = nullptr;        // slab
   int32_t* rank = nullptr;
@@ -720,7 +774,7 @@ static void cg_free_all(rumgpu_cg* c) {
   void* bufs[] = {c->masks32, c->masks64, c->offsets, c->scores, c->spos, c->V, c->pred, c->nodes, c->R, c->rank,
                   c->binom, c->vbuf[0], c->vbuf[1], c->lbuf[0], c->lbuf[1], c->mbuf[0], c->mbuf[1], c->ebuf[0],
                   c->ebuf[1], c->lstep, c->subV, c->subpred, c->subnodes, c->submasks, c->subscores, c->suboffsets,
-                  c->Vb, c->predb, c->scoresb, c->vmaxb, c->rankb, c->V64, c->r64, c->clcm, c->clF, c->cltmax,
+                  c->Vb, c->predb, c->scoresb, c->vmaxb, c->rankb, c->V64, c->r64, c->V64b, c->r64b, c->clcm, c->clF, c->cltmax,
                   c->clbmax, c->clvmax, c->clbmax64, c->clv64, c->clerr};
   for (void* q : bufs) rg_free(q);
   for (auto* p : c->predk) rg_free(p);
@@ -1290,6 +1344,60 @@ extern "C" int rumgpu_cg_vmax64(rumgpu_cg* c, const double* r_x, double* vmax) {
     return RG_OK;
   }
   RG_HIP(rg_copy(st, vmax, c->V64 + (c->NN - 1), 8, hipMemcpyDeviceToHost));
+  return RG_OK;
+}
+
+#define CG_LB_BATCH 8
+
+template <int DB>
+static int mask_sweep64_batch(rumgpu_cg* c, hipStream_t st) {
+  hipLaunchKernelGGL(cg_fill_f64, dim3(1), dim3(64), 0, st, c->V64b, 0.0, (i64)DB);     // V[empty set] = 0
+  for (int k = 1; k <= c->n; ++k) {
+    const i64 lo = c->bounds[k], ns = c->bounds[k + 1] - lo;
+    if (ns <= 0) continue;
+    hipLaunchKernelGGL(cg_mask_layer64_batch<DB>, dim3(grid_for(ns, 256)), dim3(256), 0, st, c->V64b, c->masks32,
+                       c->r64b, c->offsets, c->nodes + lo, ns, c->n, c->zmask);
+  }
+  RG_LAUNCH_CHECK();
+  return RG_OK;
+}
+
+extern "C" int rumgpu_cg_vmax64_batch(rumgpu_cg* c, int D, const double* r, double* vmax) {
+  RG_CHECK(c && r && vmax && D >= 1, RG_EINVAL, "cg_vmax64_batch: bad argument");
+  RG_CHECK(c->layout == 0, RG_EUNSUP, "cg_vmax64_batch: the batched float64 sweep uses the mask layout");
+  RG_HIP(hipSetDevice(c->ctx->device));
+  hipStream_t st = c->ctx->stream;
+  const i64 M = c->M;
+  if (!c->V64b) {
+    int rc = RG_OK;
+    if ((rc = cg_alloc(c, (void**)&c->V64b, (size_t)c->NN * CG_LB_BATCH * 8, "batched float64 V")) != RG_OK ||
+        (rc = cg_alloc(c, (void**)&c->r64b, (size_t)M * CG_LB_BATCH * 8, "batched float64 residuals")) != RG_OK) {
+      rg_free(c->V64b), rg_free(c->r64b);
+      c->V64b = nullptr, c->r64b = nullptr;
+      return rc;
+    }
+  }
+  const u64 full = (u64)(c->NN - 1);
+  std::vector<double> tr((size_t)M * CG_LB_BATCH), out(CG_LB_BATCH);
+  for (int first = 0; first < D; first += CG_LB_BATCH) {
+    const int nd = std::min(CG_LB_BATCH, D - first);
+    const int DB = nd <= 1 ? 1 : nd <= 2 ? 2 : nd <= 4 ? 4 : 8;
+    for (i64 j = 0; j < M; ++j)
+      for (int d = 0; d < DB; ++d) tr[(size_t)j * DB + d] = d < nd ? r[(size_t)(first + d) * M + j] : 0.0;
+    RG_HIP(hipMemcpyAsync(c->r64b, tr.data(), (size_t)M * DB * 8, hipMemcpyHostToDevice, st));
+    switch (DB) {
+      case 1: RG_TRY(mask_sweep64_batch<1>(c, st)); break;
+      case 2: RG_TRY(mask_sweep64_batch<2>(c, st)); break;
+      case 4: RG_TRY(mask_sweep64_batch<4>(c, st)); break;
+      default: RG_TRY(mask_sweep64_batch<8>(c, st)); break;
+    }
+    if ((full & c->zmask) == 0) {              // no rows at all: every node is 0
+      for (int d = 0; d < nd; ++d) vmax[first + d] = 0.0;
+      continue;
+    }
+    RG_HIP(rg_copy(st, out.data(), c->V64b + full * DB, (size_t)nd * 8, hipMemcpyDeviceToHost));
+    for (int d = 0; d < nd; ++d) vmax[first + d] = out[d];
+  }
   return RG_OK;
 }
 

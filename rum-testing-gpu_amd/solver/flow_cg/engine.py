@@ -72,6 +72,14 @@ class Pricer:
         _lib.check(self.lib.rumgpu_cg_vmax64(self.h, _lib.ptr(r), ctypes.byref(v)), "cg_vmax64")
         return v.value
 
+    def vmax64_batch(self, R):
+        """vmax64 for D residual vectors at once (mask layout): R (D, M) x-major -> vmax[D]"""
+        r = np.ascontiguousarray(R, dtype=np.float64)
+        v = np.empty(r.shape[0])
+        _lib.check(self.lib.rumgpu_cg_vmax64_batch(self.h, int(r.shape[0]), _lib.ptr(r), _lib.ptr(v)),
+                   "cg_vmax64_batch")
+        return v
+
     def lb64(self, perm, y, p):
         """float64 Frank-Wolfe lower bound at y (input row order), fw_lb_rows_gpu's formula:
         r = p - y, lb = sqrt(max(0, |r|^2 - 2 (vmax(r) - r.y))) with compensated sums"""

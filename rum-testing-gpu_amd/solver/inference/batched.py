@@ -292,8 +292,8 @@ class BatchedCG:
         one_alt = np.unique(boot.rum._geom.alternative).size == 1 and sess._args.get("_sweep", "layers") == "auto"
         if one_alt:     # the closure sweep: 4 bytes per subset of the other alternatives, and the scores
             need = 4 * (1 << (n - 1)) + 32 * 4 * boot.rum.M
-        else:           # batched V, pred, float64 V, scores
-            need = (32 * 5 + 8) * (1 << n) + 32 * 4 * boot.rum.M
+        else:           # batched V, pred, float64 V (single and 8 draws), scores
+            need = (32 * 5 + 8 + 64) * (1 << n) + 32 * 4 * boot.rum.M
         budget = defaults.AUTO["gpu_memory_fraction"] * _machine.gpu_free_bytes()
         if need > budget:
             raise MemoryError("out of memory: batched pricing at n = %d needs %.1f GB, %.1f GB may be used"
@@ -415,7 +415,8 @@ class BatchedCG:
 
     def _serve(self, pricer, slots, reqs):
         """Replies to the requests of `slots` (reqs: slot -> request): one batched sweep for every
-        pricing request, the float64 longest paths of finishing draws one by one."""
+        pricing request, one batched float64 sweep for the longest paths of the finishing draws
+        (each draw's value equals its own sweep's)."""
         get = reqs.__getitem__
         price = [q for q in slots if not isinstance(get(q), tuple)]
         out = {}
@@ -426,10 +427,15 @@ class BatchedCG:
             self.stats["draw_rounds"] += len(price)
             for i, q in enumerate(price):
                 out[q] = (float(vmax[i]), rank[i].copy(), 0.0)
-        for q in slots:
-            if q not in out:
+        lbs = [q for q in slots if q not in out]
+        if len(lbs) == 1 or (lbs and pricer.layout != "mask"):
+            for q in lbs:
                 out[q] = pricer.vmax64(get(q)[1])
-                self.stats["lb_sweeps"] = self.stats.get("lb_sweeps", 0) + 1
+        elif lbs:
+            v = pricer.vmax64_batch(np.stack([get(q)[1] for q in lbs]))
+            for i, q in enumerate(lbs):
+                out[q] = float(v[i])
+        self.stats["lb_sweeps"] = self.stats.get("lb_sweeps", 0) + len(lbs)
         return out
 
     def close(self):

@@ -258,6 +258,24 @@ def test_mask_lb64_matches_slab_bound(n, alt0):
     pr.close()
 
 
+@pytest.mark.parametrize("n,D,alt0", [(7, 3, False), (12, 8, False), (13, 11, True), (16, 2, False)])
+def test_batched_vmax64_equals_single(n, D, alt0):
+    """the float64 longest paths of D residual vectors in one batched sweep equal the single-draw
+    sweep of each bit for bit (D above 8 runs in chunks)"""
+    rng = np.random.default_rng(700 + n)
+    menu, alt, p = _medium(n, min(5000, n << (n - 3)), rng)
+    if alt0:
+        keep = alt == 0
+        menu, alt, p = menu[keep], alt[keep], p[keep]
+    perm, masks_x, _alts, offsets = xmajor(menu, alt, n)
+    pr = Pricer(n, masks_x, offsets, layout="mask")
+    R = rng.standard_normal((D, menu.size))
+    vb = pr.vmax64_batch(R)
+    for d in range(D):
+        assert vb[d] == pr.vmax64(R[d]), d
+    pr.close()
+
+
 @pytest.mark.parametrize("graph", [0, 5])
 def test_set_beta_carries_the_state(graph):
     """rumgpu_bm_set_beta settles the owed s, y update with the old penalty, then iterates with the
