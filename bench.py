@@ -13,7 +13,9 @@ Reported beside the value (one JSON line on stdout):
   plain      the same certified solve with the reference's own schedule in float64 (separate
              elementwise kernels, four full butterflies per iteration) on this GPU: tuned vs plain
   roofline   the dominant fused pass: algorithmic bytes per launch / its mean device time, from
-             HIP events on the engine's stream right after the timed region (rumgpu_bm_profile)
+             a start/stop HIP event pair recorded in each pass launch's own dispatch
+             (hipExtLaunchKernelGGL) on the engine's stream, right after the timed region
+             (rumgpu_bm_profile)
   cpu_baseline  the CPU oracle (oracle/rum_oracle.cpp, one core) on a bounded sample of the same
              solve: a few float64 iterations plus one push and one lower bound, scaled to a
              certified solve by the GPU run's iteration and check counts

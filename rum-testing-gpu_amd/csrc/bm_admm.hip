@@ -25,6 +25,8 @@
 #include "rumgpu_internal.hpp"
 #include "transform.hpp"
 
+#include <hip/hip_ext.h>
+
 #include <cmath>
 #include <algorithm>
 
@@ -728,6 +730,21 @@ __global__ void k_final(const double* part, int nblk, int Q, double* out) {
 // Host side.
 namespace {
 
+// Start and stop events for the next pass launch (the per-pass profiler, rumgpu_bm_profile): with
+// them the launch records both events in its own dispatch (hipExtLaunchKernelGGL), so the interval
+// is the kernel's execution, not the gaps between eager launches. Consumed by that launch.
+thread_local hipEvent_t g_pass_ev0 = nullptr, g_pass_ev1 = nullptr;
+
+template <typename F, typename... Args>
+void pass_launch(F kernel, const dim3& grid, const dim3& block, size_t shm, hipStream_t st, Args... args) {
+  if (g_pass_ev0) {
+    hipExtLaunchKernelGGL(kernel, grid, block, (std::uint32_t)shm, st, g_pass_ev0, g_pass_ev1, 0, args...);
+    g_pass_ev0 = g_pass_ev1 = nullptr;
+  } else {
+    hipLaunchKernelGGL(kernel, grid, block, shm, st, args...);
+  }
+}
+
 template <typename T, int KIND>
 int launch_pass_rl(int RL, const PassArgs<T>& P, hipStream_t st) {
   const WinGeom& g = P.g;
@@ -738,11 +755,11 @@ int launch_pass_rl(int RL, const PassArgs<T>& P, hipStream_t st) {
                 KIND == K_FULL_FIRST) {
     if (RL == 3 && g.s == 0 && g.C == 0 && g.B >= 8 && g.B <= 12) {
       switch (g.B) {
-        case 8: hipLaunchKernelGGL((bm_pass<T, 3, KIND, 8, 0>), grid, block, shm, st, P); break;
-        case 9: hipLaunchKernelGGL((bm_pass<T, 3, KIND, 9, 0>), grid, block, shm, st, P); break;
-        case 10: hipLaunchKernelGGL((bm_pass<T, 3, KIND, 10, 0>), grid, block, shm, st, P); break;
-        case 11: hipLaunchKernelGGL((bm_pass<T, 3, KIND, 11, 0>), grid, block, shm, st, P); break;
-        default: hipLaunchKernelGGL((bm_pass<T, 3, KIND, 12, 0>), grid, block, shm, st, P); break;
+        case 8: pass_launch((bm_pass<T, 3, KIND, 8, 0>), grid, block, shm, st, P); break;
+        case 9: pass_launch((bm_pass<T, 3, KIND, 9, 0>), grid, block, shm, st, P); break;
+        case 10: pass_launch((bm_pass<T, 3, KIND, 10, 0>), grid, block, shm, st, P); break;
+        case 11: pass_launch((bm_pass<T, 3, KIND, 11, 0>), grid, block, shm, st, P); break;
+        default: pass_launch((bm_pass<T, 3, KIND, 12, 0>), grid, block, shm, st, P); break;
       }
       RG_LAUNCH_CHECK();
       return RG_OK;
@@ -754,8 +771,8 @@ int launch_pass_rl(int RL, const PassArgs<T>& P, hipStream_t st) {
     if ((RL == 3 || RL == 4) && g.C == CW && g.B >= 4 && g.B <= 8) {
 #define RG_HI_CASE(BB)                                                                                   \
   case BB:                                                                                               \
-    if (RL == 3) hipLaunchKernelGGL((bm_pass<T, 3, KIND, BB, CW>), grid, block, shm, st, P);             \
-    else hipLaunchKernelGGL((bm_pass<T, 4, KIND, BB, CW>), grid, block, shm, st, P);                     \
+    if (RL == 3) pass_launch((bm_pass<T, 3, KIND, BB, CW>), grid, block, shm, st, P);             \
+    else pass_launch((bm_pass<T, 4, KIND, BB, CW>), grid, block, shm, st, P);                     \
     break;
       switch (g.B) {
         RG_HI_CASE(4) RG_HI_CASE(5) RG_HI_CASE(6) RG_HI_CASE(7)
@@ -767,11 +784,11 @@ int launch_pass_rl(int RL, const PassArgs<T>& P, hipStream_t st) {
     }
   }
   switch (RL) {
-    case 1: hipLaunchKernelGGL((bm_pass<T, 1, KIND>), grid, block, shm, st, P); break;
-    case 2: hipLaunchKernelGGL((bm_pass<T, 2, KIND>), grid, block, shm, st, P); break;
-    case 3: hipLaunchKernelGGL((bm_pass<T, 3, KIND>), grid, block, shm, st, P); break;
-    case 4: hipLaunchKernelGGL((bm_pass<T, 4, KIND>), grid, block, shm, st, P); break;
-    case 5: hipLaunchKernelGGL((bm_pass<T, 5, KIND>), grid, block, shm, st, P); break;
+    case 1: pass_launch((bm_pass<T, 1, KIND>), grid, block, shm, st, P); break;
+    case 2: pass_launch((bm_pass<T, 2, KIND>), grid, block, shm, st, P); break;
+    case 3: pass_launch((bm_pass<T, 3, KIND>), grid, block, shm, st, P); break;
+    case 4: pass_launch((bm_pass<T, 4, KIND>), grid, block, shm, st, P); break;
+    case 5: pass_launch((bm_pass<T, 5, KIND>), grid, block, shm, st, P); break;
     default: rg_set_error("bad RL %d", RL); return RG_EINVAL;
   }
   RG_LAUNCH_CHECK();
@@ -803,7 +820,7 @@ int launch_prox(int n, const T* xk, T* out, T* z, const u32* om, const int* oo, 
 #define RG_PROX_CASE(NV)                                                                                   \
   case NV:                                                                                                 \
     if constexpr (MODE == 0)                                                                               \
-      hipLaunchKernelGGL((bm_prox_iter<T, NV>), grid, block, 0, st, xk, out, z, om, oo, pc, b2, m2, M, fresh); \
+      pass_launch((bm_prox_iter<T, NV>), grid, block, 0, st, xk, out, z, om, oo, pc, b2, m2, M, fresh); \
     else                                                                                                   \
       hipLaunchKernelGGL((bm_prox<T, NV, MODE>), grid, block, 0, st, xk, out, z, om, oo, pc, b2, m2, M);  \
     break;
@@ -890,11 +907,13 @@ static void bm_free_all(rumgpu_bm* b) {
 }
 
 // profiler kinds: 0 SY/FIRST, 1 RHS, 2 D, 3 XK, 4 prox, 5 middle-window transform, 6 FULL (one window)
-static inline void prof_mark(rumgpu_bm* b, int kind) {
-  if (!b->prof_on || b->prof_n >= b->prof_ev.size()) return;
-  (void)hipEventRecord(b->prof_ev[b->prof_n], b->ctx->stream);
+// arm the next pass launch with an event pair charged to `kind`
+static inline void prof_arm(rumgpu_bm* b, int kind) {
+  if (!b->prof_on || b->prof_n + 2 > b->prof_ev.size()) return;
+  g_pass_ev0 = b->prof_ev[b->prof_n];
+  g_pass_ev1 = b->prof_ev[b->prof_n + 1];
   b->prof_kind[b->prof_n] = kind;
-  b->prof_n++;
+  b->prof_n += 2;
 }
 
 template <typename T>
@@ -929,34 +948,33 @@ static int run_pass(rumgpu_bm* b, int kind, int w, const Coef<T>* ct = nullptr) 
   return launch_pass<T>(kind, b->wins[w].RL, P, b->ctx->stream);
 }
 
-// One fused iteration (see the header comment). With the profiler on, an event is recorded after
-// every launch; the interval ending at an event is charged to that launch's pass kind.
+// One fused iteration (see the header comment). With the profiler on, every launch carries a start
+// and stop event pair charged to its pass kind.
 template <typename T>
 static int fused_iter(rumgpu_bm* b) {
   const int k = (int)b->wins.size();
   hipStream_t st = b->ctx->stream;
   const PassArgs<T> P0 = pass_args<T>(b, 0);
-  if (b->prof_on) prof_mark(b, -1);
   if (k == 1) {
+    prof_arm(b, 6);
     RG_TRY(run_pass<T>(b, b->pending ? K_FULL : K_FULL_FIRST, 0));
-    prof_mark(b, 6);
   } else {
+    prof_arm(b, 0);
     RG_TRY(run_pass<T>(b, b->pending ? K_SY : K_FIRST, 0));
-    prof_mark(b, 0);
-    for (int w = 1; w < k - 1; ++w) { RG_TRY(run_pass<T>(b, K_T, w, &P0.cAp)); prof_mark(b, 5); }
+    for (int w = 1; w < k - 1; ++w) { prof_arm(b, 5); RG_TRY(run_pass<T>(b, K_T, w, &P0.cAp)); }
+    prof_arm(b, 1);
     RG_TRY(run_pass<T>(b, K_RHS, k - 1));
-    prof_mark(b, 1);
-    for (int w = k - 2; w >= 1; --w) { RG_TRY(run_pass<T>(b, K_T, w, &P0.cQt)); prof_mark(b, 5); }
+    for (int w = k - 2; w >= 1; --w) { prof_arm(b, 5); RG_TRY(run_pass<T>(b, K_T, w, &P0.cQt)); }
+    prof_arm(b, 2);
     RG_TRY(run_pass<T>(b, K_D, 0));
-    prof_mark(b, 2);
-    for (int w = 1; w < k - 1; ++w) { RG_TRY(run_pass<T>(b, K_T, w, &P0.cQ)); prof_mark(b, 5); }
+    for (int w = 1; w < k - 1; ++w) { prof_arm(b, 5); RG_TRY(run_pass<T>(b, K_T, w, &P0.cQ)); }
+    prof_arm(b, 3);
     RG_TRY(run_pass<T>(b, K_XK, k - 1));
-    prof_mark(b, 3);
   }
+  prof_arm(b, 4);
   RG_TRY((launch_prox<T, 0>(b->n, (const T*)b->xk, (T*)b->x2, (T*)b->z, b->obsmask, b->obsoff, b->pc, (T)b->beta, st,
                             b->D, b->m2, b->M, b->fresh)));
-  prof_mark(b, 4);
-  for (int w = k - 2; w >= 1; --w) { RG_TRY(run_pass<T>(b, K_T, w, &P0.cA)); prof_mark(b, 5); }
+  for (int w = k - 2; w >= 1; --w) { prof_arm(b, 5); RG_TRY(run_pass<T>(b, K_T, w, &P0.cA)); }
   b->pending = 1;
   b->iters += 1;
   return RG_OK;
@@ -1818,7 +1836,7 @@ extern "C" int rumgpu_bm_profile(rumgpu_bm* b, int64_t iters, double* ms_by_kind
   RG_CHECK(!b->plain, RG_EINVAL, "bm_profile: the plain schedule has no fused passes");
   RG_HIP(hipSetDevice(b->ctx->device));
   for (int q = 0; q < 7; ++q) { ms_by_kind[q] = 0.0; launches_by_kind[q] = 0; }
-  const int per_iter = 2 + 2 * (int)b->wins.size() + 4;
+  const int per_iter = 2 * (2 + 2 * (int)b->wins.size() + 4);     // an event pair per launch
   const i64 batch = 16;
   if (b->prof_ev.size() < (size_t)(per_iter * batch)) {
     for (auto e : b->prof_ev) (void)hipEventDestroy(e);
@@ -1836,16 +1854,16 @@ extern "C" int rumgpu_bm_profile(rumgpu_bm* b, int64_t iters, double* ms_by_kind
     b->prof_on = 0;
     if (rc != RG_OK) break;
     RG_HIP(hipStreamSynchronize(b->ctx->stream));
-    for (size_t e = 1; e < b->prof_n; ++e) {
+    for (size_t e = 0; e + 1 < b->prof_n; e += 2) {
       const int kind = b->prof_kind[e];
-      if (kind < 0) continue;
       float ms = 0.0f;
-      RG_HIP(hipEventElapsedTime(&ms, b->prof_ev[e - 1], b->prof_ev[e]));
+      RG_HIP(hipEventElapsedTime(&ms, b->prof_ev[e], b->prof_ev[e + 1]));
       ms_by_kind[kind] += ms;
       launches_by_kind[kind] += 1;
     }
     done += now;
   }
   b->prof_on = 0;
+  g_pass_ev0 = g_pass_ev1 = nullptr;       // an armed pair that no launch consumed (an error path)
   return rc;
 }
