@@ -233,7 +233,7 @@ def main():
                 config=dict(workload="certified BM-ADMM solve (abs gap 1e-5, rel 0), n=18, 1%% density "
                                      "(config2 seed 18001, %d cells)" % mp, n=n, cells=mp, abs_tol=a.abs_tol,
                             beta_rule=a.beta, beta=r_last.method_args["beta"],
-                            beta_final=r_last.info.get("beta_final"), check=50,
+                            beta_final=r_last.info.get("beta_final"), check=r_last.method_args["check"],
                             iterations=[r.iterations for r in results],
                             iters32=[r.info.get("iters32") for r in results],
                             iters64=[r.info.get("iters64") for r in results],

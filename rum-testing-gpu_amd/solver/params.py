@@ -31,8 +31,8 @@ def bm_beta_cert(mp):
     return float(max(BM_BETA_MIN * BM_BETA_CERT_FACTOR, BM_BETA_CERT_FACTOR * bm_beta_default(mp)))
 
 
-# Adaptive penalty (beta="adaptive"): residual balancing on the certified gap. Every `every`
-# checks the gap ub - lb splits at phi = ||x2[obs] - p||: ub - phi is what the push to a feasible
+# Adaptive penalty (beta="adaptive"): residual balancing on the certified gap. Every `every_iters`
+# iterations (a whole number of checks) the gap ub - lb splits at phi = ||x2[obs] - p||: ub - phi is what the push to a feasible
 # point costs (primal infeasibility), phi - lb what the iterate still lacks in optimality (the
 # dual side). When one part exceeds `ratio` times the other, beta is multiplied (infeasibility) or
 # divided (optimality) by `factor`; the first reversal of direction takes a half step (sqrt of
@@ -41,7 +41,11 @@ def bm_beta_cert(mp):
 # density): 37 % fewer iterations than beta="cert" in total, from -47 % (n18_dense, n18_m231,
 # n19_m102) to +2 % (n18_m8); without the cap on changes n18_sparse drifted to 4.5 times the race
 # rule and needed 12 times the iterations.
-BM_BETA_ADAPT = dict(start=0.05, every=10, ratio=10.0, factor=2.0, freeze_on_flip=True, max_changes=3)
+BM_BETA_ADAPT = dict(start=0.05, every_iters=500, ratio=10.0, factor=2.0, freeze_on_flip=True, max_changes=3)
+# Certificate cadence of beta="adaptive" when method_args names no `check`: a push and a lower bound
+# every 100 iterations instead of 50. The rule above acts on iterations, so its schedule is the
+# same; over the same ten instances the iterations stay within 0.1 % and the time drops 5.5 %.
+BM_BETA_ADAPT_CHECK = 100
 
 
 def bm_beta_adaptive_start(mp):

@@ -362,6 +362,8 @@ class RUM:
         if bad:
             raise ValueError("unknown method_args for %s: %s; allowed: %s" % (name, sorted(bad), sorted(args)))
         args.update(method_args or {})
+        if name == "bm_admm" and args.get("beta") == "adaptive" and "check" not in (method_args or {}):
+            args["check"] = params.BM_BETA_ADAPT_CHECK
         return name, reason, _PLAN[name](n, mp, weighted, prof, args)
 
     def _result(self, t0, t_run, name, reason, args, abs_tol, rel_tol, seed, st, stop, message, const, prof):
@@ -593,7 +595,9 @@ def _plan_bm_admm(n, mp, weighted, prof, a):
     elif a["beta"] == "cert":
         a["beta"] = params.bm_beta_cert(mp)
     elif a["beta"] == "adaptive":
-        a["beta"], a["_beta_adapt"] = params.bm_beta_adaptive_start(mp), dict(params.BM_BETA_ADAPT)
+        ad = dict(params.BM_BETA_ADAPT)
+        ad["every"] = max(1, int(round(ad.pop("every_iters") / int(a["check"]))))     # in checks
+        a["beta"], a["_beta_adapt"] = params.bm_beta_adaptive_start(mp), ad
     else:
         a["beta"] = float(a["beta"])
         if not a["beta"] > 0:
