@@ -161,6 +161,26 @@ def test_planner_rules_and_beta():
     assert abs(params.bm_beta_default(306) - 0.6) < 0.05 and abs(params.bm_beta_default(24000) - 5.0) < 1.0
 
 
+def test_adaptive_penalty_cadence_in_iterations():
+    """beta="adaptive" certifies every params.BM_BETA_ADAPT_CHECK iterations unless method_args names
+    check, and its rule acts every BM_BETA_ADAPT["every_iters"] iterations, counted in checks"""
+    prof = dict(use_gpu=True, gpu=dict(supported=True, name="stub"), ram_bytes=64 << 30)
+    import solver.machine as machine
+    saved = machine.gpu_free_bytes
+    machine.gpu_free_bytes = lambda device=0: 250 << 30
+    try:
+        rum = RUM.from_rows(instances.warmup_instance(12))
+        _, _, a = rum._plan("bm_admm", {"beta": "adaptive"}, False, prof)
+        assert a["check"] == params.BM_BETA_ADAPT_CHECK == 100
+        assert a["_beta_adapt"]["every"] * a["check"] == params.BM_BETA_ADAPT["every_iters"] == 500
+        _, _, a = rum._plan("bm_admm", {"beta": "adaptive", "check": 50}, False, prof)
+        assert a["check"] == 50 and a["_beta_adapt"]["every"] == 10
+        _, _, a = rum._plan("bm_admm", {"beta": "cert"}, False, prof)
+        assert a["check"] == defaults.METHOD_ARGS["bm_admm"]["check"] == 50 and "_beta_adapt" not in a
+    finally:
+        machine.gpu_free_bytes = saved
+
+
 def test_no_gpu_means_no_method():
     rum = RUM.from_rows(instances.warmup_instance(6))
     prof = dict(use_gpu=False, gpu=None, ram_bytes=1 << 34)
