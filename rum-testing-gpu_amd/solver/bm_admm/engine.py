@@ -321,9 +321,15 @@ def solve_kadmm(h, max_iter=10 ** 9, check=50, ub_target=None, ub_eps=1e-2, time
 def solve_mixed(h32, h64, gap_stall=(3, 0.9), **kw):
     """Tuned certificate: float32 leg until the certified gap stops shrinking by 10 % per three
     checks (the float32 iterate's precision floor), then the same state in float64 (device
-    conversion) until the tolerance. Iterations and traces are summed over both legs."""
+    conversion) until the tolerance. Iterations and traces are summed over both legs. Without a
+    lower bound (a race) there is no certified gap: the float32 leg ends when phi, then the pushed
+    ub, stops improving by 1e-3 over three checks, the reference hybrid's plateau rule
+    (bm_admm/solver.py:380-425, stall_checks=3, stall_rtol=1e-3)."""
     t0 = time.perf_counter()
-    r32 = solve_kadmm(h32, gap_stall=gap_stall, **kw)
+    if kw.get("lb_fn") is None or all(kw.get(k) is None for k in ("lb_target", "gap_abs_tol", "gap_rel_tol")):
+        r32 = solve_kadmm(h32, **dict(dict(stall_checks=3, stall_rtol=1e-3), **kw))
+    else:
+        r32 = solve_kadmm(h32, gap_stall=gap_stall, **kw)
     if r32["stop_reason"] != "stall":
         r32.update(iters32=r32["iters"], iters64=0, wall32_s=r32["wall_s"], wall64_s=0.0)
         return r32

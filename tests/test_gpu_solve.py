@@ -63,6 +63,23 @@ def test_reference_opt_brackets(key, method, args):
     assert r.lb <= ref["opt"] + 1e-9 and r.ub >= ref["lb"] - 1e-9
 
 
+@pytest.mark.parametrize("method,args", [("bm_admm", {}), ("bm_admm", {"dtype": "mixed"}), ("flow_cg", {})])
+def test_race_to_reference_opt(method, args):
+    """calculate_distance(opt=...) races to ub <= (1 + opt_rel_tol) opt and computes no lower bound
+    (rum_solver.py:333-453 of the reference); the portable criterion of its stored race records
+    (tests/test_gpu.py:75-104 there) is ub <= 1.01 OPT. The ub is a true upper bound, so it also
+    stays above the reference's certified lower bound."""
+    refs = json.load(open(os.path.join(GOLDEN, "opt_refs_subset.json")))
+    ref = refs["n18_d1pct"]
+    cd, _mp = instances.get_instance(18, "d1pct")
+    rum = RUM.from_rows(cd, 18)
+    r = rum.calculate_distance(opt=ref["opt"], opt_rel_tol=0.01, method=method, method_args=args, max_time=120)
+    print(r)
+    assert r.mode == "race" and r.exit_flag == 4, r.message
+    assert r.ub <= 1.01 * ref["opt"] + 1e-12 and r.ub >= ref["lb"] - 1e-9
+    assert np.isnan(r.lb) and r.target == pytest.approx(1.01 * ref["opt"])
+
+
 def test_session_thresholds_and_warm_start():
     cd, _mp = instances.get_instance(14, "m8")
     n = 14
