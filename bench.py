@@ -19,6 +19,8 @@ Reported beside the value (one JSON line on stdout):
   cpu_baseline  the CPU oracle (oracle/rum_oracle.cpp, one core) on a bounded sample of the same
              solve: a few float64 iterations plus one push and one lower bound, scaled to a
              certified solve by the GPU run's iteration and check counts
+  dsr_bootstrap  the reference's stored DSR bootstrap (n = 25, B = 10,000) run whole through the
+             batched path: its time beside the stored run's, and the draws whose decision differs
 Inputs are resident on the GPU before the timed region (the engines are built in warmup); each
 solve call returns after its last stream synchronization, so the host clock brackets the device
 work exactly.
@@ -60,6 +62,8 @@ def _parse():
     ap.add_argument("--boot-plain-draws", type=int, default=128,
                     help="of those, solved one by one as the reference does (the plain leg)")
     ap.add_argument("--boot-slots", type=int, default=64)
+    ap.add_argument("--no-dsr", action="store_true",
+                    help="skip the whole DSR alternative-0 bootstrap (n=25, B=10,000) checked against the stored run")
     ap.add_argument("--bootstrap-only", action="store_true",
                     help="only the bootstrap leg (prints its JSON object): the workload of its rocprof summary")
     ap.add_argument("--traffic-run", action="store_true",
@@ -225,6 +229,7 @@ def main():
     if not a.no_cpu:
         cpu = _cpu_baseline(n, rum, p, r_last, a)
     boot = _bootstrap_bench(a) if a.boot_draws > 0 else None
+    dsr = None if a.no_dsr else _dsr_bench()
 
     line = dict(metric=METRIC, value=round(value, 5), unit="certified solves/s", n_gpus=1, steps=a.steps,
                 warmup=a.warmup, ms_per_step=round(1e3 * elapsed / a.steps, 2), higher_is_better=True,
@@ -238,7 +243,7 @@ def main():
                             iters32=[r.info.get("iters32") for r in results],
                             iters64=[r.info.get("iters64") for r in results],
                             bracket=[r_last.lb, r_last.ub], certified=certified),
-                roofline=roofline, cpu_baseline=cpu, plain=plain, bootstrap=boot,
+                roofline=roofline, cpu_baseline=cpu, plain=plain, bootstrap=boot, dsr_bootstrap=dsr,
                 tuned_vs_plain=round(value / plain["value"], 3) if plain and plain["value"] > 0 else None)
     print(json.dumps(line), flush=True)
 
@@ -319,6 +324,48 @@ def _bootstrap_bench(a):
                 slots=a.boot_slots, rounds=st["rounds"], price_s=round(st["price_s"], 3), host_s=round(st["host_s"], 3),
                 start_s=round(t_start, 3),
                 learned="not applied to flow_cg draws; the MLP warm-starts BM-ADMM slots (tests/test_gpu_learned.py)")
+
+
+def _dsr_bench():
+    """The whole bootstrap of the reference's stored DSR run (tests/golden: the n = 25 overload panel,
+    alternative 0, 301 cells, 520 respondents; B = 10,000, multinomial_reject seed 0, abs 5e-4,
+    the default method), end to end through rum.bootstrap(..., batch={"slots": 64}).p_value(). The
+    stored record holds that run's per-draw brackets and its time on an RTX 5090; every draw's
+    decision here is compared with the stored one."""
+    import numpy as np
+    import pandas as pd
+    from solver.inference import cells_from_panel, hong_li
+    from solver.rum_solver import RUM
+    gold = os.path.join(ROOT, "tests", "golden")
+    panel = pd.read_csv(os.path.join(gold, "dsr_overload_data_25_long.csv"))
+    with open(os.path.join(gold, "dsr_alt0_record.json")) as f:
+        rec = json.load(f)
+    menu, alt, p = cells_from_panel(panel, 25, alternatives=[0])
+    rum = RUM(menu, alt, p, num_alternatives=25)
+    t0 = time.perf_counter()
+    with rum.bootstrap(panel=panel, B=10000, batch={"slots": 64}) as boot:
+        pv = boot.p_value()
+        t = time.perf_counter() - t0
+        dec = hong_li.decide(boot.L, boot.U, *boot.tau)
+        L, U = boot.L, boot.U
+        same_draws = boot.fingerprints["draws"] == rec["fingerprints"]["draws"]
+    dr = rec["draws"]
+    rL, rU = np.asarray(dr["lb"], dtype=np.float64), np.asarray(dr["ub"], dtype=np.float64)
+    rdec = hong_li.decide(rL, rU, *rec["tau"])
+    both = np.isfinite(U) & np.isfinite(rU)
+    overlap = int(((L[both] <= rU[both] + 1e-9) & (rL[both] <= U[both] + 1e-9)).sum())
+    rpv = rec["results"]["p_value"]["0.05"]
+    print("[bench] DSR bootstrap: p %.4f in %.2f s (stored run: p %.4f in %.1f s); %d draws decided differently"
+          % (pv.p, t, rpv["p"], rpv["time"], int((dec != rdec).sum())), file=sys.stderr, flush=True)
+    return dict(workload="the reference's stored DSR run: n=25 overload panel, alternative 0 (301 cells, 520 "
+                         "respondents), B=10,000, multinomial_reject seed 0, abs 5e-4, whole p_value() end to end",
+                seconds=round(t, 2), draws_per_s=round(10000 / t, 1), p=pv.p,
+                below=pv.n_below, above=pv.n_above, undecided=pv.n_undecided, screened=pv.n_screened,
+                stored_run=dict(seconds=round(rpv["time"], 1), gpu="RTX 5090", p=rpv["p"], below=rpv["n_below"],
+                                above=rpv["n_above"], undecided=rpv["n_undecided"]),
+                speedup_vs_stored_run=round(rpv["time"] / t, 2), draws_fingerprint_equal=bool(same_draws),
+                decisions_differing=int((dec != rdec).sum()), brackets_overlapping=overlap,
+                brackets_compared=int(both.sum()))
 
 
 def _cpu_baseline(n, rum, p, r_last, a):

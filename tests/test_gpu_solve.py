@@ -125,3 +125,32 @@ def test_bootstrap_small_panel_end_to_end(tmp_path):
         assert int((b2.flag != 127).sum()) == solved
         pv2 = b2.p_value()
         assert pv2.p == pv.p
+
+
+@pytest.mark.timeout(600)
+def test_dsr_bootstrap_matches_stored_run():
+    """The whole stored DSR run (n = 25 overload panel, alternative 0; B = 10,000, seed 0, abs
+    5e-4, the default method) through the batched driver: the same draws (fingerprint), the same
+    p-value and the same decision for every draw as the reference's stored brackets give, and every
+    draw's bracket overlaps the stored one (both hold the draw's true distance)."""
+    from solver.inference import hong_li
+    panel = pd.read_csv(os.path.join(GOLDEN, "dsr_overload_data_25_long.csv"))
+    menu, alt, p = cells_from_panel(panel, 25, alternatives=[0])
+    rum = RUM(menu, alt, p, num_alternatives=25)
+    rec = json.load(open(os.path.join(GOLDEN, "dsr_alt0_record.json")))
+    with rum.bootstrap(panel=panel, B=10000, batch={"slots": 64}) as boot:
+        pv = boot.p_value()
+        print(pv)
+        assert boot.fingerprints == rec["fingerprints"]
+        dec = hong_li.decide(boot.L, boot.U, *boot.tau)
+        L, U = boot.L.copy(), boot.U.copy()
+    stored = rec["results"]["p_value"]["0.05"]
+    assert (pv.p, pv.n_below, pv.n_above, pv.n_undecided, pv.n_screened) == \
+        (stored["p"], stored["n_below"], stored["n_above"], stored["n_undecided"], stored["n_screened"])
+    rL, rU = np.asarray(rec["draws"]["lb"], dtype=np.float64), np.asarray(rec["draws"]["ub"], dtype=np.float64)
+    rdec = hong_li.decide(rL, rU, *rec["tau"])
+    assert np.flatnonzero(dec != rdec).size == 0
+    both = np.isfinite(U) & np.isfinite(rU)
+    # screened draws carry their screen's finite upper bound in both runs
+    assert both.sum() == 10000
+    assert np.all(L[both] <= rU[both] + 1e-9) and np.all(rL[both] <= U[both] + 1e-9)
