@@ -145,7 +145,8 @@ int rumgpu_cg_create(rumgpu_ctx* ctx, int n, const uint64_t* masks_x, const uint
                      int dtype, int k_pred, rumgpu_cg** out);
 int rumgpu_cg_destroy(rumgpu_cg* cg);
 /* one pricing sweep: scores (float32, x-major) -> vmax (float64 decode), rank_pos[n] (position
- * of each alternative, 0 = best), margin (u16/u24 quantization bound, 0 otherwise) */
+ * of each alternative, 0 = best), margin (u16/u24 quantization bound, 0 otherwise). scores_x =
+ * NULL prices the scores rumgpu_cg_cols_scores left on the device. */
 int rumgpu_cg_price(rumgpu_cg* cg, const float* scores_x, double* vmax, int32_t* rank_pos, double* margin);
 /* D pricing sweeps in one pass over the lattice (mask layout): scores is D x M (draw-major, x-major
  * rows), vmax[D], rank_pos[D x n]. Draw d's result equals rumgpu_cg_price on row d of scores bit
@@ -175,6 +176,26 @@ int rumgpu_cg_info(rumgpu_cg* cg, size_t* device_bytes, double* last_kernel_ms);
  * alternative the closure sweep uses, or -1 when the layered sweep runs. */
 int rumgpu_cg_set_sweep(rumgpu_cg* cg, int mode);
 int rumgpu_cg_sweep_info(rumgpu_cg* cg, int* one_alternative);
+
+/* Column store of one column-generation run, on the pricer's device and rows. Replaces the host
+ * column bookkeeping of solve_cgbfqp_cholbitset (flow_cg/solver.py:92-115, :145, :240): the packed
+ * column bitsets and Gram rows of BitsetGramMaintainer (flow_cg/qp_bitset_incremental.py:36-79),
+ * p_out = V lam of PoutMaintainer (:99-130) and the ranking-to-column map (map_ranking_to_choice,
+ * flow_cg/cg_core.py:64-80). Columns are bitsets over the x-major rows: bit r % 64 of word r / 64,
+ * W = ceil(M / 64) words per column.
+ * cols_init: p_x[M] (float64, x-major) and room for cap columns; drops stored columns.
+ * cols_load: k columns (k x W words) replace the stored ones (a warm start's columns).
+ * cols_add: the column of a ranking (rank_pos[n], a permutation; row r is 1 iff its alternative
+ *   is its menu's best-ranked member) is appended; gram[K + 1] receives its inner products with
+ *   columns 0..K (itself last, exact integers); bits[W] receives it unless null.
+ * cols_scores: p_out = sum_t lam[t] col[idx[t]] (in t order) and scores = float32(p - p_out) stay
+ *   on the device; rumgpu_cg_price(cg, NULL, ...) then prices those scores.
+ * cols_pout: p_out[M] of the last cols_scores. */
+int rumgpu_cg_cols_init(rumgpu_cg* cg, const double* p_x, int64_t cap);
+int rumgpu_cg_cols_load(rumgpu_cg* cg, int64_t k, const uint64_t* bits);
+int rumgpu_cg_cols_add(rumgpu_cg* cg, const int32_t* rank_pos, double* gram, uint64_t* bits);
+int rumgpu_cg_cols_scores(rumgpu_cg* cg, int32_t nA, const int32_t* idx, const double* lam);
+int rumgpu_cg_cols_pout(rumgpu_cg* cg, double* p_out);
 
 /* Flow-CG restricted-master QP on the host: min 1/2 lam'Q lam - h'lam, lam >= 0, sum lam = 1, by
  * the active set of solve_qp_native (common/qp_native.py:37-100) with its warm working set

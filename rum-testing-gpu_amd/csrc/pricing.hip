@@ -749,6 +749,16 @@ struct rumgpu_cg {
   u64 cap64 = 0;
   float* R = nullptr;        // slab
   int32_t* rank = nullptr;
+  // column store of one column-generation run (rumgpu_cg_cols_*)
+  u64* cbits = nullptr;      // [ccap][cW]: bit r % 64 of word r / 64 = x-major row r is in the column
+  i64 ccap = 0, cK = 0, cW = 0;
+  double* cp = nullptr;      // observed p per x-major row
+  double* cpout = nullptr;   // V lam of the last rumgpu_cg_cols_scores
+  unsigned long long* cq = nullptr;   // inner products of the last added column [ccap]
+  int32_t* cidx = nullptr;   // support of lam [ccap]
+  double* clam = nullptr;    // lam on that support [ccap]
+  int32_t* crank = nullptr;  // rank positions of the column being added
+  bool scores_resident = false;   // c->scores holds float(p - V lam) from rumgpu_cg_cols_scores
   // stream
   u64* binom = nullptr;
   std::vector<u64> h_binom;
@@ -775,7 +785,8 @@ static void cg_free_all(rumgpu_cg* c) {
                   c->binom, c->vbuf[0], c->vbuf[1], c->lbuf[0], c->lbuf[1], c->mbuf[0], c->mbuf[1], c->ebuf[0],
                   c->ebuf[1], c->lstep, c->subV, c->subpred, c->subnodes, c->submasks, c->subscores, c->suboffsets,
                   c->Vb, c->predb, c->scoresb, c->vmaxb, c->rankb, c->V64, c->r64, c->V64b, c->r64b, c->clcm, c->clF, c->cltmax,
-                  c->clbmax, c->clvmax, c->clbmax64, c->clv64, c->clerr};
+                  c->clbmax, c->clvmax, c->clbmax64, c->clv64, c->clerr, c->cbits, c->cp, c->cpout, c->cq,
+                  c->cidx, c->clam, c->crank};
   for (void* q : bufs) rg_free(q);
   for (auto* p : c->predk) rg_free(p);
 }
@@ -1030,12 +1041,17 @@ extern "C" int rumgpu_cg_sweep_info(rumgpu_cg* c, int* one_alternative) {
 }
 
 extern "C" int rumgpu_cg_price(rumgpu_cg* c, const float* scores_x, double* vmax, int32_t* rank_pos, double* margin) {
-  RG_CHECK(c && scores_x && vmax && rank_pos, RG_EINVAL, "cg_price: null argument");
+  RG_CHECK(c && vmax && rank_pos, RG_EINVAL, "cg_price: null argument");
+  RG_CHECK(scores_x || c->scores_resident, RG_EINVAL,
+           "cg_price: no scores (pass them, or compute them on the device with rumgpu_cg_cols_scores)");
   RG_HIP(hipSetDevice(c->ctx->device));
   hipStream_t st = c->ctx->stream;
   const int n = c->n;
   auto t0 = std::chrono::steady_clock::now();
-  RG_HIP(hipMemcpyAsync(c->scores, scores_x, (size_t)c->M * 4, hipMemcpyHostToDevice, st));
+  if (scores_x) {
+    RG_HIP(hipMemcpyAsync(c->scores, scores_x, (size_t)c->M * 4, hipMemcpyHostToDevice, st));
+    c->scores_resident = false;
+  }
   if (margin) *margin = 0.0;
   if (use_closure(c)) {
     RG_TRY(cl_sweep<1>(c, st, c->scores + c->h_offsets[c->single_a], 1, c->rank, c->clvmax));
@@ -1140,7 +1156,15 @@ extern "C" int rumgpu_cg_price(rumgpu_cg* c, const float* scores_x, double* vmax
     for (int k = 1; k <= n; ++k) mg += (double)steps[k];
   }
   if (margin) *margin = mg;
-  // backtrace: stored predecessors down to layer k_pred, then the exact sub-lattice
+  // backtrace: stored predecessors down to layer k_pred, then the exact sub-lattice, whose rows
+  // are gathered on the host (from the device copy when the scores are resident)
+  std::vector<float> resident;
+  const float* sx = scores_x;
+  if (!sx) {
+    resident.resize((size_t)c->M);
+    RG_HIP(rg_copy(st, resident.data(), c->scores, (size_t)c->M * 4, hipMemcpyDeviceToHost));
+    sx = resident.data();
+  }
   u64 D = (n >= 64) ? ~0ull : ((1ull << n) - 1ull);
   int pos = 0;
   for (int k = n; k > c->k_pred; --k) {
@@ -1167,7 +1191,7 @@ extern "C" int rumgpu_cg_price(rumgpu_cg* c, const float* scores_x, double* vmax
       u32 sm = 0;
       for (int e = 0; e < K; ++e)
         if ((m >> elems[e]) & 1ull) sm |= 1u << e;
-      rows[posof[x]].push_back({sm, scores_x[j]});
+      rows[posof[x]].push_back({sm, sx[j]});
     }
   }
   std::vector<u32> sm;
@@ -1405,5 +1429,216 @@ extern "C" int rumgpu_cg_info(rumgpu_cg* c, size_t* device_bytes, double* last_k
   RG_CHECK(c, RG_EINVAL, "cg_info: null handle");
   if (device_bytes) *device_bytes = c->device_bytes;
   if (last_kernel_ms) *last_kernel_ms = c->last_ms;
+  return RG_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Column store of one column-generation run. The ranking columns are bitsets over the x-major rows
+// (bit r % 64 of word r / 64), built on the device from the ranking the sweep returned; the new
+// column's Gram row and the residual scores float(p - V lam) of the next sweep are computed where
+// the columns live, so a run's per-column host work is the master QP on the K x K Gram matrix.
+
+// one word per wavefront: lane l of the wave that starts at row 64 w decides row 64 w + l
+template <typename MT>
+__global__ __launch_bounds__(256) void cg_col_bits(const MT* __restrict__ masks, const u64* __restrict__ offsets,
+                                                   int n, i64 M, i64 W, const int32_t* __restrict__ rank_pos,
+                                                   u64* __restrict__ out) {
+  __shared__ int rk[64];
+  __shared__ u64 off[65];
+  if (threadIdx.x < (unsigned)n) rk[threadIdx.x] = rank_pos[threadIdx.x];
+  if (threadIdx.x <= (unsigned)n) off[threadIdx.x] = offsets[threadIdx.x];
+  __syncthreads();
+  const i64 r = (i64)blockIdx.x * 256 + threadIdx.x;
+  bool on = false;
+  if (r < M) {
+    int x = 0;                                   // the row's alternative: offsets[x] <= r < offsets[x + 1]
+    while (off[x + 1] <= (u64)r) ++x;
+    u64 S = (u64)masks[r];
+    int best = 1 << 30, top = -1;                // the menu's top member under the ranking
+    while (S) {
+      const int b = __builtin_ctzll(S);
+      S &= S - 1;
+      if (rk[b] < best) {
+        best = rk[b];
+        top = b;
+      }
+    }
+    on = top == x;
+  }
+  const u64 word = __ballot(on);
+  if ((threadIdx.x & 63) == 0 && (r >> 6) < W) out[r >> 6] = word;
+}
+
+// q[j] = |col_j AND col_k| for j = 0..k (block j)
+__global__ __launch_bounds__(256) void cg_col_gram(const u64* __restrict__ bits, i64 W, i64 k,
+                                                   unsigned long long* __restrict__ q) {
+  __shared__ unsigned long long part[4];
+  const u64* a = bits + (i64)blockIdx.x * W;
+  const u64* b = bits + k * W;
+  unsigned long long s = 0;
+  for (i64 w = threadIdx.x; w < W; w += 256) s += (unsigned long long)__popcll(a[w] & b[w]);
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) q[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
+}
+
+// p_out[r] = sum_t lam[t] [r in col idx[t]] (in t order), scores[r] = float(p[r] - p_out[r])
+__global__ __launch_bounds__(256) void cg_col_scores(const u64* __restrict__ bits, i64 W, i64 M, int nA,
+                                                     const int32_t* __restrict__ idx,
+                                                     const double* __restrict__ lam, const double* __restrict__ p,
+                                                     double* __restrict__ pout, float* __restrict__ scores) {
+  __shared__ i64 base[256];
+  __shared__ double sl[256];
+  const i64 r = (i64)blockIdx.x * 256 + threadIdx.x;
+  const i64 w = r < M ? (r >> 6) : 0;
+  const int l = (int)(r & 63);
+  double acc = 0.0;
+  for (int a0 = 0; a0 < nA; a0 += 256) {
+    const int cnt = nA - a0 < 256 ? nA - a0 : 256;
+    __syncthreads();
+    if ((int)threadIdx.x < cnt) {
+      base[threadIdx.x] = (i64)idx[a0 + threadIdx.x] * W;
+      sl[threadIdx.x] = lam[a0 + threadIdx.x];
+    }
+    __syncthreads();
+#pragma unroll 8
+    for (int t = 0; t < cnt; ++t) {
+      const u64 word = bits[base[t] + w];
+      acc += ((word >> l) & 1ull) ? sl[t] : 0.0;
+    }
+  }
+  if (r < M) {
+    pout[r] = acc;
+    scores[r] = (float)(p[r] - acc);
+  }
+}
+
+static int cols_grow(rumgpu_cg* c, i64 need) {
+  if (need <= c->ccap) return RG_OK;
+  i64 cap = c->ccap ? 2 * c->ccap : 64;
+  if (cap < need) cap = need;
+  u64* nb = nullptr;
+  unsigned long long* nq = nullptr;
+  int32_t* ni = nullptr;
+  double* nl = nullptr;
+  int rc = RG_OK;
+  if ((rc = rg_malloc((void**)&nb, (size_t)cap * c->cW * 8, "column bitsets")) != RG_OK) return rc;
+  if ((rc = rg_malloc((void**)&nq, (size_t)cap * 8, "Gram row")) != RG_OK ||
+      (rc = rg_malloc((void**)&ni, (size_t)cap * 4, "lambda support")) != RG_OK ||
+      (rc = rg_malloc((void**)&nl, (size_t)cap * 8, "lambda")) != RG_OK) {
+    rg_free(nb);
+    rg_free(nq);
+    rg_free(ni);
+    return rc;
+  }
+  if (c->cK > 0)
+    RG_HIP(hipMemcpyAsync(nb, c->cbits, (size_t)c->cK * c->cW * 8, hipMemcpyDeviceToDevice, c->ctx->stream));
+  RG_HIP(hipStreamSynchronize(c->ctx->stream));
+  rg_free(c->cbits);
+  rg_free(c->cq);
+  rg_free(c->cidx);
+  rg_free(c->clam);
+  c->device_bytes += (size_t)(cap - c->ccap) * ((size_t)c->cW * 8 + 20);
+  c->cbits = nb;
+  c->cq = nq;
+  c->cidx = ni;
+  c->clam = nl;
+  c->ccap = cap;
+  return RG_OK;
+}
+
+extern "C" int rumgpu_cg_cols_init(rumgpu_cg* c, const double* p_x, int64_t cap) {
+  RG_CHECK(c && p_x, RG_EINVAL, "cg_cols_init: null argument");
+  RG_CHECK(cap >= 1, RG_EINVAL, "cg_cols_init: cap must be >= 1");
+  RG_CHECK(c->masks32 || c->masks64, RG_EINVAL, "cg_cols_init: pricer has no rows");
+  RG_HIP(hipSetDevice(c->ctx->device));
+  int rc = RG_OK;
+  if (!c->cp) {
+    c->cW = (c->M + 63) / 64;
+    if ((rc = cg_alloc(c, (void**)&c->cp, (size_t)c->M * 8, "column store p")) != RG_OK) return rc;
+    if ((rc = cg_alloc(c, (void**)&c->cpout, (size_t)c->M * 8, "column store p_out")) != RG_OK) return rc;
+    if ((rc = cg_alloc(c, (void**)&c->crank, 64 * 4, "column ranking")) != RG_OK) return rc;
+  }
+  c->cK = 0;
+  c->scores_resident = false;
+  RG_TRY(cols_grow(c, cap));
+  RG_HIP(rg_copy(c->ctx->stream, c->cp, p_x, (size_t)c->M * 8, hipMemcpyHostToDevice));
+  return RG_OK;
+}
+
+extern "C" int rumgpu_cg_cols_load(rumgpu_cg* c, int64_t k, const uint64_t* bits) {
+  RG_CHECK(c && (bits || k == 0), RG_EINVAL, "cg_cols_load: null argument");
+  RG_CHECK(c->cp, RG_EINVAL, "cg_cols_load: call rumgpu_cg_cols_init first");
+  RG_CHECK(k >= 0, RG_EINVAL, "cg_cols_load: k must be >= 0");
+  RG_HIP(hipSetDevice(c->ctx->device));
+  c->cK = 0;
+  RG_TRY(cols_grow(c, k));
+  if (k) RG_HIP(rg_copy(c->ctx->stream, c->cbits, bits, (size_t)k * c->cW * 8, hipMemcpyHostToDevice));
+  c->cK = k;
+  c->scores_resident = false;
+  return RG_OK;
+}
+
+extern "C" int rumgpu_cg_cols_add(rumgpu_cg* c, const int32_t* rank_pos, double* gram, uint64_t* bits) {
+  RG_CHECK(c && rank_pos && gram, RG_EINVAL, "cg_cols_add: null argument");
+  RG_CHECK(c->cp, RG_EINVAL, "cg_cols_add: call rumgpu_cg_cols_init first");
+  const int n = c->n;
+  std::vector<char> seen(n, 0);
+  for (int b = 0; b < n; ++b) {
+    RG_CHECK(rank_pos[b] >= 0 && rank_pos[b] < n && !seen[rank_pos[b]], RG_EINVAL,
+             "cg_cols_add: rank_pos must be a permutation of 0..n-1");
+    seen[rank_pos[b]] = 1;
+  }
+  RG_HIP(hipSetDevice(c->ctx->device));
+  hipStream_t st = c->ctx->stream;
+  RG_TRY(cols_grow(c, c->cK + 1));
+  const i64 k = c->cK;
+  u64* col = c->cbits + k * c->cW;
+  RG_HIP(hipMemcpyAsync(c->crank, rank_pos, (size_t)n * 4, hipMemcpyHostToDevice, st));
+  const i64 blocks = (c->cW * 64 + 255) / 256;
+  if (c->masks32)
+    hipLaunchKernelGGL(cg_col_bits<u32>, dim3((unsigned)blocks), dim3(256), 0, st, c->masks32, c->offsets, n, c->M,
+                       c->cW, c->crank, col);
+  else
+    hipLaunchKernelGGL(cg_col_bits<u64>, dim3((unsigned)blocks), dim3(256), 0, st, c->masks64, c->offsets, n, c->M,
+                       c->cW, c->crank, col);
+  hipLaunchKernelGGL(cg_col_gram, dim3((unsigned)(k + 1)), dim3(256), 0, st, c->cbits, c->cW, k, c->cq);
+  RG_LAUNCH_CHECK();
+  std::vector<unsigned long long> q((size_t)k + 1);
+  RG_HIP(hipMemcpyAsync(q.data(), c->cq, (size_t)(k + 1) * 8, hipMemcpyDeviceToHost, st));
+  if (bits) RG_HIP(hipMemcpyAsync(bits, col, (size_t)c->cW * 8, hipMemcpyDeviceToHost, st));
+  RG_HIP(hipStreamSynchronize(st));
+  for (i64 j = 0; j <= k; ++j) gram[j] = (double)q[j];
+  c->cK = k + 1;
+  return RG_OK;
+}
+
+extern "C" int rumgpu_cg_cols_scores(rumgpu_cg* c, int32_t nA, const int32_t* idx, const double* lam) {
+  RG_CHECK(c && (nA == 0 || (idx && lam)), RG_EINVAL, "cg_cols_scores: null argument");
+  RG_CHECK(c->cp, RG_EINVAL, "cg_cols_scores: call rumgpu_cg_cols_init first");
+  RG_CHECK(nA >= 0 && nA <= c->cK, RG_EINVAL, "cg_cols_scores: nA = %d outside 0..%lld", nA, (long long)c->cK);
+  for (int t = 0; t < nA; ++t)
+    RG_CHECK(idx[t] >= 0 && idx[t] < c->cK, RG_EINVAL, "cg_cols_scores: column %d outside 0..%lld", idx[t],
+             (long long)c->cK - 1);
+  RG_HIP(hipSetDevice(c->ctx->device));
+  hipStream_t st = c->ctx->stream;
+  if (nA) {
+    RG_HIP(hipMemcpyAsync(c->cidx, idx, (size_t)nA * 4, hipMemcpyHostToDevice, st));
+    RG_HIP(hipMemcpyAsync(c->clam, lam, (size_t)nA * 8, hipMemcpyHostToDevice, st));
+  }
+  hipLaunchKernelGGL(cg_col_scores, dim3((unsigned)((c->M + 255) / 256)), dim3(256), 0, st, c->cbits, c->cW, c->M,
+                     (int)nA, c->cidx, c->clam, c->cp, c->cpout, c->scores);
+  RG_LAUNCH_CHECK();
+  RG_HIP(hipStreamSynchronize(st));
+  c->scores_resident = true;
+  return RG_OK;
+}
+
+extern "C" int rumgpu_cg_cols_pout(rumgpu_cg* c, double* p_out) {
+  RG_CHECK(c && p_out, RG_EINVAL, "cg_cols_pout: null argument");
+  RG_CHECK(c->cp && c->scores_resident, RG_EINVAL, "cg_cols_pout: no rumgpu_cg_cols_scores since the last change");
+  RG_HIP(hipSetDevice(c->ctx->device));
+  RG_HIP(rg_copy(c->ctx->stream, p_out, c->cpout, (size_t)c->M * 8, hipMemcpyDeviceToHost));
   return RG_OK;
 }

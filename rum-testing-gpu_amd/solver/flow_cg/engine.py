@@ -49,11 +49,43 @@ class Pricer:
         self.rank = np.empty(self.n, dtype=np.int32)
 
     def price(self, scores_x):
-        s = _lib.c_array(scores_x, np.float32)
+        """one sweep over scores_x (float32, x-major), or over the scores cols_scores left on the
+        device when scores_x is None"""
+        s = None if scores_x is None else _lib.c_array(scores_x, np.float32)
         vmax, margin = ctypes.c_double(), ctypes.c_double()
-        _lib.check(self.lib.rumgpu_cg_price(self.h, _lib.ptr(s), ctypes.byref(vmax), _lib.ptr(self.rank),
-                                            ctypes.byref(margin)), "cg_price")
+        _lib.check(self.lib.rumgpu_cg_price(self.h, None if s is None else _lib.ptr(s), ctypes.byref(vmax),
+                                            _lib.ptr(self.rank), ctypes.byref(margin)), "cg_price")
         return vmax.value, self.rank.copy(), margin.value
+
+    # column store of one run (rumgpu_cg_cols_*)
+    def cols_init(self, p_x, cap):
+        p = _lib.c_array(p_x, np.float64)
+        _lib.check(self.lib.rumgpu_cg_cols_init(self.h, _lib.ptr(p), int(cap)), "cg_cols_init")
+
+    def cols_load(self, bits):
+        b = np.ascontiguousarray(bits, dtype=np.uint64)
+        _lib.check(self.lib.rumgpu_cg_cols_load(self.h, int(b.shape[0]), _lib.ptr(b) if b.size else None),
+                   "cg_cols_load")
+
+    def cols_add(self, rank_pos, k):
+        """append the column of a ranking; returns (its inner products with columns 0..k, the
+        packed bitset)"""
+        r = _lib.c_array(rank_pos, np.int32)
+        gram = np.empty(k + 1)
+        bits = np.empty((self.M + 63) // 64, dtype=np.uint64)
+        _lib.check(self.lib.rumgpu_cg_cols_add(self.h, _lib.ptr(r), _lib.ptr(gram), _lib.ptr(bits)), "cg_cols_add")
+        return gram, bits
+
+    def cols_scores(self, idx, lam):
+        i = _lib.c_array(idx, np.int32)
+        v = _lib.c_array(lam, np.float64)
+        _lib.check(self.lib.rumgpu_cg_cols_scores(self.h, int(i.size), _lib.ptr(i) if i.size else None,
+                                                  _lib.ptr(v) if v.size else None), "cg_cols_scores")
+
+    def cols_pout(self):
+        out = np.empty(self.M)
+        _lib.check(self.lib.rumgpu_cg_cols_pout(self.h, _lib.ptr(out)), "cg_cols_pout")
+        return out
 
     def price_batch(self, scores):
         """D sweeps at once (mask layout): scores (D, M) x-major -> vmax[D], rank_pos (D, n)"""
@@ -214,6 +246,75 @@ class Columns:
         return self.Q[:self.K, :self.K], self.h[:self.K]
 
 
+class DeviceColumns:
+    """The ranking columns of one run in the pricer's device column store: the bitsets, p_out = V lam
+    and the float32 residual scores of the next sweep live on the GPU (rumgpu_cg_cols_*); the host
+    keeps the Gram matrix Q (exact integer counts, as Columns has them), h = V'p and the keys.
+    set_lambda(lam) recomputes p_out and the scores from lam's support."""
+
+    def __init__(self, pricer, M, p, max_K, start=64):
+        self.pricer = pricer
+        self.M, self.W = int(M), (int(M) + 63) // 64
+        self.p = np.asarray(p, dtype=np.float64)
+        self.max_K = int(max_K)
+        self.cap = 0
+        self.Q = np.zeros((0, 0))
+        self.h = np.zeros(0)
+        self._grow(min(self.max_K, max(1, int(start))))
+        self.keys = []
+        self.K = 0
+        pricer.cols_init(self.p, self.cap)
+
+    def _grow(self, cap):
+        cap = min(self.max_K, cap)
+        k = self.cap
+        Q, h = np.zeros((cap, cap)), np.zeros(cap)
+        Q[:k, :k], h[:k] = self.Q[:k, :k], self.h[:k]
+        self.Q, self.h, self.cap = Q, h, cap
+
+    def add_ranking(self, rank_pos):
+        k = self.K
+        if k >= self.max_K:
+            raise RuntimeError("max_K = %d columns exceeded" % self.max_K)
+        if k >= self.cap:
+            self._grow(2 * self.cap)
+        gram, bits = self.pricer.cols_add(rank_pos, k)
+        self.Q[k, :k] = gram[:k]
+        self.Q[:k, k] = gram[:k]
+        self.Q[k, k] = gram[k]
+        col = np.unpackbits(bits.view(np.uint8), bitorder="little")[:self.M]
+        self.h[k] = float(self.p[np.flatnonzero(col)].sum())
+        self.keys.append(col.tobytes())
+        self.K += 1
+
+    def load_template(self, tpl):
+        k = tpl.K
+        if k > self.max_K:
+            raise RuntimeError("max_K = %d columns exceeded" % self.max_K)
+        if k > self.cap:
+            self._grow(max(k + 64, 2 * self.cap))
+        self.Q[:k, :k] = tpl.Q[:k, :k]
+        for i in range(k):
+            self.h[i] = float(self.p[tpl.pos[i]].sum())
+        self.pricer.cols_load(tpl.bits[:k])
+        self.keys = list(tpl.keys)
+        self.K = k
+
+    def set_lambda(self, lam):
+        lam = np.asarray(lam, dtype=np.float64)[:self.K]
+        idx = np.flatnonzero(lam)
+        self.pricer.cols_scores(idx.astype(np.int32), lam[idx])
+
+    def view(self):
+        return self.Q[:self.K, :self.K], self.h[:self.K]
+
+    @property
+    def p_out(self):
+        return self.pricer.cols_pout()
+
+
+RESIDENT = "resident"      # a request to price the scores DeviceColumns.set_lambda left on the device
+
 _TEMPLATES = []
 
 
@@ -258,6 +359,8 @@ def answer(pricer, req):
     longest path for ("vmax64", r)."""
     if isinstance(req, tuple):
         return pricer.vmax64(req[1])
+    if isinstance(req, str) and req == RESIDENT:
+        return pricer.price(None)
     return pricer.price(req)
 
 
@@ -273,10 +376,14 @@ def drive(gen, pricer):
 
 def cg_steps(n, menu, alt, p, pricer, rc_tol=0.0, max_iters=4000, time_limit_s=0.0, seed=0, verbose=False,
              phi_ub_target=None, gap_rel_tol=None, gap_abs_tol=None, init_columns=None, init_lambda=None,
-             early_ub_check=False, lb_stop=None, qp="native"):
+             early_ub_check=False, lb_stop=None, qp="native", resident=False):
     """solve_cg's loop as a generator: yields the float32 scores of each pricing sweep, receives
     (vmax, rank_pos, margin), returns the result dict. A driver may price many of these loops in
-    one batched sweep (inference/batched.py); each loop's arithmetic is the same either way."""
+    one batched sweep (inference/batched.py); each loop's arithmetic is the same either way.
+    resident=True (a driver that gives this loop the pricer to itself) keeps the columns of a run
+    with more than Columns.DENSE_ROWS rows in the pricer's device column store (DeviceColumns) and
+    yields RESIDENT instead of the scores: the column of each ranking, its Gram row, p_out and the
+    scores are then computed on the GPU."""
     t_setup = time.perf_counter()
     qp_state = QPState() if qp == "native" else None
     qp_solve = {"native": lambda Q, h, A_init: solve_qp_native(Q, h, A_init=A_init, state=qp_state),
@@ -285,24 +392,31 @@ def cg_steps(n, menu, alt, p, pricer, rc_tol=0.0, max_iters=4000, time_limit_s=0
     p32 = np.asarray(p, dtype=np.float64)[perm].astype(np.float32)
     p_obs = p32.astype(np.float64)
     M = p_obs.size
-    memb = ((masks_x[:, None] >> np.arange(n, dtype=np.uint64)[None, :]) & np.uint64(1)).astype(bool)
+    dev = bool(resident) and M > Columns.DENSE_ROWS
+    memb = None if dev else ((masks_x[:, None] >> np.arange(n, dtype=np.uint64)[None, :]) & np.uint64(1)).astype(bool)
     setup_s = time.perf_counter() - t_setup
     p_sq = float(p_obs @ p_obs)
     n_init = len(init_columns) if init_columns is not None else 0
-    cols = Columns(M, p_obs, max_iters + 2 + n_init)
+    cols = (DeviceColumns(pricer, M, p_obs, max_iters + 2 + n_init) if dev
+            else Columns(M, p_obs, max_iters + 2 + n_init))
     if n_init:
         cols.load_template(_warm_template(init_columns, M))
         lam = (np.asarray(init_lambda, dtype=np.float64).copy() if init_lambda is not None
                else np.full(n_init, 1.0 / n_init))
         lam = np.maximum(lam, 0.0)
         lam /= lam.sum()
-        cols.apply_delta(lam)
+        if not dev:
+            cols.apply_delta(lam)
         A_warm = [int(i) for i in np.flatnonzero(lam > 1e-12)] or [0]
     else:
         rank0 = np.random.RandomState(seed).permutation(n)
-        cols.add(ranking_column(memb, alt_x, rank0, n))
+        if dev:
+            cols.add_ranking(rank0)
+        else:
+            cols.add(ranking_column(memb, alt_x, rank0, n))
         lam = np.array([1.0])
-        cols.apply_delta(lam)
+        if not dev:
+            cols.apply_delta(lam)
         A_warm = [0]
     UB, lb_best, gap_rel, rc_final = float("inf"), 0.0, float("inf"), float("nan")
     stop = "max_iters"
@@ -317,7 +431,8 @@ def cg_steps(n, menu, alt, p, pricer, rc_tol=0.0, max_iters=4000, time_limit_s=0
         tq = time.perf_counter()
         Qv, hv = cols.view()
         lam_new, A_warm = qp_solve(Qv, hv, A_init=A_warm)
-        cols.apply_delta(lam_new - lam)
+        if not dev:
+            cols.apply_delta(lam_new - lam)
         lam = lam_new
         lQl = float(lam @ (Qv @ lam))
         hl = float(hv @ lam)
@@ -329,9 +444,13 @@ def cg_steps(n, menu, alt, p, pricer, rc_tol=0.0, max_iters=4000, time_limit_s=0
             if UB <= phi_ub_target:
                 stop = "phi_ub"
                 break
-        resid = p_obs - cols.p_out
+        if dev:
+            cols.set_lambda(lam)
+            req = RESIDENT
+        else:
+            req = (p_obs - cols.p_out).astype(np.float32)
         tpz = time.perf_counter()
-        vmax, rank_pos, margin = yield resid.astype(np.float32)
+        vmax, rank_pos, margin = yield req
         price_s += time.perf_counter() - tpz
         if margin:
             margins.append(margin)
@@ -358,10 +477,15 @@ def cg_steps(n, menu, alt, p, pricer, rc_tol=0.0, max_iters=4000, time_limit_s=0
         if fw_gap < rc_tol:
             stop = "rc"
             break
-        cols.add(ranking_column(memb, alt_x, rank_pos, n))
+        if dev:
+            cols.add_ranking(rank_pos)
+        else:
+            cols.add(ranking_column(memb, alt_x, rank_pos, n))
         lam = np.concatenate([lam, [0.0]])
     wall = time.perf_counter() - t0
     dev_bytes = pricer.info()[0]
+    if dev:
+        cols.set_lambda(lam)
     return dict(wall_s=wall, setup_s=setup_s, trace=trace, margins=margins, iters=it, n_cols=cols.K, lam=lam,
                 p_out=cols.p_out.copy(), perm=perm, columns=[np.frombuffer(k, dtype=np.uint8).copy() for k in cols.keys],
                 rc_final=rc_final, distance=UB, lb=lb_best, gap_rel=gap_rel, stop_reason=stop, qp_s=qp_s,

@@ -854,12 +854,14 @@ def _run_flow_cg(t):
     one resume from the engine's columns when the float64 bracket misses the tolerance."""
     from .flow_cg.engine import drive
     pricer = _cg_pricer(t)
-    return drive(_flow_cg_steps(t, pricer), pricer)
+    # "_cols": "host" keeps the columns on the host as the reference does (qp_bitset_incremental.py)
+    return drive(_flow_cg_steps(t, pricer, resident=t.args.get("_cols", "device") == "device"), pricer)
 
 
-def _flow_cg_steps(t, pricer):
+def _flow_cg_steps(t, pricer, resident=False):
     """_run_flow_cg's body as a generator of pricing requests (flow_cg.engine.cg_steps); returns the
-    stop reason. The batched bootstrap drives many of these with one batched sweep per round."""
+    stop reason. The batched bootstrap drives many of these with one batched sweep per round.
+    resident: the pricer is this run's alone, so large runs keep their columns on the device."""
     from .flow_cg.engine import cg_steps
     g, a, st = t.geom, t.args, t.state
     n = g.num_alternatives
@@ -876,7 +878,7 @@ def _flow_cg_steps(t, pricer):
             return "time"
         t_call = time.perf_counter()
         kw = dict(max_iters=int(a["max_iters"]), time_limit_s=remaining, seed=t.seed, verbose=t.verbose, **warm,
-                  **stops, qp=a.get("_qp", "native"))
+                  **stops, qp=a.get("_qp", "native"), resident=resident)
         if not race:
             kw.update(gap_rel_tol=rel, gap_abs_tol=abs_e if abs_e else None)
         r = yield from cg_steps(n, g.menu, g.alternative, t.p, pricer, **kw)

@@ -228,6 +228,59 @@ def test_stream_pricing_bit_exact(n, mode):
     pr.close()
 
 
+@pytest.mark.parametrize("layout", ["mask", "slab", "stream"])
+@pytest.mark.parametrize("n,cells", [(9, 1000), (14, 7001)])
+def test_column_store_matches_oracle(n, cells, layout):
+    """rumgpu_cg_cols_*: each ranking's column bitset equals the oracle's ranking column on the
+    x-major rows, its Gram row equals the integer overlaps, p_out = sum_t lam_t col_(idx_t) equals the
+    same sum taken in t order on the host bit for bit, and pricing the resident scores equals pricing
+    those scores passed from the host. 70 columns from an initial room of 2 also cover the store's
+    growth; a reload of the first 5 columns continues at index 5."""
+    rng = np.random.default_rng(300 + n)
+    menu, alt, p = _medium(n, cells, rng)
+    perm, masks_x, alt_x, offsets = xmajor(menu, alt, n)
+    M = masks_x.size
+    W = (M + 63) // 64
+    pr = Pricer(n, masks_x, offsets, layout=layout)
+    p_x = p[perm].astype(np.float32).astype(np.float64)
+    pr.cols_init(p_x, 2)
+    cols, bits_all = [], []
+    for k in range(70):
+        rank = rng.permutation(n).astype(np.int32)
+        gram, bits = pr.cols_add(rank, k)
+        col = O.ranking_column(n, masks_x, alt_x, rank).astype(bool)
+        assert np.array_equal(np.unpackbits(bits.view(np.uint8), bitorder="little")[:M].astype(bool), col), k
+        assert bits.size == W and not np.any(np.unpackbits(bits.view(np.uint8), bitorder="little")[M:])
+        cols.append(col)
+        bits_all.append(bits)
+        assert np.array_equal(gram, [float((c & col).sum()) for c in cols]), k
+    for trial in range(3):
+        idx = np.sort(rng.choice(70, size=int(rng.integers(1, 70)), replace=False)).astype(np.int32)
+        lam = rng.random(idx.size)
+        lam /= lam.sum()
+        pr.cols_scores(idx, lam)
+        acc = np.zeros(M)
+        for i, v in zip(idx, lam):
+            acc = acc + np.where(cols[i], v, 0.0)
+        assert np.array_equal(pr.cols_pout(), acc)
+        scores = (p_x - acc).astype(np.float32)
+        a = pr.price(None)
+        b = pr.price(scores)
+        assert a[0] == b[0] and np.array_equal(a[1], b[1]) and a[2] == b[2]
+    pr.cols_load(np.stack(bits_all[:5]))
+    rank = rng.permutation(n).astype(np.int32)
+    gram, bits = pr.cols_add(rank, 5)
+    col = O.ranking_column(n, masks_x, alt_x, rank).astype(bool)
+    assert np.array_equal(gram, [float((c & col).sum()) for c in cols[:5] + [col]])
+    pr.cols_scores(np.array([5], dtype=np.int32), np.array([1.0]))
+    assert np.array_equal(pr.cols_pout(), np.where(col, 1.0, 0.0))
+    with pytest.raises(ValueError):
+        pr.cols_add(np.zeros(n, dtype=np.int32), 6)          # not a permutation
+    with pytest.raises(ValueError):
+        pr.cols_scores(np.array([6], dtype=np.int32), np.array([1.0]))   # only columns 0..5 are stored
+    pr.close()
+
+
 @pytest.mark.parametrize("n,alt0", [(7, False), (12, False), (16, False), (13, True)])
 def test_mask_lb64_matches_slab_bound(n, alt0):
     """flow_cg's float64 lower bound on the mask lattice and on the stream layout's two float64

@@ -80,6 +80,32 @@ def test_race_to_reference_opt(method, args):
     assert np.isnan(r.lb) and r.target == pytest.approx(1.01 * ref["opt"])
 
 
+@pytest.mark.parametrize("key", ["n18_m15", "n19_m29"])
+def test_flow_cg_device_columns_equal_host_columns(key):
+    """A run with more than 4,096 rows keeps its columns in the pricer's device store; with
+    _cols = "host" the same run keeps them on the host. Both certify the reference's optimum
+    bracket, and warm restarts from either run's columns certify too."""
+    refs = json.load(open(os.path.join(GOLDEN, "opt_refs_subset.json")))
+    ref = refs[key]
+    n, cat = int(key.split("_")[0][1:]), key.split("_", 1)[1]
+    cd, _mp = instances.get_instance(n, cat)
+    rum = RUM.from_rows(cd, n)
+    out = {}
+    for cols in ("device", "host"):
+        with rum.session(method="flow_cg") as s:
+            s._args["_cols"] = cols
+            r = s.solve(rum.choice_probability, abs_tol=0.0, rel_tol=1e-3, keep=True)
+            p2 = np.clip(rum.choice_probability * 0.98 + 0.02 / n, 0, 1)
+            rw = s.solve(p2, abs_tol=1e-4, rel_tol=0.0, warm=True)
+        print(cols, r, rw)
+        assert r.exit_flag in (2, 3) and r.lb <= ref["opt"] + 1e-9 and r.ub >= ref["lb"] - 1e-9
+        assert rw.exit_flag in (1, 3) and rw.ub - rw.lb <= 1e-4
+        out[cols] = (r, rw)
+    (d, dw), (h, hw) = out["device"], out["host"]
+    assert d.lb <= h.ub + 1e-9 and h.lb <= d.ub + 1e-9
+    assert dw.lb <= hw.ub + 1e-9 and hw.lb <= dw.ub + 1e-9
+
+
 def test_session_thresholds_and_warm_start():
     cd, _mp = instances.get_instance(14, "m8")
     n = 14
