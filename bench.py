@@ -64,6 +64,9 @@ def _parse():
     ap.add_argument("--boot-slots", type=int, default=64)
     ap.add_argument("--no-dsr", action="store_true",
                     help="skip the whole DSR alternative-0 bootstrap (n=25, B=10,000) checked against the stored run")
+    ap.add_argument("--reference-instances", action="store_true",
+                    help="only the instances with the reference's certified optima (tests/golden/opt_refs_subset.json):"
+                         " each solved to rel 1e-3 by the method of its stored record, timed beside the record")
     ap.add_argument("--bootstrap-only", action="store_true",
                     help="only the bootstrap leg (prints its JSON object): the workload of its rocprof summary")
     ap.add_argument("--traffic-run", action="store_true",
@@ -103,6 +106,9 @@ def main():
 
     if a.bootstrap_only:
         print(json.dumps(_bootstrap_bench(a)), flush=True)
+        return
+    if a.reference_instances:
+        print(json.dumps(_reference_instances()), flush=True)
         return
 
     cd, n = instances.config2()
@@ -324,6 +330,42 @@ def _bootstrap_bench(a):
                 slots=a.boot_slots, rounds=st["rounds"], price_s=round(st["price_s"], 3), host_s=round(st["host_s"], 3),
                 start_s=round(t_start, 3),
                 learned="not applied to flow_cg draws; the MLP warm-starts BM-ADMM slots (tests/test_gpu_learned.py)")
+
+
+def _reference_instances():
+    """The 41 instances whose certified optimum brackets the reference stores (data/opt_refs.json,
+    subset in tests/golden; generated by solver/instances.py with the stored cd_hash fingerprints),
+    smallest first. Each is solved cold through calculate_distance to rel 1e-3 with the method of
+    its stored record (CB-GPU -> flow_cg, K-ADMM -> bm_admm), the record's stop rule; the time is
+    the call's wall clock (the HIP context exists beforehand) beside the record's wall_s, and the
+    bracket must overlap [lb, OPT]."""
+    from solver import _lib, instances
+    from solver.rum_solver import RUM
+    with open(os.path.join(ROOT, "tests", "golden", "opt_refs_subset.json")) as f:
+        refs = json.load(f)
+    _lib.context(0)         # the HIP runtime starts before the first timed solve
+    rows = []
+    for key in sorted(refs, key=lambda k: refs[k]["mp"]):
+        ref = refs[key]
+        n, cat = int(key.split("_")[0][1:]), key.split("_", 1)[1]
+        cd, mp = instances.get_instance(n, cat)
+        method = "bm_admm" if ref["method"].startswith("K-ADMM") else "flow_cg"
+        rum = RUM.from_rows(cd, n)
+        t0 = time.perf_counter()
+        r = rum.calculate_distance(abs_tol=0.0, rel_tol=1e-3, method=method, max_time=300)
+        t = time.perf_counter() - t0
+        ok = bool(r.exit_flag in (2, 3) and r.lb <= ref["opt"] + 1e-9 and r.ub >= ref["lb"] - 1e-9)
+        rows.append(dict(instance=key, cells=mp, method=method, exit_flag=r.exit_flag, lb=r.lb, ub=r.ub,
+                         seconds=round(t, 3), stored_seconds=round(ref["wall_s"], 3),
+                         speedup=round(ref["wall_s"] / t, 2), certified_overlap=ok, iterations=r.iterations))
+        print("[bench] %-11s %9d cells %-7s %7.2f s (stored %7.2f s) flag %d overlap %s"
+              % (key, mp, method, t, ref["wall_s"], r.exit_flag, ok), file=sys.stderr, flush=True)
+    sp = [r["speedup"] for r in rows]
+    return dict(workload="the reference's certified-optimum instances (n=18-20 and n25_sparse, 41), rel 1e-3, cold",
+                instances=rows, all_certified_overlap=all(r["certified_overlap"] for r in rows),
+                geomean_speedup=round(math.exp(sum(math.log(x) for x in sp) / len(sp)), 2),
+                seconds=round(sum(r["seconds"] for r in rows), 1),
+                stored_seconds=round(sum(r["stored_seconds"] for r in rows), 1))
 
 
 def _dsr_bench():
