@@ -67,6 +67,8 @@ def _parse():
     ap.add_argument("--reference-instances", action="store_true",
                     help="only the instances with the reference's certified optima (tests/golden/opt_refs_subset.json):"
                          " each solved to rel 1e-3 by the method of its stored record, timed beside the record")
+    ap.add_argument("--only", default="",
+                    help="with --reference-instances: a comma-separated list of instance names to run")
     ap.add_argument("--bootstrap-only", action="store_true",
                     help="only the bootstrap leg (prints its JSON object): the workload of its rocprof summary")
     ap.add_argument("--traffic-run", action="store_true",
@@ -108,7 +110,7 @@ def main():
         print(json.dumps(_bootstrap_bench(a)), flush=True)
         return
     if a.reference_instances:
-        print(json.dumps(_reference_instances()), flush=True)
+        print(json.dumps(_reference_instances([k for k in a.only.split(",") if k])), flush=True)
         return
 
     cd, n = instances.config2()
@@ -332,7 +334,7 @@ def _bootstrap_bench(a):
                 learned="not applied to flow_cg draws; the MLP warm-starts BM-ADMM slots (tests/test_gpu_learned.py)")
 
 
-def _reference_instances():
+def _reference_instances(only=()):
     """The 41 instances whose certified optimum brackets the reference stores (data/opt_refs.json,
     subset in tests/golden; generated by solver/instances.py with the stored cd_hash fingerprints),
     smallest first. Each is solved cold through calculate_distance to rel 1e-3 with the method of
@@ -346,6 +348,8 @@ def _reference_instances():
     _lib.context(0)         # the HIP runtime starts before the first timed solve
     rows = []
     for key in sorted(refs, key=lambda k: refs[k]["mp"]):
+        if only and key not in only:
+            continue
         ref = refs[key]
         n, cat = int(key.split("_")[0][1:]), key.split("_", 1)[1]
         cd, mp = instances.get_instance(n, cat)

@@ -1,5 +1,5 @@
 // Flow-CG's restricted-master QP on the host, native: min 1/2 lam'Q lam - h'lam over the simplex
-// (lam >= 0, sum lam = 1), by the reference's active set (solver/common/qp_native.py:37-100):
+// (lam >= 0, sum lam = 1; Q symmetric), by the reference's active set (solver/common/qp_native.py:37-100):
 //   on the working set W: lam_W = Q_WW^-1 (h_W - mu 1), mu = (1'Q_WW^-1 h_W - 1) / (1'Q_WW^-1 1)
 //   (Q_WW + reg I, Cholesky); drop the most negative weight, else add the inactive column with
 //   the most negative gradient (below -tol), else stop; then a Euclidean simplex projection and
@@ -54,9 +54,13 @@ bool chol_append(Chol& c, const double* q, double d0) {
   double* row = &c.L[(size_t)k * c.cap];
   double s = d0;
   for (int j = 0; j < k; ++j) {
-    double v = q[j];
-    for (int t = 0; t < j; ++t) v -= row[t] * c.L[(size_t)j * c.cap + t];
-    v /= c.L[(size_t)j * c.cap + j];
+    const double* lj = &c.L[(size_t)j * c.cap];
+    double pp[4] = {0.0, 0.0, 0.0, 0.0};    // the dot product in four partial sums
+    int t = 0;
+    for (; t + 4 <= j; t += 4)
+      for (int u = 0; u < 4; ++u) pp[u] += row[t + u] * lj[t + u];
+    for (int u = 0; t < j; ++t, ++u) pp[u] += row[t] * lj[t];
+    const double v = (q[j] - ((pp[0] + pp[1]) + (pp[2] + pp[3]))) / lj[j];
     row[j] = v;
     s -= v * v;
   }
@@ -93,20 +97,36 @@ void chol_delete(Chol& c, int i) {
   c.k = k - 1;
 }
 
-// x = (L L')^-1 b
-void chol_solve(const Chol& c, const double* b, double* x) {
+// x1 = (L L')^-1 b1 and x2 = (L L')^-1 b2 in one pass over L: forward substitution by rows (each
+// row's dot product in four partial sums per right-hand side, eight independent chains), then
+// L' x = y by rows of L (x_i is final once the rows below have been applied)
+void chol_solve2(const Chol& c, const double* b1, const double* b2, double* x1, double* x2) {
   const int k = c.k;
   for (int i = 0; i < k; ++i) {
-    double s = b[i];
-    for (int t = 0; t < i; ++t) s -= c.L[(size_t)i * c.cap + t] * x[t];
-    x[i] = s / c.L[(size_t)i * c.cap + i];
+    const double* row = &c.L[(size_t)i * c.cap];
+    double p1[4] = {0.0, 0.0, 0.0, 0.0}, p2[4] = {0.0, 0.0, 0.0, 0.0};
+    int t = 0;
+    for (; t + 4 <= i; t += 4)
+      for (int u = 0; u < 4; ++u) {
+        p1[u] += row[t + u] * x1[t + u];
+        p2[u] += row[t + u] * x2[t + u];
+      }
+    for (int u = 0; t < i; ++t, ++u) {
+      p1[u] += row[t] * x1[t];
+      p2[u] += row[t] * x2[t];
+    }
+    x1[i] = (b1[i] - ((p1[0] + p1[1]) + (p1[2] + p1[3]))) / row[i];
+    x2[i] = (b2[i] - ((p2[0] + p2[1]) + (p2[2] + p2[3]))) / row[i];
   }
-  // L' x = y by rows of L: x_i is final once rows below have been applied
   for (int i = k - 1; i >= 0; --i) {
     const double* row = &c.L[(size_t)i * c.cap];
-    const double xi = x[i] / row[i];
-    x[i] = xi;
-    for (int t = 0; t < i; ++t) x[t] -= row[t] * xi;
+    const double xi1 = x1[i] / row[i], xi2 = x2[i] / row[i];
+    x1[i] = xi1;
+    x2[i] = xi2;
+    for (int t = 0; t < i; ++t) {
+      x1[t] -= row[t] * xi1;
+      x2[t] -= row[t] * xi2;
+    }
   }
 }
 
@@ -194,6 +214,8 @@ void project_simplex(std::vector<double>& lam) {
 struct rumgpu_qp {
   std::vector<int> work;
   Chol ch;
+  // the last solve on that working set and factor: its right-hand side h_W and Q_WW^-1 (h_W, 1)
+  std::vector<double> bh, qh, qe;
 };
 
 extern "C" int rumgpu_qp_create(rumgpu_qp** out) {
@@ -242,7 +264,7 @@ static int qp_simplex_impl(int K, const double* Q, int64_t ldq, const double* h,
   }
   Chol ch;
   ch.reserve(std::min(K, std::max(64, (int)work.size() + 16)));
-  std::vector<double> a, qh(K), qe(K), bh(K), be(K, 1.0), lam_w;
+  std::vector<double> a, qh(K), qe(K), bh(K), be(K, 1.0), lam_w, grad, lam_full;
   std::vector<int> solved;     // the working set lam_w belongs to
   auto rebuild = [&]() {
     const int k = (int)work.size();
@@ -270,8 +292,14 @@ static int qp_simplex_impl(int K, const double* Q, int64_t ldq, const double* h,
       for (int w : work) in[w] = 1;
     }
   }
+  // the last call ended on a solve with this working set and factor: when h_W is also the same, its
+  // solution is this call's first
+  bool cached = false;
   if (reuse) {
     ch = std::move(state->ch);                 // the factor of this working set, from the last call
+    const size_t k = work.size();
+    cached = state->bh.size() == k && state->qh.size() == k && state->qe.size() == k;
+    for (size_t i = 0; cached && i < k; ++i) cached = state->bh[i] == h[work[i]];
   } else {
     rebuild();
   }
@@ -279,9 +307,11 @@ static int qp_simplex_impl(int K, const double* Q, int64_t ldq, const double* h,
   for (int outer = 0; outer < max_outer; ++outer) {
     const int k = (int)work.size();
     for (int i = 0; i < k; ++i) bh[i] = h[work[i]];
-    if (ch.ok && ch.k == k) {
-      chol_solve(ch, bh.data(), qh.data());
-      chol_solve(ch, be.data(), qe.data());
+    if (outer == 0 && cached) {
+      std::copy(state->qh.begin(), state->qh.end(), qh.begin());
+      std::copy(state->qe.begin(), state->qe.end(), qe.begin());
+    } else if (ch.ok && ch.k == k) {
+      chol_solve2(ch, bh.data(), be.data(), qh.data(), qe.data());
     } else {
       a.assign((size_t)k * k, 0.0);
       for (int i = 0; i < k; ++i)
@@ -306,14 +336,36 @@ static int qp_simplex_impl(int K, const double* Q, int64_t ldq, const double* h,
       continue;
     }
     if (K > k) {
+      // gradient of the inactive columns: g_j = mu - h_j + sum_i Q[j, work_i] lam_i. Q is symmetric,
+      // so it is read by whole rows, whichever are fewer: the K - k inactive rows (a dot product
+      // with lam spread over all K columns) or the k working rows (axpys into every g_j)
+      grad.resize(K);
+      if (K - k < k) {
+        lam_full.assign(K, 0.0);
+        for (int i = 0; i < k; ++i) lam_full[work[i]] = lam_w[i];
+        for (int j = 0; j < K; ++j) {
+          if (in[j]) continue;
+          const double* qj = Q + (int64_t)j * ldq;
+          double pp[4] = {0.0, 0.0, 0.0, 0.0};
+          int t = 0;
+          for (; t + 4 <= K; t += 4)
+            for (int u = 0; u < 4; ++u) pp[u] += qj[t + u] * lam_full[t + u];
+          for (int u = 0; t < K; ++t, ++u) pp[u] += qj[t] * lam_full[t];
+          grad[j] = (mu - h[j]) + ((pp[0] + pp[1]) + (pp[2] + pp[3]));
+        }
+      } else {
+        for (int j = 0; j < K; ++j) grad[j] = mu - h[j];
+        for (int i = 0; i < k; ++i) {
+          const double* qi = Q + (int64_t)work[i] * ldq;
+          const double li = lam_w[i];
+          for (int j = 0; j < K; ++j) grad[j] += qi[j] * li;
+        }
+      }
       int jbest = -1;
       double gbest = 0.0;
       for (int j = 0; j < K; ++j) {
         if (in[j]) continue;
-        double g = mu - h[j];
-        const double* qj = Q + (int64_t)j * ldq;
-        for (int i = 0; i < k; ++i) g += qj[work[i]] * lam_w[i];
-        if (jbest < 0 || g < gbest) jbest = j, gbest = g;
+        if (jbest < 0 || grad[j] < gbest) jbest = j, gbest = grad[j];
       }
       if (jbest >= 0 && gbest < -tol) {
         std::vector<double> q(k);
@@ -330,6 +382,15 @@ static int qp_simplex_impl(int K, const double* Q, int64_t ldq, const double* h,
     state->work = work;
     if (ch.ok && ch.k == (int)work.size()) state->ch = std::move(ch);
     else state->ch.ok = false;
+    // the loop's last solve was on the final working set unless it ended at max_outer after a change
+    const int k = (int)work.size();
+    if (state->ch.ok && solved == work) {
+      state->bh.assign(bh.begin(), bh.begin() + k);
+      state->qh.assign(qh.begin(), qh.begin() + k);
+      state->qe.assign(qe.begin(), qe.begin() + k);
+    } else {
+      state->bh.clear(), state->qh.clear(), state->qe.clear();
+    }
   }
   std::vector<double> lam(K, 0.0);
   for (size_t i = 0; i < solved.size(); ++i) lam[solved[i]] = lam_w[i];

@@ -1483,30 +1483,33 @@ __global__ __launch_bounds__(256) void cg_col_gram(const u64* __restrict__ bits,
   if (threadIdx.x == 0) q[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
 }
 
-// p_out[r] = sum_t lam[t] [r in col idx[t]] (in t order), scores[r] = float(p[r] - p_out[r])
+// p_out[r] = sum_t lam[t] [r in col idx[t]] (in t order), scores[r] = float(p[r] - p_out[r]).
+// A block owns 4 words (256 rows). Per chunk of 64 support columns its threads load the 4 words of
+// each column at once (thread i: column i / 4, word i % 4) into LDS; then every lane adds, in t
+// order, the weights of the columns whose bit for its row is set (broadcast LDS reads).
 __global__ __launch_bounds__(256) void cg_col_scores(const u64* __restrict__ bits, i64 W, i64 M, int nA,
                                                      const int32_t* __restrict__ idx,
                                                      const double* __restrict__ lam, const double* __restrict__ p,
                                                      double* __restrict__ pout, float* __restrict__ scores) {
-  __shared__ i64 base[256];
-  __shared__ double sl[256];
-  const i64 r = (i64)blockIdx.x * 256 + threadIdx.x;
-  const i64 w = r < M ? (r >> 6) : 0;
-  const int l = (int)(r & 63);
+  __shared__ u64 wl[64][4];
+  __shared__ double sl[64];
+  const i64 w0 = (i64)blockIdx.x * 4;
+  const int tid = threadIdx.x;
+  const int myw = tid >> 6, lane = tid & 63;
+  const i64 r = w0 * 64 + tid;
   double acc = 0.0;
-  for (int a0 = 0; a0 < nA; a0 += 256) {
-    const int cnt = nA - a0 < 256 ? nA - a0 : 256;
+  for (int a0 = 0; a0 < nA; a0 += 64) {
+    const int cnt = nA - a0 < 64 ? nA - a0 : 64;
     __syncthreads();
-    if ((int)threadIdx.x < cnt) {
-      base[threadIdx.x] = (i64)idx[a0 + threadIdx.x] * W;
-      sl[threadIdx.x] = lam[a0 + threadIdx.x];
+    {
+      const int tc = tid >> 2, wq = tid & 3;
+      u64 word = 0;
+      if (tc < cnt && w0 + wq < W) word = bits[(i64)idx[a0 + tc] * W + w0 + wq];
+      wl[tc][wq] = word;
+      if (tid < cnt) sl[tid] = lam[a0 + tid];
     }
     __syncthreads();
-#pragma unroll 8
-    for (int t = 0; t < cnt; ++t) {
-      const u64 word = bits[base[t] + w];
-      acc += ((word >> l) & 1ull) ? sl[t] : 0.0;
-    }
+    for (int t = 0; t < cnt; ++t) acc += ((wl[t][myw] >> lane) & 1ull) ? sl[t] : 0.0;
   }
   if (r < M) {
     pout[r] = acc;
@@ -1627,7 +1630,7 @@ extern "C" int rumgpu_cg_cols_scores(rumgpu_cg* c, int32_t nA, const int32_t* id
     RG_HIP(hipMemcpyAsync(c->cidx, idx, (size_t)nA * 4, hipMemcpyHostToDevice, st));
     RG_HIP(hipMemcpyAsync(c->clam, lam, (size_t)nA * 8, hipMemcpyHostToDevice, st));
   }
-  hipLaunchKernelGGL(cg_col_scores, dim3((unsigned)((c->M + 255) / 256)), dim3(256), 0, st, c->cbits, c->cW, c->M,
+  hipLaunchKernelGGL(cg_col_scores, dim3((unsigned)((c->cW + 3) / 4)), dim3(256), 0, st, c->cbits, c->cW, c->M,
                      (int)nA, c->cidx, c->clam, c->cp, c->cpout, c->scores);
   RG_LAUNCH_CHECK();
   RG_HIP(hipStreamSynchronize(st));
