@@ -483,15 +483,16 @@ __global__ __launch_bounds__(256) void cg_slab_layer(float* __restrict__ V, uint
   float best = 0.0f;
   bool seen = false;
   int bx = 0;
-  for (int x = 0; x < n; ++x) {
-    if ((D >> x) & 1u) {
-      const u32 P = D ^ (1u << x);
-      const float cand = V[P] + R[(i64)x * Nloc + (i64)compress_bit(P, x)];
-      if (!seen || cand > best) {
-        best = cand;
-        seen = true;
-        bx = x;
-      }
+  // the members of D in ascending order: every node of a layer has the same count, so the lanes
+  // of a wavefront run the same number of iterations
+  for (u32 S = D; S; S &= S - 1) {
+    const int x = __builtin_ctz(S);
+    const u32 P = D ^ (1u << x);
+    const float cand = V[P] + R[(i64)x * Nloc + (i64)compress_bit(P, x)];
+    if (!seen || cand > best) {
+      best = cand;
+      seen = true;
+      bx = x;
     }
   }
   V[D] = best;
