@@ -64,6 +64,8 @@ def _parse():
     ap.add_argument("--boot-slots", type=int, default=64)
     ap.add_argument("--no-dsr", action="store_true",
                     help="skip the whole DSR alternative-0 bootstrap (n=25, B=10,000) checked against the stored run")
+    ap.add_argument("--dsr-all-alternatives", action="store_true",
+                    help="only the DSR bootstrap on all 901 cells (every alternative), against the published run")
     ap.add_argument("--reference-instances", action="store_true",
                     help="only the instances with the reference's certified optima (tests/golden/opt_refs_subset.json):"
                          " each solved to rel 1e-3 by the method of its stored record, timed beside the record")
@@ -108,6 +110,9 @@ def main():
 
     if a.bootstrap_only:
         print(json.dumps(_bootstrap_bench(a)), flush=True)
+        return
+    if a.dsr_all_alternatives:
+        print(json.dumps(_dsr_all_alternatives()), flush=True)
         return
     if a.reference_instances:
         print(json.dumps(_reference_instances([k for k in a.only.split(",") if k])), flush=True)
@@ -393,8 +398,9 @@ def _dsr_bench():
         pv = boot.p_value()
         t = time.perf_counter() - t0
         dec = hong_li.decide(boot.L, boot.U, *boot.tau)
-        L, U = boot.L, boot.U
+        L, U = boot.L.copy(), boot.U.copy()
         same_draws = boot.fingerprints["draws"] == rec["fingerprints"]["draws"]
+        ci = boot.confidence_interval()
     dr = rec["draws"]
     rL, rU = np.asarray(dr["lb"], dtype=np.float64), np.asarray(dr["ub"], dtype=np.float64)
     rdec = hong_li.decide(rL, rU, *rec["tau"])
@@ -411,7 +417,40 @@ def _dsr_bench():
                                 above=rpv["n_above"], undecided=rpv["n_undecided"]),
                 speedup_vs_stored_run=round(rpv["time"] / t, 2), draws_fingerprint_equal=bool(same_draws),
                 decisions_differing=int((dec != rdec).sum()), brackets_overlapping=overlap,
-                brackets_compared=int(both.sum()))
+                brackets_compared=int(both.sum()),
+                confidence_interval=dict(lower=ci.lower, lower_bracket=list(ci.lower_bracket), seconds=round(ci.time, 2),
+                                         draws_solved=ci.n_solved,
+                                         published=dict(lower=-0.0476, seconds=237, draws_solved=592,
+                                                        source="the reference's solver/DESIGN_NOTES.md:422")))
+
+
+# the reference's published run of the same bootstrap on all 901 cells (solver/DESIGN_NOTES.md:423)
+_DSR_ALL_PUBLISHED = dict(p=0.1392, below=8608, above=1392, undecided=0, seconds=1499)
+
+
+def _dsr_all_alternatives():
+    """The DSR bootstrap on every alternative's cells (n = 25, 901 cells, 520 respondents; B = 10,000,
+    seed 0, abs 5e-4, the default method), p_value() end to end through the batched driver, beside
+    the reference's published p, counts and time for the same run."""
+    import pandas as pd
+    from solver import _lib
+    from solver.inference import cells_from_panel
+    from solver.rum_solver import RUM
+    panel = pd.read_csv(os.path.join(ROOT, "tests", "golden", "dsr_overload_data_25_long.csv"))
+    menu, alt, p = cells_from_panel(panel, 25)
+    rum = RUM(menu, alt, p, num_alternatives=25)
+    _lib.context(0)
+    t0 = time.perf_counter()
+    with rum.bootstrap(panel=panel, B=10000, batch={"slots": 64}, verbose=True) as boot:
+        pv = boot.p_value()
+    t = time.perf_counter() - t0
+    pub = _DSR_ALL_PUBLISHED
+    return dict(workload="DSR bootstrap on all 901 cells (n=25, 520 respondents), B=10,000, seed 0, abs 5e-4",
+                seconds=round(t, 1), p=pv.p, below=pv.n_below, above=pv.n_above, undecided=pv.n_undecided,
+                screened=pv.n_screened, published=pub,
+                same_as_published=bool((pv.p, pv.n_below, pv.n_above, pv.n_undecided)
+                                       == (pub["p"], pub["below"], pub["above"], pub["undecided"])),
+                speedup_vs_published=round(pub["seconds"] / t, 2))
 
 
 def _cpu_baseline(n, rum, p, r_last, a):

@@ -284,7 +284,12 @@ class BatchedCG:
         if sess.method != "flow_cg":
             raise ValueError("BatchedCG runs flow_cg draws (the session's method is %s)" % sess.method)
         if sess._args.get("pricing") != "mask":
-            raise ValueError("batched sweeps need the mask pricing layout (the plan uses %s)" % sess._args["pricing"])
+            # the batched sweep is a mask sweep: the session's draws then price on the mask layout
+            # (the same pricing value; float32 scores only)
+            if sess._args.get("dtype", "float32") != "float32":
+                raise ValueError("batched sweeps price float32 scores on the mask layout (the plan uses %s %s)"
+                                 % (sess._args.get("pricing"), sess._args.get("dtype")))
+            sess._args["pricing"] = "mask"
         self.boot, self.sess, self.D, self.verbose = boot, sess, int(slots), verbose
         self.overlap = bool(overlap) and self.D >= 2
         from .. import machine as _machine

@@ -328,9 +328,14 @@ class Bootstrap:
             self._batched = kind(self, **(self.batch or {}))
         tau_lo, tau_hi = self.tau
         P_of = lambda sel: hong_li.perturb(self.p_hat, self.draws.rows(sel), self.r_n, self.eps_n)  # noqa: E731
-        for b, lb, ub, flag in self._batched.solve(idx, P_of, abs_tol, rel_tol, tau_lo if stops else None,
-                                                   tau_hi if stops else None, self.draw_time):
+        report = max(1, len(idx) // 10)
+        for done, (b, lb, ub, flag) in enumerate(self._batched.solve(idx, P_of, abs_tol, rel_tol,
+                                                                     tau_lo if stops else None,
+                                                                     tau_hi if stops else None, self.draw_time), 1):
             self._record(b, lb, ub, flag, abs_tol, stops)
+            if self.verbose and done % report == 0:
+                self._say("  %d / %d draws at abs_tol %g (%.0f s)" % (done, len(idx), abs_tol,
+                                                                     time.perf_counter() - t0))
             self._save_checkpoint()
         self.batch_stats = self._batched.stats
         self.time["solve"] += time.perf_counter() - t0

@@ -245,6 +245,28 @@ def test_batched_flow_cg_bootstrap_identical_brackets():
     assert pv1.p == pv2.p and pv1.n_solved >= 20
 
 
+def test_batched_flow_cg_bootstrap_from_a_slab_plan():
+    """A flow_cg plan on the slab layout (pricing="slab" here; the default plan takes it above 16
+    rows per alternative) runs its batched draws on the mask layout: the session switches to mask,
+    and the decisions equal the per-draw solves on the mask layout."""
+    import pandas as pd
+    panel, n = _small_panel()
+    panel = pd.DataFrame(panel)
+    menu, alt, p = cells_from_panel(panel, n)
+    rum = RUM(menu, alt, p, num_alternatives=n)
+    B = 64
+    with rum.bootstrap(panel=panel, B=B, method="flow_cg", method_args={"pricing": "mask"}, abs_tol=1e-4,
+                       refine=1) as one:
+        pv1 = one.p_value()
+    with rum.bootstrap(panel=panel, B=B, method="flow_cg", method_args={"pricing": "slab"}, abs_tol=1e-4,
+                       refine=1, batch=dict(slots=16)) as bat:
+        pv2 = bat.p_value()
+        assert bat.session._args["pricing"] == "mask"
+    print(pv1)
+    print(pv2)
+    assert (pv1.n_below, pv1.n_above, pv1.n_undecided) == (pv2.n_below, pv2.n_above, pv2.n_undecided)
+
+
 def test_batched_confidence_interval_equals_per_draw():
     """Bootstrap.confidence_interval solves upper-tail draws to tolerance with no band stops
     (bootstrap.py:212-240); batched flow_cg gives the per-draw brackets, so the same interval"""
