@@ -653,6 +653,13 @@ def _plan_flow_cg(n, mp, weighted, prof, a):
         need = _flow_cg_bytes(n, lay, d, mp)
         if need <= budget:
             a.update(pricing=lay, dtype=d, _gpu_bytes=need)
+            # a single run over 4,096 rows keeps its column bitsets, p and p_out on the device
+            # (flow_cg.engine.DeviceColumns), else on the host as the reference does
+            cols = 8 * k * ((mp + 63) // 64) + 16 * mp + 20 * k if mp > 4096 else 0
+            if cols and need + cols > budget:
+                a.setdefault("_cols", "host")
+            elif cols:
+                a["_gpu_bytes"] = need + cols
             return a
         tried.append("%s %s %.1f GB" % (lay, d, need / 2 ** 30))
     raise RuntimeError("flow_cg at n = %d does not fit the GPU: %.1f GB may be used (%s)"
