@@ -97,27 +97,31 @@ void chol_delete(Chol& c, int i) {
   c.k = k - 1;
 }
 
-// x1 = (L L')^-1 b1 and x2 = (L L')^-1 b2 in one pass over L: forward substitution by rows (each
-// row's dot product in four partial sums per right-hand side, eight independent chains), then
-// L' x = y by rows of L (x_i is final once the rows below have been applied)
-void chol_solve2(const Chol& c, const double* b1, const double* b2, double* x1, double* x2) {
-  const int k = c.k;
-  for (int i = 0; i < k; ++i) {
-    const double* row = &c.L[(size_t)i * c.cap];
-    double p1[4] = {0.0, 0.0, 0.0, 0.0}, p2[4] = {0.0, 0.0, 0.0, 0.0};
-    int t = 0;
-    for (; t + 4 <= i; t += 4)
-      for (int u = 0; u < 4; ++u) {
-        p1[u] += row[t + u] * x1[t + u];
-        p2[u] += row[t + u] * x2[t + u];
-      }
-    for (int u = 0; t < i; ++t, ++u) {
-      p1[u] += row[t] * x1[t];
-      p2[u] += row[t] * x2[t];
+// Solves with the working set's factor for two right-hand sides at once (h_W and 1).
+// Forward substitution L y = b by rows, row i's dot product in four partial sums per right-hand
+// side (eight independent chains). Rows 0..i-1 of L do not change when a row is appended, so the
+// y of a grown factor is the old y plus one fwd_row step.
+void fwd_row(const Chol& c, int i, const double* b1, const double* b2, double* y1, double* y2) {
+  const double* row = &c.L[(size_t)i * c.cap];
+  double p1[4] = {0.0, 0.0, 0.0, 0.0}, p2[4] = {0.0, 0.0, 0.0, 0.0};
+  int t = 0;
+  for (; t + 4 <= i; t += 4)
+    for (int u = 0; u < 4; ++u) {
+      p1[u] += row[t + u] * y1[t + u];
+      p2[u] += row[t + u] * y2[t + u];
     }
-    x1[i] = (b1[i] - ((p1[0] + p1[1]) + (p1[2] + p1[3]))) / row[i];
-    x2[i] = (b2[i] - ((p2[0] + p2[1]) + (p2[2] + p2[3]))) / row[i];
+  for (int u = 0; t < i; ++t, ++u) {
+    p1[u] += row[t] * y1[t];
+    p2[u] += row[t] * y2[t];
   }
+  y1[i] = (b1[i] - ((p1[0] + p1[1]) + (p1[2] + p1[3]))) / row[i];
+  y2[i] = (b2[i] - ((p2[0] + p2[1]) + (p2[2] + p2[3]))) / row[i];
+}
+
+// L' x = y by rows of L: x_i is final once the rows below have been applied
+void bwd2(const Chol& c, const double* y1, const double* y2, double* x1, double* x2) {
+  const int k = c.k;
+  for (int i = 0; i < k; ++i) x1[i] = y1[i], x2[i] = y2[i];
   for (int i = k - 1; i >= 0; --i) {
     const double* row = &c.L[(size_t)i * c.cap];
     const double xi1 = x1[i] / row[i], xi2 = x2[i] / row[i];
@@ -214,8 +218,9 @@ void project_simplex(std::vector<double>& lam) {
 struct rumgpu_qp {
   std::vector<int> work;
   Chol ch;
-  // the last solve on that working set and factor: its right-hand side h_W and Q_WW^-1 (h_W, 1)
-  std::vector<double> bh, qh, qe;
+  // the last solve on that working set and factor: its right-hand side h_W, Q_WW^-1 (h_W, 1), and
+  // the forward solutions L^-1 h_W and L^-1 1
+  std::vector<double> bh, qh, qe, yh, ye;
 };
 
 extern "C" int rumgpu_qp_create(rumgpu_qp** out) {
@@ -295,11 +300,18 @@ static int qp_simplex_impl(int K, const double* Q, int64_t ldq, const double* h,
   // the last call ended on a solve with this working set and factor: when h_W is also the same, its
   // solution is this call's first
   bool cached = false;
+  bool fwd = false;                            // yh, ye hold L^-1 h_W and L^-1 1 for work and ch
+  std::vector<double> yh(K + 1), ye(K + 1);
   if (reuse) {
     ch = std::move(state->ch);                 // the factor of this working set, from the last call
     const size_t k = work.size();
     cached = state->bh.size() == k && state->qh.size() == k && state->qe.size() == k;
     for (size_t i = 0; cached && i < k; ++i) cached = state->bh[i] == h[work[i]];
+    if (cached && state->yh.size() == k && state->ye.size() == k) {
+      std::copy(state->yh.begin(), state->yh.end(), yh.begin());
+      std::copy(state->ye.begin(), state->ye.end(), ye.begin());
+      fwd = true;
+    }
   } else {
     rebuild();
   }
@@ -311,7 +323,11 @@ static int qp_simplex_impl(int K, const double* Q, int64_t ldq, const double* h,
       std::copy(state->qh.begin(), state->qh.end(), qh.begin());
       std::copy(state->qe.begin(), state->qe.end(), qe.begin());
     } else if (ch.ok && ch.k == k) {
-      chol_solve2(ch, bh.data(), be.data(), qh.data(), qe.data());
+      if (!fwd) {
+        for (int i = 0; i < k; ++i) fwd_row(ch, i, bh.data(), be.data(), yh.data(), ye.data());
+        fwd = true;
+      }
+      bwd2(ch, yh.data(), ye.data(), qh.data(), qe.data());
     } else {
       a.assign((size_t)k * k, 0.0);
       for (int i = 0; i < k; ++i)
@@ -333,6 +349,7 @@ static int qp_simplex_impl(int K, const double* Q, int64_t ldq, const double* h,
       work.erase(work.begin() + imin);
       if (ch.ok && ch.k == k) chol_delete(ch, imin);
       else rebuild();
+      fwd = false;
       continue;
     }
     if (K > k) {
@@ -371,8 +388,15 @@ static int qp_simplex_impl(int K, const double* Q, int64_t ldq, const double* h,
         std::vector<double> q(k);
         for (int i = 0; i < k; ++i) q[i] = Q[(int64_t)jbest * ldq + work[i]];
         work.push_back(jbest), in[jbest] = 1;
-        if (!(ch.ok && ch.k == k && k + 1 <= ch.cap && chol_append(ch, q.data(), Q[(int64_t)jbest * ldq + jbest] + reg)))
+        if (ch.ok && ch.k == k && k + 1 <= ch.cap && chol_append(ch, q.data(), Q[(int64_t)jbest * ldq + jbest] + reg)) {
+          if (fwd) {                           // the grown factor's forward solutions: one more row
+            bh[k] = h[jbest];
+            fwd_row(ch, k, bh.data(), be.data(), yh.data(), ye.data());
+          }
+        } else {
           rebuild();
+          fwd = false;
+        }
         continue;
       }
     }
@@ -388,8 +412,14 @@ static int qp_simplex_impl(int K, const double* Q, int64_t ldq, const double* h,
       state->bh.assign(bh.begin(), bh.begin() + k);
       state->qh.assign(qh.begin(), qh.begin() + k);
       state->qe.assign(qe.begin(), qe.begin() + k);
+      if (fwd) {
+        state->yh.assign(yh.begin(), yh.begin() + k);
+        state->ye.assign(ye.begin(), ye.begin() + k);
+      } else {
+        state->yh.clear(), state->ye.clear();
+      }
     } else {
-      state->bh.clear(), state->qh.clear(), state->qe.clear();
+      state->bh.clear(), state->qh.clear(), state->qe.clear(), state->yh.clear(), state->ye.clear();
     }
   }
   std::vector<double> lam(K, 0.0);
