@@ -187,13 +187,14 @@ int rumgpu_cg_sweep_info(rumgpu_cg* cg, int* one_alternative);
  * cols_load: k columns (k x W words) replace the stored ones (a warm start's columns).
  * cols_add: the column of a ranking (rank_pos[n], a permutation; row r is 1 iff its alternative
  *   is its menu's best-ranked member) is appended; gram[K + 1] receives its inner products with
- *   columns 0..K (itself last, exact integers); bits[W] receives it unless null.
+ *   columns 0..K (itself last, exact integers), h its inner product with p (a fixed-order sum
+ *   within each block of 256 rows, then over the blocks in order), bits[W] the column unless null.
  * cols_scores: p_out = sum_t lam[t] col[idx[t]] (in t order) and scores = float32(p - p_out) stay
  *   on the device; rumgpu_cg_price(cg, NULL, ...) then prices those scores.
  * cols_pout: p_out[M] of the last cols_scores. */
 int rumgpu_cg_cols_init(rumgpu_cg* cg, const double* p_x, int64_t cap);
 int rumgpu_cg_cols_load(rumgpu_cg* cg, int64_t k, const uint64_t* bits);
-int rumgpu_cg_cols_add(rumgpu_cg* cg, const int32_t* rank_pos, double* gram, uint64_t* bits);
+int rumgpu_cg_cols_add(rumgpu_cg* cg, const int32_t* rank_pos, double* gram, double* h, uint64_t* bits);
 int rumgpu_cg_cols_scores(rumgpu_cg* cg, int32_t nA, const int32_t* idx, const double* lam);
 int rumgpu_cg_cols_pout(rumgpu_cg* cg, double* p_out);
 

@@ -759,6 +759,7 @@ struct rumgpu_cg {
   int32_t* cidx = nullptr;   // support of lam [ccap]
   double* clam = nullptr;    // lam on that support [ccap]
   int32_t* crank = nullptr;  // rank positions of the column being added
+  double* cpart = nullptr;   // per block of cg_col_bits: the sum of p over the new column's rows
   bool scores_resident = false;   // c->scores holds float(p - V lam) from rumgpu_cg_cols_scores
   // stream
   u64* binom = nullptr;
@@ -787,7 +788,7 @@ static void cg_free_all(rumgpu_cg* c) {
                   c->ebuf[1], c->lstep, c->subV, c->subpred, c->subnodes, c->submasks, c->subscores, c->suboffsets,
                   c->Vb, c->predb, c->scoresb, c->vmaxb, c->rankb, c->V64, c->r64, c->V64b, c->r64b, c->clcm, c->clF, c->cltmax,
                   c->clbmax, c->clvmax, c->clbmax64, c->clv64, c->clerr, c->cbits, c->cp, c->cpout, c->cq,
-                  c->cidx, c->clam, c->crank};
+                  c->cidx, c->clam, c->crank, c->cpart};
   for (void* q : bufs) rg_free(q);
   for (auto* p : c->predk) rg_free(p);
 }
@@ -1439,11 +1440,13 @@ extern "C" int rumgpu_cg_info(rumgpu_cg* c, size_t* device_bytes, double* last_k
 // column's Gram row and the residual scores float(p - V lam) of the next sweep are computed where
 // the columns live, so a run's per-column host work is the master QP on the K x K Gram matrix.
 
-// one word per wavefront: lane l of the wave that starts at row 64 w decides row 64 w + l
+// one word per wavefront: lane l of the wave that starts at row 64 w decides row 64 w + l; each
+// block also writes the sum of p over its rows that are in the column (a fixed reduction order)
 template <typename MT>
 __global__ __launch_bounds__(256) void cg_col_bits(const MT* __restrict__ masks, const u64* __restrict__ offsets,
                                                    int n, i64 M, i64 W, const int32_t* __restrict__ rank_pos,
-                                                   u64* __restrict__ out) {
+                                                   const double* __restrict__ p, u64* __restrict__ out,
+                                                   double* __restrict__ part) {
   __shared__ int rk[64];
   __shared__ u64 off[65];
   if (threadIdx.x < (unsigned)n) rk[threadIdx.x] = rank_pos[threadIdx.x];
@@ -1468,6 +1471,12 @@ __global__ __launch_bounds__(256) void cg_col_bits(const MT* __restrict__ masks,
   }
   const u64 word = __ballot(on);
   if ((threadIdx.x & 63) == 0 && (r >> 6) < W) out[r >> 6] = word;
+  __shared__ double wsum[4];
+  double v = on ? p[r] : 0.0;                  // on implies r < M
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]);
 }
 
 // q[j] = |col_j AND col_k| for j = 0..k (block j)
@@ -1563,6 +1572,8 @@ extern "C" int rumgpu_cg_cols_init(rumgpu_cg* c, const double* p_x, int64_t cap)
     if ((rc = cg_alloc(c, (void**)&c->cp, (size_t)c->M * 8, "column store p")) != RG_OK) return rc;
     if ((rc = cg_alloc(c, (void**)&c->cpout, (size_t)c->M * 8, "column store p_out")) != RG_OK) return rc;
     if ((rc = cg_alloc(c, (void**)&c->crank, 64 * 4, "column ranking")) != RG_OK) return rc;
+    if ((rc = cg_alloc(c, (void**)&c->cpart, (size_t)((c->cW * 64 + 255) / 256) * 8, "column sums")) != RG_OK)
+      return rc;
   }
   c->cK = 0;
   c->scores_resident = false;
@@ -1584,8 +1595,8 @@ extern "C" int rumgpu_cg_cols_load(rumgpu_cg* c, int64_t k, const uint64_t* bits
   return RG_OK;
 }
 
-extern "C" int rumgpu_cg_cols_add(rumgpu_cg* c, const int32_t* rank_pos, double* gram, uint64_t* bits) {
-  RG_CHECK(c && rank_pos && gram, RG_EINVAL, "cg_cols_add: null argument");
+extern "C" int rumgpu_cg_cols_add(rumgpu_cg* c, const int32_t* rank_pos, double* gram, double* h, uint64_t* bits) {
+  RG_CHECK(c && rank_pos && gram && h, RG_EINVAL, "cg_cols_add: null argument");
   RG_CHECK(c->cp, RG_EINVAL, "cg_cols_add: call rumgpu_cg_cols_init first");
   const int n = c->n;
   std::vector<char> seen(n, 0);
@@ -1603,17 +1614,22 @@ extern "C" int rumgpu_cg_cols_add(rumgpu_cg* c, const int32_t* rank_pos, double*
   const i64 blocks = (c->cW * 64 + 255) / 256;
   if (c->masks32)
     hipLaunchKernelGGL(cg_col_bits<u32>, dim3((unsigned)blocks), dim3(256), 0, st, c->masks32, c->offsets, n, c->M,
-                       c->cW, c->crank, col);
+                       c->cW, c->crank, c->cp, col, c->cpart);
   else
     hipLaunchKernelGGL(cg_col_bits<u64>, dim3((unsigned)blocks), dim3(256), 0, st, c->masks64, c->offsets, n, c->M,
-                       c->cW, c->crank, col);
+                       c->cW, c->crank, c->cp, col, c->cpart);
   hipLaunchKernelGGL(cg_col_gram, dim3((unsigned)(k + 1)), dim3(256), 0, st, c->cbits, c->cW, k, c->cq);
   RG_LAUNCH_CHECK();
   std::vector<unsigned long long> q((size_t)k + 1);
   RG_HIP(hipMemcpyAsync(q.data(), c->cq, (size_t)(k + 1) * 8, hipMemcpyDeviceToHost, st));
   if (bits) RG_HIP(hipMemcpyAsync(bits, col, (size_t)c->cW * 8, hipMemcpyDeviceToHost, st));
+  std::vector<double> part((size_t)blocks);
+  RG_HIP(hipMemcpyAsync(part.data(), c->cpart, (size_t)blocks * 8, hipMemcpyDeviceToHost, st));
   RG_HIP(hipStreamSynchronize(st));
   for (i64 j = 0; j <= k; ++j) gram[j] = (double)q[j];
+  double hs = 0.0;
+  for (i64 b = 0; b < blocks; ++b) hs += part[(size_t)b];
+  *h = hs;
   c->cK = k + 1;
   return RG_OK;
 }

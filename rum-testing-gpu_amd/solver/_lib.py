@@ -68,7 +68,7 @@ SIGNATURES = {
     "rumgpu_cg_sweep_info": [_P, _P],
     "rumgpu_cg_cols_init": [_P, _P, _I64],
     "rumgpu_cg_cols_load": [_P, _I64, _P],
-    "rumgpu_cg_cols_add": [_P, _P, _P, _P],
+    "rumgpu_cg_cols_add": [_P, _P, _P, _P, _P],
     "rumgpu_cg_cols_scores": [_P, _I, _P, _P],
     "rumgpu_cg_cols_pout": [_P, _P],
     "rumgpu_qp_simplex": [_I, _P, _I64, _P, _P, _I, _I, _D, _D, _P, _P, _P],

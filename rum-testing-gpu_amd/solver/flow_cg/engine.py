@@ -8,6 +8,7 @@ LB = sqrt(max(err^2 - 2 (Vmax + margin - inner), 0)) (a running max), the stop t
 priced ranking's 0/1 column. Q = V'V and h = V'p are grown from packed column bitsets
 (qp_bitset_incremental.py:36-141 of the reference): Q[k, j] = popcount(bits_k & bits_j).
 """
+import collections.abc
 import ctypes
 import math
 import time
@@ -68,13 +69,15 @@ class Pricer:
                    "cg_cols_load")
 
     def cols_add(self, rank_pos, k):
-        """append the column of a ranking; returns (its inner products with columns 0..k, the
-        packed bitset)"""
+        """append the column of a ranking; returns (its inner products with columns 0..k, its inner
+        product with p, the packed bitset)"""
         r = _lib.c_array(rank_pos, np.int32)
         gram = np.empty(k + 1)
+        hv = ctypes.c_double()
         bits = np.empty((self.M + 63) // 64, dtype=np.uint64)
-        _lib.check(self.lib.rumgpu_cg_cols_add(self.h, _lib.ptr(r), _lib.ptr(gram), _lib.ptr(bits)), "cg_cols_add")
-        return gram, bits
+        _lib.check(self.lib.rumgpu_cg_cols_add(self.h, _lib.ptr(r), _lib.ptr(gram), ctypes.byref(hv),
+                                               _lib.ptr(bits)), "cg_cols_add")
+        return gram, hv.value, bits
 
     def cols_scores(self, idx, lam):
         i = _lib.c_array(idx, np.int32)
@@ -246,11 +249,28 @@ class Columns:
         return self.Q[:self.K, :self.K], self.h[:self.K]
 
 
+class PackedColumns(collections.abc.Sequence):
+    """0/1 uint8 columns over M rows, kept as packed bitsets (bit r % 64 of word r / 64) and
+    unpacked when one is read."""
+
+    def __init__(self, packed, M):
+        self._packed, self.M = list(packed), int(M)
+
+    def __len__(self):
+        return len(self._packed)
+
+    def __getitem__(self, i):
+        if isinstance(i, slice):
+            return [self[j] for j in range(*i.indices(len(self)))]
+        return np.unpackbits(self._packed[i].view(np.uint8), bitorder="little")[:self.M]
+
+
 class DeviceColumns:
     """The ranking columns of one run in the pricer's device column store: the bitsets, p_out = V lam
     and the float32 residual scores of the next sweep live on the GPU (rumgpu_cg_cols_*); the host
-    keeps the Gram matrix Q (exact integer counts, as Columns has them), h = V'p and the keys.
-    set_lambda(lam) recomputes p_out and the scores from lam's support."""
+    keeps the Gram matrix Q (exact integer counts, as Columns has them), h = V'p (summed on the
+    device) and the packed bitsets. set_lambda(lam) recomputes p_out and the scores from lam's
+    support."""
 
     def __init__(self, pricer, M, p, max_K, start=64):
         self.pricer = pricer
@@ -261,7 +281,7 @@ class DeviceColumns:
         self.Q = np.zeros((0, 0))
         self.h = np.zeros(0)
         self._grow(min(self.max_K, max(1, int(start))))
-        self.keys = []
+        self.packed = []
         self.K = 0
         pricer.cols_init(self.p, self.cap)
 
@@ -278,13 +298,12 @@ class DeviceColumns:
             raise RuntimeError("max_K = %d columns exceeded" % self.max_K)
         if k >= self.cap:
             self._grow(2 * self.cap)
-        gram, bits = self.pricer.cols_add(rank_pos, k)
+        gram, hk, bits = self.pricer.cols_add(rank_pos, k)
         self.Q[k, :k] = gram[:k]
         self.Q[:k, k] = gram[:k]
         self.Q[k, k] = gram[k]
-        col = np.unpackbits(bits.view(np.uint8), bitorder="little")[:self.M]
-        self.h[k] = float(self.p[np.flatnonzero(col)].sum())
-        self.keys.append(col.tobytes())
+        self.h[k] = hk
+        self.packed.append(bits)
         self.K += 1
 
     def load_template(self, tpl):
@@ -297,7 +316,7 @@ class DeviceColumns:
         for i in range(k):
             self.h[i] = float(self.p[tpl.pos[i]].sum())
         self.pricer.cols_load(tpl.bits[:k])
-        self.keys = list(tpl.keys)
+        self.packed = [tpl.bits[i].copy() for i in range(k)]
         self.K = k
 
     def set_lambda(self, lam):
@@ -311,6 +330,9 @@ class DeviceColumns:
     @property
     def p_out(self):
         return self.pricer.cols_pout()
+
+    def columns(self):
+        return PackedColumns(self.packed[:self.K], self.M)
 
 
 RESIDENT = "resident"      # a request to price the scores DeviceColumns.set_lambda left on the device
@@ -487,6 +509,6 @@ def cg_steps(n, menu, alt, p, pricer, rc_tol=0.0, max_iters=4000, time_limit_s=0
     if dev:
         cols.set_lambda(lam)
     return dict(wall_s=wall, setup_s=setup_s, trace=trace, margins=margins, iters=it, n_cols=cols.K, lam=lam,
-                p_out=cols.p_out.copy(), perm=perm, columns=[np.frombuffer(k, dtype=np.uint8).copy() for k in cols.keys],
+                p_out=cols.p_out.copy(), perm=perm, columns=(cols.columns() if dev else [np.frombuffer(k, dtype=np.uint8).copy() for k in cols.keys]),
                 rc_final=rc_final, distance=UB, lb=lb_best, gap_rel=gap_rel, stop_reason=stop, qp_s=qp_s,
                 price_s=price_s, device_bytes=dev_bytes, layout=pricer.layout, dtype=pricer.dtype)

@@ -232,7 +232,8 @@ def test_stream_pricing_bit_exact(n, mode):
 @pytest.mark.parametrize("n,cells", [(9, 1000), (14, 7001)])
 def test_column_store_matches_oracle(n, cells, layout):
     """rumgpu_cg_cols_*: each ranking's column bitset equals the oracle's ranking column on the
-    x-major rows, its Gram row equals the integer overlaps, p_out = sum_t lam_t col_(idx_t) equals the
+    x-major rows, its Gram row equals the integer overlaps, its h = col'p equals the exactly rounded
+    sum to 1e-12 (relative), p_out = sum_t lam_t col_(idx_t) equals the
     same sum taken in t order on the host bit for bit, and pricing the resident scores equals pricing
     those scores passed from the host. 70 columns from an initial room of 2 also cover the store's
     growth; a reload of the first 5 columns continues at index 5."""
@@ -247,8 +248,9 @@ def test_column_store_matches_oracle(n, cells, layout):
     cols, bits_all = [], []
     for k in range(70):
         rank = rng.permutation(n).astype(np.int32)
-        gram, bits = pr.cols_add(rank, k)
+        gram, hk, bits = pr.cols_add(rank, k)
         col = O.ranking_column(n, masks_x, alt_x, rank).astype(bool)
+        assert abs(hk - math.fsum(p_x[col])) <= 1e-12 * max(1.0, abs(hk)), k
         assert np.array_equal(np.unpackbits(bits.view(np.uint8), bitorder="little")[:M].astype(bool), col), k
         assert bits.size == W and not np.any(np.unpackbits(bits.view(np.uint8), bitorder="little")[M:])
         cols.append(col)
@@ -269,7 +271,7 @@ def test_column_store_matches_oracle(n, cells, layout):
         assert a[0] == b[0] and np.array_equal(a[1], b[1]) and a[2] == b[2]
     pr.cols_load(np.stack(bits_all[:5]))
     rank = rng.permutation(n).astype(np.int32)
-    gram, bits = pr.cols_add(rank, 5)
+    gram, _hk, bits = pr.cols_add(rank, 5)
     col = O.ranking_column(n, masks_x, alt_x, rank).astype(bool)
     assert np.array_equal(gram, [float((c & col).sum()) for c in cols[:5] + [col]])
     pr.cols_scores(np.array([5], dtype=np.int32), np.array([1.0]))
