@@ -54,8 +54,9 @@ class QPState:
 
 
 def solve_qp_native(Q, h, A_init=None, max_outer=200, tol=1e-10, reg=1e-12, state=None):
-    """Active-set solve of the simplex QP in librumgpu; returns (lam, support list). With a
-    QPState the factor of a working set equal to the last call's is reused."""
+    """Active-set solve of the simplex QP in librumgpu; returns (lam, support as an int32 array, in
+    working-set order). With a QPState the factor of a working set equal to the last call's is
+    reused."""
     K = int(h.size)
     Q = np.asarray(Q, dtype=np.float64)
     if Q.strides[1] != 8 or Q.shape != (K, K):
@@ -75,7 +76,7 @@ def solve_qp_native(Q, h, A_init=None, max_outer=200, tol=1e-10, reg=1e-12, stat
                                         int(A.size), int(max_outer), float(tol), float(reg), _lib.ptr(lam),
                                         _lib.ptr(sup), ctypes.byref(ns))
     _lib.check(rc, "qp_simplex")
-    return lam, [int(i) for i in sup[:ns.value]]
+    return lam, sup[:ns.value].copy()
 
 
 def solve_qp_numpy(Q, h, A_init=None, max_outer=200, tol=1e-10, reg=1e-12):
