@@ -222,6 +222,47 @@ def test_native_master_qp_matches_numpy_and_oracle():
         assert abs(f(lam4) - f(lam2)) <= 1e-9 * max(1.0, abs(f(lam2)))
 
 
+def test_column_store_planning_and_packed_columns():
+    """The flow_cg plan charges a single run's device column store (bitsets of max_iters + 2
+    columns, p, p_out) to the GPU budget, and keeps the columns on the host when the store does not
+    fit beside the sweep; packed columns read back as the 0/1 columns they were packed from."""
+    from solver.flow_cg.engine import PackedColumns
+    prof = dict(use_gpu=True, gpu=dict(supported=True, name="stub"), ram_bytes=64 << 30)
+    import solver.machine as machine
+    saved = machine.gpu_free_bytes
+    mp = 104885
+    try:
+        machine.gpu_free_bytes = lambda device=0: 250 << 30
+        a = _plan_flow_cg(20, mp, False, prof, dict(defaults.METHOD_ARGS["flow_cg"]))
+        k = int(a["max_iters"]) + 2
+        cols = 8 * k * ((mp + 63) // 64) + 16 * mp + 20 * k
+        assert "_cols" not in a and a["pricing"] == "slab"
+        need = a["_gpu_bytes"] - cols
+        # a budget that holds the sweep but not the store beside it
+        frac = defaults.AUTO["gpu_memory_fraction"]
+        machine.gpu_free_bytes = lambda device=0: int((need + cols // 2) / frac)
+        b = _plan_flow_cg(20, mp, False, prof, dict(defaults.METHOD_ARGS["flow_cg"]))
+        assert b["pricing"] == "slab" and b["_cols"] == "host" and b["_gpu_bytes"] == need
+        # at most 4,096 rows the columns stay on the host and nothing is charged
+        machine.gpu_free_bytes = lambda device=0: 250 << 30
+        c = _plan_flow_cg(16, 250, False, prof, dict(defaults.METHOD_ARGS["flow_cg"]))
+        assert "_cols" not in c
+    finally:
+        machine.gpu_free_bytes = saved
+    rng = np.random.default_rng(5)
+    M = 1000
+    C = (rng.random((7, M)) < 0.3).astype(np.uint8)
+    packed = []
+    for col in C:
+        pad = np.zeros(((M + 63) // 64) * 64, np.uint8)
+        pad[:M] = col
+        packed.append(np.packbits(pad, bitorder="little").view(np.uint64))
+    pc = PackedColumns(packed, M)
+    assert len(pc) == 7 and all(np.array_equal(pc[i], C[i]) for i in range(7))
+    assert [x.tolist() for x in pc[2:4]] == [C[2].tolist(), C[3].tolist()]
+    assert all(np.array_equal(x, y) for x, y in zip(pc, C))
+
+
 def test_adaptive_penalty_rule():
     """solve_kadmm's beta_adapt on a scripted handle: doubles while ub - phi dominates, halves while
     phi - lb dominates, a reversal takes sqrt(2) and freezes, at most max_changes changes"""
