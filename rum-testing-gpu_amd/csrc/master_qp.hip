@@ -18,14 +18,25 @@ namespace {
 
 struct Chol {
   int k = 0;
-  std::vector<double> L;   // row-major lower factor, row stride cap (grown by doubling)
+  std::vector<double> L;   // row-major lower factor, row stride cap (grown by doubling, plus 8 so that
+                           // the stride is no power of two and rows do not alias in the caches)
   int cap = 0;
   bool ok = false;
   void reserve(int need) {
     if (need <= cap) return;
-    cap = std::max(need, 2 * cap);
+    cap = std::max(need, 2 * cap) + 8;
     L.assign((size_t)cap * cap, 0.0);
     ok = false;
+  }
+  // room for `need` rows keeping the factor: its k rows move to the wider stride (O(k^2))
+  void grow(int need) {
+    if (need <= cap) return;
+    const int nc = std::max(need, 2 * cap) + 8;
+    std::vector<double> nl((size_t)nc * nc, 0.0);
+    for (int i = 0; i < k; ++i)
+      std::copy(L.begin() + (size_t)i * cap, L.begin() + (size_t)i * cap + i + 1, nl.begin() + (size_t)i * nc);
+    L.swap(nl);
+    cap = nc;
   }
 };
 
@@ -101,8 +112,9 @@ void chol_delete(Chol& c, int i) {
 // Forward substitution L y = b by rows, row i's dot product in four partial sums per right-hand
 // side (eight independent chains). Rows 0..i-1 of L do not change when a row is appended, so the
 // y of a grown factor is the old y plus one fwd_row step.
-void fwd_row(const Chol& c, int i, const double* b1, const double* b2, double* y1, double* y2) {
-  const double* row = &c.L[(size_t)i * c.cap];
+void fwd_row(const Chol& c, int i, const double* __restrict__ b1, const double* __restrict__ b2,
+             double* __restrict__ y1, double* __restrict__ y2) {
+  const double* __restrict__ row = &c.L[(size_t)i * c.cap];
   double p1[4] = {0.0, 0.0, 0.0, 0.0}, p2[4] = {0.0, 0.0, 0.0, 0.0};
   int t = 0;
   for (; t + 4 <= i; t += 4)
@@ -119,11 +131,12 @@ void fwd_row(const Chol& c, int i, const double* b1, const double* b2, double* y
 }
 
 // L' x = y by rows of L: x_i is final once the rows below have been applied
-void bwd2(const Chol& c, const double* y1, const double* y2, double* x1, double* x2) {
+void bwd2(const Chol& c, const double* __restrict__ y1, const double* __restrict__ y2, double* __restrict__ x1,
+          double* __restrict__ x2) {
   const int k = c.k;
   for (int i = 0; i < k; ++i) x1[i] = y1[i], x2[i] = y2[i];
   for (int i = k - 1; i >= 0; --i) {
-    const double* row = &c.L[(size_t)i * c.cap];
+    const double* __restrict__ row = &c.L[(size_t)i * c.cap];
     const double xi1 = x1[i] / row[i], xi2 = x2[i] / row[i];
     x1[i] = xi1;
     x2[i] = xi2;
@@ -388,7 +401,8 @@ static int qp_simplex_impl(int K, const double* Q, int64_t ldq, const double* h,
         std::vector<double> q(k);
         for (int i = 0; i < k; ++i) q[i] = Q[(int64_t)jbest * ldq + work[i]];
         work.push_back(jbest), in[jbest] = 1;
-        if (ch.ok && ch.k == k && k + 1 <= ch.cap && chol_append(ch, q.data(), Q[(int64_t)jbest * ldq + jbest] + reg)) {
+        if (ch.ok && ch.k == k) ch.grow(k + 1);
+        if (ch.ok && ch.k == k && chol_append(ch, q.data(), Q[(int64_t)jbest * ldq + jbest] + reg)) {
           if (fwd) {                           // the grown factor's forward solutions: one more row
             bh[k] = h[jbest];
             fwd_row(ch, k, bh.data(), be.data(), yh.data(), ye.data());
