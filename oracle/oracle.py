@@ -242,6 +242,47 @@ def price_stream(n, mode, masks_x, scores_x, offsets, k_pred=None):
     return vmax, margin.value, pred_top
 
 
+def stream_ranking(n, masks_x, scores_x, offsets, pred_top, k_pred=None):
+    """The stream pricer's ranking (StreamPricer.backtrace / _solve_sublattice,
+    flow_cg/pricing_stream.py:348-405): from the full set, take the stored predecessor of each layer
+    k = n .. k_pred+1 (pred_top holds layers k_pred..n back to back, each in colex order); the node
+    reached has k_pred members, whose order comes from the exact float32 mask DP on its sub-lattice:
+    the rows whose menu lies inside it, members relabelled 0..K-1 in ascending order, x-major by the
+    new label with the rows' order kept. Returns rank_pos[n] (0 = most preferred)."""
+    k_pred = max(1, min(n - 10 if k_pred is None else int(k_pred), n))
+    masks_x = np.asarray(masks_x, dtype=np.uint64)
+    scores_x = np.asarray(scores_x, dtype=np.float32)
+    offsets = np.asarray(offsets, dtype=np.uint64)
+    layer_off, acc = {}, 0
+    for k in range(k_pred, n + 1):
+        layer_off[k] = acc
+        acc += math.comb(n, k)
+    rank_pos = np.zeros(n, dtype=np.int32)
+    D, pos = (1 << n) - 1, 0
+    for k in range(n, k_pred, -1):
+        x = int(pred_top[layer_off[k] + colex_rank(D)])
+        rank_pos[x] = pos
+        pos += 1
+        D ^= 1 << x
+    elems = [b for b in range(n) if (D >> b) & 1]
+    K = len(elems)
+    assert K == k_pred
+    alt_x = np.repeat(np.arange(n), np.diff(offsets.astype(np.int64)))
+    inside = (masks_x & np.uint64(D)) == masks_x
+    sub_m = np.zeros(int(inside.sum()), dtype=np.uint64)
+    for j, e in enumerate(elems):
+        sub_m |= ((masks_x[inside] >> np.uint64(e)) & np.uint64(1)) << np.uint64(j)
+    relabel = {e: j for j, e in enumerate(elems)}
+    sub_a = np.array([relabel[int(a)] for a in alt_x[inside]], dtype=np.int64)
+    sub_s = scores_x[inside]
+    order = np.argsort(sub_a, kind="stable")
+    sub_off = np.concatenate(([0], np.cumsum(np.bincount(sub_a, minlength=K)))).astype(np.uint64)
+    _v, sub_rank, _V, _p = price_mask(K, sub_m[order], sub_s[order], sub_off)
+    for j, e in enumerate(elems):
+        rank_pos[e] = pos + int(sub_rank[j])
+    return rank_pos
+
+
 def colex_rank(mask):
     return int(lib().orc_colex_rank(int(mask)))
 

@@ -244,6 +244,7 @@ __global__ void cg_backtrace_batch(const uint8_t* __restrict__ pred, const float
 // per tile, a second settles the global max, the closure runs in windows, and a thread per draw
 // walks the path.
 #define CL_TILE 256
+#define CL_MAX_N 20
 #define CG_BATCH 32
 
 // dynamic LDS: the rows' scores, [row][DB] float32 (a chunk of at most rows_lds rows)
@@ -1032,6 +1033,10 @@ static int cl_check(rumgpu_cg* c) {
 
 extern "C" int rumgpu_cg_set_sweep(rumgpu_cg* c, int mode) {
   RG_CHECK(c && (mode == 0 || mode == 1), RG_EINVAL, "cg_set_sweep: mode must be 0 (layers) or 1 (auto)");
+  // The closure sweep's first run at n = 25 ended in a GPU fault whose cause has not been shown; it
+  // passed at n <= 20, so it is refused above that until the cause is known.
+  RG_CHECK(mode == 0 || c->n <= CL_MAX_N, RG_EUNSUP,
+           "cg_set_sweep: the one-alternative closure sweep is not run above n = %d (n = %d)", CL_MAX_N, c->n);
   c->sweep = mode;
   return RG_OK;
 }

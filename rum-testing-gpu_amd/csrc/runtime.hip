@@ -98,7 +98,14 @@ extern "C" int rumgpu_device_props(int device, char* name, int name_len, char* a
   hipDeviceProp_t pr;
   RG_HIP(hipGetDeviceProperties(&pr, device));
   if (name && name_len > 0) {
-    strncpy(name, pr.name, (size_t)name_len - 1);
+    // some boxes report an empty marketing name: fall back to the architecture and CU count
+    char fallback[96];
+    const char* nm = pr.name;
+    if (!nm[0]) {
+      snprintf(fallback, sizeof(fallback), "AMD %s (%d CUs)", pr.gcnArchName, pr.multiProcessorCount);
+      nm = fallback;
+    }
+    strncpy(name, nm, (size_t)name_len - 1);
     name[name_len - 1] = 0;
   }
   if (arch && arch_len > 0) {

@@ -126,11 +126,13 @@ class Pricer:
         return math.sqrt(max(0.0, f - 2.0 * (vmax - ry)))
 
     SWEEPS = {"layers": 0, "auto": 1}
+    CLOSURE_MAX_N = 20      # the library refuses the closure sweep above this n
 
     def set_sweep(self, mode):
         """'layers' (the default): the layer-by-layer sweep for any rows (the reference's); 'auto': the
         one-alternative closure sweep when every row belongs to one alternative (same results, 4 bytes
-        per node; opt-in, tested on the GPU up to n = 20)"""
+        per node; opt-in, refused above n = 20: its first run at n = 25 ended in a GPU fault whose cause
+        has not been shown)"""
         _lib.check(self.lib.rumgpu_cg_set_sweep(self.h, self.SWEEPS[mode]), "cg_set_sweep")
 
     def one_alternative(self):

@@ -21,6 +21,7 @@ import time
 import numpy as np
 
 from .. import defaults
+from ..flow_cg.engine import Pricer as _Pricer
 
 SLOT_CHOICES = (4, 8, 16, 32, 64, 128, 256)
 _TUNED = {}
@@ -294,7 +295,8 @@ class BatchedCG:
         self.overlap = bool(overlap) and self.D >= 2
         from .. import machine as _machine
         n = boot.rum.n
-        one_alt = np.unique(boot.rum._geom.alternative).size == 1 and sess._args.get("_sweep", "layers") == "auto"
+        one_alt = (np.unique(boot.rum._geom.alternative).size == 1 and sess._args.get("_sweep", "layers") == "auto"
+                   and n <= _Pricer.CLOSURE_MAX_N)
         if one_alt:     # the closure sweep: 4 bytes per subset of the other alternatives, and the scores
             need = 4 * (1 << (n - 1)) + 32 * 4 * boot.rum.M
         else:           # batched V, pred, float64 V (single and 8 draws), scores

@@ -851,7 +851,7 @@ def _cg_pricer(t):
     if pricer.layout == "mask":
         # "auto" opts in to the one-alternative closure sweep; "layers" (the default) is the
         # reference's layer-by-layer sweep
-        pricer.set_sweep(a.get("_sweep", "layers"))
+        pricer.set_sweep(a.get("_sweep", "layers") if g.num_alternatives <= Pricer.CLOSURE_MAX_N else "layers")
     return pricer
 
 

@@ -9,7 +9,7 @@ reach the per-draw bootstrap's decision on every draw.
 import numpy as np
 import pytest
 
-from solver import instances
+from solver import _lib, instances
 from solver.bm_admm.engine import Handle
 from solver.inference import cells_from_panel
 from solver.rum_solver import RUM
@@ -220,6 +220,22 @@ def test_one_alternative_sweep_equals_layers(n, a, D, quant):
             vo, ro, _V, _pred = orc.price_mask(n, masks_x, S[d], offsets)
             assert vo == v0 and np.array_equal(ro, r0), d
     assert l1 == l0
+    pr.close()
+
+
+def test_one_alternative_sweep_refused_above_n20():
+    """Above n = 20 the library refuses the closure sweep (status 4) and the layered sweep stays on;
+    nothing is launched: the refusal happens on the host."""
+    from solver.flow_cg.engine import Pricer, xmajor
+    n = 21
+    rng = np.random.default_rng(21)
+    menu = np.array([int(m) | 1 for m in rng.integers(2, 1 << n, size=40)], dtype=np.uint64)
+    alt = np.zeros(menu.size, dtype=np.int32)
+    perm, masks_x, _alts, offsets = xmajor(menu, alt, n)
+    pr = Pricer(n, masks_x, offsets, layout="mask")
+    with pytest.raises(_lib.GPUError, match="not run above n = 20"):
+        pr.set_sweep("auto")
+    assert pr.one_alternative() == -1
     pr.close()
 
 

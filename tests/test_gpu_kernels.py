@@ -211,20 +211,35 @@ def test_slab_pricing_equals_mask_on_quantized_duals(n):
 
 
 @pytest.mark.parametrize("mode", ["float32", "uint16", "uint24"])
-@pytest.mark.parametrize("n", [9, 14, 17])
+@pytest.mark.parametrize("n", [9, 14, 17, 18])
 def test_stream_pricing_bit_exact(n, mode):
+    """Stream pricing (float32, uint16, uint24 layers): vmax, margin and the ranking (the column that
+    enters the master) equal the oracle's bit for bit: the stored predecessors walked from the full
+    set, then the sub-lattice DP (pricing_stream.py:348-405). n = 18 prices config2's rows (23,595
+    cells). Both random scores and quantized duals (multiples of 1/4, where ties are everywhere and
+    the lowest-member rule decides) are priced."""
     rng = np.random.default_rng(200 + n)
-    menu, alt, p = _medium(n, min(3000, n << (n - 3)), rng)
+    if n == 18:
+        cd, _n = instances.config2()
+        menu, alt = cd[:, 0].astype(np.uint64), cd[:, 1].astype(np.int32)
+    else:
+        menu, alt, p = _medium(n, min(3000, n << (n - 3)), rng)
     perm, masks_x, _alts, offsets = xmajor(menu, alt, n)
     pr = Pricer(n, masks_x, offsets, layout="stream", dtype=mode)
     omode = {"float32": "f32", "uint16": "u16", "uint24": "u24"}[mode]
-    scores = rng.standard_normal(menu.size).astype(np.float32)
-    vmax, rank, margin = pr.price(scores)
-    vo, mo, _pred = O.price_stream(n, omode, masks_x, scores, offsets)
-    assert vmax == vo and abs(margin - mo) <= 1e-12 * max(1.0, mo)
-    if mode == "float32":
-        vmask, rank_mask, _V, _p = O.price_mask(n, masks_x, scores, offsets)
-        assert vmax == vmask and np.array_equal(rank, rank_mask)
+    for quant in (False, True):
+        scores = rng.standard_normal(menu.size).astype(np.float32)
+        if quant:
+            scores = (np.round(scores * 4) / 4).astype(np.float32)
+        print("[case] stream %s n=%d M=%d quantized=%s" % (mode, n, menu.size, quant), flush=True)
+        vmax, rank, margin = pr.price(scores)
+        vo, mo, pred_top = O.price_stream(n, omode, masks_x, scores, offsets)
+        assert vmax == vo and abs(margin - mo) <= 1e-12 * max(1.0, mo)
+        rank_o = O.stream_ranking(n, masks_x, scores, offsets, pred_top)
+        assert np.array_equal(rank, rank_o), (quant, rank, rank_o)
+        if mode == "float32":
+            vmask, rank_mask, _V, _p = O.price_mask(n, masks_x, scores, offsets)
+            assert vmax == vmask and np.array_equal(rank, rank_mask)
     pr.close()
 
 

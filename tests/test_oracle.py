@@ -191,6 +191,51 @@ def test_stream_quantized_within_margin(mode):
     assert abs(vq - vmask) <= margin + 1e-6 * abs(vmask)
 
 
+@pytest.mark.parametrize("n,k_pred", [(7, 1), (7, 4), (7, 7), (10, 1), (10, 3), (12, 2), (12, 6)])
+def test_stream_ranking_f32_equals_mask_ranking(n, k_pred):
+    """In float32 mode the stream sweep's predecessors are the mask sweep's (lowest member on ties),
+    so its ranking must be the mask ranking for every k_pred: k_pred = 1 walks the stored
+    predecessors only, k_pred = n is the sub-lattice DP alone (every row inside the full set,
+    nothing relabelled), values between split the walk at every depth. A wrong colex rank, layer
+    offset or relabelling changes the ranking."""
+    rng = np.random.default_rng(60 + n + k_pred)
+    menu, alt = _rows_all_menus(n, rng, frac=min(1.0, 400.0 / (1 << n)))
+    perm, masks_x, offsets = O.xmajor(menu, alt, n)
+    for quant in (False, True):
+        scores = rng.standard_normal(menu.size).astype(np.float32)
+        if quant:            # multiples of 1/4: ties everywhere, exact sums
+            scores = (np.round(scores * 4) / 4).astype(np.float32)
+        sx = scores[perm]
+        _v, rank_mask, _V, _pred = O.price_mask(n, masks_x, sx, offsets)
+        _vs, _m, pred_top = O.price_stream(n, "f32", masks_x, sx, offsets, k_pred=k_pred)
+        rank = O.stream_ranking(n, masks_x, sx, offsets, pred_top, k_pred=k_pred)
+        assert np.array_equal(rank, rank_mask), (quant, rank, rank_mask)
+
+
+@pytest.mark.parametrize("mode", ["u16", "u24"])
+@pytest.mark.parametrize("n", [7, 11])
+def test_stream_ranking_quantized_value(mode, n):
+    """u16 / u24: the ranking is a permutation whose exact value (its column against the scores) is
+    within two margins of the exact maximum (brute force over the n! rankings at n = 7, the mask
+    sweep at n = 11), and at most that maximum."""
+    rng = np.random.default_rng(80 + n)
+    menu, alt = _rows_all_menus(n, rng, frac=min(1.0, 500.0 / (1 << n)))
+    scores = rng.standard_normal(menu.size).astype(np.float32)
+    perm, masks_x, offsets = O.xmajor(menu, alt, n)
+    sx = scores[perm]
+    for k_pred in (1, max(1, n - 4)):
+        _vq, margin, pred_top = O.price_stream(n, mode, masks_x, sx, offsets, k_pred=k_pred)
+        rank = O.stream_ranking(n, masks_x, sx, offsets, pred_top, k_pred=k_pred)
+        assert sorted(rank.tolist()) == list(range(n))
+        col = O.ranking_column(n, masks_x, np.repeat(np.arange(n), np.diff(offsets.astype(np.int64))), rank)
+        val = float((sx.astype(np.float64) * col).sum())
+        if n <= 7:
+            best, _ = O.brute_pricing(n, menu, alt, scores.astype(np.float64))
+        else:
+            best = O.price_mask(n, masks_x, sx, offsets)[0]
+        assert val <= best + 1e-5 and val >= best - 2.0 * margin - 1e-5, (val, best, margin)
+
+
 def test_colex_rank_is_ascending_order():
     n = 9
     for k in range(n + 1):
