@@ -131,6 +131,17 @@ int rumgpu_prox_slab(rumgpu_bm* bm, const double* xk, double* out);
  * fw_lb_rows_gpu, common/transforms.py:101-114 (flow_cg's final bound, rum_solver.py:928-938) */
 int rumgpu_lb64(rumgpu_ctx* ctx, int n, const uint64_t* menu, const int32_t* alt, int64_t M, const double* y,
                 const double* p, double* lb);
+/* The same lower bound with no allocation. No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). menu_dev, alt_dev, y_dev and p_dev are M rows in device memory of ctx's device;
+ * lb_dev is one double in device memory. Every buffer rumgpu_lb64 allocates per call lives instead in
+ * one fixed device array of the library, sized for n <= 18 (22,028,800 bytes, held by each device the
+ * library runs on until the process ends); larger n returns status 4. Work runs on ctx's stream and
+ * has finished when the call returns. Calls on one device share that array: calls that overlap on one
+ * GPU (from several threads or contexts) race. A row whose alternative is outside 0..n-1 or not in its
+ * menu, or whose menu has bits at n or above, makes lb = NaN. Same kernels and order as rumgpu_lb64. */
+int rumgpu_lb64_noalloc(rumgpu_ctx* ctx, int n, const uint64_t* menu_dev, const int32_t* alt_dev, int64_t M,
+                        const double* y_dev, const double* p_dev, double* lb_dev);
 
 /* ---- Flow-CG pricing (replaces bf_update, flow_cg/bf_pricing.py:8-47 with the sweep and
  *      backtrace of flow_cg/solver.py:168-200, and StreamPricer, flow_cg/pricing_stream.py:267-412)

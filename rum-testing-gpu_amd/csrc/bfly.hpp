@@ -56,18 +56,22 @@ inline std::vector<WinPlan> plan_windows_explicit(int nb, int itemsize, int lo_b
   return out;
 }
 
-inline std::vector<WinPlan> plan_windows(int nb, int itemsize) {
-  std::vector<WinPlan> out;
-  if (nb <= 0) return out;
+// At most 4 windows for nb <= 36 (12 + 3 x 8 bits).
+constexpr int kMaxWindows = 4;
+
+// The built-in plan written into out[kMaxWindows]; returns the number of windows (no allocation).
+inline int plan_windows_into(int nb, int itemsize, WinPlan* out) {
+  if (nb <= 0) return 0;
   // bit 0 window up to 12 bits (a 4096-value contiguous tile, 33 KiB of float64 LDS); windows
   // above it up to 8 bits with 16-32 columns. As few windows as that allows: each window is one
   // fused pass per transform.
   int k = nb <= 12 ? 1 : 1 + (nb - 12 + 7) / 8;
+  if (k > kMaxWindows) k = kMaxWindows;
   int lo = (nb + k - 1) / k;
   if (lo < nb - 8 * (k - 1)) lo = nb - 8 * (k - 1);
   if (lo > 12) lo = 12;
   int rem = nb - lo;
-  out.push_back({0, lo, 0, lo < 3 ? lo : 3});
+  out[0] = {0, lo, 0, lo < 3 ? lo : 3};
   int s = lo;
   for (int i = 1; i < k; ++i) {
     int left = k - i;
@@ -75,11 +79,17 @@ inline std::vector<WinPlan> plan_windows(int nb, int itemsize) {
     int Cw = itemsize == 8 ? 4 : 5;
     if (Cw > s) Cw = s;
     int RL = B < 4 ? B : 4;
-    out.push_back({s, B, Cw, RL});
+    out[i] = {s, B, Cw, RL};
     s += B;
     rem -= B;
   }
-  return out;
+  return k;
+}
+
+inline std::vector<WinPlan> plan_windows(int nb, int itemsize) {
+  WinPlan w[kMaxWindows];
+  const int k = plan_windows_into(nb, itemsize, w);
+  return std::vector<WinPlan>(w, w + k);
 }
 
 inline WinGeom make_geom(int n, const WinPlan& w, int slots = 1) {

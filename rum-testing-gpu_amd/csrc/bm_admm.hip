@@ -29,6 +29,7 @@
 
 #include <cmath>
 #include <algorithm>
+#include <mutex>
 
 enum PassKind { K_FIRST, K_SY, K_SYONLY, K_RHS, K_D, K_XK, K_FULL, K_FULL_FIRST, K_T };
 
@@ -1826,6 +1827,151 @@ extern "C" int rumgpu_lb64(rumgpu_ctx* ctx, int n, const uint64_t* menu, const i
   }
   rg_free(g); rg_free(dp); rg_free(dy); rg_free(dpv); rg_free(obs); rg_free(order); rg_free(part); rg_free(res);
   return rc;
+}
+
+// ---------------------------------------------------------------------------------------------
+// The same float64 lower bound without any device allocation (rumgpu_lb64_noalloc): rows, y and p
+// are device arrays, and every buffer rumgpu_lb64 allocates per call lives in one fixed device
+// array of this library. Layout (byte offsets, sized for n = LB64_NOALLOC_MAX_N, each a multiple of
+// 256): the float64 slab g, dp[2^n], the node order int[2^n], the block partials (256 blocks x 2
+// quantities x (sum, compensation)), the two reduced sums, and a bad-row flag.
+#define LB64_NOALLOC_MAX_N 18
+static constexpr size_t kNaSlab = (size_t)LB64_NOALLOC_MAX_N << (LB64_NOALLOC_MAX_N - 1);   // doubles
+static constexpr size_t kNaNodes = (size_t)1 << LB64_NOALLOC_MAX_N;
+static constexpr size_t kNaOffDp = kNaSlab * 8;
+static constexpr size_t kNaOffOrder = kNaOffDp + kNaNodes * 8;
+static constexpr size_t kNaOffPart = kNaOffOrder + kNaNodes * 4;
+static constexpr size_t kNaOffRes = kNaOffPart + 256 * 4 * 8;
+static constexpr size_t kNaOffFlag = kNaOffRes + 256;
+static constexpr size_t kNaBytes = kNaOffFlag + 256;                                      // 22,028,800
+
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+
+// Device address of `scratch` per device, taken once with that device current (hipGetSymbolAddress);
+// host code uses only these addresses, never the array's name.
+static std::mutex g_na_mu;
+static void* g_na_ptr[64] = {};
+static hipError_t g_na_err[64] = {};
+
+int rg_noalloc_setup(int device) {
+  if (device < 0 || device >= 64) return RG_EUNSUP;
+  std::lock_guard<std::mutex> lk(g_na_mu);
+  if (g_na_ptr[device]) return RG_OK;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  g_na_err[device] = e;
+  if (e != hipSuccess || !p) {
+    (void)hipGetLastError();
+    rg_set_error("lb64_noalloc: hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+    return RG_EHIP;
+  }
+  g_na_ptr[device] = p;
+  return RG_OK;
+}
+
+// r = p - y at row j (the slab index from its menu and alternative), g[index] = r, f = sum r^2,
+// ry = sum r y: the arithmetic and the order of k_lb_rows. A row whose alternative is outside 0..n-1
+// or not in its menu, or whose menu has bits at n or above, is skipped and raises the flag.
+__global__ __launch_bounds__(256) void k_lb_rows_dev(const u64* __restrict__ menu, const int32_t* __restrict__ alt,
+                                                     int n, i64 Nloc, const double* __restrict__ yv,
+                                                     const double* __restrict__ p, i64 M, double* g, double* part,
+                                                     int* flag) {
+  double s[2] = {0.0, 0.0}, c[2] = {0.0, 0.0};
+  for (i64 j = (i64)blockIdx.x * blockDim.x + threadIdx.x; j < M; j += (i64)gridDim.x * blockDim.x) {
+    const u64 m = menu[j];
+    const int a = alt[j];
+    if (a < 0 || a >= n || !((m >> a) & 1ull) || (m >> n) != 0ull) {
+      flag[0] = 1;
+      continue;
+    }
+    const i64 o = (i64)a * Nloc + (i64)compress_bit(m & ~(1ull << a), a);
+    const double y = yv[j];
+    const double r = p[j] - y;
+    g[o] = r;
+    two_sum(s[0], c[0], r * r);
+    two_sum(s[1], c[1], r * y);
+  }
+  block_reduce_store<2>(s, c, part);
+}
+
+// Node order by popcount, ascending masks within a layer: node v sits at (number of masks of lower
+// popcount) + (colex rank of v among the masks of its popcount), the order rumgpu_lb64 builds on the host.
+__device__ inline i64 na_binom(int a, int b) {
+  if (b < 0 || b > a) return 0;
+  i64 r = 1;
+  for (int i = 1; i <= b; ++i) r = r * (a - b + i) / i;
+  return r;
+}
+__global__ void k_layer_order(int* order, int n) {
+  const i64 v = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= (1ll << n)) return;
+  const int k = __popcll((u64)v);
+  i64 pos = 0;
+  for (int j = 0; j < k; ++j) pos += na_binom(n, j);
+  int i = 0;
+  for (int b = 0; b < n; ++b)
+    if ((v >> b) & 1) pos += na_binom(b, ++i);
+  order[pos] = (int)v;
+}
+
+// lb = sqrt(max(0, f - 2 (dp[full] - ry))), NaN when a row was bad (the flag is cleared at the start of
+// every call).
+__global__ void k_lb_finish(const double* res, const double* dp, i64 full, int* flag, double* lb) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double l2 = res[0] - 2.0 * (dp[full] - res[1]);
+  lb[0] = flag[0] ? __longlong_as_double(0x7ff8000000000000ll) : sqrt(l2 > 0.0 ? l2 : 0.0);
+}
+
+extern "C" int rumgpu_lb64_noalloc(rumgpu_ctx* ctx, int n, const uint64_t* menu_dev, const int32_t* alt_dev,
+                                   int64_t M, const double* y_dev, const double* p_dev, double* lb_dev) {
+  RG_CHECK(ctx && menu_dev && alt_dev && y_dev && p_dev && lb_dev && M >= 1, RG_EINVAL,
+           "lb64_noalloc: bad argument");
+  RG_CHECK(n >= 2 && n <= LB64_NOALLOC_MAX_N, RG_EUNSUP,
+           "lb64_noalloc: n = %d outside 2..%d (its fixed scratch holds %zu bytes per device)", n,
+           LB64_NOALLOC_MAX_N, kNaBytes);
+  RG_HIP(hipSetDevice(ctx->device));
+  void* base = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_na_mu);
+    if (ctx->device >= 0 && ctx->device < 64) base = g_na_ptr[ctx->device];
+  }
+  RG_CHECK(base, RG_EHIP, "lb64_noalloc: the scratch of device %d has no address (set-up failed: %s)", ctx->device,
+           hipGetErrorString(ctx->device >= 0 && ctx->device < 64 ? g_na_err[ctx->device] : hipErrorInvalidDevice));
+  unsigned char* sb = (unsigned char*)base;
+  double* g = (double*)sb;
+  double* dp = (double*)(sb + kNaOffDp);
+  int* order = (int*)(sb + kNaOffOrder);
+  double* part = (double*)(sb + kNaOffPart);
+  double* res = (double*)(sb + kNaOffRes);
+  int* flag = (int*)(sb + kNaOffFlag);
+  const i64 Nloc = 1ll << (n - 1), m2 = (i64)n * Nloc, NN = 1ll << n;
+  hipStream_t st = ctx->stream;
+  // the scratch keeps the last call's contents: clear the slab, dp[0] and the flag
+  hipLaunchKernelGGL((ew_fill<double>), dim3(ew_grid(m2)), dim3(256), 0, st, g, 0.0, m2);
+  hipLaunchKernelGGL((ew_fill<int>), dim3(1), dim3(64), 0, st, flag, 0, (i64)1);
+  hipLaunchKernelGGL(k_layer_order, dim3(grid_for(NN, 256)), dim3(256), 0, st, order, n);
+  const int nblk = (int)std::min<i64>(256, std::max<i64>(1, (M + 255) / 256));
+  hipLaunchKernelGGL(k_lb_rows_dev, dim3(nblk), dim3(256), 0, st, (const u64*)menu_dev, alt_dev, n, Nloc, y_dev,
+                     p_dev, (i64)M, g, part, flag);
+  hipLaunchKernelGGL(k_final, dim3(1), dim3(64), 0, st, part, nblk, 2, res);
+  RG_LAUNCH_CHECK();
+  WinPlan wins[kMaxWindows];
+  const int nw = plan_windows_into(n - 1, 8, wins);
+  for (int w = 0; w < nw; ++w)
+    RG_TRY((launch_tr_rl<double, double, 0, 0>(wins[w].RL, g, g, coef<double>(1.0, 0.0, 1.0, 1.0),
+                                                make_geom(n, wins[w]), nullptr, nullptr, st)));
+  hipLaunchKernelGGL((ew_fill<double>), dim3(1), dim3(64), 0, st, dp, 0.0, (i64)1);
+  i64 lo = 1;                                   // layer c starts after the C(n, j) masks of j < c
+  i64 ns = n;
+  for (int c = 1; c <= n; ++c) {
+    launch_lpath(n, dim3(grid_for(ns, 256)), st, g, dp, order + lo, ns, Nloc);
+    lo += ns;
+    ns = ns * (n - c) / (c + 1);
+  }
+  hipLaunchKernelGGL(k_lb_finish, dim3(1), dim3(64), 0, st, res, dp, NN - 1, flag, lb_dev);
+  RG_LAUNCH_CHECK();
+  RG_HIP(hipStreamSynchronize(st));
+  return RG_OK;
 }
 
 // Per-pass device time of `iters` fused iterations, measured with HIP events on the context stream

@@ -59,6 +59,7 @@ SIGNATURES = {
     "rumgpu_butterfly32": [_P, _I, _P, _F, _F, _F, _F],
     "rumgpu_prox_slab": [_P, _P, _P],
     "rumgpu_lb64": [_P, _I, _P, _P, _I64, _P, _P, _P],
+    "rumgpu_lb64_noalloc": [_P, _I, _P, _P, _I64, _P, _P, _P],
     "rumgpu_cg_create": [_P, _I, _P, _P, _I64, _I, _I, _I, _P],
     "rumgpu_cg_destroy": [_P],
     "rumgpu_cg_price_batch": [_P, _I, _P, _P, _P],
