@@ -68,6 +68,44 @@ def test_dsr_draws_reproduce_stored_fingerprints():
     assert D.n_redraws == rec["panel"]["redraws"]
 
 
+def _ai_luna():
+    z = np.load(os.path.join(GOLDEN, "ai_luna_replies.npz"))          # allow_pickle=False (numpy's default)
+    table = pd.DataFrame({c: z[c].astype(np.int64) for c in ("respondent", "menu", "chosen")})
+    menu, alt, p = cells_from_panel(table, 18)
+    return table, menu, alt, p
+
+
+def test_ai_luna_panel_draws_reproduce_stored_fingerprints():
+    """The reference's AI bootstrap record (gpt-5.6-luna, n = 18, 100 replicates as respondents,
+    B = 10,000 multinomial_reject draws, seed 0): the cells (73,492 rows), the panel and the whole
+    B x M draw matrix hash to the stored fingerprints."""
+    rec = json.load(open(os.path.join(GOLDEN, "ai_luna_record.json")))["panel"]
+    table, menu, alt, p = _ai_luna()
+    assert menu.size == 73492
+    assert _sha(np.column_stack([menu.astype(np.float64), alt.astype(np.float64), p])) == rec["fingerprints"]["data"]
+    D = Draws(table, menu, alt, p, B=rec["settings"]["B"], seed=rec["settings"]["seed"])
+    assert D.input_fingerprint == rec["fingerprints"]["panel"]
+    for _ in D.blocks():
+        pass
+    assert D.fingerprint == rec["fingerprints"]["draws"]
+    assert D.n_redraws == rec["panel"]["redraws"]
+
+
+def test_ai_luna_menu_draws_and_distance_interval_reproduce_stored_record():
+    """The same data resampled menu by menu (MenuDraws, B = 10,000, seed 0): the replies and the draw
+    matrix hash to the stored fingerprints, and the two-sided 95 % distance interval from the stored
+    dist(p_hat) bracket is the stored one to the last bit (c = the 95 % quantile of ||p_b - p_hat||)."""
+    rec = json.load(open(os.path.join(GOLDEN, "ai_luna_record.json")))
+    table, menu, alt, p = _ai_luna()
+    D = MenuDraws(table, menu, alt, p, B=rec["replies"]["settings"]["B"], seed=rec["replies"]["settings"]["seed"])
+    assert D.input_fingerprint == rec["replies"]["fingerprints"]["replies"]
+    dev = np.concatenate([np.sqrt(((blk - p) ** 2).sum(1)) for _lo, _hi, blk in D.blocks()])
+    assert D.fingerprint == rec["replies"]["fingerprints"]["draws"]
+    di = rec["replies"]["distance_interval"]
+    lo, hi, c = hong_li.distance_interval(rec["p_hat"]["lb"], rec["p_hat"]["ub"], dev, 0.05)
+    assert (lo, hi, c) == (di["lower"], di["upper"], di["half_width"])
+
+
 def test_hong_li_reproduces_stored_decisions():
     rec = json.load(open(os.path.join(GOLDEN, "dsr_alt0_record.json")))
     s = rec["settings"]
