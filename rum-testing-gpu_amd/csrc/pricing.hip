@@ -475,29 +475,90 @@ __global__ void cg_scatter(float* __restrict__ g, const i64* __restrict__ pos, c
   if (j < M) g[pos[j]] = scores[j];
 }
 
-__global__ __launch_bounds__(256) void cg_slab_layer(float* __restrict__ V, uint8_t* __restrict__ pred,
-                                                     const float* __restrict__ R, const u32* __restrict__ nodes, i64 ns,
-                                                     int n, i64 Nloc) {
-  const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= ns) return;
-  const u32 D = nodes[i];
-  float best = 0.0f;
-  bool seen = false;
-  int bx = 0;
-  // the members of D in ascending order: every node of a layer has the same count, so the lanes
-  // of a wavefront run the same number of iterations
-  for (u32 S = D; S; S &= S - 1) {
-    const int x = __builtin_ctz(S);
-    const u32 P = D ^ (1u << x);
-    const float cand = V[P] + R[(i64)x * Nloc + (i64)compress_bit(P, x)];
-    if (!seen || cand > best) {
-      best = cand;
-      seen = true;
-      bx = x;
-    }
+// Slab layout's layered max, tile by tile. V[D] = max over members x of D of V[D - x] + R_x[D - x]
+// (R_x at slab position x Nloc + compress(D - x, x)), the lowest x on ties, for every subset D. The n
+// bits are split into L low bits and n - L high bits; tile h holds the 2^L subsets h | l, which are
+// contiguous in V and pred. A member x < L leads to a subset of the same tile, and its arc weights
+// for the tile are one contiguous run R[x Nloc + (h >> 1) + c], c < 2^(L-1); a member x >= L leads
+// to tile h - x, at the same l, with contiguous arc weights too. Tiles are launched by the popcount of
+// h, so the tiles they read are finished; inside a tile the subsets go by the popcount of l in LDS,
+// with the tile's low arcs staged in LDS first. Every load from HBM is a contiguous run, each arc
+// weight is read once, and V and pred are written once, in order. A node's value and choice are
+// those of the one-node-per-thread form: each candidate is V + R rounded once, the members are
+// taken in ascending order with a strict >, and the high members (all above the low ones) are
+// folded in after the low ones with a strict >, which selects the same member.
+#define CG_SLAB_TILE_L 10
+template <int L>
+__global__ __launch_bounds__(256) void cg_slab_tile(float* __restrict__ V, uint8_t* __restrict__ pred,
+                                                    const float* __restrict__ R, const u32* __restrict__ hlist,
+                                                    const uint16_t* __restrict__ lord, i64 Nloc) {
+  constexpr int T = 1 << L, H = T >> 1;
+  __shared__ float vt[T];
+  __shared__ float hv[T];
+  __shared__ uint8_t hx[T];
+  __shared__ float rl[L][H];
+  const int tid = threadIdx.x;
+  const u32 h = hlist[blockIdx.x];
+  const u32 hm = h << L;
+  for (int e = tid; e < L * H; e += 256) {
+    const int x = e / H, cidx = e - x * H;
+    rl[x][cidx] = R[(i64)x * Nloc + (i64)(hm >> 1) + cidx];
   }
-  V[D] = best;
-  pred[D] = (uint8_t)bx;
+  // the high members, ascending, for every subset of the tile
+  for (int l = tid; l < T; l += 256) {
+    const u32 D = hm | (u32)l;
+    float b = 0.0f;
+    int bx = 0;
+    bool seen = false;
+    for (u32 S = h; S; S &= S - 1) {
+      const int x = L + __builtin_ctz(S);
+      const u32 P = D ^ (1u << x);
+      const float cand = V[P] + R[(i64)x * Nloc + (i64)compress_bit(P, x)];
+      if (!seen || cand > b) {
+        b = cand;
+        bx = x;
+        seen = true;
+      }
+    }
+    hv[l] = b;
+    hx[l] = (uint8_t)bx;
+  }
+  __syncthreads();
+  if (tid == 0) vt[0] = h ? hv[0] : 0.0f;       // the tile's own empty low part (V[empty set] = 0)
+  __syncthreads();
+  int lo = 1, cnt = L;
+  for (int c = 1; c <= L; ++c) {
+    for (int i = tid; i < cnt; i += 256) {
+      const u32 l = lord[lo + i];
+      float b = 0.0f;
+      int bx = 0;
+      bool seen = false;
+      for (u32 S = l; S; S &= S - 1) {
+        const int x = __builtin_ctz(S);
+        const u32 pl = l ^ (1u << x);
+        const float cand = vt[pl] + rl[x][compress_bit(pl, x)];
+        if (!seen || cand > b) {
+          b = cand;
+          bx = x;
+          seen = true;
+        }
+      }
+      if (h && hv[l] > b) {
+        b = hv[l];
+        bx = hx[l];
+      }
+      vt[l] = b;
+      hx[l] = (uint8_t)bx;
+    }
+    __syncthreads();
+    lo += cnt;
+    cnt = cnt * (L - c) / (c + 1);
+  }
+  for (int l = tid; l < T; l += 256) {
+    if ((hm | (u32)l) == 0u) continue;           // the empty set keeps V = 0 and has no choice
+    V[hm | (u32)l] = vt[l];
+    pred[hm | (u32)l] = hx[l];
+  }
 }
 
 // walk pred from the full set (one thread): rank_pos[x] = position of x, 0 = best
@@ -750,6 +811,11 @@ struct rumgpu_cg {
   double* r64b = nullptr;
   u64 cap64 = 0;
   float* R = nullptr;        // slab
+  // tiles of the slab layout's layered max: high subsets by popcount, local subsets by popcount
+  int tL = 0;
+  u32* thlist = nullptr;
+  uint16_t* tlord = nullptr;
+  std::vector<i64> thbnd;
   int32_t* rank = nullptr;
   // column store of one column-generation run (rumgpu_cg_cols_*)
   u64* cbits = nullptr;      // [ccap][cW]: bit r % 64 of word r / 64 = x-major row r is in the column
@@ -789,7 +855,7 @@ static void cg_free_all(rumgpu_cg* c) {
                   c->ebuf[1], c->lstep, c->subV, c->subpred, c->subnodes, c->submasks, c->subscores, c->suboffsets,
                   c->Vb, c->predb, c->scoresb, c->vmaxb, c->rankb, c->V64, c->r64, c->V64b, c->r64b, c->clcm, c->clF, c->cltmax,
                   c->clbmax, c->clvmax, c->clbmax64, c->clv64, c->clerr, c->cbits, c->cp, c->cpout, c->cq,
-                  c->cidx, c->clam, c->crank, c->cpart};
+                  c->cidx, c->clam, c->crank, c->cpart, c->thlist, c->tlord};
   for (void* q : bufs) rg_free(q);
   for (auto* p : c->predk) rg_free(p);
 }
@@ -897,6 +963,16 @@ extern "C" int rumgpu_cg_create(rumgpu_ctx* ctx, int n, const uint64_t* masks_x,
       if ((rc = cg_alloc(c, (void**)&c->spos, (size_t)M * 8, "slab positions")) != RG_OK) return fail(rc);
       RG_HIP(rg_copy(ctx->stream, c->spos, pos.data(), (size_t)M * 8, hipMemcpyHostToDevice));
       if ((rc = cg_alloc(c, (void**)&c->R, (size_t)n * c->Nloc * 4, "pricing slab")) != RG_OK) return fail(rc);
+      c->tL = std::min(n, CG_SLAB_TILE_L);
+      std::vector<u32> hord, lord32;
+      std::vector<i64> lb;
+      layer_order(n - c->tL, hord, c->thbnd);
+      layer_order(c->tL, lord32, lb);
+      std::vector<uint16_t> lord(lord32.begin(), lord32.end());
+      if ((rc = cg_alloc(c, (void**)&c->thlist, hord.size() * 4, "slab tiles")) != RG_OK) return fail(rc);
+      if ((rc = cg_alloc(c, (void**)&c->tlord, lord.size() * 2, "slab tile order")) != RG_OK) return fail(rc);
+      RG_HIP(rg_copy(ctx->stream, c->thlist, hord.data(), hord.size() * 4, hipMemcpyHostToDevice));
+      RG_HIP(rg_copy(ctx->stream, c->tlord, lord.data(), lord.size() * 2, hipMemcpyHostToDevice));
     }
   } else {
     c->k_pred = k_pred <= 0 ? n - 10 : k_pred;
@@ -964,6 +1040,24 @@ static int mask_sweep(hipStream_t st, int n, float* V, uint8_t* pred, const u32*
     if (ns <= 0) continue;
     hipLaunchKernelGGL(cg_mask_layer, dim3(grid_for(ns, 256)), dim3(256), 0, st, V, pred, masks, scores, offsets,
                        nodes + lo, ns, n);
+  }
+  RG_LAUNCH_CHECK();
+  return RG_OK;
+}
+
+// the tiles of the slab layout's layered max, one launch per popcount of the high part
+static int slab_tiles(rumgpu_cg* c, hipStream_t st) {
+  const int L = c->tL;
+  for (int j = 0; j <= c->n - L; ++j) {
+    const i64 lo = c->thbnd[j], ns = c->thbnd[j + 1] - lo;
+    if (ns <= 0) continue;
+    const dim3 g((unsigned)ns), b(256);
+    switch (L) {
+#define RG_T(LV) case LV: hipLaunchKernelGGL(cg_slab_tile<LV>, g, b, 0, st, c->V, c->pred, c->R, c->thlist + lo, c->tlord, c->Nloc); break;
+      RG_T(2) RG_T(3) RG_T(4) RG_T(5) RG_T(6) RG_T(7) RG_T(8) RG_T(9) RG_T(10)
+#undef RG_T
+      default: RG_CHECK(false, RG_EUNSUP, "slab pricing: tile of %d bits", L);
+    }
   }
   RG_LAUNCH_CHECK();
   return RG_OK;
@@ -1083,13 +1177,7 @@ extern "C" int rumgpu_cg_price(rumgpu_cg* c, const float* scores_x, double* vmax
         RG_TRY((launch_tr_rl<float, float, 0, 0>(wins[w].RL, c->R, c->R, coef<float>(1.0, 0.0, 1.0, 1.0),
                                                  make_geom(n, wins[w]), nullptr, nullptr, st)));
       hipLaunchKernelGGL(cg_fill_f32, dim3(1), dim3(64), 0, st, c->V, 0.0f, (i64)1);
-      for (int k = 1; k <= n; ++k) {
-        const i64 lo = c->bounds[k], ns = c->bounds[k + 1] - lo;
-        if (ns <= 0) continue;
-        hipLaunchKernelGGL(cg_slab_layer, dim3(grid_for(ns, 256)), dim3(256), 0, st, c->V, c->pred, c->R,
-                           c->nodes + lo, ns, n, c->Nloc);
-      }
-      RG_LAUNCH_CHECK();
+      RG_TRY(slab_tiles(c, st));
     }
     hipLaunchKernelGGL(cg_backtrace, dim3(1), dim3(64), 0, st, c->pred, n, c->rank, (u64)(c->NN - 1));
     float vf = 0.0f;
