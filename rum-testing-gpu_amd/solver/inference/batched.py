@@ -92,6 +92,10 @@ class BatchedSolver:
         self.model, self.collect, self.pairs = model, collect, []
         kept = sess._kept
         self.src = kept["handle"] if kept is not None else None
+        if kept is not None and "beta" in kept:
+            # the penalty the kept p_hat state was reached with (an adaptive penalty ends elsewhere
+            # than it starts); its unscaled multipliers belong to that penalty
+            self.beta = float(kept["beta"])
         p0 = boot.p_hat[self.rest]
         self.p_hat_rest = p0
         from .. import machine as _machine
@@ -139,6 +143,9 @@ class BatchedSolver:
         D, h = self.D, self.h
         kept = self.boot.session._kept               # a refinement round may have re-kept p_hat
         self.src = kept["handle"] if kept is not None else None
+        if kept is not None and "beta" in kept and float(kept["beta"]) != h.beta:
+            self.beta = float(kept["beta"])
+            h.set_beta(self.beta)
         if self.src is not None and (self.model is not None or self.collect):
             h.slot_load(0, self.src)
             self.x_hat_obs, self.z_hat_obs = h.slot_get_obs(0)

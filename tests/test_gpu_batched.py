@@ -300,3 +300,42 @@ def test_batched_confidence_interval_equals_per_draw():
     print(ci2)
     assert ci1.n_solved >= 1
     assert ci1.lower == ci2.lower and ci1.lower_bracket == ci2.lower_bracket and ci1.n_solved == ci2.n_solved
+
+
+def test_config4_alt0_slots_reach_per_draw_decisions():
+    """Batched bootstrap at n = 16: the panel generator of the n = 25 workload at n = 16 (301 cells of
+    alternative 0, 520 respondents x 15 menus, seed 16002; config4 as specified is decided by the
+    screen alone, see the module notes). p_hat is certified once; its open draws are then solved
+    twice from the same screened brackets: one float64 Session.solve per draw (the reference's
+    loop) and the draw slots of one float64 batch object (graph-replayed iterations, retire and
+    refill, float64 certificates per slot). Every draw's decision must agree, and every pair of
+    brackets must overlap."""
+    import time
+    import pandas as pd
+    from solver.inference.bootstrap import _Batched, _PerDraw
+    panel, n = instances.config4_alt0()
+    panel = pd.DataFrame(panel)
+    menu, alt, p = cells_from_panel(panel, n, alternatives=[0])
+    rum = RUM(menu, alt, p, num_alternatives=n)
+    with rum.bootstrap(panel=panel, B=210, method="bm_admm", method_args={"dtype": "float64"}) as boot:
+        t0 = time.perf_counter()
+        boot._start()
+        open_ = np.flatnonzero(boot._undecided())
+        print("[case] config4_alt0: p_hat [%.8f, %.8f] in %.1f s, %d screened, %d open"
+              % (boot.L0, boot.U0, time.perf_counter() - t0, boot.n_screened, open_.size), flush=True)
+        assert open_.size >= 64
+        screened = (boot.L.copy(), boot.U.copy())
+        t0 = time.perf_counter()
+        _PerDraw(boot).run(open_, boot.abs_tol, boot.rel_tol, stops=True, until=None)
+        t_one = time.perf_counter() - t0
+        d_one, L1, U1 = boot._decisions().copy(), boot.L.copy(), boot.U.copy()
+        boot.L[:], boot.U[:] = screened
+        runner = _Batched(boot, {"slots": 32})
+        t0 = time.perf_counter()
+        runner.run(open_, boot.abs_tol, boot.rel_tol, stops=True, until=None)
+        t_bat = time.perf_counter() - t0
+        runner.close()
+        d_bat = boot._decisions()
+        print("per draw %.1f s, slots %.1f s for %d draws; %s" % (t_one, t_bat, open_.size, boot.batch_stats))
+        assert np.array_equal(d_one, d_bat), np.flatnonzero(d_one != d_bat)
+        assert (boot.L[open_] <= U1[open_] + 1e-9).all() and (L1[open_] <= boot.U[open_] + 1e-9).all()
