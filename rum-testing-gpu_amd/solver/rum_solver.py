@@ -47,6 +47,12 @@ _FLAG = {"threshold": 5, "target": 4, "time": 0, "iters": -1, "stalled": -2, "oo
 _UNIT = {"bm_admm": "ADMM iterations", "flow_cg": "columns"}
 
 
+def _table(rows):
+    """Right-aligned 'label : text' lines."""
+    width = max(len(k) for k, _ in rows)
+    return "\n".join("%s : %s" % (k.rjust(width), v) for k, v in rows)
+
+
 @dataclass
 class Result:
     exit_flag: int
@@ -80,29 +86,34 @@ class Result:
     abs_to_opt: float = None
     rel_to_opt: float = None
 
-    def __repr__(self):
-        rows = [("exit_flag", "%d (%s)" % (self.exit_flag, self.status)), ("distance", "%.10g" % self.distance)]
+    def _rows(self):
+        """(label, text) pairs of the printed result, in order."""
+        yield "exit_flag", "%d (%s)" % (self.exit_flag, self.status)
+        yield "distance", "%.10g" % self.distance
         if self.mode == "race":
-            rows += [("mode", "race to opt (no lower bound)"), ("opt, target", "%.10g, %.10g" % (self.opt, self.target)),
-                     ("ub - opt", "-" if self.abs_to_opt is None else "%.3g" % self.abs_to_opt)]
+            yield "mode", "race to opt (no lower bound)"
+            yield "opt, target", "%.10g, %.10g" % (self.opt, self.target)
+            yield "ub - opt", "-" if self.abs_to_opt is None else "%.3g" % self.abs_to_opt
         else:
-            rows += [("lb, ub", "%.10g, %.10g" % (self.lb, self.ub)),
-                     ("abs_gap", "%.3g (abs_tol %.3g, %s)" % (self.abs_gap, self.abs_tol, "met" if self.met_abs else "not met")),
-                     ("rel_gap", "%.3g (rel_tol %.3g, %s)" % (self.rel_gap, self.rel_tol, "met" if self.met_rel else "not met"))]
-        rows += [("method", self.method + ("  [%s]" % self.auto_reason if self.auto_reason else "")),
-                 ("iterations", "%d %s" % (self.iterations, self.iteration_unit)),
-                 ("time", "%.3f s (setup %.3f s, solve %.3f s)" % (self.time, self.time_setup, self.time_solve)),
-                 ("seed", str(self.seed))]
+            yield "lb, ub", "%.10g, %.10g" % (self.lb, self.ub)
+            for name, gap, tol, met in (("abs", self.abs_gap, self.abs_tol, self.met_abs),
+                                        ("rel", self.rel_gap, self.rel_tol, self.met_rel)):
+                yield name + "_gap", "%.3g (%s_tol %.3g, %s)" % (gap, name, tol, "met" if met else "not met")
+        yield "method", self.method + ("  [%s]" % self.auto_reason if self.auto_reason else "")
+        yield "iterations", "%d %s" % (self.iterations, self.iteration_unit)
+        yield "time", "%.3f s (setup %.3f s, solve %.3f s)" % (self.time, self.time_setup, self.time_solve)
+        yield "seed", str(self.seed)
         if self.message:
-            rows.append(("message", self.message))
-        rows.append(("method_args", str(dict(self.method_args))))
-        rows.append(("p_fit", "None" if self.p_fit is None else "array of %d" % len(self.p_fit)))
-        rows.append(("trace", "%d entries" % len(self.trace)))
-        rows.append(("machine", self.machine))
+            yield "message", self.message
+        yield "method_args", str(dict(self.method_args))
+        yield "p_fit", "None" if self.p_fit is None else "array of %d" % len(self.p_fit)
+        yield "trace", "%d entries" % len(self.trace)
+        yield "machine", self.machine
         if self.info:
-            rows.append(("info", str(self.info)))
-        w = max(len(k) for k, _ in rows)
-        return "\n".join("%s : %s" % (k.rjust(w), v) for k, v in rows)
+            yield "info", str(self.info)
+
+    def __repr__(self):
+        return _table(list(self._rows()))
 
     def to_dict(self, p_fit=False):
         d = {f.name: getattr(self, f.name) for f in fields(self) if f.name != "p_fit"}
@@ -134,37 +145,67 @@ class _Geometry:
                                 np.asarray(p, dtype=np.float64)])
 
 
+def _tolerances(abs_tol, rel_tol, max_time):
+    """The stop rule of one call, defaults filled in and checked."""
+    abs_tol = float(defaults.ABS_TOL if abs_tol is None else abs_tol)
+    rel_tol = float(defaults.REL_TOL if rel_tol is None else rel_tol)
+    max_time = float(defaults.MAX_TIME if max_time is None else max_time)
+    if abs_tol < 0 or rel_tol < 0 or not max_time > 0:
+        raise ValueError("abs_tol and rel_tol must be >= 0 and max_time > 0")
+    return abs_tol, rel_tol, max_time
+
+
+def _task(rum, p, W, args, abs_tol, rel_tol, deadline, seed, verbose, t0, const, ram, **extra):
+    """What a runner reads and writes: the non-singleton rows and their p, the plan, the stop rule,
+    and the running state (bounds, fitted point, iterations, trace, info)."""
+    state = dict(lb=0.0, ub=math.inf, p_fit=None, iterations=0, trace=[], info={}, setup=0.0)
+    base = dict(geom=rum._geom, p=p[rum._rest], W=W, args=args, abs_tol=abs_tol, rel_tol=rel_tol, deadline=deadline,
+                seed=seed, verbose=verbose, t0=t0, state=state, const=const, ram=ram, engines=None)
+    base.update(extra)
+    return types.SimpleNamespace(**base)
+
+
+def _check_rows(menu, alternative, num_alternatives):
+    """(menu, alternative, n) as int64 arrays and the number of alternatives, after the row checks
+    of the reference's RUM constructor (same messages), each run only once the ones before it pass."""
+    menu, alternative = _as_int(menu, "menu"), _as_int(alternative, "alternative")
+    if menu.ndim != 1 or alternative.shape != menu.shape:
+        raise ValueError("menu and alternative must be 1-D arrays of the same length")
+    if menu.size == 0:
+        raise ValueError("no observations")
+    for bad, msg in ((menu <= 0, "row %d: empty menu"), (alternative < 0, "row %d: negative alternative")):
+        i = _first_true(bad)
+        if i is not None:
+            raise ValueError(msg % i)
+    need = max(int(menu.max()).bit_length(), int(alternative.max()) + 1)
+    n = need if num_alternatives is None else int(num_alternatives)
+    if n < need:
+        raise ValueError("num_alternatives = %d, but the data use alternatives up to %d" % (n, need - 1))
+    if n > 62:
+        raise ValueError("menus are 64-bit masks: at most 62 alternatives")
+    i = _first_true(((menu >> alternative) & 1) == 0)
+    if i is not None:
+        raise ValueError("row %d: alternative %d is not in its menu (mask %d)" % (i, alternative[i], menu[i]))
+    # a repeated (menu, alternative) pair: the smallest such pair in (menu, alternative) order, at
+    # its first row (a stable sort keeps equal pairs in row order)
+    order = np.lexsort((alternative, menu))
+    sm, sa = menu[order], alternative[order]
+    j = _first_true((sm[1:] == sm[:-1]) & (sa[1:] == sa[:-1]))
+    if j is not None:
+        i = int(order[j])
+        raise ValueError("row %d: (menu %d, alternative %d) appears more than once" % (i, menu[i], alternative[i]))
+    return menu, alternative, n
+
+
+def _first_true(mask):
+    hit = np.flatnonzero(mask)
+    return int(hit[0]) if hit.size else None
+
+
 class RUM:
     def __init__(self, menu, alternative, choice_probability, num_alternatives=None):
-        menu = _as_int(menu, "menu")
-        alternative = _as_int(alternative, "alternative")
-        if menu.ndim != 1 or alternative.shape != menu.shape:
-            raise ValueError("menu and alternative must be 1-D arrays of the same length")
-        if menu.size == 0:
-            raise ValueError("no observations")
-        if (menu <= 0).any():
-            raise ValueError("row %d: empty menu" % int(np.flatnonzero(menu <= 0)[0]))
-        if (alternative < 0).any():
-            raise ValueError("row %d: negative alternative" % int(np.flatnonzero(alternative < 0)[0]))
-        need = max(int(menu.max()).bit_length(), int(alternative.max()) + 1)
-        n = need if num_alternatives is None else int(num_alternatives)
-        if n < need:
-            raise ValueError("num_alternatives = %d, but the data use alternatives up to %d" % (n, need - 1))
-        if n > 62:
-            raise ValueError("menus are 64-bit masks: at most 62 alternatives")
-        off = ((menu >> alternative) & 1) == 0
-        if off.any():
-            i = int(np.flatnonzero(off)[0])
-            raise ValueError("row %d: alternative %d is not in its menu (mask %d)" % (i, alternative[i], menu[i]))
-        _, first, counts = np.unique(np.column_stack([menu, alternative]), axis=0, return_index=True,
-                                     return_counts=True)
-        if (counts > 1).any():
-            i = int(first[np.flatnonzero(counts > 1)[0]])
-            raise ValueError("row %d: (menu %d, alternative %d) appears more than once" % (i, menu[i], alternative[i]))
-        self.n = n
-        self.M = int(menu.size)
-        self.menu = menu
-        self.alternative = alternative
+        menu, alternative, n = _check_rows(menu, alternative, num_alternatives)
+        self.n, self.M, self.menu, self.alternative = n, int(menu.size), menu, alternative
         self.singleton = (menu & (menu - 1)) == 0
         self._rest = np.flatnonzero(~self.singleton)
         self._single = np.flatnonzero(self.singleton)
@@ -195,41 +236,28 @@ class RUM:
         if opt_rel_tol is not None or opt_floor is not None:
             raise ValueError("opt_rel_tol and opt_floor set a race target and need opt")
         t0 = time.perf_counter()
-        abs_tol = float(defaults.ABS_TOL if abs_tol is None else abs_tol)
-        rel_tol = float(defaults.REL_TOL if rel_tol is None else rel_tol)
-        max_time = float(defaults.MAX_TIME if max_time is None else max_time)
-        if abs_tol < 0 or rel_tol < 0 or not max_time > 0:
-            raise ValueError("abs_tol and rel_tol must be >= 0 and max_time > 0")
+        abs_tol, rel_tol, max_time = _tolerances(abs_tol, rel_tol, max_time)
         if seed is None:
             seed = int(np.random.SeedSequence().entropy % (2 ** 31))
         p = self.choice_probability if p is None else self._check_p(p)
         W, const = self._weights(vcov, p)
         prof = _machine.load() if machine is None else machine
-        geom, pr = self._geom, p[self._rest]
-        if geom.M == 0:
-            d = math.sqrt(const)
-            return self._result(t0, t0, "none", "only singleton menus", {}, abs_tol, rel_tol, seed,
-                                dict(lb=d, ub=d, p_fit=np.zeros(0), iterations=0, trace=[], info={}, setup=0.0),
-                                "tol", "", 0.0, prof)
+        if self._geom.M == 0:
+            return self._singletons_only(t0, abs_tol, rel_tol, seed, const, prof)
         name, reason, args = self._plan(method, method_args, W is not None, prof)
-        st = dict(lb=0.0, ub=math.inf, p_fit=None, iterations=0, trace=[], info={}, setup=0.0)
-        task = types.SimpleNamespace(geom=geom, p=pr, W=W, args=args, abs_tol=abs_tol, rel_tol=rel_tol,
-                                     deadline=t0 + max_time, seed=seed, verbose=verbose, t0=t0, state=st, const=const,
-                                     ram=prof.get("ram_bytes"), engines=None)
+        task = _task(self, p, W, args, abs_tol, rel_tol, t0 + max_time, seed, verbose, t0, const,
+                     prof.get("ram_bytes"))
         t_run = time.perf_counter()
-        message = ""
-        try:
-            stop = _RUN[name](task)
-        except _TimeUp:
-            stop = "time"
-        except KeyboardInterrupt:
-            stop, message = "interrupted", "interrupted by the user"
-        except Exception as e:                # noqa: BLE001 -- reported through exit_flag
-            stop = "oom" if _is_oom(e) else "error"
-            message = "%s: %s" % (type(e).__name__, e)
-        finally:
-            _close_engines(task)
-        return self._result(t0, t_run, name, reason, args, abs_tol, rel_tol, seed, st, stop, message, const, prof)
+        stop, message = _execute(name, task, own_engines=True)
+        return self._result(t0, t_run, name, reason, args, abs_tol, rel_tol, seed, task.state, stop, message, const,
+                            prof)
+
+    def _singletons_only(self, t0, abs_tol, rel_tol, seed, const, prof):
+        """Every row is a singleton menu: the distance is the constant part, exactly."""
+        d = math.sqrt(const)
+        return self._result(t0, t0, "none", "only singleton menus", {}, abs_tol, rel_tol, seed,
+                            dict(lb=d, ub=d, p_fit=np.zeros(0), iterations=0, trace=[], info={}, setup=0.0),
+                            "tol", "", 0.0, prof)
 
     # ------------------------------------------------------------------ session / bootstrap
     def session(self, method=None, method_args=None, vcov=None, machine=None, seed=0):
@@ -276,25 +304,11 @@ class RUM:
         prof = _machine.load() if machine is None else machine
         name, reason, args = self._plan(method, method_args, False, prof)
         rest = math.sqrt(target ** 2 - const) if const else target
-        st = dict(lb=0.0, ub=math.inf, p_fit=None, iterations=0, trace=[], info={}, setup=0.0)
-        task = types.SimpleNamespace(geom=self._geom, p=p[self._rest], W=None, args=args, abs_tol=0.0, rel_tol=0.0,
-                                     deadline=t0 + max_time, seed=seed, verbose=verbose, t0=t0, state=st, const=const,
-                                     ram=prof.get("ram_bytes"), engines=None, below=rest, above=None, race=True)
-        message, stop = "", "time"
+        task = _task(self, p, None, args, 0.0, 0.0, t0 + max_time, seed, verbose, t0, const, prof.get("ram_bytes"),
+                     below=rest, above=None, race=True)
+        st = task.state
         t_run = time.perf_counter()
-        try:
-            stop = _RUN[name](task)
-        except _TimeUp:
-            stop = "time"
-        except _Decided:
-            stop = "threshold"
-        except KeyboardInterrupt:
-            stop, message = "interrupted", "interrupted by the user"
-        except Exception as e:                # noqa: BLE001
-            stop = "oom" if _is_oom(e) else "error"
-            message = "%s: %s" % (type(e).__name__, e)
-        finally:
-            _close_engines(task)
+        stop, message = _execute(name, task, own_engines=True)
         t_end = time.perf_counter()
         ub = math.sqrt(st["ub"] ** 2 + const) if math.isfinite(st["ub"]) else math.inf
         flag = 4 if (stop == "threshold" or ub <= target) else _FLAG.get(stop, -2)
@@ -434,45 +448,23 @@ class Session:
         if isinstance(task, Result):
             return task
         task.t_run = time.perf_counter()
-        message = ""
-        try:
-            stop = _RUN[self.method](task)
-        except _TimeUp:
-            stop = "time"
-        except _Decided:
-            stop = "threshold"
-        except KeyboardInterrupt:
-            stop, message = "interrupted", "interrupted by the user"
-        except Exception as e:                # noqa: BLE001
-            stop = "oom" if _is_oom(e) else "error"
-            message = "%s: %s" % (type(e).__name__, e)
+        stop, message = _execute(self.method, task, own_engines=False)
         return self._finish(task, stop, message)
 
     def _task(self, p, abs_tol, rel_tol, max_time, below, above, warm, keep, verbose):
         """The runner's task for one solve (or the Result itself when only singleton menus remain)."""
         rum, t0 = self.rum, time.perf_counter()
-        abs_tol = float(defaults.ABS_TOL if abs_tol is None else abs_tol)
-        rel_tol = float(defaults.REL_TOL if rel_tol is None else rel_tol)
-        max_time = float(defaults.MAX_TIME if max_time is None else max_time)
-        if abs_tol < 0 or rel_tol < 0 or not max_time > 0:
-            raise ValueError("abs_tol and rel_tol must be >= 0 and max_time > 0")
+        abs_tol, rel_tol, max_time = _tolerances(abs_tol, rel_tol, max_time)
         if warm and self._kept is None:
             raise ValueError("warm=True starts from a kept solve: call solve(..., keep=True) first")
         p = rum._finite_p(p)
         W, const = rum._weights(self.vcov, p)
-        geom, pr = rum._geom, p[rum._rest]
-        if geom.M == 0:
-            d = math.sqrt(const)
-            return rum._result(t0, t0, "none", "only singleton menus", {}, abs_tol, rel_tol, self.seed,
-                               dict(lb=d, ub=d, p_fit=np.zeros(0), iterations=0, trace=[], info={}, setup=0.0),
-                               "tol", "", 0.0, self.prof)
-        st = dict(lb=0.0, ub=math.inf, p_fit=None, iterations=0, trace=[], info={}, setup=0.0)
-        return types.SimpleNamespace(geom=geom, p=pr, W=W, args=self._args, abs_tol=abs_tol, rel_tol=rel_tol,
-                                     deadline=t0 + max_time, seed=self.seed, verbose=verbose, t0=t0, state=st,
-                                     const=const, ram=self.prof.get("ram_bytes"),
-                                     below=_on_rest(below, const, False), above=_on_rest(above, const, True),
-                                     warm=self._kept if warm else None, keep=bool(keep), engines=self._engines,
-                                     t_run=t0)
+        if rum._geom.M == 0:
+            return rum._singletons_only(t0, abs_tol, rel_tol, self.seed, const, self.prof)
+        return _task(rum, p, W, self._args, abs_tol, rel_tol, t0 + max_time, self.seed, verbose, t0, const,
+                     self.prof.get("ram_bytes"), below=_on_rest(below, const, False),
+                     above=_on_rest(above, const, True), warm=self._kept if warm else None, keep=bool(keep),
+                     engines=self._engines, t_run=t0)
 
     def _finish(self, task, stop, message):
         st = task.state
@@ -504,47 +496,58 @@ class Session:
 
 
 def _on_rest(tau, const, above):
+    """A threshold on the whole distance moved onto the non-singleton part: dist^2 = rest^2 + const.
+    A threshold the singleton rows alone reach is met at once (above: from 0 on) or never (below)."""
     if tau is None:
         return None
     tau = float(tau)
     if not math.isfinite(tau):
         raise ValueError("below and above must be finite")
-    if tau * tau <= const:
-        return 0.0 if above else None
-    return math.sqrt(tau * tau - const)
+    room = tau * tau - const
+    if room > 0:
+        return math.sqrt(room)
+    return 0.0 if above else None
 
 
 # ---------------------------------------------------------------------- choosing a method
 def choose_method(n, mp, weighted, prof):
-    """(method, reason): bm_admm from the density defaults.AUTO sets for this n, flow_cg below it and
-    whenever bm_admm does not fit (rum_solver.py:675-711 of the reference)."""
-    A = defaults.AUTO
+    """(method, reason) when none is named (the rule of the reference's rum_solver.py:675-711): BM-ADMM
+    from the density defaults.AUTO sets for this n, Flow-CG below that density, for small n, and
+    whenever BM-ADMM does not fit. The reason lists every step that decided it."""
     density = mp / (n << (n - 1))
     why = ["n = %d, %d rows, density %.3g%%" % (n, mp, 100 * density)]
-    if weighted:
-        why.append("weighted norm: only the Gurobi methods take vcov, and they are not part of this package")
-    elif not _machine.can_use_gpu(prof):
-        g = prof.get("gpu")
-        why.append("no usable GPU (%s)" % ((g or {}).get("reason") or "none found"
-                                          if prof.get("use_gpu", True) else "use_gpu = False"))
+    blocked = _why_no_gpu_method(weighted, prof)
+    candidates = []
+    if blocked:
+        why.append(blocked)
     else:
         d_min = bm_admm_min_density(n)
         if d_min is None:
-            why.append("n <= %d -> flow_cg at any density" % (min(A["bm_admm_min_density"]) - 1))
+            why.append("n <= %d -> flow_cg at any density" % (min(defaults.AUTO["bm_admm_min_density"]) - 1))
         elif density < d_min:
             why.append("density < %.2g%% at n = %d -> flow_cg" % (100 * d_min, n))
         else:
-            try:
-                _plan_bm_admm(n, mp, False, prof, dict(defaults.METHOD_ARGS["bm_admm"]))
-                return "bm_admm", "; ".join(why + ["density >= %.2g%% at n = %d -> bm_admm" % (100 * d_min, n)])
-            except RuntimeError as e:
-                why.append(str(e))
+            candidates.append(("bm_admm", "density >= %.2g%% at n = %d -> bm_admm" % (100 * d_min, n)))
+        candidates.append(("flow_cg", None))
+    for name, because in candidates:
         try:
-            _plan_flow_cg(n, mp, False, prof, dict(defaults.METHOD_ARGS["flow_cg"]))
-            return "flow_cg", "; ".join(why)
-        except RuntimeError as e:
+            _PLAN[name](n, mp, False, prof, dict(defaults.METHOD_ARGS[name]))
+        except RuntimeError as e:           # does not fit: the next candidate
             why.append(str(e))
+            continue
+        return name, "; ".join(why + ([because] if because else []))
     raise RuntimeError("no method can run on this machine: " + "; ".join(why))
+
+
+def _why_no_gpu_method(weighted, prof):
+    """Why neither GPU method can take this problem, or None."""
+    if weighted:
+        return "weighted norm: only the Gurobi methods take vcov, and they are not part of this package"
+    if not _machine.can_use_gpu(prof):
+        g = prof.get("gpu")
+        return "no usable GPU (%s)" % ((g or {}).get("reason") or "none found" if prof.get("use_gpu", True)
+                                      else "use_gpu = False")
+    return None
 
 
 def bm_admm_min_density(n):
@@ -621,13 +624,8 @@ _CG_ENGINE = (("mask", "float32"), ("stream", "float32"), ("stream", "uint24"), 
 _CG_OLD = {"stream_f32": ("stream", "float32"), "stream_u24": ("stream", "uint24"), "stream_u16": ("stream", "uint16")}
 
 
-def _plan_flow_cg(n, mp, weighted, prof, a):
-    _need_gpu("flow_cg", weighted, prof)
-    k = int(a["max_iters"]) + 2
-    host = 8 * k * k + 8 * k * ((mp + 63) // 64)
-    if prof.get("ram_bytes") and host > 0.5 * prof["ram_bytes"]:
-        raise RuntimeError("flow_cg max_iters = %d needs %.1f GB of host memory for its Gram matrix"
-                           % (a["max_iters"], host / 2 ** 30))
+def _cg_storage(a):
+    """The requested (layout, dtype) of flow_cg's pricing, old names resolved and checked."""
     layout, dt = a["pricing"], a["dtype"]
     if layout in _CG_OLD:
         if dt not in ("auto", _CG_OLD[layout][1]):
@@ -640,28 +638,45 @@ def _plan_flow_cg(n, mp, weighted, prof, a):
         raise ValueError("flow_cg dtype must be 'auto', 'float32', 'uint24' or 'uint16'")
     if layout in ("mask", "slab") and dt not in ("auto", "float32"):
         raise ValueError("%s pricing stores float32 only; dtype=%r needs pricing='stream' or 'auto'" % (layout, dt))
-    budget = defaults.AUTO["gpu_memory_fraction"] * _machine.gpu_free_bytes()
-    engines = list(_CG_ENGINE) if layout != "slab" else [("slab", "float32")]
+    return layout, dt
+
+
+def _cg_candidates(layout, dt, n, mp):
+    """Pricing engines in order of preference for the request. Many rows per alternative put the
+    slab form first: the row-order sweeps test every row at every subset (mp 2^(n-1) tests), the
+    slab form reads each arc weight once from a subset-sum butterfly."""
+    pool = [("slab", "float32")] if layout == "slab" else list(_CG_ENGINE)
     if layout == "auto" and dt in ("auto", "float32") and mp > defaults.AUTO["cg_slab_rows_per_alt"] * n:
-        # many rows: the row-order sweeps test every row at every subset (mp 2^(n-1) tests), the
-        # slab form reads each arc weight once from a subset-sum butterfly (16 n 2^(n-1) bytes)
-        engines.insert(0, ("slab", "float32"))
+        pool.insert(0, ("slab", "float32"))
+    for lay, d in pool:
+        if layout in ("auto", lay) and dt in ("auto", d) and not (lay in ("mask", "slab") and n > 31):
+            yield lay, d
+
+
+def _plan_flow_cg(n, mp, weighted, prof, a):
+    _need_gpu("flow_cg", weighted, prof)
+    k = int(a["max_iters"]) + 2
+    host = 8 * k * k + 8 * k * ((mp + 63) // 64)
+    if prof.get("ram_bytes") and host > 0.5 * prof["ram_bytes"]:
+        raise RuntimeError("flow_cg max_iters = %d needs %.1f GB of host memory for its Gram matrix"
+                           % (a["max_iters"], host / 2 ** 30))
+    layout, dt = _cg_storage(a)
+    budget = defaults.AUTO["gpu_memory_fraction"] * _machine.gpu_free_bytes()
+    # a single run over 4,096 rows keeps its column bitsets, p and p_out on the device
+    # (flow_cg.engine.DeviceColumns) when they fit beside the pricer, else on the host
+    cols = 8 * k * ((mp + 63) // 64) + 16 * mp + 20 * k if mp > 4096 else 0
     tried = []
-    for lay, d in engines:
-        if layout not in ("auto", lay) or dt not in ("auto", d) or (lay in ("mask", "slab") and n > 31):
-            continue
+    for lay, d in _cg_candidates(layout, dt, n, mp):
         need = _flow_cg_bytes(n, lay, d, mp)
-        if need <= budget:
-            a.update(pricing=lay, dtype=d, _gpu_bytes=need)
-            # a single run over 4,096 rows keeps its column bitsets, p and p_out on the device
-            # (flow_cg.engine.DeviceColumns), else on the host as the reference does
-            cols = 8 * k * ((mp + 63) // 64) + 16 * mp + 20 * k if mp > 4096 else 0
-            if cols and need + cols > budget:
-                a.setdefault("_cols", "host")
-            elif cols:
-                a["_gpu_bytes"] = need + cols
-            return a
-        tried.append("%s %s %.1f GB" % (lay, d, need / 2 ** 30))
+        if need > budget:
+            tried.append("%s %s %.1f GB" % (lay, d, need / 2 ** 30))
+            continue
+        a.update(pricing=lay, dtype=d, _gpu_bytes=need)
+        if cols and need + cols > budget:
+            a.setdefault("_cols", "host")
+        elif cols:
+            a["_gpu_bytes"] = need + cols
+        return a
     raise RuntimeError("flow_cg at n = %d does not fit the GPU: %.1f GB may be used (%s)"
                        % (n, budget / 2 ** 30, "; ".join(tried) or "mask needs n <= 31"))
 
@@ -798,7 +813,7 @@ def _run_bm_admm(t):
                       beta_changes=len(r["beta_changes"]))
     if getattr(t, "keep", False):
         final.keep()
-        st["keep"] = dict(handle=final)
+        st["keep"] = dict(handle=final, beta=float(final.beta))
     return {"gap_abs": "tol", "gap_rel": "tol", "time_limit": "time", "max_iter": "iters", "ub_target": "threshold",
             "lb_target": "threshold"}.get(r["stop_reason"], "stalled")
 
@@ -944,13 +959,35 @@ def _flow_cg_steps(t, pricer, resident=False):
 _RUN = {"bm_admm": _run_bm_admm, "flow_cg": _run_flow_cg}
 
 
+def _execute(name, task, own_engines):
+    """Run the method's runner on the task: (stop reason, message). A time limit, a decided
+    threshold, Ctrl-C, running out of memory and any other failure end the run with a reason (the
+    exit flag) instead of an exception. A one-shot call owns its engines and frees them here; a
+    session keeps them for its next solve."""
+    try:
+        return _RUN[name](task), ""
+    except _TimeUp:
+        return "time", ""
+    except _Decided:
+        return "threshold", ""
+    except KeyboardInterrupt:
+        return "interrupted", "interrupted by the user"
+    except Exception as e:                # noqa: BLE001 -- reported through the exit flag
+        return ("oom" if _is_oom(e) else "error"), "%s: %s" % (type(e).__name__, e)
+    finally:
+        if own_engines:
+            _close_engines(task)
+
+
 # ---------------------------------------------------------------------- small helpers
 def _method_name(method):
-    name = str(method).strip().lower().replace("-", "_")
-    name = ALIASES.get(name, name)
-    if name not in METHODS:
-        raise ValueError("unknown method %r; choose from %s" % (method, METHODS))
-    return name
+    """The canonical method name: case, surrounding blanks and '-' vs '_' do not matter, and the
+    reference's aliases (cgbfqp, cgbf, ...) map to their methods."""
+    key = str(method).strip().lower().replace("-", "_")
+    name = ALIASES.get(key, key)
+    if name in METHODS:
+        return name
+    raise ValueError("unknown method %r; choose from %s" % (method, METHODS))
 
 
 def _as_int(x, what):
