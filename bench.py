@@ -16,9 +16,9 @@ Reported beside the value (one JSON line on stdout):
              a start/stop HIP event pair recorded in each pass launch's own dispatch
              (hipExtLaunchKernelGGL) on the engine's stream, right after the timed region
              (rumgpu_bm_profile)
-  cpu_baseline  the CPU oracle (oracle/rum_oracle.cpp, one core) on a bounded sample of the same
-             solve: a few float64 iterations plus one push and one lower bound, scaled to a
-             certified solve by the GPU run's iteration and check counts
+  cpu_baseline  the CPU oracle (oracle/rum_oracle.cpp, OpenMP on every host thread) on a bounded
+             sample of the same solve: a few float64 iterations plus one push and one lower bound,
+             scaled to a certified solve by the GPU run's iteration, check and lower-bound counts
   dsr_bootstrap  the reference's stored DSR bootstrap (n = 25, B = 10,000) run whole through the
              batched path: its time beside the stored run's, and the draws whose decision differs
 Inputs are resident on the GPU before the timed region (the engines are built in warmup); each
@@ -55,7 +55,7 @@ def _parse():
     ap.add_argument("--max-time", type=float, default=120.0)
     ap.add_argument("--profile-iters", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-iters", type=int, default=20)
+    ap.add_argument("--cpu-iters", type=int, default=60)
     ap.add_argument("--no-graph", action="store_true", help="eager launches instead of HIP graph replays")
     ap.add_argument("--boot-draws", type=int, default=512,
                     help="open bootstrap draws of the n=25 panel solved by the batched (tuned) leg; 0 skips it")
@@ -453,31 +453,48 @@ def _dsr_all_alternatives():
                 speedup_vs_published=round(pub["seconds"] / t, 2))
 
 
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def _cpu_baseline(n, rum, p, r_last, a):
-    """The oracle on one host core: cpu_iters float64 iterations, one push and one lower bound of
-    the same instance, scaled to one certified solve with the GPU solve's counts."""
+    """The oracle (oracle/rum_oracle.cpp, OpenMP over every host thread it may use) on a bounded
+    sample of the same certified solve: cpu_iters float64 iterations of the reference schedule, one
+    push and one lower bound, scaled to the GPU solve's iteration count, its checks (iterations /
+    check) and its lower bounds (every lb_every-th check)."""
     import numpy as np
     from oracle import oracle as O
     g = rum._geom
-    menu = g.menu
-    alt = g.alternative.astype(np.int32)
     pr = p[rum._rest]
-    st = O.AdmmState(n, menu, alt, pr, r_last.method_args["beta"])
+    st = O.AdmmState(n, g.menu, g.alternative.astype(np.int32), pr, r_last.method_args["beta"])
     t0 = time.perf_counter()
     st.run(a.cpu_iters)
     t_it = (time.perf_counter() - t0) / a.cpu_iters
     t1 = time.perf_counter()
     O.push_ub64(n, st.x2, st.obs, pr)
+    t_push = time.perf_counter() - t1
+    t1 = time.perf_counter()
     O.fw_lb64(n, st.obs, st.x2[st.obs], pr)
-    t_chk = time.perf_counter() - t1
+    t_lb = time.perf_counter() - t1
     iters = r_last.iterations
-    checks = max(1, iters // 50)
-    t_solve = iters * t_it + checks * t_chk
-    return dict(value=round(1.0 / t_solve, 8), unit="certified solves/s", cores=1, kind="port",
-                sample="%d float64 iterations + 1 push + 1 lower bound of the n=18 1%% instance (%.1f s on 1 core), "
-                       "scaled to %d iterations and %d checks" % (a.cpu_iters, a.cpu_iters * t_it + t_chk, iters, checks),
-                seconds_per_iteration=round(t_it, 4), seconds_per_check=round(t_chk, 4),
-                host_cpus=os.cpu_count())
+    check, lb_every = int(r_last.method_args["check"]), int(r_last.method_args["lb_every"])
+    checks = max(1, iters // check)
+    lbs = -(-checks // lb_every)
+    t_solve = iters * t_it + checks * t_push + lbs * t_lb
+    threads = O.threads()
+    return dict(value=round(1.0 / t_solve, 8), unit="certified solves/s", cores=threads, kind="port",
+                sample="%d float64 iterations + 1 push + 1 lower bound of the n=18 1%% instance (%.1f s on %d "
+                       "threads of %s), scaled to %d iterations, %d checks (every %d) and %d lower bounds"
+                       % (a.cpu_iters, a.cpu_iters * t_it + t_push + t_lb, threads, _cpu_model(), iters, checks,
+                          check, lbs),
+                seconds_per_iteration=round(t_it, 5), seconds_per_push=round(t_push, 5),
+                seconds_per_lb=round(t_lb, 5), host_cpus=os.cpu_count(), cpu_model=_cpu_model())
 
 
 if __name__ == "__main__":

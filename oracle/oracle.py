@@ -58,6 +58,7 @@ def lib():
             "orc_colex_rank": (ctypes.c_uint64, [ctypes.c_uint64]),
             "orc_price_stream": (f64, [i32, i32, i32, P, P, P, P, P]),
             "orc_ranking_column": (None, [i32, i64, P, P, P, P]),
+            "orc_threads": (i32, []),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -281,6 +282,11 @@ def stream_ranking(n, masks_x, scores_x, offsets, pred_top, k_pred=None):
     for j, e in enumerate(elems):
         rank_pos[e] = pos + int(sub_rank[j])
     return rank_pos
+
+
+def threads():
+    """host threads the oracle's parallel loops use"""
+    return int(lib().orc_threads())
 
 
 def colex_rank(mask):

@@ -13,6 +13,7 @@
 #include <cstdint>
 #include <cstring>
 #include <vector>
+#include <omp.h>
 #include <algorithm>
 
 typedef int64_t i64;
@@ -32,6 +33,7 @@ static inline u64 expand_bit(u64 rest, int i) {           // arcflow_kernels.py:
 template <typename T>
 static void butterfly(T* v, int n, T a, T b, T c, T d) {
   const i64 Nloc = 1ll << (n - 1);
+#pragma omp parallel for schedule(static)
   for (int x = 0; x < n; ++x) {
     T* col = v + (i64)x * Nloc;
     for (int j = 0; j < n - 1; ++j) {
@@ -47,6 +49,9 @@ static void butterfly(T* v, int n, T a, T b, T c, T d) {
     }
   }
 }
+
+// Threads the parallel loops use (OMP_NUM_THREADS, else every core the process may run on).
+extern "C" int orc_threads() { return omp_get_max_threads(); }
 
 extern "C" void orc_butterfly_f64(double* v, int n, double a, double b, double c, double d) {
   butterfly<double>(v, n, a, b, c, d);
@@ -72,10 +77,11 @@ template <typename T>
 static void prox_all(int n, const T* slab_in, T* slab_out, const T* xhat, const T* wmask, T beta2) {
   const i64 Nloc = 1ll << (n - 1);
   const u64 NN = 1ull << n;
-  T vals[64];
-  i64 pos_arr[64];
-  double srt[64];
+#pragma omp parallel for schedule(dynamic, 4096)
   for (u64 S = 1; S < NN; ++S) {
+    T vals[64];
+    i64 pos_arr[64];
+    double srt[64];
     int k = __builtin_popcountll(S);
     u64 SS = S;
     int kk = 0;
@@ -178,22 +184,28 @@ static void admm_iters(int n, i64 iters, T* x2, T* s, T* y, T* z, const T* xhat,
   const T b1 = (T)beta1, b2 = (T)beta2, ib1 = (T)(1.0 / beta1), ib2 = (T)(1.0 / beta2), a = (T)alpha_relax;
   const T one = (T)1;
   for (i64 it = 0; it < iters; ++it) {
+#pragma omp parallel for schedule(static)
     for (i64 i = 0; i < m2; ++i) w[i] = y[i] - b1 * s[i];                                  // fuse_w
     std::memcpy(AT.data(), w.data(), m2 * sizeof(T));
     butterfly<T>(AT.data(), n, (T)1.0, (T)0.0, (T)-1.0, (T)p);                              // raw A'
+#pragma omp parallel for schedule(static)
     for (i64 i = 0; i < m2; ++i) rhs[i] = (z[i] + b2 * x2[i] - AT[i]) * ib1;               // fuse_rhs
     std::memcpy(x.data(), rhs.data(), m2 * sizeof(T));
     butterfly<T>(x.data(), n, (T)Q[0], (T)Q[2], (T)Q[1], (T)Q[3]);                          // Q^T
+#pragma omp parallel for schedule(static)
     for (i64 i = 0; i < m2; ++i) x[i] *= inv_gl[i % Nloc];                                  // diag_mul
     butterfly<T>(x.data(), n, (T)Q[0], (T)Q[1], (T)Q[2], (T)Q[3]);                          // Q
     std::memcpy(Ax.data(), x.data(), m2 * sizeof(T));
     butterfly<T>(Ax.data(), n, (T)1.0, (T)-1.0, (T)0.0, (T)p);                              // raw A
+#pragma omp parallel for schedule(static)
     for (i64 i = 0; i < m2; ++i) xk[i] = a * x[i] + (one - a) * x2[i] - ib2 * z[i];        // fuse_xk
     prox_all<T>(n, xk.data(), x2n.data(), xhat, wmask, b2);
+#pragma omp parallel for schedule(static)
     for (i64 i = 0; i < m2; ++i) {                                                          // fuse_z
       T xo = a * x[i] + (one - a) * x2[i];
       z[i] -= b2 * (xo - x2n[i]);
     }
+#pragma omp parallel for schedule(static)
     for (i64 i = 0; i < m2; ++i) {                                                          // fuse_sy
       T axor = -a * Ax[i] - (one - a) * s[i];
       T sn = ib1 * y[i] - axor;
@@ -252,8 +264,10 @@ extern "C" double orc_push_ub64(int n, const double* x, i64 M, const i64* obs, c
   const i64 NN = 1ll << n;
   std::vector<double> w(x, x + m2);
   butterfly<double>(w.data(), n, 1.0, -1.0, 0.0, 1.0);                   // superset Mobius
+#pragma omp parallel for schedule(static)
   for (i64 i = 0; i < m2; ++i) w[i] = (w[i] > 0.0 ? w[i] : 0.0) + 1e-12;   // wmax64
   std::vector<double> outw(NN), mass(NN, 0.0), rf(m2);
+#pragma omp parallel for schedule(static)
   for (i64 D = 0; D < NN; ++D) {                                          // outwt64
     double sacc = 0.0;
     for (int xx = 0; xx < n; ++xx) {
@@ -266,6 +280,7 @@ extern "C" double orc_push_ub64(int n, const double* x, i64 M, const i64* obs, c
   build_layers(n, order, bounds);
   mass[0] = 1.0;
   for (int c = 1; c <= n; ++c) {                                          // massg64 per layer
+#pragma omp parallel for schedule(static)
     for (i64 t = bounds[c]; t < bounds[c + 1]; ++t) {
       i64 C = order[t];
       double sacc = 0.0;
@@ -280,6 +295,7 @@ extern "C" double orc_push_ub64(int n, const double* x, i64 M, const i64* obs, c
       mass[C] = sacc;
     }
   }
+#pragma omp parallel for schedule(static)
   for (i64 a = 0; a < m2; ++a) {                                          // extract64
     i64 xx = a / Nloc, rest = a - xx * Nloc;
     i64 T = (i64)expand_bit((u64)rest, (int)xx);
@@ -318,6 +334,7 @@ extern "C" double orc_fw_lb64(int n, i64 M, const i64* obs, const double* y, con
   std::vector<double> dp(NN);
   dp[0] = 0.0;
   for (int c = 1; c <= n; ++c) {
+#pragma omp parallel for schedule(static)
     for (i64 t = bounds[c]; t < bounds[c + 1]; ++t) {
       i64 C = order[t];
       double best = -1e300;
