@@ -75,7 +75,23 @@ def _parse():
                     help="only the bootstrap leg (prints its JSON object): the workload of its rocprof summary")
     ap.add_argument("--traffic-run", action="store_true",
                     help="only the workload for HBM counters: --profile-iters fused iterations on the n=18 "
-                         "instance, then a device copy of the 4 state slabs (a copy kernel of known size)")
+                         "instance (or config3 with --traffic-n 22) with the autotuned window plan, then a device "
+                         "copy of the 4 state slabs (a copy kernel of known size)")
+    ap.add_argument("--traffic-n", type=int, default=18, choices=[18, 22])
+    ap.add_argument("--config3-only", action="store_true",
+                    help="only the n=22 (config3) iteration roofline (prints its JSON object)")
+    ap.add_argument("--no-config3", action="store_true", help="skip the n=22 iteration roofline")
+    ap.add_argument("--no-ai", action="store_true", help="skip the reference's stored AI run (n=18, 73,492 cells)")
+    ap.add_argument("--pricing-sweeps", default="",
+                    help="only slab pricing sweeps on one instance (n20_d1pct, n25_sparse or config3), "
+                         "--sweeps of them: the workload of the slab pricing profile")
+    ap.add_argument("--sweeps", type=int, default=20)
+    ap.add_argument("--summarize-stats", default="",
+                    help="print the per-kernel table of a rocprofv3 --kernel-trace --stats database (rocpd .db)")
+    ap.add_argument("--summarize-traffic", nargs=3, metavar=("FETCH_DB", "WRITE_DB", "COPY_BYTES"),
+                    help="print FETCH_SIZE / WRITE_SIZE per dispatch of two rocprofv3 --pmc databases, "
+                         "calibrated on the ew_copy kernel of --traffic-run (COPY_BYTES read and written)")
+    ap.add_argument("--traffic-json", default="", help="with --summarize-traffic: write bytes per dispatch by kernel")
     return ap.parse_args()
 
 
@@ -108,6 +124,22 @@ def main():
     from solver.rum_solver import RUM
     from solver.bm_admm.engine import Handle, solve_kadmm
 
+    if a.summarize_stats:
+        print(_summarize_stats(a.summarize_stats))
+        return
+    if a.summarize_traffic:
+        text, per_kernel = _summarize_traffic(*a.summarize_traffic)
+        print(text)
+        if a.traffic_json:
+            with open(a.traffic_json, "w") as f:
+                json.dump(per_kernel, f, indent=1)
+        return
+    if a.pricing_sweeps:
+        _pricing_sweeps(a.pricing_sweeps, a.sweeps)
+        return
+    if a.config3_only:
+        print(json.dumps(_config3_bench(a)), flush=True)
+        return
     if a.bootstrap_only:
         print(json.dumps(_bootstrap_bench(a)), flush=True)
         return
@@ -118,20 +150,22 @@ def main():
         print(json.dumps(_reference_instances([k for k in a.only.split(",") if k])), flush=True)
         return
 
-    cd, n = instances.config2()
+    cd, n = instances.config2() if a.traffic_n == 18 or not a.traffic_run else instances.config3()
     rum = RUM.from_rows(cd, n)
     if a.traffic_run:
-        from solver import params
+        from solver import params, machine
+        from solver.bm_admm.autotune import tune
         g = rum._geom
         h = Handle(n, g.menu, g.alternative.astype(np.int32), rum.choice_probability[rum._rest], dtype=a.dtype,
                    beta=params.bm_beta_cert(g.M))
+        plan, _t = tune(h, device=machine.gpu_name())
+        h.init_cold()
         h2 = Handle(n, g.menu, g.alternative.astype(np.int32), rum.choice_probability[rum._rest], dtype=a.dtype,
                     beta=params.bm_beta_cert(g.M))
         h.run(a.profile_iters)
         h2.copy_state_from(h)           # 4 x ew_copy: each reads and writes n 2^(n-1) values
-        print("[bench] traffic run: %d iterations, state copy of 4 x %d bytes" % (a.profile_iters,
-                                                                                 (n << (n - 1)) * h.dtype.itemsize),
-              file=sys.stderr, flush=True)
+        print("[bench] traffic run: n=%d, window plan %s, %d iterations, state copy of 4 x %d bytes"
+              % (n, list(plan), a.profile_iters, (n << (n - 1)) * h.dtype.itemsize), file=sys.stderr, flush=True)
         return
     p = rum.choice_probability
     mp = rum._geom.M
@@ -165,6 +199,7 @@ def main():
 
     # roofline of the dominant fused pass (profiled on the session's handles, same state)
     eng = sess._engines["bm"].items
+    sess_plan = r_last.info.get("window_plan")
     prof = {}
     import ctypes
     for dname, h in eng.items():
@@ -187,15 +222,23 @@ def main():
     passes = {KIND_NAMES[k]: dict(avg_us=round(1e3 * v[0] / v[1], 3), launches=v[1],
                                   gbs=round(_pass_bytes(k, m2, isz, n, mp) / (v[0] / v[1] * 1e-3) / 1e9, 1))
               for k, v in kinds.items()}
-    traffic = None
+    # HBM bytes per launch of the timed kernel, from the committed rocprofv3 FETCH_SIZE + WRITE_SIZE
+    # table of the same window plan (keyed by the kernel's template name); None when no table has it
+    traffic, dom_kernel = None, _kernel_name(dom, dom_dtype, sess_plan, n)
     tf = os.path.join(ROOT, "profiles", "traffic_%s_n18.json" % dom_dtype)
-    if os.path.exists(tf):
+    if os.path.exists(tf) and dom_kernel:
         try:
-            traffic = json.load(open(tf)).get(str(dom))
+            table = json.load(open(tf))
+            hits = [v for k, v in table.items() if k.startswith(dom_kernel)]
+            traffic = hits[0] if len(hits) == 1 else None
         except Exception:
             traffic = None
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, kernel=KIND_NAMES[dom],
+                    kernel_template=dom_kernel, window_plan=list(sess_plan) if sess_plan else None,
+                    working_set="%d float64 slabs of %.1f MB (%.0f MB) fit the 256 MB Infinity Cache: this fraction "
+                                "is cache-served; config3's (n=22) is the HBM-bound one"
+                                % (6, m2 * isz / 1e6, 6 * m2 * isz / 1e6),
                     dtype=dom_dtype, alg_bytes_per_launch=alg, avg_launch_us=round(avg_ms * 1e3, 3),
                     iteration=dict(us=round(iter_ms * 1e3, 2), alg_bytes=int(iter_bytes),
                                    gbs=round(iter_bytes / (iter_ms * 1e-3) / 1e9, 1)),
@@ -243,6 +286,8 @@ def main():
         cpu = _cpu_baseline(n, rum, p, r_last, a)
     boot = _bootstrap_bench(a) if a.boot_draws > 0 else None
     dsr = None if a.no_dsr else _dsr_bench()
+    cfg3 = None if a.no_config3 else _config3_bench(a)
+    ai = None if a.no_ai else _ai_bench()
 
     line = dict(metric=METRIC, value=round(value, 5), unit="certified solves/s", n_gpus=1, steps=a.steps,
                 warmup=a.warmup, ms_per_step=round(1e3 * elapsed / a.steps, 2), higher_is_better=True,
@@ -257,6 +302,7 @@ def main():
                             iters64=[r.info.get("iters64") for r in results],
                             bracket=[r_last.lb, r_last.ub], certified=certified),
                 roofline=roofline, cpu_baseline=cpu, plain=plain, bootstrap=boot, dsr_bootstrap=dsr,
+                config3_iteration=cfg3, ai_stored_run=ai,
                 tuned_vs_plain=round(value / plain["value"], 3) if plain and plain["value"] > 0 else None)
     print(json.dumps(line), flush=True)
 
@@ -451,6 +497,207 @@ def _dsr_all_alternatives():
                 same_as_published=bool((pv.p, pv.n_below, pv.n_above, pv.n_undecided)
                                        == (pub["p"], pub["below"], pub["above"], pub["undecided"])),
                 speedup_vs_published=round(pub["seconds"] / t, 2))
+
+
+def _kernel_name(kind, dtype, plan, n):
+    """rocprof's name (prefix) of the kernel that runs pass `kind` (0 SY, 1 RHS, 2 D, 3 XK, 4 prox) for
+    window plan (lo_bits, lo_rl, hi_rl): bm_pass<T, RL, KIND, B, C> with the geometry fixed at compile
+    time where the launcher has that variant, else bm_pass<T, RL, KIND, 0, -1>."""
+    if not plan:
+        return None
+    T = "double" if dtype == "float64" else "float"
+    lo, lrl, hrl = (int(v) for v in plan)
+    if kind == 4:
+        return "void bm_prox_iter<%s, %d>" % (T, n)
+    if kind in (0, 2):
+        K = {0: 1, 2: 4}[kind]
+        if lrl == 3 and 8 <= lo <= 12:
+            return "void bm_pass<%s, 3, %d, %d, 0>(PassArgs<%s>)" % (T, K, lo, T)
+        return "void bm_pass<%s, %d, %d, 0, -1>(PassArgs<%s>)" % (T, lrl, K, T)
+    K, B, CW = {1: 3, 3: 5}[kind], n - 1 - lo, (4 if T == "double" else 5)
+    if hrl in (3, 4) and 4 <= B <= 8:
+        return "void bm_pass<%s, %d, %d, %d, %d>(PassArgs<%s>)" % (T, hrl, K, B, CW, T)
+    return "void bm_pass<%s, %d, %d, 0, -1>(PassArgs<%s>)" % (T, hrl, K, T)
+
+
+def _config3_bench(a):
+    """Fused BM-ADMM iterations at n = 22, 0.5 % (config3, 230,691 cells): one float64 slab is 369 MB,
+    so the working set (6 slabs) streams from HBM; no cache serves it. Per dtype: the autotuned window
+    plan, the time per iteration (HIP-graph replays, host clock around 100 iterations after a
+    stream synchronization), the per-pass device times from start/stop events recorded in each
+    pass's own dispatch (rumgpu_bm_profile), the algorithmic bytes per iteration (20 n 2^(n-1) w) and
+    the bytes the passes actually move (28 slab transfers with three windows)."""
+    import ctypes
+    import numpy as np
+    from solver import _lib, instances, machine, params
+    from solver.bm_admm.autotune import tune
+    from solver.bm_admm.engine import Handle
+    from solver.rum_solver import RUM
+    cd, n = instances.config3()
+    rum = RUM.from_rows(cd, n)
+    g = rum._geom
+    p = rum.choice_probability[rum._rest]
+    m2 = n << (n - 1)
+    out = dict(workload="fused BM-ADMM iterations, n=22, 0.5%% density (config3 seed 22001, %d cells)" % g.M,
+               fingerprint=instances.fingerprint(cd))
+    for dt in ("float64", "float32"):
+        isz = 8 if dt == "float64" else 4
+        h = Handle(n, g.menu, g.alternative.astype(np.int32), p, dtype=dt, beta=params.bm_beta_cert(g.M))
+        plan, _t = tune(h, device=machine.gpu_name())
+        h.init_cold()
+        if not a.no_graph:
+            h.set_graph(10)
+        h.run(20)
+        h.phi()
+        t0 = time.perf_counter()
+        h.run(100)
+        h.phi()
+        ms_it = (time.perf_counter() - t0) * 10.0
+        ms = (ctypes.c_double * 7)()
+        cnt = (ctypes.c_int64 * 7)()
+        iters = 50
+        _lib.check(h.lib.rumgpu_bm_profile(h.h, iters, ms, cnt), "bm_profile")
+        kinds = {k: (ms[k], cnt[k]) for k in range(7) if cnt[k]}
+        dev_ms = sum(v[0] for v in kinds.values()) / iters
+        moved = sum(_pass_bytes(k, m2, isz, n, g.M) * v[1] for k, v in kinds.items()) / iters
+        alg = 20 * m2 * isz
+        passes = {KIND_NAMES[k]: dict(avg_us=round(1e3 * v[0] / v[1], 1), launches_per_iteration=v[1] / iters,
+                                      gbs=round(_pass_bytes(k, m2, isz, n, g.M) / (v[0] / v[1] * 1e-3) / 1e9, 1),
+                                      frac=round(_pass_bytes(k, m2, isz, n, g.M) / (v[0] / v[1] * 1e-3) / 1e9
+                                                 / HBM_PEAK_GBS, 3))
+                  for k, v in kinds.items()}
+        out[dt] = dict(window_plan=list(plan), windows=h.windows(), ms_per_iteration=round(ms_it, 4),
+                       device_ms_per_iteration=round(dev_ms, 4), alg_bytes_per_iteration=alg,
+                       alg_gbs=round(alg / (ms_it * 1e-3) / 1e9, 1),
+                       alg_frac=round(alg / (ms_it * 1e-3) / 1e9 / HBM_PEAK_GBS, 3),
+                       moved_bytes_per_iteration=int(moved),
+                       moved_gbs=round(moved / (dev_ms * 1e-3) / 1e9, 1),
+                       moved_frac=round(moved / (dev_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 3), passes=passes)
+        print("[bench] config3 %s: %.3f ms per iteration, %.0f GB/s algorithmic (%.2f of peak), %.0f GB/s moved"
+              % (dt, ms_it, out[dt]["alg_gbs"], out[dt]["alg_frac"], out[dt]["moved_gbs"]), file=sys.stderr, flush=True)
+        h.close()
+    return out
+
+
+def _ai_bench():
+    """The reference's stored AI run (gpt-5.6-luna, n = 18, 73,492 cells, 100 replicates): its
+    float64 BM-ADMM certificate of dist(p_hat) with the record's settings (beta 5, check 25, lower
+    bound every 4th check, abs 5e-4), the same certificate with the adaptive penalty, and the whole
+    bootstrap (B = 10,000 panel draws, p_value()), each beside the stored time on an RTX 5090."""
+    import numpy as np
+    import pandas as pd
+    from solver import _lib
+    from solver.inference import cells_from_panel
+    from solver.rum_solver import RUM
+    gold = os.path.join(ROOT, "tests", "golden")
+    z = np.load(os.path.join(gold, "ai_luna_replies.npz"))
+    table = pd.DataFrame({c: z[c].astype(np.int64) for c in ("respondent", "menu", "chosen")})
+    rec = json.load(open(os.path.join(gold, "ai_luna_record.json")))
+    menu, alt, p = cells_from_panel(table, 18)
+    rum = RUM(menu, alt, p, num_alternatives=18)
+    cert = rec["p_hat"]["certificate"]
+    margs = {k: cert["method_args"][k] for k in ("beta", "check", "lb_every", "dtype")}
+    _lib.context(0)
+    out = dict(workload="AI gpt-5.6-luna: n=18, 73,492 cells, 100 replicates as respondents", stored_gpu="RTX 5090")
+    for name, args in (("certificate_reference_settings", margs),
+                       ("certificate_adaptive", {"beta": "adaptive", "dtype": "float64"})):
+        rum.calculate_distance(abs_tol=cert["abs_tol"], rel_tol=0.0, method="bm_admm", method_args=args,
+                               max_time=120)                       # window plan tuned, kernels loaded
+        t0 = time.perf_counter()
+        r = rum.calculate_distance(abs_tol=cert["abs_tol"], rel_tol=0.0, method="bm_admm", method_args=args,
+                                   max_time=120)
+        t = time.perf_counter() - t0
+        out[name] = dict(seconds=round(t, 3), iterations=r.iterations, bracket=[r.lb, r.ub], exit_flag=r.exit_flag,
+                         stored_seconds=round(cert["time"], 2), stored_iterations=cert["iterations"],
+                         stored_bracket=[cert["lb"], cert["ub"]], speedup=round(cert["time"] / t, 2))
+    s = rec["panel"]["settings"]
+    t0 = time.perf_counter()
+    with rum.bootstrap(panel=table, B=s["B"], seed=s["seed"], method=s["method"], method_args=s["method_args"],
+                       abs_tol=s["abs_tol"], rel_tol=s["rel_tol"], refine=s["refine"]) as boot:
+        pv = boot.p_value()
+        tm = dict(boot.time)
+    t = time.perf_counter() - t0
+    want = rec["panel"]["p_value"]
+    out["bootstrap"] = dict(seconds=round(t, 2), p=pv.p, below=pv.n_below, screened=pv.n_screened,
+                            certificate_s=round(tm["certificate"], 2), screen_s=round(tm["screen"], 2),
+                            stored_seconds=round(want["time"], 2), stored_p=want["p"],
+                            stored_certificate_s=round(rec["panel"]["time"]["certificate"], 2),
+                            stored_screen_s=round(rec["panel"]["time"]["screen"], 2),
+                            speedup=round(want["time"] / t, 2))
+    print("[bench] AI luna: certificate %.2f s (stored %.2f s), bootstrap %.2f s (stored %.2f s)"
+          % (out["certificate_reference_settings"]["seconds"], cert["time"], t, want["time"]), file=sys.stderr,
+          flush=True)
+    return out
+
+
+def _pricing_sweeps(key, sweeps):
+    """Slab-layout pricing sweeps on one instance (random float32 scores), for a kernel profile."""
+    import numpy as np
+    from solver import instances
+    from solver.flow_cg.engine import Pricer, xmajor
+    if key == "config3":
+        cd, n = instances.config3()
+    else:
+        n, cat = int(key.split("_")[0][1:]), key.split("_", 1)[1]
+        cd, _mp = instances.get_instance(n, cat)
+    menu, alt = cd[:, 0].astype(np.uint64), cd[:, 1].astype(np.int32)
+    perm, masks_x, _a, offsets = xmajor(menu, alt, n)
+    rng = np.random.default_rng(1)
+    pr = Pricer(n, masks_x, offsets, layout="slab")
+    S = rng.standard_normal((2, menu.size)).astype(np.float32)
+    pr.price(S[0])
+    t0 = time.perf_counter()
+    for k in range(sweeps):
+        pr.price(S[k % 2])
+    dt = (time.perf_counter() - t0) / sweeps
+    pr.close()
+    print("[bench] slab pricing %s (n=%d, %d rows): %.3f ms per sweep over %d sweeps"
+          % (key, n, menu.size, dt * 1e3, sweeps), flush=True)
+
+
+def _summarize_stats(db):
+    """Per-kernel calls, total and mean time, share: the top_kernels view of a rocpd database."""
+    import sqlite3
+    con = sqlite3.connect(db)
+    rows = con.execute("select name, total_calls, total_duration, average, percentage from top_kernels").fetchall()
+    out = ["| kernel | calls | total ms | mean us | share % |", "|---|---|---|---|---|"]
+    for name, calls, tot, avg, pct in rows[:24]:
+        out.append("| `%s` | %d | %.1f | %.3f | %.1f |" % (name[:100], calls, tot / 1e3, avg, pct))
+    return "\n".join(out)
+
+
+def _counters(db, cname):
+    import collections
+    import sqlite3
+    con = sqlite3.connect(db)
+    d = collections.defaultdict(list)
+    for k, v in con.execute("select kernel_name, value from counters_collection where counter_name = ?", (cname,)):
+        d[k].append(float(v))
+    return d
+
+
+def _summarize_traffic(fetch_db, write_db, copy_bytes):
+    """FETCH_SIZE and WRITE_SIZE (KiB per dispatch) per kernel, each scaled so that the ew_copy kernel
+    (which reads and writes copy_bytes per dispatch) reads what it reads: HBM bytes per dispatch."""
+    copy_bytes = int(copy_bytes)
+    f, w = _counters(fetch_db, "FETCH_SIZE"), _counters(write_db, "WRITE_SIZE")
+    copy = [k for k in f if k.startswith("void ew_copy<")]
+    if not copy:
+        raise SystemExit("no ew_copy dispatch to calibrate on")
+    cf = copy_bytes / (1024.0 * sum(f[copy[0]]) / len(f[copy[0]]))
+    cw = copy_bytes / (1024.0 * sum(w[copy[0]]) / len(w[copy[0]]))
+    out = ["calibration on ew_copy (reads and writes %d bytes per dispatch): bytes = FETCH_SIZE KiB x 1024 x %.4f, "
+           "WRITE_SIZE KiB x 1024 x %.4f" % (copy_bytes, cf, cw), "",
+           "| kernel | dispatches | FETCH_SIZE KiB | WRITE_SIZE KiB | HBM read MB | HBM write MB |",
+           "|---|---|---|---|---|---|"]
+    per_kernel = {}
+    for k in sorted(f, key=lambda k: -sum(f[k]) / len(f[k])):
+        fv = sum(f[k]) / len(f[k])
+        wv = sum(w.get(k, [0.0])) / max(1, len(w.get(k, [])))
+        rb, wb = fv * 1024 * cf, wv * 1024 * cw
+        out.append("| `%s` | %d | %.1f | %.1f | %.2f | %.2f |" % (k[:90], len(f[k]), fv, wv, rb / 1e6, wb / 1e6))
+        per_kernel[k] = int(rb + wb)
+    return "\n".join(out), per_kernel
 
 
 def _cpu_model():
