@@ -71,8 +71,9 @@ int rumgpu_bm_set_graph(rumgpu_bm* bm, int64_t len);
  * over. The reference fixes beta per solve (solver.py:148-154); this is the hook of an adaptive
  * penalty (residual balancing on the certified gap). */
 int rumgpu_bm_set_beta(rumgpu_bm* bm, double beta);
-/* window plan of the fused passes: lo_bits at bit 0 (lo_rl register bits), the rest in windows
- * of <= 8 bits (hi_rl register bits). The reference plans its butterfly passes by a calibrated
+/* window plan of the fused passes: lo_bits (1..13) at bit 0 (lo_rl register bits), the rest in
+ * windows of <= 8 bits (hi_rl register bits); a plan whose tile needs more threads (512) or LDS
+ * than a workgroup may have on the device is refused (status 1). The reference plans its butterfly passes by a calibrated
  * DP (common/butterfly.py:125-195); here the autotuner (solver/bm_admm/autotune.py) times plans
  * per n. Results are the same for every plan up to rounding order. */
 int rumgpu_bm_set_plan(rumgpu_bm* bm, int lo_bits, int lo_rl, int hi_rl);
@@ -187,6 +188,11 @@ int rumgpu_cg_info(rumgpu_cg* cg, size_t* device_bytes, double* last_kernel_ms);
  * alternative the closure sweep uses, or -1 when the layered sweep runs. */
 int rumgpu_cg_set_sweep(rumgpu_cg* cg, int mode);
 int rumgpu_cg_sweep_info(rumgpu_cg* cg, int* one_alternative);
+/* Tile bits L of the slab layout's layered max (2..10, 0 = the built-in choice): the subsets are
+ * worked in tiles of 2^L, one wavefront per tile, one launch per popcount of the other n - L bits.
+ * Results are the same for every L. No reference counterpart: a tuning knob of this engine's
+ * tiling of the layered DP (flow_cg/solver.py:168-200 sweeps one layer at a time). */
+int rumgpu_cg_set_slab_tile(rumgpu_cg* cg, int bits);
 
 /* Column store of one column-generation run, on the pricer's device and rows. Replaces the host
  * column bookkeeping of solve_cgbfqp_cholbitset (flow_cg/solver.py:92-115, :145, :240): the packed

@@ -754,6 +754,24 @@ int launch_pass_rl(int RL, const PassArgs<T>& P, hipStream_t st) {
   // the window at bit 0 of the fused passes with 3 register bits: geometry fixed at compile time
   if constexpr (KIND == K_FIRST || KIND == K_SY || KIND == K_SYONLY || KIND == K_D || KIND == K_FULL ||
                 KIND == K_FULL_FIRST) {
+    // 13 bits at bit 0 (4 or 5 register bits, 512 or 256 threads): with 8 bits above it, two windows cover
+    // n = 22; the float64 tile needs 68 KiB of LDS, which gfx950's 160 KiB per workgroup allows
+    if ((RL == 4 || RL == 5) && g.s == 0 && g.C == 0 && g.B == 13) {
+      if (RL == 4) pass_launch((bm_pass<T, 4, KIND, 13, 0>), grid, block, shm, st, P);
+      else pass_launch((bm_pass<T, 5, KIND, 13, 0>), grid, block, shm, st, P);
+      RG_LAUNCH_CHECK();
+      return RG_OK;
+    }
+    // 4 register bits on 10-12 bits at bit 0: one LDS exchange fewer per transform than 3
+    if (RL == 4 && g.s == 0 && g.C == 0 && g.B >= 10 && g.B <= 12) {
+      switch (g.B) {
+        case 10: pass_launch((bm_pass<T, 4, KIND, 10, 0>), grid, block, shm, st, P); break;
+        case 11: pass_launch((bm_pass<T, 4, KIND, 11, 0>), grid, block, shm, st, P); break;
+        default: pass_launch((bm_pass<T, 4, KIND, 12, 0>), grid, block, shm, st, P); break;
+      }
+      RG_LAUNCH_CHECK();
+      return RG_OK;
+    }
     if (RL == 3 && g.s == 0 && g.C == 0 && g.B >= 8 && g.B <= 12) {
       switch (g.B) {
         case 8: pass_launch((bm_pass<T, 3, KIND, 8, 0>), grid, block, shm, st, P); break;
@@ -1475,14 +1493,14 @@ extern "C" int rumgpu_bm_set_graph(rumgpu_bm* b, int64_t len) {
 extern "C" int rumgpu_bm_set_plan(rumgpu_bm* b, int lo_bits, int lo_rl, int hi_rl) {
   RG_CHECK(b, RG_EINVAL, "bm_set_plan: null handle");
   RG_CHECK(!b->plain, RG_EINVAL, "bm_set_plan: the plain schedule has no window plan");
-  RG_CHECK(lo_bits >= 1 && lo_bits <= 12 && lo_rl >= 1 && lo_rl <= 5 && hi_rl >= 1 && hi_rl <= 5, RG_EINVAL,
-           "bm_set_plan: lo_bits 1..12, register bits 1..5");
+  RG_CHECK(lo_bits >= 1 && lo_bits <= 13 && lo_rl >= 1 && lo_rl <= 5 && hi_rl >= 1 && hi_rl <= 5, RG_EINVAL,
+           "bm_set_plan: lo_bits 1..13, register bits 1..5");
   std::vector<WinPlan> wins = plan_windows_explicit(b->nb, b->isz, lo_bits, lo_rl, hi_rl);
   std::vector<WinGeom> geoms;
   for (auto& w : wins) {
     const WinGeom g = make_geom(b->n, w, b->D);
     const size_t lds = (size_t)g.TPB * tile_lds_elems(g) * b->isz;
-    RG_CHECK(g.TT * g.TPB <= 512 && lds <= 65536, RG_EINVAL,
+    RG_CHECK(g.TT * g.TPB <= 512 && lds <= b->ctx->lds_per_block, RG_EINVAL,
              "bm_set_plan: a tile of %d bits with %d register bits needs %d threads and %zu bytes of LDS per block",
              w.B + w.C, w.RL, g.TT * g.TPB, lds);
     geoms.push_back(g);

@@ -49,6 +49,7 @@ void rg_set_error(const char* fmt, ...);
 struct rumgpu_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  size_t lds_per_block = 65536;   // the device's LDS per workgroup (160 KiB on gfx950)
 };
 
 // Device allocation that records a readable error (with the size) on failure.

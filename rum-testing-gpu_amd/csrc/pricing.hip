@@ -469,10 +469,36 @@ __global__ void cl_backtrace(const u32* __restrict__ F, const u32* __restrict__ 
   }
 }
 
-// slab layout: scatter of the scores, then one layer of the max over members
-__global__ void cg_scatter(float* __restrict__ g, const i64* __restrict__ pos, const float* __restrict__ scores, i64 M) {
-  const i64 j = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j < M) g[pos[j]] = scores[j];
+// Slab layout, first window of the subset-sum butterfly (at bit 0, 2^B consecutive positions per
+// tile): the tile starts at zero with the rows whose slab position falls inside it (rows sorted by
+// position, zoff[tile] .. zoff[tile + 1]), then takes the window's stages. The same values as
+// scattering the scores on a zeroed slab and running the plain pass on it.
+template <int RL>
+__global__ __launch_bounds__(512) void cg_zeta_first(float* __restrict__ R, WinGeom g, const i64* __restrict__ zoff,
+                                                     const i64* __restrict__ zpos, const int32_t* __restrict__ zrow,
+                                                     const float* __restrict__ scores) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  Tile<float, RL> t;
+  t.init(g, reinterpret_cast<float*>(smem));
+  constexpr int NR = Tile<float, RL>::R;
+  const int tl = threadIdx.x / g.TT, tt = threadIdx.x - tl * g.TT;
+  const i64 tile = (i64)blockIdx.x * g.TPB + tl;
+  const int E = 1 << g.B;
+  for (int e = tt; e < E; e += g.TT) t.lds[e ^ ((e >> 4) & 15)] = 0.0f;
+  __syncthreads();
+  if (t.active) {
+    for (i64 q = zoff[tile] + tt; q < zoff[tile + 1]; q += g.TT) {
+      const int e = (int)(zpos[q] - t.base);
+      t.lds[e ^ ((e >> 4) & 15)] = scores[zrow[q]];
+    }
+  }
+  __syncthreads();
+  if (t.active) {
+#pragma unroll
+    for (int j = 0; j < NR; ++j) t.v[j] = t.lds[t.lidx(t.cur, j)];
+  }
+  t.transform(Coef<float>{1.0f, 0.0f, 1.0f, 1.0f});
+  t.store(R);
 }
 
 // Slab layout's layered max, tile by tile. V[D] = max over members x of D of V[D - x] + R_x[D - x]
@@ -482,17 +508,20 @@ __global__ void cg_scatter(float* __restrict__ g, const i64* __restrict__ pos, c
 // for the tile are one contiguous run R[x Nloc + (h >> 1) + c], c < 2^(L-1); a member x >= L leads
 // to tile h - x, at the same l, with contiguous arc weights too. Tiles are launched by the popcount of
 // h, so the tiles they read are finished; inside a tile the subsets go by the popcount of l in LDS,
-// with the tile's low arcs staged in LDS first. Every load from HBM is a contiguous run, each arc
-// weight is read once, and V and pred are written once, in order. A node's value and choice are
-// those of the one-node-per-thread form: each candidate is V + R rounded once, the members are
-// taken in ascending order with a strict >, and the high members (all above the low ones) are
-// folded in after the low ones with a strict >, which selects the same member.
-#define CG_SLAB_TILE_L 10
+// with the tile's low arcs staged in LDS first. One wavefront works one tile, so the steps between
+// the tile's popcount layers cost no more than a wavefront's own LDS ordering, and many tiles are
+// resident per CU. Every load from HBM is a contiguous run, each arc weight is read once, and V and
+// pred are written once, in order. A node's value and choice are those of the one-node-per-thread
+// form: each candidate is V + R rounded once, the members are taken in ascending order with a
+// strict >, and the high members (all above the low ones) are folded in after the low ones with a
+// strict >, which selects the same member.
+#define CG_SLAB_TILE_THREADS 64
 template <int L>
-__global__ __launch_bounds__(256) void cg_slab_tile(float* __restrict__ V, uint8_t* __restrict__ pred,
-                                                    const float* __restrict__ R, const u32* __restrict__ hlist,
-                                                    const uint16_t* __restrict__ lord, i64 Nloc) {
-  constexpr int T = 1 << L, H = T >> 1;
+__global__ __launch_bounds__(CG_SLAB_TILE_THREADS) void cg_slab_tile(float* __restrict__ V, uint8_t* __restrict__ pred,
+                                                                     const float* __restrict__ R,
+                                                                     const u32* __restrict__ hlist,
+                                                                     const uint16_t* __restrict__ lord, i64 Nloc) {
+  constexpr int T = 1 << L, H = T >> 1, NT = CG_SLAB_TILE_THREADS;
   __shared__ float vt[T];
   __shared__ float hv[T];
   __shared__ uint8_t hx[T];
@@ -500,12 +529,12 @@ __global__ __launch_bounds__(256) void cg_slab_tile(float* __restrict__ V, uint8
   const int tid = threadIdx.x;
   const u32 h = hlist[blockIdx.x];
   const u32 hm = h << L;
-  for (int e = tid; e < L * H; e += 256) {
+  for (int e = tid; e < L * H; e += NT) {
     const int x = e / H, cidx = e - x * H;
     rl[x][cidx] = R[(i64)x * Nloc + (i64)(hm >> 1) + cidx];
   }
   // the high members, ascending, for every subset of the tile
-  for (int l = tid; l < T; l += 256) {
+  for (int l = tid; l < T; l += NT) {
     const u32 D = hm | (u32)l;
     float b = 0.0f;
     int bx = 0;
@@ -528,7 +557,7 @@ __global__ __launch_bounds__(256) void cg_slab_tile(float* __restrict__ V, uint8
   __syncthreads();
   int lo = 1, cnt = L;
   for (int c = 1; c <= L; ++c) {
-    for (int i = tid; i < cnt; i += 256) {
+    for (int i = tid; i < cnt; i += NT) {
       const u32 l = lord[lo + i];
       float b = 0.0f;
       int bx = 0;
@@ -554,7 +583,7 @@ __global__ __launch_bounds__(256) void cg_slab_tile(float* __restrict__ V, uint8
     lo += cnt;
     cnt = cnt * (L - c) / (c + 1);
   }
-  for (int l = tid; l < T; l += 256) {
+  for (int l = tid; l < T; l += NT) {
     if ((hm | (u32)l) == 0u) continue;           // the empty set keeps V = 0 and has no choice
     V[hm | (u32)l] = vt[l];
     pred[hm | (u32)l] = hx[l];
@@ -793,7 +822,12 @@ struct rumgpu_cg {
   u64* masks64 = nullptr;
   u64* offsets = nullptr;
   float* scores = nullptr;
-  i64* spos = nullptr;       // slab positions (slab layout)
+  // slab layout: the rows' slab positions sorted ascending (zpos) with their rows (zrow), and per
+  // tile of the first butterfly window the range of them inside it (zoff)
+  i64* zpos = nullptr;
+  int32_t* zrow = nullptr;
+  i64* zoff = nullptr;
+  int zB = 0;                // bits of the first window (its tiles are 2^zB consecutive positions)
   // mask / slab
   float* V = nullptr;
   uint8_t* pred = nullptr;
@@ -850,7 +884,7 @@ struct rumgpu_cg {
 };
 
 static void cg_free_all(rumgpu_cg* c) {
-  void* bufs[] = {c->masks32, c->masks64, c->offsets, c->scores, c->spos, c->V, c->pred, c->nodes, c->R, c->rank,
+  void* bufs[] = {c->masks32, c->masks64, c->offsets, c->scores, c->zpos, c->zrow, c->zoff, c->V, c->pred, c->nodes, c->R, c->rank,
                   c->binom, c->vbuf[0], c->vbuf[1], c->lbuf[0], c->lbuf[1], c->mbuf[0], c->mbuf[1], c->ebuf[0],
                   c->ebuf[1], c->lstep, c->subV, c->subpred, c->subnodes, c->submasks, c->subscores, c->suboffsets,
                   c->Vb, c->predb, c->scoresb, c->vmaxb, c->rankb, c->V64, c->r64, c->V64b, c->r64b, c->clcm, c->clF, c->cltmax,
@@ -875,6 +909,46 @@ static void layer_order(int n, std::vector<u32>& ord, std::vector<i64>& bounds) 
   for (int k = 0; k <= n; ++k) bounds[k + 1] = bounds[k] + cnt[k];
   std::vector<i64> fill(bounds.begin(), bounds.end());
   for (i64 v = 0; v < NN; ++v) ord[fill[__builtin_popcountll(v)]++] = (u32)v;
+}
+
+// Tile bits of the slab layout's layered max for n alternatives: 7 (128-subset tiles, 2.6 KiB of LDS,
+// one wavefront each), n when smaller. Measured on whole sweeps over L = 5..10: n20_d1pct 0.31 ms at
+// L = 7 (0.41 at 6, 0.40 at 8), config3 (n = 22) 0.57 ms (0.54 at 6), n25_sparse 3.60 ms (3.68 at 6,
+// 3.78 at 8); larger tiles leave the wavefront too much serial work per tile.
+static int slab_tile_bits(int n) { return n < 7 ? n : 7; }
+
+// The tile tables for L tile bits: the high subsets by popcount (thlist, thbnd) and the subsets of a
+// tile by popcount (tlord). Replaces earlier tables.
+static int slab_tile_tables(rumgpu_cg* c, int L) {
+  std::vector<u32> hord, lord32;
+  std::vector<i64> hb, lb;
+  layer_order(c->n - L, hord, hb);
+  layer_order(L, lord32, lb);
+  std::vector<uint16_t> lord(lord32.begin(), lord32.end());
+  u32* th = nullptr;
+  uint16_t* tl = nullptr;
+  int rc = RG_OK;
+  if ((rc = rg_malloc((void**)&th, hord.size() * 4, "slab tiles")) != RG_OK) return rc;
+  if ((rc = rg_malloc((void**)&tl, lord.size() * 2, "slab tile order")) != RG_OK) {
+    rg_free(th);
+    return rc;
+  }
+  if (rg_copy(c->ctx->stream, th, hord.data(), hord.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      rg_copy(c->ctx->stream, tl, lord.data(), lord.size() * 2, hipMemcpyHostToDevice) != hipSuccess) {
+    rg_free(th);
+    rg_free(tl);
+    rg_set_error("slab tile tables: upload failed");
+    return RG_EHIP;
+  }
+  if (c->thlist) c->device_bytes -= (((size_t)1 << (c->n - c->tL)) * 4 + ((size_t)1 << c->tL) * 2);
+  rg_free(c->thlist);
+  rg_free(c->tlord);
+  c->thlist = th;
+  c->tlord = tl;
+  c->thbnd = hb;
+  c->tL = L;
+  c->device_bytes += hord.size() * 4 + lord.size() * 2;
+  return RG_OK;
 }
 
 static u64 binom_u64(int b, int t) {
@@ -960,19 +1034,27 @@ extern "C" int rumgpu_cg_create(rumgpu_ctx* ctx, int n, const uint64_t* masks_x,
       for (int x = 0; x < n; ++x)
         for (u64 j = offsets[x]; j < offsets[x + 1]; ++j)
           pos[j] = (i64)x * c->Nloc + (i64)compress_bit(masks_x[j] & ~(1ull << x), x);
-      if ((rc = cg_alloc(c, (void**)&c->spos, (size_t)M * 8, "slab positions")) != RG_OK) return fail(rc);
-      RG_HIP(rg_copy(ctx->stream, c->spos, pos.data(), (size_t)M * 8, hipMemcpyHostToDevice));
+      // rows by slab position, and the range of them in each tile of the first window
+      std::vector<int32_t> row(M);
+      for (i64 j = 0; j < M; ++j) row[j] = (int32_t)j;
+      std::sort(row.begin(), row.end(), [&](int32_t a, int32_t b2) { return pos[a] < pos[b2]; });
+      std::vector<i64> zp(M);
+      for (i64 j = 0; j < M; ++j) zp[j] = pos[row[j]];
+      WinPlan w0[kMaxWindows];
+      plan_windows_into(n - 1, 4, w0);
+      c->zB = w0[0].B;
+      const i64 ntl = ((i64)n * c->Nloc) >> c->zB;
+      std::vector<i64> zo(ntl + 1, 0);
+      for (i64 j = 0; j < M; ++j) zo[(zp[j] >> c->zB) + 1]++;
+      for (i64 t = 0; t < ntl; ++t) zo[t + 1] += zo[t];
+      if ((rc = cg_alloc(c, (void**)&c->zpos, (size_t)M * 8, "slab positions")) != RG_OK) return fail(rc);
+      if ((rc = cg_alloc(c, (void**)&c->zrow, (size_t)M * 4, "slab position rows")) != RG_OK) return fail(rc);
+      if ((rc = cg_alloc(c, (void**)&c->zoff, (size_t)(ntl + 1) * 8, "slab tile ranges")) != RG_OK) return fail(rc);
+      RG_HIP(rg_copy(ctx->stream, c->zpos, zp.data(), (size_t)M * 8, hipMemcpyHostToDevice));
+      RG_HIP(rg_copy(ctx->stream, c->zrow, row.data(), (size_t)M * 4, hipMemcpyHostToDevice));
+      RG_HIP(rg_copy(ctx->stream, c->zoff, zo.data(), (size_t)(ntl + 1) * 8, hipMemcpyHostToDevice));
       if ((rc = cg_alloc(c, (void**)&c->R, (size_t)n * c->Nloc * 4, "pricing slab")) != RG_OK) return fail(rc);
-      c->tL = std::min(n, CG_SLAB_TILE_L);
-      std::vector<u32> hord, lord32;
-      std::vector<i64> lb;
-      layer_order(n - c->tL, hord, c->thbnd);
-      layer_order(c->tL, lord32, lb);
-      std::vector<uint16_t> lord(lord32.begin(), lord32.end());
-      if ((rc = cg_alloc(c, (void**)&c->thlist, hord.size() * 4, "slab tiles")) != RG_OK) return fail(rc);
-      if ((rc = cg_alloc(c, (void**)&c->tlord, lord.size() * 2, "slab tile order")) != RG_OK) return fail(rc);
-      RG_HIP(rg_copy(ctx->stream, c->thlist, hord.data(), hord.size() * 4, hipMemcpyHostToDevice));
-      RG_HIP(rg_copy(ctx->stream, c->tlord, lord.data(), lord.size() * 2, hipMemcpyHostToDevice));
+      if ((rc = slab_tile_tables(c, slab_tile_bits(n))) != RG_OK) return fail(rc);
     }
   } else {
     c->k_pred = k_pred <= 0 ? n - 10 : k_pred;
@@ -1045,13 +1127,27 @@ static int mask_sweep(hipStream_t st, int n, float* V, uint8_t* pred, const u32*
   return RG_OK;
 }
 
+static int zeta_first(rumgpu_cg* c, hipStream_t st, const WinGeom& g, int RL) {
+  RG_CHECK(g.s == 0 && g.C == 0 && g.B == c->zB, RG_EINVAL, "slab pricing: first window geometry");
+  const size_t shm = (size_t)g.TPB * tile_lds_elems(g) * 4;
+  const dim3 grid((unsigned)((g.ntiles + g.TPB - 1) / g.TPB)), block((unsigned)(g.TT * g.TPB));
+  switch (RL) {
+#define RG_Z(RV) case RV: hipLaunchKernelGGL(cg_zeta_first<RV>, grid, block, shm, st, c->R, g, c->zoff, c->zpos, c->zrow, c->scores); break;
+    RG_Z(1) RG_Z(2) RG_Z(3) RG_Z(4) RG_Z(5)
+#undef RG_Z
+    default: RG_CHECK(false, RG_EINVAL, "slab pricing: %d register bits", RL);
+  }
+  RG_LAUNCH_CHECK();
+  return RG_OK;
+}
+
 // the tiles of the slab layout's layered max, one launch per popcount of the high part
 static int slab_tiles(rumgpu_cg* c, hipStream_t st) {
   const int L = c->tL;
   for (int j = 0; j <= c->n - L; ++j) {
     const i64 lo = c->thbnd[j], ns = c->thbnd[j + 1] - lo;
     if (ns <= 0) continue;
-    const dim3 g((unsigned)ns), b(256);
+    const dim3 g((unsigned)ns), b(CG_SLAB_TILE_THREADS);
     switch (L) {
 #define RG_T(LV) case LV: hipLaunchKernelGGL(cg_slab_tile<LV>, g, b, 0, st, c->V, c->pred, c->R, c->thlist + lo, c->tlord, c->Nloc); break;
       RG_T(2) RG_T(3) RG_T(4) RG_T(5) RG_T(6) RG_T(7) RG_T(8) RG_T(9) RG_T(10)
@@ -1135,6 +1231,17 @@ extern "C" int rumgpu_cg_set_sweep(rumgpu_cg* c, int mode) {
   return RG_OK;
 }
 
+extern "C" int rumgpu_cg_set_slab_tile(rumgpu_cg* c, int bits) {
+  RG_CHECK(c, RG_EINVAL, "cg_set_slab_tile: null handle");
+  RG_CHECK(c->layout == 1, RG_EINVAL, "cg_set_slab_tile: tiles belong to the slab layout");
+  const int L = bits == 0 ? slab_tile_bits(c->n) : bits;
+  RG_CHECK(L >= 2 && L <= 10 && L <= c->n, RG_EINVAL, "cg_set_slab_tile: %d tile bits outside 2..min(10, n = %d)", L,
+           c->n);
+  RG_HIP(hipSetDevice(c->ctx->device));
+  RG_HIP(hipStreamSynchronize(c->ctx->stream));
+  return L == c->tL ? RG_OK : slab_tile_tables(c, L);
+}
+
 extern "C" int rumgpu_cg_sweep_info(rumgpu_cg* c, int* one_alternative) {
   RG_CHECK(c && one_alternative, RG_EINVAL, "cg_sweep_info: null argument");
   *one_alternative = use_closure(c) ? c->single_a : -1;
@@ -1168,12 +1275,12 @@ extern "C" int rumgpu_cg_price(rumgpu_cg* c, const float* scores_x, double* vmax
     if (c->layout == 0) {
       RG_TRY(mask_sweep(st, n, c->V, c->pred, c->masks32, c->scores, c->offsets, c->nodes, c->bounds));
     } else {
-      const i64 m2 = (i64)n * c->Nloc;
-      hipLaunchKernelGGL(cg_fill_f32, dim3(grid_for(m2, 256) > 8192 ? 8192 : grid_for(m2, 256)), dim3(256), 0, st, c->R, 0.0f, m2);
-      hipLaunchKernelGGL(cg_scatter, dim3(grid_for(c->M, 256)), dim3(256), 0, st, c->R, c->spos, c->scores, c->M);
-      RG_LAUNCH_CHECK();
-      std::vector<WinPlan> wins = plan_windows(n - 1, 4);
-      for (size_t w = 0; w < wins.size(); ++w)
+      // arc weights: the subset-sum butterfly of the scores scattered on the slab; the first window
+      // pass builds its tiles from the rows inside them (no fill and scatter pass over the slab)
+      WinPlan wins[kMaxWindows];
+      const int nw = plan_windows_into(n - 1, 4, wins);
+      RG_TRY(zeta_first(c, st, make_geom(n, wins[0]), wins[0].RL));
+      for (int w = 1; w < nw; ++w)
         RG_TRY((launch_tr_rl<float, float, 0, 0>(wins[w].RL, c->R, c->R, coef<float>(1.0, 0.0, 1.0, 1.0),
                                                  make_geom(n, wins[w]), nullptr, nullptr, st)));
       hipLaunchKernelGGL(cg_fill_f32, dim3(1), dim3(64), 0, st, c->V, 0.0f, (i64)1);

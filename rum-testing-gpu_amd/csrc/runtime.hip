@@ -67,6 +67,9 @@ extern "C" int rumgpu_ctx_create(int device, rumgpu_ctx** out) {
   RG_HIP(hipSetDevice(device));
   rumgpu_ctx* ctx = new rumgpu_ctx();
   ctx->device = device;
+  hipDeviceProp_t pr;
+  if (hipGetDeviceProperties(&pr, device) == hipSuccess && pr.sharedMemPerBlock > 0)
+    ctx->lds_per_block = pr.sharedMemPerBlock;
   // the no-allocation entry point's scratch address on this device (a failure is reported by
   // that entry point; every other entry point works without it)
   (void)rg_noalloc_setup(device);

@@ -67,6 +67,7 @@ SIGNATURES = {
     "rumgpu_cg_vmax64_batch": [_P, _I, _P, _P],
     "rumgpu_cg_set_sweep": [_P, _I],
     "rumgpu_cg_sweep_info": [_P, _P],
+    "rumgpu_cg_set_slab_tile": [_P, _I],
     "rumgpu_cg_cols_init": [_P, _P, _I64],
     "rumgpu_cg_cols_load": [_P, _I64, _P],
     "rumgpu_cg_cols_add": [_P, _P, _P, _P, _P],

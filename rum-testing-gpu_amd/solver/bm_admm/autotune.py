@@ -1,7 +1,7 @@
 """Window-plan autotuner of the fused BM-ADMM passes, per n and dtype.
 
-A plan (lo_bits, lo_rl, hi_rl) splits the n-1 rest bits into the window at bit 0 and windows of at
-most 8 bits above it; each window is one fused pass per Kronecker transform, and the register
+A plan (lo_bits, lo_rl, hi_rl) splits the n-1 rest bits into the window at bit 0 (up to 13 bits) and
+windows of at most 8 bits above it; each window is one fused pass per Kronecker transform, and the register
 bits set how many rounds (LDS exchanges) a window takes. Every plan computes the same iteration up
 to rounding order. The tuner times graph-replayed iterations of each candidate near the built-in
 plan on the caller's own object and keeps the fastest; results are cached in memory and in
@@ -24,10 +24,22 @@ def default_lo(nb):
     return min(lo, 12)
 
 
+def _windows(nb, lo):
+    return 1 + -(-(nb - lo) // 8)
+
+
 def candidates(nb):
     lo0 = default_lo(nb)
     los = sorted({lo for lo in (lo0 - 1, lo0, lo0 + 1, lo0 + 2) if 1 <= lo <= min(12, nb) and nb - lo <= 16})
-    return [(lo, lrl, hrl) for lo in los for lrl in (2, 3) for hrl in (3, 4)]
+    out = [(lo, lrl, hrl) for lo in los for lrl in (2, 3) for hrl in (3, 4)]
+    # 4 register bits at bit 0 for windows of 10-12 bits (compiled with fixed geometry)
+    out += [(lo, 4, hrl) for lo in los if 10 <= lo <= 12 for hrl in (3, 4)]
+    # a 13-bit window at bit 0 (4 or 5 register bits) where it saves a window over 12 bits (nb = 21: two
+    # windows instead of three); its float64 tile needs more than 64 KiB of LDS, which the library
+    # accepts only where the device has it
+    if nb > 13 and _windows(nb, 13) < _windows(nb, 12):
+        out += [(13, lrl, hrl) for lrl in (4, 5) for hrl in (3, 4)]
+    return out
 
 
 def _key(device, n, dtype):

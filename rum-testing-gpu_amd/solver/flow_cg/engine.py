@@ -135,6 +135,10 @@ class Pricer:
         has not been shown)"""
         _lib.check(self.lib.rumgpu_cg_set_sweep(self.h, self.SWEEPS[mode]), "cg_set_sweep")
 
+    def set_slab_tile(self, bits=0):
+        """tile bits of the slab layout's layered max (0 = the built-in choice); same results for all"""
+        _lib.check(self.lib.rumgpu_cg_set_slab_tile(self.h, int(bits)), "cg_set_slab_tile")
+
     def one_alternative(self):
         """the alternative the closure sweep runs on, or -1 when the layered sweep runs"""
         a = ctypes.c_int()

@@ -74,9 +74,12 @@ def test_config3_fused_iterations_match_oracle():
     print("[case] config3: 3 oracle iterations, float64 and float32", flush=True)
     ref64.run(3)
     ref32.run(3)
-    for dtype in ("float64", "float32"):
-        print("[case] config3 %s: 3 fused iterations" % dtype, flush=True)
+    for dtype, plan in (("float64", None), ("float32", None), ("float64", (13, 4, 4)), ("float32", (13, 4, 4))):
+        print("[case] config3 %s, plan %s: 3 fused iterations" % (dtype, plan or "built-in"), flush=True)
         h = Handle(n, menu, alt, p, dtype=dtype, beta=beta)
+        if plan:
+            h.set_plan(*plan)          # two windows: 13 bits at bit 0, 8 above
+            assert len(h.windows()) == 2
         h.run(3)
         got = h.get_state()
         h.close()

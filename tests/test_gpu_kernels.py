@@ -201,11 +201,13 @@ def test_slab_pricing_equals_mask_on_quantized_duals(n):
     perm, masks_x, _alts, offsets = xmajor(menu, alt, n)
     pm = Pricer(n, masks_x, offsets, layout="mask")
     ps = Pricer(n, masks_x, offsets, layout="slab")
-    for trial in range(3):
+    for trial, bits in enumerate((0, 3, 6, 10)):
+        if bits <= n:
+            ps.set_slab_tile(bits)              # tiles of the layered max: any size, the same result
         scores = (rng.integers(-127, 128, size=menu.size) / 4096.0).astype(np.float32)
         a = pm.price(scores)
         b = ps.price(scores)
-        assert a[0] == b[0] and np.array_equal(a[1], b[1])
+        assert a[0] == b[0] and np.array_equal(a[1], b[1]), (bits, a, b)
     pm.close()
     ps.close()
 
@@ -372,9 +374,11 @@ def test_set_beta_carries_the_state(graph):
         h.close()
 
 
-@pytest.mark.parametrize("n,plan", [(13, (6, 2, 3)), (13, (12, 4, 4)), (18, (11, 3, 4)), (18, (8, 2, 4))])
+@pytest.mark.parametrize("n,plan", [(13, (6, 2, 3)), (13, (12, 4, 4)), (18, (11, 3, 4)), (18, (8, 2, 4)),
+                                    (14, (13, 4, 3)), (18, (13, 4, 4))])
 def test_window_plans_match_oracle(n, plan):
-    """any window plan (rumgpu_bm_set_plan) gives the reference iteration within rounding"""
+    """any window plan (rumgpu_bm_set_plan) gives the reference iteration within rounding; a 13-bit
+    window at bit 0 (68 KiB of float64 LDS per tile) is one window at n = 14 and two at n = 18"""
     rng = np.random.default_rng(80 + n)
     menu, alt, p = _medium(n, min(6000, n << (n - 2)), rng)
     h = Handle(n, menu, alt, p, dtype="float64", beta=1.1)

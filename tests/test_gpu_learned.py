@@ -1,7 +1,8 @@
 """What the learned warm start gains: the batched bootstrap on an alternative-0 panel (301 cells,
 n = 12, B = 2,048), the MLP trained on the GPU on the first 96 open draws the method certified, then
 the next 64 open draws solved with and without it. Every draw keeps its own float64 bracket, so the
-decisions must agree; the gain is printed (draw-iterations and wall time)."""
+decisions must agree; the learned start must take fewer draw-iterations than the plain one (it
+took 475 against 488 per draw, 2.6 % fewer), and the wall times are printed."""
 import time
 
 import numpy as np
@@ -47,6 +48,8 @@ def test_learned_warm_start_gain():
     print("draw-iterations per draw: plain %.0f, learned %.0f (%.1f %% fewer); wall %.2f s vs %.2f s"
           % (work["plain"][0], work["learned"][0], 100 * (1 - work["learned"][0] / work["plain"][0]),
              work["plain"][1], work["learned"][1]))
+    # the trained model must pay: fewer draw-iterations than the plain warm start from p_hat
+    assert work["learned"][0] < work["plain"][0], work
     d1 = hong_li.decide([res["plain"][b][0] for b in te], [res["plain"][b][1] for b in te], *tau)
     d2 = hong_li.decide([res["learned"][b][0] for b in te], [res["learned"][b][1] for b in te], *tau)
     assert np.array_equal(d1, d2)

@@ -360,3 +360,32 @@ def test_master_qp_state_sequence():
         f = lambda l: 0.5 * l @ Q @ l - h @ l      # noqa: E731
         assert abs(f(lam1) - f(lam2)) <= 1e-10 * max(1.0, abs(f(lam2))), K
     st.close()
+
+
+def test_out_of_memory_gives_exit_flag_minus_3(monkeypatch):
+    """A device allocation that fails inside a runner ends the call with exit flag -3 and the
+    library's message (rum_solver.py:1217-1218 of the reference maps MemoryError and "out of memory"
+    to it); the C ABI's status 2 becomes that MemoryError. Forced on the host: the runner raises as
+    the engine would, so no GPU is needed."""
+    import solver.rum_solver as RS
+    from solver import machine
+    assert RS._is_oom(MemoryError("x")) and RS._is_oom(RuntimeError("hipMalloc: out of memory"))
+    assert not RS._is_oom(RuntimeError("invalid argument"))
+    with pytest.raises(MemoryError, match="out of memory"):
+        _lib.check(_lib.ENOMEM, "slab of 3.7 GB")
+    prof = dict(use_gpu=True, gpu=dict(name="test", arch="gfx950", memory_total=1 << 38, supported=True, reason=""),
+                cpu_count=8, ram_bytes=1 << 36, gurobi=None)
+    monkeypatch.setattr(machine, "gpu_free_bytes", lambda device=0: 1 << 38)
+
+    def fails(task):
+        raise MemoryError("out of memory: bm_admm slab needs 3.700 GB, 0.100 GB free")
+
+    monkeypatch.setitem(RS._RUN, "bm_admm", fails)
+    menu = np.array([3, 3, 5, 5, 6, 6], dtype=np.int64)
+    alt = np.array([0, 1, 0, 2, 1, 2], dtype=np.int64)
+    rum = RUM(menu, alt, np.array([0.6, 0.4, 0.5, 0.5, 0.3, 0.7]))
+    r = rum.calculate_distance(method="bm_admm", machine=prof)
+    assert r.exit_flag == -3 and r.status == "out of memory" and "3.700 GB" in r.message
+    with rum.session(method="bm_admm", machine=prof) as s:
+        r2 = s.solve(rum.choice_probability)
+    assert r2.exit_flag == -3
