@@ -86,6 +86,8 @@ def _parse():
                     help="only slab pricing sweeps on one instance (n20_d1pct, n25_sparse or config3), "
                          "--sweeps of them: the workload of the slab pricing profile")
     ap.add_argument("--sweeps", type=int, default=20)
+    ap.add_argument("--slab-tile", type=int, default=0,
+                    help="with --pricing-sweeps: low elements per layered-max tile (2..10; 0 = the built-in 7)")
     ap.add_argument("--summarize-stats", default="",
                     help="print the per-kernel table of a rocprofv3 --kernel-trace --stats database (rocpd .db)")
     ap.add_argument("--summarize-traffic", nargs=3, metavar=("FETCH_DB", "WRITE_DB", "COPY_BYTES"),
@@ -135,7 +137,7 @@ def main():
                 json.dump(per_kernel, f, indent=1)
         return
     if a.pricing_sweeps:
-        _pricing_sweeps(a.pricing_sweeps, a.sweeps)
+        _pricing_sweeps(a.pricing_sweeps, a.sweeps, a.slab_tile)
         return
     if a.config3_only:
         print(json.dumps(_config3_bench(a)), flush=True)
@@ -630,7 +632,7 @@ def _ai_bench():
     return out
 
 
-def _pricing_sweeps(key, sweeps):
+def _pricing_sweeps(key, sweeps, tile_bits=0):
     """Slab-layout pricing sweeps on one instance (random float32 scores), for a kernel profile."""
     import numpy as np
     from solver import instances
@@ -644,6 +646,7 @@ def _pricing_sweeps(key, sweeps):
     perm, masks_x, _a, offsets = xmajor(menu, alt, n)
     rng = np.random.default_rng(1)
     pr = Pricer(n, masks_x, offsets, layout="slab")
+    pr.set_slab_tile(tile_bits)
     S = rng.standard_normal((2, menu.size)).astype(np.float32)
     pr.price(S[0])
     t0 = time.perf_counter()
@@ -651,8 +654,8 @@ def _pricing_sweeps(key, sweeps):
         pr.price(S[k % 2])
     dt = (time.perf_counter() - t0) / sweeps
     pr.close()
-    print("[bench] slab pricing %s (n=%d, %d rows): %.3f ms per sweep over %d sweeps"
-          % (key, n, menu.size, dt * 1e3, sweeps), flush=True)
+    print("[bench] slab pricing %s (n=%d, %d rows, tile bits %s): %.3f ms per sweep over %d sweeps"
+          % (key, n, menu.size, tile_bits or "7", dt * 1e3, sweeps), flush=True)
 
 
 def _summarize_stats(db):
