@@ -138,7 +138,13 @@ struct Tile {
   // start_last: begin in the last round's layout, where a window at bit 0 reads contiguous memory
   // across the lanes (the first round's layout gives each thread R consecutive values instead);
   // the first transform then runs its rounds in descending order.
-  __device__ __forceinline__ void init(const WinGeom& g, T* lds_all, bool start_last = false) {
+  //
+  // item_inner: the tiles are taken in (rest tile, item) order with the item fastest, and each of
+  // the 8 XCDs (which receive workgroups round robin) gets one contiguous run of that order, so
+  // the items of one rest tile run back to back on one XCD and share its L2 for data indexed by
+  // the rest alone (the D pass's inv_gl).
+  __device__ __forceinline__ void init(const WinGeom& g, T* lds_all, bool start_last = false,
+                                       bool item_inner = false) {
     s = g.s;
     B = BT > 0 ? BT : g.B;
     C = CT >= 0 ? CT : g.C;
@@ -146,11 +152,23 @@ struct Tile {
     cur = (start_last && C == 0) ? nr - 1 : 0;
     const int tl = threadIdx.x / g.TT;
     const int t = threadIdx.x - tl * g.TT;
-    const i64 tile = (i64)blockIdx.x * g.TPB + tl;
-    active = tile < g.ntiles;
     const int outer_bits = g.nb - B - C;
-    const i64 x = tile >> outer_bits;
-    const i64 o = tile & ((1ll << outer_bits) - 1);
+    i64 x, o;
+    if (item_inner) {
+      const u32 per = gridDim.x >> 3;
+      u32 blk = blockIdx.x;
+      if (blk < per * 8u) blk = (blk & 7u) * per + (blk >> 3);
+      const i64 lin = (i64)blk * g.TPB + tl;
+      const i64 nitems = g.ntiles >> outer_bits;
+      active = lin < g.ntiles;
+      o = lin / nitems;
+      x = lin - o * nitems;
+    } else {
+      const i64 tile = (i64)blockIdx.x * g.TPB + tl;
+      active = tile < g.ntiles;
+      x = tile >> outer_bits;
+      o = tile & ((1ll << outer_bits) - 1);
+    }
     const int lowo = s - C;
     const i64 lowpart = o & ((1ll << lowo) - 1);
     const i64 highpart = o >> lowo;

@@ -52,7 +52,7 @@ template <typename T, int RL, int KIND, int BT = 0, int CT = -1>
 __global__ __launch_bounds__(512) void bm_pass(PassArgs<T> P) {
   extern __shared__ __align__(16) unsigned char smem[];
   Tile<T, RL, BT, CT> t;
-  t.init(P.g, reinterpret_cast<T*>(smem), true);
+  t.init(P.g, reinterpret_cast<T*>(smem), true, KIND == K_D);
   const T one = (T)1;
   constexpr int R = Tile<T, RL, BT, CT>::R;
   // every global access happens in a lane-contiguous layout (t.coalesce() before it)
