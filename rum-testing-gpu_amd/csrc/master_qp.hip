@@ -11,10 +11,150 @@
 #include "rumgpu_internal.hpp"
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cmath>
+#include <condition_variable>
+#include <cstdlib>
+#include <functional>
+#include <mutex>
+#include <thread>
 #include <vector>
 
 namespace {
+
+// Host threads for the gradient of the inactive columns, the one loop of a master solve that grows
+// with K x k. The columns are cut into contiguous parts and every g_j is computed whole by one
+// thread, in the same order as the single-threaded loop, so the results do not depend on the
+// thread count. Threads: OMP_NUM_THREADS if set, else the hardware's, at most 16; created at
+// the first large gradient and kept until the process ends. A worker polls for the next run for
+// about a millisecond before it sleeps (a column generation's masters come a fraction of a
+// millisecond apart, and waking a sleeping thread costs about as much as one gradient).
+class HostPool {
+ public:
+  static HostPool& get() {
+    static HostPool pool(threads_wanted());
+    return pool;
+  }
+  int size() const { return (int)workers_.size() + 1; }
+  // f(part) for every part in [0, parts), on the pool's threads and the caller's
+  void run(int parts, const std::function<void(int)>& f) {
+    std::lock_guard<std::mutex> one(run_mu_);
+    std::unique_lock<std::mutex> lk(mu_);
+    fn_ = &f;
+    parts_ = parts;
+    next_ = 0;
+    remaining_ = parts;
+    gen_.fetch_add(1, std::memory_order_release);
+    cv_work_.notify_all();
+    drain(lk);
+    cv_done_.wait(lk, [&] { return remaining_ == 0; });
+    fn_ = nullptr;
+  }
+  ~HostPool() {
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      stop_ = true;
+    }
+    cv_work_.notify_all();
+    for (auto& w : workers_) w.join();
+  }
+
+ private:
+  explicit HostPool(int nt) {
+    for (int t = 1; t < nt; ++t) workers_.emplace_back([this] { loop(); });
+  }
+  static int threads_wanted() {
+    int nt = (int)std::thread::hardware_concurrency();
+    if (const char* e = std::getenv("OMP_NUM_THREADS")) {
+      const int v = std::atoi(e);
+      if (v > 0) nt = v;
+    }
+    return std::max(1, std::min(nt, 16));
+  }
+  void loop() {
+    unsigned long long seen = 0;
+    std::unique_lock<std::mutex> lk(mu_);
+    for (;;) {
+      if (!stop_ && gen_.load(std::memory_order_relaxed) == seen) {
+        lk.unlock();
+        const auto t0 = std::chrono::steady_clock::now();
+        for (int spin = 0; gen_.load(std::memory_order_acquire) == seen; ++spin) {
+          if ((spin & 1023) == 1023 && std::chrono::steady_clock::now() - t0 > std::chrono::microseconds(1000)) break;
+          std::this_thread::yield();
+        }
+        lk.lock();
+      }
+      cv_work_.wait(lk, [&] { return stop_ || gen_.load(std::memory_order_relaxed) != seen; });
+      if (stop_) return;
+      seen = gen_.load(std::memory_order_relaxed);
+      drain(lk);
+    }
+  }
+  // take parts of the current run until none is left (lock held on entry and exit)
+  void drain(std::unique_lock<std::mutex>& lk) {
+    while (next_ < parts_) {
+      const int p = next_++;
+      lk.unlock();
+      (*fn_)(p);
+      lk.lock();
+      if (--remaining_ == 0) cv_done_.notify_all();
+    }
+  }
+  std::mutex run_mu_, mu_;
+  std::condition_variable cv_work_, cv_done_;
+  std::vector<std::thread> workers_;
+  const std::function<void(int)>* fn_ = nullptr;
+  int parts_ = 0, next_ = 0, remaining_ = 0;
+  std::atomic<unsigned long long> gen_{0};
+  bool stop_ = false;
+};
+
+// g_j for the inactive columns j in [j0, j1): mu - h_j + sum_i Q[j, work_i] lam_i, by whole rows of
+// Q (symmetric), whichever are fewer: the inactive rows (a dot product with lam spread over all K
+// columns, in four partial sums) or the k working rows (axpys into every g_j, in working-set order)
+template <typename TQ>
+void grad_range(int j0, int j1, int K, int k, const TQ* Q, int64_t ldq, const double* h, const char* in,
+                const int* work, const double* lam_w, const double* lam_full, double mu, double* grad) {
+  if (K - k < k) {
+    for (int j = j0; j < j1; ++j) {
+      if (in[j]) continue;
+      const TQ* qj = Q + (int64_t)j * ldq;
+      double pp[4] = {0.0, 0.0, 0.0, 0.0};
+      int t = 0;
+      for (; t + 4 <= K; t += 4)
+        for (int u = 0; u < 4; ++u) pp[u] += (double)qj[t + u] * lam_full[t + u];
+      for (int u = 0; t < K; ++t, ++u) pp[u] += (double)qj[t] * lam_full[t];
+      grad[j] = (mu - h[j]) + ((pp[0] + pp[1]) + (pp[2] + pp[3]));
+    }
+  } else {
+    double* __restrict__ g = grad;
+    for (int j = j0; j < j1; ++j) g[j] = mu - h[j];
+    // four working rows per sweep over g: each g_j still takes its products in working-set order,
+    // one rounded add at a time, but is loaded and stored once per four rows
+    int i = 0;
+    for (; i + 4 <= k; i += 4) {
+      const TQ* __restrict__ q0 = Q + (int64_t)work[i] * ldq;
+      const TQ* __restrict__ q1 = Q + (int64_t)work[i + 1] * ldq;
+      const TQ* __restrict__ q2 = Q + (int64_t)work[i + 2] * ldq;
+      const TQ* __restrict__ q3 = Q + (int64_t)work[i + 3] * ldq;
+      const double l0 = lam_w[i], l1 = lam_w[i + 1], l2 = lam_w[i + 2], l3 = lam_w[i + 3];
+      for (int j = j0; j < j1; ++j) {
+        double t = g[j];
+        t += (double)q0[j] * l0;
+        t += (double)q1[j] * l1;
+        t += (double)q2[j] * l2;
+        t += (double)q3[j] * l3;
+        g[j] = t;
+      }
+    }
+    for (; i < k; ++i) {
+      const TQ* __restrict__ qi = Q + (int64_t)work[i] * ldq;
+      const double li = lam_w[i];
+      for (int j = j0; j < j1; ++j) g[j] += (double)qi[j] * li;
+    }
+  }
+}
 
 struct Chol {
   int k = 0;
@@ -234,7 +374,68 @@ struct rumgpu_qp {
   // the last solve on that working set and factor: its right-hand side h_W, Q_WW^-1 (h_W, 1), and
   // the forward solutions L^-1 h_W and L^-1 1
   std::vector<double> bh, qh, qe, yh, ye;
+  // Q in float32 for the gradient's row reads (half the bytes). A column generation's Q holds
+  // integer overlap counts, which float32 holds exactly, so the products are the same doubles;
+  // the copy is off for good once an entry is not a float32. Rows and columns 0..k32-1 are copied
+  // from the Q at q_src (ldq q_ld); a call with another Q, or whose last copied row differs from
+  // the copy, copies it all again. New rows are copied whole; the new columns of the old rows are
+  // taken from the new rows (Q is symmetric), so every read of Q is a contiguous row.
+  std::vector<float> q32;
+  int64_t ld32 = 0;
+  int k32 = 0;
+  const double* q_src = nullptr;
+  int64_t q_ld = 0;
+  bool q32_ok = true;
 };
+
+// the state's float32 copy of Q brought up to K x K (null when Q is not all float32)
+static const float* sync_q32(rumgpu_qp* s, int K, const double* Q, int64_t ldq) {
+  if (!s->q32_ok) return nullptr;
+  int from = (s->q_src == Q && s->q_ld == ldq && s->k32 <= K) ? s->k32 : 0;
+  if (from > 0) {
+    const float* cr = s->q32.data() + (int64_t)(from - 1) * s->ld32;
+    const double* qr = Q + (int64_t)(from - 1) * ldq;
+    for (int j = 0; j < from; ++j)
+      if ((double)cr[j] != qr[j]) {
+        from = 0;
+        break;
+      }
+  }
+  if (K > s->ld32) {
+    const int64_t nl = std::max<int64_t>(K, 2 * s->ld32) + 16;
+    std::vector<float> nq((size_t)(nl * nl));
+    for (int i = 0; i < from; ++i)
+      std::copy(s->q32.begin() + (size_t)(i * s->ld32), s->q32.begin() + (size_t)(i * s->ld32 + from),
+                nq.begin() + (size_t)(i * nl));
+    s->q32.swap(nq);
+    s->ld32 = nl;
+  }
+  float* c = s->q32.data();
+  bool exact = true;
+  for (int i = from; i < K; ++i) {
+    const double* qi = Q + (int64_t)i * ldq;
+    float* ci = c + (int64_t)i * s->ld32;
+    for (int j = 0; j < K; ++j) {
+      const float f = (float)qi[j];
+      exact &= (double)f == qi[j];
+      ci[j] = f;
+    }
+  }
+  for (int i = 0; i < from; ++i) {
+    float* ci = c + (int64_t)i * s->ld32;
+    for (int j = from; j < K; ++j) ci[j] = c[(int64_t)j * s->ld32 + i];
+  }
+  s->q_src = Q;
+  s->q_ld = ldq;
+  s->k32 = K;
+  if (!exact) {
+    s->q32_ok = false;
+    std::vector<float>().swap(s->q32);
+    s->ld32 = s->k32 = 0;
+    return nullptr;
+  }
+  return c;
+}
 
 extern "C" int rumgpu_qp_create(rumgpu_qp** out) {
   RG_CHECK(out, RG_EINVAL, "qp_create: null argument");
@@ -250,6 +451,9 @@ extern "C" int rumgpu_qp_destroy(rumgpu_qp* q) {
 static int qp_simplex_impl(int K, const double* Q, int64_t ldq, const double* h, const int32_t* A_init, int nA,
                            int max_outer, double tol, double reg, double* lam_out, int32_t* support, int* n_support,
                            rumgpu_qp* state);
+
+// below this many multiply-adds a gradient stays on the calling thread (a pool round costs ~10 us)
+static constexpr double kParallelFma = 131072.0;
 
 extern "C" int rumgpu_qp_simplex(int K, const double* Q, int64_t ldq, const double* h, const int32_t* A_init, int nA,
                                  int max_outer, double tol, double reg, double* lam_out, int32_t* support,
@@ -268,6 +472,8 @@ static int qp_simplex_impl(int K, const double* Q, int64_t ldq, const double* h,
                            int max_outer, double tol, double reg, double* lam_out, int32_t* support, int* n_support,
                            rumgpu_qp* state) {
   RG_CHECK(K >= 1 && Q && h && lam_out && support && n_support && ldq >= K, RG_EINVAL, "qp_simplex: bad argument");
+  const float* q32 = state ? sync_q32(state, K, Q, ldq) : nullptr;
+  const int64_t ld32 = q32 ? state->ld32 : 0;
   std::vector<int> work;
   std::vector<char> in(K, 0);
   for (int i = 0; i < nA; ++i) {
@@ -281,7 +487,6 @@ static int qp_simplex_impl(int K, const double* Q, int64_t ldq, const double* h,
     work.push_back(j), in[j] = 1;
   }
   Chol ch;
-  ch.reserve(std::min(K, std::max(64, (int)work.size() + 16)));
   std::vector<double> a, qh(K), qe(K), bh(K), be(K, 1.0), lam_w, grad, lam_full;
   std::vector<int> solved;     // the working set lam_w belongs to
   auto rebuild = [&]() {
@@ -326,6 +531,7 @@ static int qp_simplex_impl(int K, const double* Q, int64_t ldq, const double* h,
       fwd = true;
     }
   } else {
+    ch.reserve(std::min(K, std::max(64, (int)work.size() + 16)));   // room for the columns to come
     rebuild();
   }
   double mu = 0.0;
@@ -373,23 +579,26 @@ static int qp_simplex_impl(int K, const double* Q, int64_t ldq, const double* h,
       if (K - k < k) {
         lam_full.assign(K, 0.0);
         for (int i = 0; i < k; ++i) lam_full[work[i]] = lam_w[i];
-        for (int j = 0; j < K; ++j) {
-          if (in[j]) continue;
-          const double* qj = Q + (int64_t)j * ldq;
-          double pp[4] = {0.0, 0.0, 0.0, 0.0};
-          int t = 0;
-          for (; t + 4 <= K; t += 4)
-            for (int u = 0; u < 4; ++u) pp[u] += qj[t + u] * lam_full[t + u];
-          for (int u = 0; t < K; ++t, ++u) pp[u] += qj[t] * lam_full[t];
-          grad[j] = (mu - h[j]) + ((pp[0] + pp[1]) + (pp[2] + pp[3]));
-        }
+      }
+      const double work_fma = (double)K * (double)std::min(k, K - k);
+      const int parts = work_fma < kParallelFma ? 1 : std::min(HostPool::get().size(), K / 128);
+      auto part = [&](int j0, int j1) {
+        if (q32)
+          grad_range<float>(j0, j1, K, k, q32, ld32, h, in.data(), work.data(), lam_w.data(), lam_full.data(), mu,
+                            grad.data());
+        else
+          grad_range<double>(j0, j1, K, k, Q, ldq, h, in.data(), work.data(), lam_w.data(), lam_full.data(), mu,
+                             grad.data());
+      };
+      if (parts <= 1) {
+        part(0, K);
       } else {
-        for (int j = 0; j < K; ++j) grad[j] = mu - h[j];
-        for (int i = 0; i < k; ++i) {
-          const double* qi = Q + (int64_t)work[i] * ldq;
-          const double li = lam_w[i];
-          for (int j = 0; j < K; ++j) grad[j] += qi[j] * li;
-        }
+        // part boundaries on multiples of 16 columns (whole cache lines of grad and of the rows)
+        const int chunk = ((K + parts - 1) / parts + 15) & ~15;
+        HostPool::get().run(parts, [&](int p) {
+          const int j0 = std::min(K, p * chunk), j1 = std::min(K, j0 + chunk);
+          part(j0, j1);
+        });
       }
       int jbest = -1;
       double gbest = 0.0;

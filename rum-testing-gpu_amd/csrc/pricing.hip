@@ -533,24 +533,55 @@ __global__ __launch_bounds__(CG_SLAB_TILE_THREADS) void cg_slab_tile(float* __re
     const int x = e / H, cidx = e - x * H;
     rl[x][cidx] = R[(i64)x * Nloc + (i64)(hm >> 1) + cidx];
   }
-  // the high members, ascending, for every subset of the tile
-  for (int l = tid; l < T; l += NT) {
-    const u32 D = hm | (u32)l;
-    float b = 0.0f;
-    int bx = 0;
-    bool seen = false;
-    for (u32 S = h; S; S &= S - 1) {
-      const int x = L + __builtin_ctz(S);
-      const u32 P = D ^ (1u << x);
-      const float cand = V[P] + R[(i64)x * Nloc + (i64)compress_bit(P, x)];
-      if (!seen || cand > b) {
-        b = cand;
-        bx = x;
-        seen = true;
+  // the high members, ascending, for every subset of the tile. They are the tile's high part, the
+  // same for every lane, so each step takes four members and issues the V and arc-weight loads of
+  // all the lane's subsets (a contiguous run across the lanes for each member) before folding them
+  // in member order (strict >: ties keep the lowest member)
+  constexpr int PER = T > NT ? T / NT : 1;
+  float hb[PER];
+  int hbx[PER];
+#pragma unroll
+  for (int q = 0; q < PER; ++q) hb[q] = 0.0f, hbx[q] = 0;
+  const bool on = T >= NT || tid < T;
+  bool seen = false;
+  for (u32 S = h; S;) {
+    int xs[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      xs[m] = S ? L + __builtin_ctz(S) : -1;
+      S &= S - 1;
+    }
+    float cand[4][PER];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+#pragma unroll
+      for (int q = 0; q < PER; ++q) {
+        cand[m][q] = 0.0f;
+        if (xs[m] >= 0 && on) {
+          const u32 P = (hm | (u32)(tid + q * NT)) ^ (1u << xs[m]);
+          cand[m][q] = V[P] + R[(i64)xs[m] * Nloc + (i64)compress_bit(P, xs[m])];
+        }
       }
     }
-    hv[l] = b;
-    hx[l] = (uint8_t)bx;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      if (xs[m] < 0) continue;
+#pragma unroll
+      for (int q = 0; q < PER; ++q) {
+        if (!seen || cand[m][q] > hb[q]) {
+          hb[q] = cand[m][q];
+          hbx[q] = xs[m];
+        }
+      }
+      seen = true;
+    }
+  }
+  if (on) {
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      hv[tid + q * NT] = hb[q];
+      hx[tid + q * NT] = (uint8_t)hbx[q];
+    }
   }
   __syncthreads();
   if (tid == 0) vt[0] = h ? hv[0] : 0.0f;       // the tile's own empty low part (V[empty set] = 0)
@@ -881,9 +912,21 @@ struct rumgpu_cg {
   u64* suboffsets = nullptr;
   size_t device_bytes = 0;
   double last_ms = 0.0;
+  // the layered sweep (mask or slab layout) captured once and replayed per sweep; rebuilt when the
+  // slab tiles change
+  hipGraph_t sgraph = nullptr;
+  hipGraphExec_t sgexec = nullptr;
 };
 
+static void sweep_graph_drop(rumgpu_cg* c) {
+  if (c->sgexec) (void)hipGraphExecDestroy(c->sgexec);
+  if (c->sgraph) (void)hipGraphDestroy(c->sgraph);
+  c->sgexec = nullptr;
+  c->sgraph = nullptr;
+}
+
 static void cg_free_all(rumgpu_cg* c) {
+  sweep_graph_drop(c);
   void* bufs[] = {c->masks32, c->masks64, c->offsets, c->scores, c->zpos, c->zrow, c->zoff, c->V, c->pred, c->nodes, c->R, c->rank,
                   c->binom, c->vbuf[0], c->vbuf[1], c->lbuf[0], c->lbuf[1], c->mbuf[0], c->mbuf[1], c->ebuf[0],
                   c->ebuf[1], c->lstep, c->subV, c->subpred, c->subnodes, c->submasks, c->subscores, c->suboffsets,
@@ -1239,12 +1282,62 @@ extern "C" int rumgpu_cg_set_slab_tile(rumgpu_cg* c, int bits) {
            c->n);
   RG_HIP(hipSetDevice(c->ctx->device));
   RG_HIP(hipStreamSynchronize(c->ctx->stream));
-  return L == c->tL ? RG_OK : slab_tile_tables(c, L);
+  if (L == c->tL) return RG_OK;
+  sweep_graph_drop(c);
+  return slab_tile_tables(c, L);
 }
 
 extern "C" int rumgpu_cg_sweep_info(rumgpu_cg* c, int* one_alternative) {
   RG_CHECK(c && one_alternative, RG_EINVAL, "cg_sweep_info: null argument");
   *one_alternative = use_closure(c) ? c->single_a : -1;
+  return RG_OK;
+}
+
+// the layered sweep from the scores on the device to the ranking in c->rank (V of the full set in
+// c->V[NN - 1]): mask layout, or the slab layout's butterfly and tiles
+static int layered_sweep(rumgpu_cg* c, hipStream_t st) {
+  const int n = c->n;
+  if (c->layout == 0) {
+    RG_TRY(mask_sweep(st, n, c->V, c->pred, c->masks32, c->scores, c->offsets, c->nodes, c->bounds));
+  } else {
+    // arc weights: the subset-sum butterfly of the scores scattered on the slab; the first window
+    // pass builds its tiles from the rows inside them (no fill and scatter pass over the slab)
+    WinPlan wins[kMaxWindows];
+    const int nw = plan_windows_into(n - 1, 4, wins);
+    RG_TRY(zeta_first(c, st, make_geom(n, wins[0]), wins[0].RL));
+    for (int w = 1; w < nw; ++w)
+      RG_TRY((launch_tr_rl<float, float, 0, 0>(wins[w].RL, c->R, c->R, coef<float>(1.0, 0.0, 1.0, 1.0),
+                                               make_geom(n, wins[w]), nullptr, nullptr, st)));
+    hipLaunchKernelGGL(cg_fill_f32, dim3(1), dim3(64), 0, st, c->V, 0.0f, (i64)1);
+    RG_TRY(slab_tiles(c, st));
+  }
+  hipLaunchKernelGGL(cg_backtrace, dim3(1), dim3(64), 0, st, c->pred, n, c->rank, (u64)(c->NN - 1));
+  RG_LAUNCH_CHECK();
+  return RG_OK;
+}
+
+// 15-40 launches per sweep, most of them short at n <= 20: one graph launch instead (its kernels
+// and arguments do not change between sweeps; only the scores buffer's contents do)
+static int launch_layered_sweep(rumgpu_cg* c, hipStream_t st) {
+  if (!c->sgexec) {
+    RG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    const int rc = layered_sweep(c, st);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(st, &g);
+    if (rc != RG_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    RG_CHECK(e == hipSuccess, RG_EHIP, "cg pricing: sweep capture failed: %s", hipGetErrorString(e));
+    c->sgraph = g;
+    const hipError_t ei = hipGraphInstantiate(&c->sgexec, g, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
+      c->sgexec = nullptr;
+      sweep_graph_drop(c);
+      RG_CHECK(false, RG_EHIP, "cg pricing: sweep graph: %s", hipGetErrorString(ei));
+    }
+  }
+  RG_HIP(hipGraphLaunch(c->sgexec, st));
   return RG_OK;
 }
 
@@ -1272,21 +1365,7 @@ extern "C" int rumgpu_cg_price(rumgpu_cg* c, const float* scores_x, double* vmax
     return RG_OK;
   }
   if (c->layout == 0 || c->layout == 1) {
-    if (c->layout == 0) {
-      RG_TRY(mask_sweep(st, n, c->V, c->pred, c->masks32, c->scores, c->offsets, c->nodes, c->bounds));
-    } else {
-      // arc weights: the subset-sum butterfly of the scores scattered on the slab; the first window
-      // pass builds its tiles from the rows inside them (no fill and scatter pass over the slab)
-      WinPlan wins[kMaxWindows];
-      const int nw = plan_windows_into(n - 1, 4, wins);
-      RG_TRY(zeta_first(c, st, make_geom(n, wins[0]), wins[0].RL));
-      for (int w = 1; w < nw; ++w)
-        RG_TRY((launch_tr_rl<float, float, 0, 0>(wins[w].RL, c->R, c->R, coef<float>(1.0, 0.0, 1.0, 1.0),
-                                                 make_geom(n, wins[w]), nullptr, nullptr, st)));
-      hipLaunchKernelGGL(cg_fill_f32, dim3(1), dim3(64), 0, st, c->V, 0.0f, (i64)1);
-      RG_TRY(slab_tiles(c, st));
-    }
-    hipLaunchKernelGGL(cg_backtrace, dim3(1), dim3(64), 0, st, c->pred, n, c->rank, (u64)(c->NN - 1));
+    RG_TRY(launch_layered_sweep(c, st));
     float vf = 0.0f;
     RG_HIP(hipMemcpyAsync(&vf, c->V + (c->NN - 1), 4, hipMemcpyDeviceToHost, st));
     RG_HIP(hipMemcpyAsync(rank_pos, c->rank, (size_t)n * 4, hipMemcpyDeviceToHost, st));

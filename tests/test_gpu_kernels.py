@@ -212,6 +212,32 @@ def test_slab_pricing_equals_mask_on_quantized_duals(n):
     ps.close()
 
 
+@pytest.mark.parametrize("layout", ["mask", "slab"])
+@pytest.mark.parametrize("n", [9, 16])
+def test_sweep_graph_replays_match_oracle(n, layout):
+    """The layered sweep is captured at a pricer's first sweep and replayed after: eight replays on
+    alternating scores, with eager device work (float64 lower-bound sweeps) and host syncs between
+    them and a change of slab tiles (a new capture) halfway, each equal to the oracle's mask sweep
+    (quantized scores for the slab layout, where every float32 partial sum is exact)."""
+    rng = np.random.default_rng(300 + n)
+    menu, alt, p = _medium(n, min(6000, n << (n - 3)), rng)
+    perm, masks_x, _alts, offsets = xmajor(menu, alt, n)
+    pr = Pricer(n, masks_x, offsets, layout=layout)
+    pm = Pricer(n, masks_x, offsets, layout="mask")       # its float64 sweeps: eager work on the stream
+    S = [(rng.integers(-127, 128, size=menu.size) / 4096.0).astype(np.float32) for _ in range(2)]
+    if layout == "mask":
+        S[1] = rng.standard_normal(menu.size).astype(np.float32)
+    ref = [O.price_mask(n, masks_x, s, offsets)[:2] for s in S]
+    for rep in range(8):
+        if layout == "slab" and rep == 4:
+            pr.set_slab_tile(5)
+        vmax, rank, _m = pr.price(S[rep % 2])
+        assert vmax == ref[rep % 2][0] and np.array_equal(rank, ref[rep % 2][1]), (rep, vmax, ref[rep % 2][0])
+        pm.vmax64(rng.standard_normal(menu.size))          # eager kernels between replays
+    pr.close()
+    pm.close()
+
+
 @pytest.mark.parametrize("mode", ["float32", "uint16", "uint24"])
 @pytest.mark.parametrize("n", [9, 14, 17, 18])
 def test_stream_pricing_bit_exact(n, mode):

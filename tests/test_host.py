@@ -362,6 +362,61 @@ def test_master_qp_state_sequence():
     st.close()
 
 
+_QP_SEQUENCE = r"""
+import hashlib, sys
+sys.path[:0] = sys.argv[1:3]
+import numpy as np
+from solver.common.qp_native import QPState, solve_qp_native
+rng = np.random.default_rng(7)
+M, P, K_MAX = 1500, 3000, 700
+p = rng.random(M)
+p *= 50.0 / p.sum()               # far from every column's reach: the support grows to hundreds
+pool = (rng.random((P, M)) < 0.12).astype(np.float64)
+C = np.zeros((K_MAX, M))
+Q = np.zeros((K_MAX + 8, K_MAX + 8))
+h = np.zeros(K_MAX)
+st, A, p_out = QPState(), [0], np.zeros(M)
+dig, worst, kmax = hashlib.sha256(), 0.0, 0
+for k in range(K_MAX):
+    j = int(np.argmax(pool @ (p - p_out))) if k else 0
+    C[k] = pool[j]
+    row = C[:k + 1] @ C[k]
+    Q[k, :k + 1] = row
+    Q[:k + 1, k] = row
+    h[k] = C[k] @ p
+    Qk, hk = Q[:k + 1, :k + 1], h[:k + 1]
+    A_prev = A
+    lam, A = solve_qp_native(Qk, hk, A_init=A, state=st)
+    dig.update(lam.tobytes())
+    kmax = max(kmax, len(A))
+    p_out = lam @ C[:k + 1]
+    if k % 50 == 49:
+        lam2, _ = solve_qp_native(np.ascontiguousarray(Qk), hk, A_init=A_prev, max_outer=2000)
+        f = lambda l: 0.5 * l @ Qk @ l - hk @ l
+        worst = max(worst, abs(f(lam) - f(lam2)) / max(1.0, abs(f(lam2))))
+print(dig.hexdigest(), worst, kmax)
+"""
+
+
+def test_master_qp_gradient_threads_do_not_change_the_weights():
+    """A growing master large enough for the inactive-column gradient to run on host threads (K up
+    to 700, a working set of a few hundred: above the library's threshold) and to read the state's
+    float32 copy of Q: every step's weights are the same bit for bit with 1 and 8 threads, and
+    reach the optimum of a stateless solve from the same warm working set (every 50 steps)."""
+    import subprocess
+    import sys
+    outs = []
+    for t in (1, 8):
+        env = dict(os.environ, OMP_NUM_THREADS=str(t))
+        r = subprocess.run([sys.executable, "-c", _QP_SEQUENCE, os.path.join(ROOT, "rum-testing-gpu_amd"), ROOT],
+                           env=env, capture_output=True, text=True, timeout=600)
+        assert r.returncode == 0, r.stderr[-2000:]
+        outs.append(r.stdout.split())
+    assert outs[0][0] == outs[1][0], outs
+    assert float(outs[0][1]) <= 1e-10, outs
+    assert int(outs[0][2]) >= 200, outs       # K x k above the threaded gradient's threshold
+
+
 def test_out_of_memory_gives_exit_flag_minus_3(monkeypatch):
     """A device allocation that fails inside a runner ends the call with exit flag -3 and the
     library's message (rum_solver.py:1217-1218 of the reference maps MemoryError and "out of memory"
