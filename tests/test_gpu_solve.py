@@ -153,6 +153,37 @@ def test_bootstrap_small_panel_end_to_end(tmp_path):
         assert pv2.p == pv.p
 
 
+@pytest.mark.timeout(300)
+def test_resume_from_the_reference_record(tmp_path):
+    """A checkpoint the reference wrote resumes here. The stored DSR record (B = 10,000: every draw's
+    bracket, tolerances and exit flag) is this run's checkpoint, with settings' `input` key added
+    (the reference's writer emits it; the stored run predates it). p_hat is certified again, the
+    draws are regenerated and must hash to the record's fingerprint, and since the stored brackets
+    decide every draw, nothing is solved: the p-value and its counts are the stored ones."""
+    rec = json.load(open(os.path.join(GOLDEN, "dsr_alt0_record.json")))
+    rec["settings"]["input"] = "panel"
+    ck = str(tmp_path / "reference_checkpoint.json")
+    with open(ck, "w") as f:
+        json.dump(rec, f)
+    panel = pd.read_csv(os.path.join(GOLDEN, "dsr_overload_data_25_long.csv"))
+    menu, alt, p = cells_from_panel(panel, 25, alternatives=[0])
+    rum = RUM(menu, alt, p, num_alternatives=25)
+    with rum.bootstrap(panel=panel, B=10000, checkpoint=ck) as boot:
+        assert boot.ledger.n_solved() == int((np.asarray(rec["draws"]["exit_flag"]) != 127).sum()) == 7622
+        pv = boot.p_value()
+        print(pv)
+        assert boot.fingerprints == rec["fingerprints"]
+        assert boot.time["solve"] == rec["time"]["solve"]          # no draw was solved again
+        assert boot.L0 >= rec["p_hat"]["lb"] and boot.U0 <= rec["p_hat"]["ub"]
+    stored = rec["results"]["p_value"]["0.05"]
+    assert (pv.p, pv.n_below, pv.n_above, pv.n_undecided) == \
+        (stored["p"], stored["n_below"], stored["n_above"], stored["n_undecided"])
+    # the screen also counts solved draws whose stored upper bound is below tau_lo
+    assert pv.n_screened >= stored["n_screened"]
+    after = json.load(open(ck))
+    assert after["draws"]["lb"] == rec["draws"]["lb"] and after["draws"]["exit_flag"] == rec["draws"]["exit_flag"]
+
+
 @pytest.mark.timeout(600)
 def test_dsr_bootstrap_matches_stored_run():
     """The whole stored DSR run (n = 25 overload panel, alternative 0; B = 10,000, seed 0, abs

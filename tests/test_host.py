@@ -122,6 +122,54 @@ def test_hong_li_reproduces_stored_decisions():
     assert p_hi == pv["p"]
 
 
+def _key_tree(d):
+    """{dotted path: kind} of a record's dict levels (lists and scalars are leaves)"""
+    out = {}
+    for k, v in d.items():
+        if isinstance(v, dict):
+            out[k] = "dict"
+            out.update({k + "." + kk: vv for kk, vv in _key_tree(v).items()})
+        else:
+            out[k] = "none" if v is None else "list" if isinstance(v, list) else "str" if isinstance(v, str) \
+                else "bool" if isinstance(v, bool) else "number"
+    return out
+
+
+def test_record_layout_matches_reference_record(tmp_path):
+    """A record written here has the layout of the reference's stored record of the same run (its
+    bootstrap.py:356-368 writes both): the same keys at every level with the same kinds of value,
+    and every draws field as a list of B values. Levels filled only by a solve (the plan, p_hat's
+    certificates, the results) are compared by their keys once present, values the screen fills
+    by their keys. Settings also carry
+    `input` ('panel' or 'replies'), which the reference's writer emits and its stored run predates."""
+    ref = json.load(open(os.path.join(GOLDEN, "dsr_alt0_record.json")))
+    panel, menu, alt, p = _dsr()
+    rum = RUM(menu, alt, p, num_alternatives=25)
+    boot = rum.bootstrap(panel=panel, B=10000)           # nothing solved: no device is touched
+    path = str(tmp_path / "record.json")
+    boot.save(path)
+    ours = json.load(open(path))
+    assert set(ours) == set(ref)
+    assert set(ours["settings"]) == set(ref["settings"]) | {"input"} and ours["settings"]["input"] == "panel"
+    t_ours, t_ref = _key_tree(ours), _key_tree(ref)
+    solved_only = ("plan", "p_hat.", "results.", "tau", "when")
+    for k, kind in t_ref.items():
+        if k.startswith(solved_only) or k in ("p_hat.lb", "p_hat.ub"):
+            continue
+        # values filled by the screen (draw fingerprint, redraw counts) are None until it runs
+        assert k in t_ours and t_ours[k] in (kind, "none"), (k, t_ours.get(k), kind)
+    for k in ref["draws"]:
+        assert len(ours["draws"][k]) == len(ref["draws"][k]) == 10000
+    assert set(ours["p_hat"]) == set(ref["p_hat"]) and set(ours["results"]) >= set(ref["results"])
+    assert set(ours["scaling"]) == set(ref["scaling"]) and ours["settings"]["r_n"] == ref["settings"]["r_n"]
+    # and a checkpoint of another run is refused with the reference's message
+    other = dict(ref, settings=dict(ref["settings"], input="panel", B=500))
+    with open(path, "w") as f:
+        json.dump(other, f)
+    with pytest.raises(ValueError, match="belongs to another run"):
+        rum.bootstrap(panel=panel, B=10000, checkpoint=path)
+
+
 def test_menu_draws_statistics():
     rng = np.random.default_rng(7)
     rows = []
