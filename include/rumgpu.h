@@ -122,6 +122,33 @@ int rumgpu_bm_slot_get_obs(rumgpu_bm* bm, int slot, double* x2_obs, double* z_ob
 int rumgpu_bm_certify(rumgpu_bm* bm, int want_lb, double* ub, double* lb, double* phi);
 /* the fitted point of one slot's last certificate: p_fit, rum_solver.py:884-889 */
 int rumgpu_bm_slot_fit(rumgpu_bm* bm, int slot, double* p_fit);
+/* ---- Device draw queue of a batch object: the open draws' loop of Bootstrap._solve
+ * (inference/bootstrap.py:308-335; each draw's stops as Session.solve applies them through
+ * bm_admm/solver.py:294-377) run with retirement and refill on the device. A cycle is `check`
+ * iterations of every slot, their float64 certificates (the lower bound every lb_every-th cycle),
+ * the stop tests of every live slot in slot order (ub below `below`, lb at or above `above`,
+ * ub - lb within abs_tol or rel_tol * lb, the host's time stop, max_iter) and the refill of each
+ * retired slot with the next pending draw, warm from the state `src` kept (plus corrections at the
+ * observed cells when corr is 1: dx2, 2: dz, 3: both). Cycles replay as HIP graphs; the host only
+ * tops up the ring of pending draws and reads results between runs of many cycles. */
+int rumgpu_bm_queue_start(rumgpu_bm* bm, rumgpu_bm* src, int64_t total, int64_t capacity, int check, int lb_every,
+                          int64_t max_iter, double abs_tol, double rel_tol, int corr);
+/* k pending draws: p (k x M, input row order), thresholds on the non-singleton part (NaN: none),
+ * corrections (k x M) when the queue takes them; at most `capacity` untaken draws at a time */
+int rumgpu_bm_queue_push(rumgpu_bm* bm, int64_t k, const double* p, const double* below, const double* above,
+                         const double* dx2, const double* dz);
+/* `cycles` cycles; status[5] = draws taken, pushed, retired, live slots, checks */
+int rumgpu_bm_queue_run(rumgpu_bm* bm, int64_t cycles, int64_t* status);
+/* per draw (sequence numbers first..first+count-1): lb, ub of the non-singleton part, iterations,
+ * retirement rank, stop (0 open, 1 threshold, 2 tolerance, 3 time, 4 max_iter); outputs may be null */
+int rumgpu_bm_queue_results(rumgpu_bm* bm, int64_t first, int64_t count, double* lb, double* ub, int64_t* iters,
+                            int64_t* order, int32_t* stop);
+/* per slot: the draw it holds (-1 idle) and its iterations */
+int rumgpu_bm_queue_slots(rumgpu_bm* bm, int64_t* draw, int64_t* iters);
+/* time stops (draw_time, rum_solver.py:914-925 max_time per draw): kill[slot] != 0 stops that
+ * slot's draw at the next decision */
+int rumgpu_bm_queue_expire(rumgpu_bm* bm, const int32_t* kill);
+int rumgpu_bm_queue_end(rumgpu_bm* bm);
 /* building blocks exposed for tests: one full (a,b,c,d) butterfly of a host slab
  * (common/butterfly.py:99-140 butterfly_fast), and the prox of a host slab (solver.py:66-139) */
 int rumgpu_butterfly64(rumgpu_ctx* ctx, int n, double* v, double a, double b, double c, double d);

@@ -912,12 +912,17 @@ struct rumgpu_bm {
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph = nullptr;
   i64 graph_len = 0;
+  int graph_gen = 0;          // bumped whenever the iteration graph is rebuilt (beta, plan, length)
+  struct BmQueue* queue = nullptr;   // the device draw queue of a batch object (rumgpu_bm_queue_*)
 };
+
+static void queue_free(rumgpu_bm* b);
 
 static void bm_free_all(rumgpu_bm* b) {
   void* bufs[] = {b->x2, b->s, b->y, b->z, b->tb, b->xk, b->inv_gl, b->kx2, b->ks, b->ky, b->kz,
                   b->pw, b->pAT, b->prhs, b->px, b->pAx, b->px2n, b->obsmask, b->obsoff, b->pc, b->obs,
                   b->p, b->cert, b->outw, b->mass, b->order, b->part, b->res, b->fit, b->fresh, b->dwarm};
+  queue_free(b);
   for (void* q : bufs) rg_free(q);
   if (b->h_res) (void)hipHostFree(b->h_res);
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
@@ -1087,6 +1092,7 @@ static int build_graph(rumgpu_bm* b, i64 len) {
   if (b->gexec) { (void)hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
   if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
   b->graph_len = 0;
+  b->graph_gen += 1;
   if (len <= 0 || b->plain) return RG_OK;
   hipStream_t st = b->ctx->stream;
   const int pend = b->pending;
@@ -1111,24 +1117,32 @@ static int build_graph(rumgpu_bm* b, i64 len) {
 // Certificates.
 // The certificates of slots 0..D-1 (D = 1: the single-solve entry points, which act on slot 0).
 template <typename T>
+static int phi_launch(rumgpu_bm* b, const void* x, int want_fit, int D, double* out) {
+  const int nblk = (int)std::min<i64>(256, std::max<i64>(1, (b->M + 255) / 256));
+  hipLaunchKernelGGL(k_phi<T>, dim3(nblk, D), dim3(256), 0, b->ctx->stream, (const T*)x, b->obs, b->p, b->M,
+                     want_fit ? b->fit : nullptr, b->part, b->m2);
+  hipLaunchKernelGGL(k_final, dim3(D), dim3(64), 0, b->ctx->stream, b->part, nblk, 1, out);
+  RG_LAUNCH_CHECK();
+  return RG_OK;
+}
+
+template <typename T>
 static int phi_impl(rumgpu_bm* b, const void* x, double* out_sq, int want_fit, int D = 1) {
   hipStream_t st = b->ctx->stream;
-  const int nblk = (int)std::min<i64>(256, std::max<i64>(1, (b->M + 255) / 256));
-  hipLaunchKernelGGL(k_phi<T>, dim3(nblk, D), dim3(256), 0, st, (const T*)x, b->obs, b->p, b->M,
-                     want_fit ? b->fit : nullptr, b->part, b->m2);
-  hipLaunchKernelGGL(k_final, dim3(D), dim3(64), 0, st, b->part, nblk, 1, b->res);
-  RG_LAUNCH_CHECK();
+  RG_TRY(phi_launch<T>(b, x, want_fit, D, b->res));
   RG_HIP(hipMemcpyAsync(b->h_res, b->res, (size_t)D * sizeof(double), hipMemcpyDeviceToHost, st));
   RG_HIP(hipStreamSynchronize(st));
   for (int d = 0; d < D; ++d) out_sq[d] = b->h_res[d];
   return RG_OK;
 }
 
+// The pushed point's squared distance of every slot into out[D] (device), its gathered values
+// into fit: Mobius of x2 in float64, w = max(q, 0) + 1e-12, the layered mass push, extract and
+// zeta, then phi of the pushed point. No host synchronisation (the draw queue captures it).
 template <typename T>
-static int push_impl(rumgpu_bm* b, double* ub, int D = 1) {
+static int push_launch(rumgpu_bm* b, int D, double* out) {
   hipStream_t st = b->ctx->stream;
   const int n = b->n;
-  // q = Mobius(x2) in float64, then w = max(q, 0) + 1e-12
   RG_TRY((full_transform<T, double>(b, (const T*)b->x2, b->cert, coef<double>(1.0, -1.0, 0.0, 1.0), 1, 0, D)));
   launch_outw(n, dim3(grid_for(b->NN, 256), D), st, b->cert, b->outw, b->Nloc);
   hipLaunchKernelGGL(k_set_strided, dim3(grid_for(D, 64)), dim3(64), 0, st, b->mass, b->NN, D, 1.0);
@@ -1140,13 +1154,23 @@ static int push_impl(rumgpu_bm* b, double* ub, int D = 1) {
   RG_LAUNCH_CHECK();
   // rf = extract, x_ub = zeta(rf) (in place)
   RG_TRY((full_transform<double, double>(b, b->cert, b->cert, coef<double>(1.0, 1.0, 0.0, 1.0), 0, 1, D)));
-  RG_TRY(phi_impl<double>(b, b->cert, ub, 1, D));
-  for (int d = 0; d < D; ++d) ub[d] = std::sqrt(ub[d]);
-  return RG_OK;
+  return phi_launch<double>(b, b->cert, 1, D, out);
 }
 
 template <typename T>
-static int lb_impl(rumgpu_bm* b, double* lb, int D = 1) {
+static int push_impl(rumgpu_bm* b, double* ub, int D = 1) {
+  hipStream_t st = b->ctx->stream;
+  RG_TRY(push_launch<T>(b, D, b->res));
+  RG_HIP(hipMemcpyAsync(b->h_res, b->res, (size_t)D * sizeof(double), hipMemcpyDeviceToHost, st));
+  RG_HIP(hipStreamSynchronize(st));
+  for (int d = 0; d < D; ++d) ub[d] = std::sqrt(b->h_res[d]);
+  return RG_OK;
+}
+
+// The Frank-Wolfe bound's parts of every slot, on the device: res[2d] = f = |r|^2,
+// res[2d + 1] = r.y, res[2D + d] = the longest path (vmax) to the full set.
+template <typename T>
+static int lb_launch(rumgpu_bm* b, int D) {
   hipStream_t st = b->ctx->stream;
   const int n = b->n;
   hipLaunchKernelGGL((ew_fill<double>), dim3(ew_grid(b->m2 * D)), dim3(256), 0, st, b->cert, 0.0, b->m2 * D);
@@ -1162,18 +1186,322 @@ static int lb_impl(rumgpu_bm* b, double* lb, int D = 1) {
     if (ns <= 0) continue;
     launch_lpath(n, dim3(grid_for(ns, 256), D), st, b->cert, b->mass, b->order + lo, ns, b->Nloc);
   }
-  // longest path to the full set of each slot: res[2 D + d]
   hipLaunchKernelGGL(k_get_strided, dim3(grid_for(D, 64)), dim3(64), 0, st, b->mass + (b->NN - 1), b->NN, D,
                      b->res + 2 * D);
   RG_LAUNCH_CHECK();
+  return RG_OK;
+}
+
+// lb = sqrt(max(0, f - 2 (vmax - r.y))): the same expression on the host and in the draw queue.
+__host__ __device__ inline double fw_lb_value(double f, double ry, double vmax) {
+  const double rc = vmax - ry;
+  const double l2 = f - 2.0 * rc;
+  return sqrt(l2 > 0.0 ? l2 : 0.0);
+}
+
+template <typename T>
+static int lb_impl(rumgpu_bm* b, double* lb, int D = 1) {
+  hipStream_t st = b->ctx->stream;
+  RG_TRY(lb_launch<T>(b, D));
   RG_HIP(hipMemcpyAsync(b->h_res, b->res, (size_t)3 * D * sizeof(double), hipMemcpyDeviceToHost, st));
   RG_HIP(hipStreamSynchronize(st));
-  for (int d = 0; d < D; ++d) {
-    const double f = b->h_res[2 * d], ry = b->h_res[2 * d + 1], vmax = b->h_res[2 * D + d];
-    const double rc = vmax - ry;
-    const double l2 = f - 2.0 * rc;
-    lb[d] = std::sqrt(l2 > 0.0 ? l2 : 0.0);
+  for (int d = 0; d < D; ++d) lb[d] = fw_lb_value(b->h_res[2 * d], b->h_res[2 * d + 1], b->h_res[2 * D + d]);
+  return RG_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Draw queue of a batch object: retirement and refill on the device.
+//
+// The host-driven slots go back to the host after every check (certificates copied out, the
+// stop tests run per slot, the next draw's p and start uploaded). Here a cycle is `check` fused
+// iterations, the float64 certificates of every slot, a decision kernel and the refill kernels,
+// all enqueued on the stream and replayed as one HIP graph (two: with and without the lower
+// bound). The decision kernel runs the host driver's stop tests in the same order with the same
+// float64 arithmetic, so every draw retires at the same check with the same bracket; the next
+// draws come from a ring of pending draws (p, thresholds, optional warm-start corrections) that
+// the host tops up between runs of many cycles.
+
+enum { QS_NONE = 0, QS_THRESH = 1, QS_TOL = 2, QS_TIME = 3, QS_ITERS = 4 };
+enum { QC_HEAD = 0, QC_TAIL = 1, QC_RETIRED = 2, QC_LIVE = 3, QC_CHECKS = 4, QC_NREF = 5, QC_N = 8 };
+
+struct BmQueue {
+  i64 cap = 0, total = 0, tail = 0;   // ring entries; draws of the run; draws pushed so far
+  int check = 1, lb_every = 1, corr = 0, primed = 0;
+  double abs_tol = 0, rel_tol = 0;
+  i64 max_iter = 0;
+  rumgpu_bm* src = nullptr;           // the kept state every refill starts from
+  double *rP = nullptr, *rDx = nullptr, *rDz = nullptr, *rBelow = nullptr, *rAbove = nullptr;  // ring
+  i64* ctr = nullptr;                 // QC_* counters
+  i64* sdraw = nullptr;               // per slot: the draw's sequence number, -1 when idle
+  i64* sit = nullptr;                 // per slot: iterations of its draw
+  double *sub = nullptr, *slb = nullptr, *sbelow = nullptr, *sabove = nullptr;
+  int* skill = nullptr;               // per slot: the host's time stop
+  i64* refill = nullptr;              // (slot, ring entry) pairs of the last decision
+  double* ubsq = nullptr;             // per slot: squared distance of the pushed point
+  i64* pcidx = nullptr;               // input row -> p position in (menu, alternative) order
+  double *qlb = nullptr, *qub = nullptr;   // per draw at retirement: running max lb, running min ub
+  i64 *qit = nullptr, *qorder = nullptr;   // per draw: iterations, retirement rank
+  int* qstop = nullptr;               // per draw: QS_* (QS_NONE while open)
+  i64* h_ctr = nullptr;               // pinned copy of ctr after each run
+  hipGraph_t g[2] = {nullptr, nullptr};
+  hipGraphExec_t gx[2] = {nullptr, nullptr};
+  int gen = -1;                       // the handle's graph_gen when the cycle graphs were captured
+  i64 cycles = 0;                     // cycles run; the lower bound runs when cycles % lb_every == 0
+};
+
+struct QueueArgs {
+  i64 *ctr, *sdraw, *sit, *refill, *qit, *qorder;
+  double *sub, *slb, *sbelow, *sabove, *qlb, *qub;
+  const double *rBelow, *rAbove, *ubsq, *res;
+  int *skill, *qstop;
+  i64 cap, max_iter;
+  double abs_tol, rel_tol;
+  int D, check;
+};
+
+// numpy's minimum / maximum: NaN if either operand is NaN.
+__device__ inline double nan_min(double a, double b) {
+  return (isnan(a) || isnan(b)) ? __longlong_as_double(0x7ff8000000000000ll) : (b < a ? b : a);
+}
+__device__ inline double nan_max(double a, double b) {
+  return (isnan(a) || isnan(b)) ? __longlong_as_double(0x7ff8000000000000ll) : (b > a ? b : a);
+}
+
+// One thread walks the slots in ascending order, as the host driver does. A live slot adds the
+// cycle's iterations, folds the certificate into its running bracket and takes the stop tests
+// (thresholds, tolerance, the host's time stop, max_iter); a slot that retires (every slot when
+// priming) takes the next pending draw, or goes idle.
+__global__ void k_queue_decide(QueueArgs a, int want_lb, int prime) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  i64 head = a.ctr[QC_HEAD], retired = a.ctr[QC_RETIRED], nref = 0, live = 0;
+  const i64 tail = a.ctr[QC_TAIL];
+  for (int d = 0; d < a.D; ++d) {
+    const i64 g = a.sdraw[d];
+    bool open_slot = prime != 0;
+    if (!prime && g >= 0) {
+      const i64 it = a.sit[d] + a.check;
+      a.sit[d] = it;
+      const double ub = nan_min(a.sub[d], sqrt(a.ubsq[d]));
+      a.sub[d] = ub;
+      double lb = a.slb[d];
+      if (want_lb) {
+        lb = nan_max(lb, fw_lb_value(a.res[2 * d], a.res[2 * d + 1], a.res[2 * a.D + d]));
+        a.slb[d] = lb;
+      }
+      const double below = a.sbelow[d], above = a.sabove[d];
+      int stop = QS_NONE;
+      if (!isnan(below) && below > 0 && ub <= below) stop = QS_THRESH;
+      else if (!isnan(above) && lb >= above) stop = QS_THRESH;
+      else {
+        const double gap = ub - lb;
+        if (gap <= a.abs_tol || (lb > 0 && gap / lb <= a.rel_tol)) stop = QS_TOL;
+        else if (a.skill[d]) stop = QS_TIME;
+        else if (it >= a.max_iter) stop = QS_ITERS;
+      }
+      if (stop != QS_NONE) {
+        a.qlb[g] = lb;
+        a.qub[g] = ub;
+        a.qit[g] = it;
+        a.qorder[g] = retired++;
+        a.qstop[g] = stop;
+        open_slot = true;
+      }
+    }
+    if (open_slot) {
+      a.skill[d] = 0;
+      if (head < tail) {
+        const i64 e = head % a.cap;
+        a.sdraw[d] = head++;
+        a.sit[d] = 0;
+        a.sub[d] = __longlong_as_double(0x7ff0000000000000ll);
+        a.slb[d] = 0.0;
+        a.sbelow[d] = a.rBelow[e];
+        a.sabove[d] = a.rAbove[e];
+        a.refill[2 * nref] = d;
+        a.refill[2 * nref + 1] = e;
+        ++nref;
+      } else {
+        a.sdraw[d] = -1;
+      }
+    }
+    if (a.sdraw[d] >= 0) ++live;
   }
+  a.ctr[QC_HEAD] = head;
+  a.ctr[QC_RETIRED] = retired;
+  a.ctr[QC_LIVE] = live;
+  a.ctr[QC_NREF] = nref;
+  if (!prime) a.ctr[QC_CHECKS] += 1;
+}
+
+// p of every refilled slot from its ring entry, in input order and in (menu, alternative) order.
+__global__ __launch_bounds__(256) void k_queue_load_p(const i64* __restrict__ ctr, const i64* __restrict__ refill,
+                                                      const double* __restrict__ rP, const i64* __restrict__ pcidx,
+                                                      double* p, double* pc, i64 M) {
+  const i64 total = ctr[QC_NREF] * M;
+  for (i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (i64)gridDim.x * blockDim.x) {
+    const i64 k = t / M, j = t - k * M;
+    const i64 d = refill[2 * k], e = refill[2 * k + 1];
+    const double v = rP[e * M + j];
+    p[d * M + j] = v;
+    pc[d * M + pcidx[j]] = v;
+  }
+}
+
+// The kept state into every refilled slot (the conversions of rumgpu_bm_slot_load), marked fresh.
+template <typename TA, typename TB>
+__global__ __launch_bounds__(256) void k_queue_load_state(const i64* __restrict__ ctr, const i64* __restrict__ refill,
+                                                          const TA* __restrict__ kx2, const TA* __restrict__ ks,
+                                                          const TA* __restrict__ ky, const TA* __restrict__ kz,
+                                                          TB* x2, TB* s, TB* y, TB* z, int* fresh, i64 m2) {
+  const i64 total = ctr[QC_NREF] * m2;
+  for (i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (i64)gridDim.x * blockDim.x) {
+    const i64 k = t / m2, i = t - k * m2;
+    const i64 d = refill[2 * k], o = d * m2 + i;
+    x2[o] = (TB)kx2[i];
+    s[o] = (TB)ks[i];
+    y[o] = (TB)ky[i];
+    z[o] = (TB)kz[i];
+    if (i == 0) fresh[d] = 1;
+  }
+}
+
+// Warm-start corrections of every refilled slot at its observed cells (rumgpu_bm_slot_add_obs).
+template <typename T>
+__global__ __launch_bounds__(256) void k_queue_add_obs(const i64* __restrict__ ctr, const i64* __restrict__ refill,
+                                                       T* x2, T* z, const i64* __restrict__ obs,
+                                                       const double* __restrict__ rDx, const double* __restrict__ rDz,
+                                                       i64 M, i64 m2) {
+  const i64 total = ctr[QC_NREF] * M;
+  for (i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (i64)gridDim.x * blockDim.x) {
+    const i64 k = t / M, j = t - k * M;
+    const i64 d = refill[2 * k], e = refill[2 * k + 1];
+    const i64 q = d * m2 + obs[j];
+    if (rDx) x2[q] = (T)((double)x2[q] + rDx[e * M + j]);
+    if (rDz) z[q] = (T)((double)z[q] + rDz[e * M + j]);
+  }
+}
+
+__global__ void k_set_i64(i64* a, i64 v) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *a = v;
+}
+
+static void queue_free(rumgpu_bm* b) {
+  BmQueue* q = b->queue;
+  if (!q) return;
+  for (int k = 0; k < 2; ++k) {
+    if (q->gx[k]) (void)hipGraphExecDestroy(q->gx[k]);
+    if (q->g[k]) (void)hipGraphDestroy(q->g[k]);
+  }
+  void* bufs[] = {q->rP, q->rDx, q->rDz, q->rBelow, q->rAbove, q->ctr, q->sdraw, q->sit, q->sub, q->slb,
+                  q->sbelow, q->sabove, q->skill, q->refill, q->ubsq, q->pcidx, q->qlb, q->qub, q->qit,
+                  q->qorder, q->qstop};
+  for (void* v : bufs) rg_free(v);
+  if (q->h_ctr) (void)hipHostFree(q->h_ctr);
+  delete q;
+  b->queue = nullptr;
+}
+
+static QueueArgs queue_args(rumgpu_bm* b) {
+  BmQueue* q = b->queue;
+  QueueArgs a;
+  a.ctr = q->ctr; a.sdraw = q->sdraw; a.sit = q->sit; a.refill = q->refill; a.qit = q->qit; a.qorder = q->qorder;
+  a.sub = q->sub; a.slb = q->slb; a.sbelow = q->sbelow; a.sabove = q->sabove; a.qlb = q->qlb; a.qub = q->qub;
+  a.rBelow = q->rBelow; a.rAbove = q->rAbove; a.ubsq = q->ubsq; a.res = b->res;
+  a.skill = q->skill; a.qstop = q->qstop;
+  a.cap = q->cap; a.max_iter = q->max_iter;
+  a.abs_tol = q->abs_tol; a.rel_tol = q->rel_tol;
+  a.D = b->D; a.check = q->check;
+  return a;
+}
+
+static const unsigned QUEUE_GRID = 1024;
+
+// Decision and refill kernels (enqueued only).
+template <typename T>
+static int queue_decide_refill(rumgpu_bm* b, int want_lb, int prime) {
+  hipStream_t st = b->ctx->stream;
+  BmQueue* q = b->queue;
+  hipLaunchKernelGGL(k_queue_decide, dim3(1), dim3(64), 0, st, queue_args(b), want_lb, prime);
+  hipLaunchKernelGGL(k_queue_load_p, dim3(QUEUE_GRID), dim3(256), 0, st, (const i64*)q->ctr, (const i64*)q->refill,
+                     (const double*)q->rP, (const i64*)q->pcidx, b->p, b->pc, b->M);
+  rumgpu_bm* s = q->src;
+  if (s->isz == 8)
+    hipLaunchKernelGGL((k_queue_load_state<double, T>), dim3(QUEUE_GRID), dim3(256), 0, st, (const i64*)q->ctr,
+                       (const i64*)q->refill, (const double*)s->kx2, (const double*)s->ks, (const double*)s->ky,
+                       (const double*)s->kz, (T*)b->x2, (T*)b->s, (T*)b->y, (T*)b->z, b->fresh, b->m2);
+  else
+    hipLaunchKernelGGL((k_queue_load_state<float, T>), dim3(QUEUE_GRID), dim3(256), 0, st, (const i64*)q->ctr,
+                       (const i64*)q->refill, (const float*)s->kx2, (const float*)s->ks, (const float*)s->ky,
+                       (const float*)s->kz, (T*)b->x2, (T*)b->s, (T*)b->y, (T*)b->z, b->fresh, b->m2);
+  if (q->corr)
+    hipLaunchKernelGGL(k_queue_add_obs<T>, dim3(QUEUE_GRID), dim3(256), 0, st, (const i64*)q->ctr,
+                       (const i64*)q->refill, (T*)b->x2, (T*)b->z, (const i64*)b->obs,
+                       (const double*)((q->corr & 1) ? q->rDx : nullptr),
+                       (const double*)((q->corr & 2) ? q->rDz : nullptr), b->M, b->m2);
+  RG_LAUNCH_CHECK();
+  return RG_OK;
+}
+
+// One cycle: `check` fused iterations, the pushed upper bound (and the lower bound when want_lb)
+// of every slot, the decision and the refills. Enqueued only; captured by the cycle graphs.
+template <typename T>
+static int queue_cycle(rumgpu_bm* b, int want_lb) {
+  BmQueue* q = b->queue;
+  for (int i = 0; i < q->check; ++i) RG_TRY(fused_iter<T>(b));
+  RG_TRY(push_launch<T>(b, b->D, q->ubsq));
+  if (want_lb) RG_TRY(lb_launch<T>(b, b->D));
+  return queue_decide_refill<T>(b, want_lb, 0);
+}
+
+template <typename T>
+static int queue_graphs(rumgpu_bm* b) {
+  BmQueue* q = b->queue;
+  if (q->gen == b->graph_gen && q->gx[0] && q->gx[1]) return RG_OK;
+  hipStream_t st = b->ctx->stream;
+  for (int k = 0; k < 2; ++k) {
+    if (q->gx[k]) { (void)hipGraphExecDestroy(q->gx[k]); q->gx[k] = nullptr; }
+    if (q->g[k]) { (void)hipGraphDestroy(q->g[k]); q->g[k] = nullptr; }
+  }
+  const int pend = b->pending;
+  const i64 it0 = b->iters;
+  for (int k = 0; k < 2; ++k) {
+    b->pending = 1;                             // steady state: every slot owes its s, y update
+    RG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    const int rc = queue_cycle<T>(b, k);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(st, &g);
+    b->pending = pend;
+    b->iters = it0;
+    if (rc != RG_OK) { if (g) (void)hipGraphDestroy(g); return rc; }
+    if (e != hipSuccess) { rg_set_error("queue graph capture failed: %s", hipGetErrorString(e)); return RG_EHIP; }
+    q->g[k] = g;
+    RG_HIP(hipGraphInstantiate(&q->gx[k], g, nullptr, nullptr, 0));
+  }
+  q->gen = b->graph_gen;
+  return RG_OK;
+}
+
+template <typename T>
+static int queue_run_t(rumgpu_bm* b, i64 cycles) {
+  BmQueue* q = b->queue;
+  hipStream_t st = b->ctx->stream;
+  if (!q->primed) {
+    RG_TRY(queue_decide_refill<T>(b, 0, 1));
+    q->primed = 1;
+  }
+  for (i64 c = 0; c < cycles; ++c) {
+    const int want_lb = (q->cycles % q->lb_every) == 0;
+    if (!b->pending) {
+      RG_TRY(queue_cycle<T>(b, want_lb));       // the first iteration after a load runs eagerly
+    } else {
+      RG_TRY(queue_graphs<T>(b));
+      RG_HIP(hipGraphLaunch(q->gx[want_lb], st));
+      b->iters += q->check;
+    }
+    q->cycles += 1;
+  }
+  RG_HIP(hipMemcpyAsync(q->h_ctr, q->ctr, QC_N * sizeof(i64), hipMemcpyDeviceToHost, st));
+  RG_HIP(hipStreamSynchronize(st));
   return RG_OK;
 }
 
@@ -1508,6 +1836,7 @@ extern "C" int rumgpu_bm_set_plan(rumgpu_bm* b, int lo_bits, int lo_rl, int hi_r
   RG_TRY(BM_DISPATCH(b, finalize, b));
   b->wins = wins;
   b->geoms = geoms;
+  b->graph_gen += 1;
   if (b->gexec) return BM_DISPATCH(b, build_graph, b, b->graph_len);
   return RG_OK;
 }
@@ -1520,6 +1849,7 @@ extern "C" int rumgpu_bm_set_beta(rumgpu_bm* b, double beta) {
   if (beta == b->beta) return RG_OK;
   RG_TRY(BM_DISPATCH(b, finalize, b));       // the owed s, y update still uses the old penalty
   b->beta = beta;
+  b->graph_gen += 1;
   if (b->gexec) return BM_DISPATCH(b, build_graph, b, b->graph_len);
   return RG_OK;
 }
@@ -1695,6 +2025,164 @@ extern "C" int rumgpu_bm_certify(rumgpu_bm* b, int want_lb, double* ub, double* 
     if (want_lb) RG_TRY(lb_impl<float>(b, lb, D));
   }
   for (int d = 0; d < D; ++d) phi[d] = std::sqrt(phi[d]);
+  return RG_OK;
+}
+
+// ---- the device draw queue (see the comment above k_queue_decide)
+extern "C" int rumgpu_bm_queue_start(rumgpu_bm* b, rumgpu_bm* src, int64_t total, int64_t capacity, int check,
+                                     int lb_every, int64_t max_iter, double abs_tol, double rel_tol, int corr) {
+  RG_CHECK(b && src, RG_EINVAL, "bm_queue_start: null handle");
+  RG_CHECK(!b->plain, RG_EINVAL, "bm_queue_start: the plain schedule has no draw slots");
+  RG_CHECK(src->n == b->n && src->M == b->M && src->ctx->stream == b->ctx->stream, RG_EINVAL,
+           "bm_queue_start: incompatible source handle");
+  RG_CHECK(src->kept_valid, RG_EINVAL, "bm_queue_start: the source handle keeps no state (bm_keep)");
+  RG_CHECK(total >= 1 && capacity >= 1 && check >= 1 && lb_every >= 1 && max_iter >= 1 && corr >= 0 && corr <= 3,
+           RG_EINVAL, "bm_queue_start: bad argument");
+  RG_CHECK(abs_tol >= 0 && rel_tol >= 0, RG_EINVAL, "bm_queue_start: tolerances must be >= 0");
+  RG_HIP(hipSetDevice(b->ctx->device));
+  queue_free(b);
+  BmQueue* q = new BmQueue();
+  b->queue = q;
+  q->cap = capacity < total ? capacity : total;
+  q->total = total;
+  q->check = check;
+  q->lb_every = lb_every;
+  q->max_iter = max_iter;
+  q->abs_tol = abs_tol;
+  q->rel_tol = rel_tol;
+  q->corr = corr;
+  q->src = src;
+  const i64 D = b->D, M = b->M, cap = q->cap;
+  int rc = RG_OK;
+  struct { void** p; size_t bytes; const char* what; } al[] = {
+      {(void**)&q->rP, (size_t)(cap * M * 8), "queue ring p"},
+      {(void**)&q->rBelow, (size_t)cap * 8, "queue ring thresholds"},
+      {(void**)&q->rAbove, (size_t)cap * 8, "queue ring thresholds"},
+      {(void**)&q->ctr, QC_N * 8, "queue counters"},
+      {(void**)&q->sdraw, (size_t)D * 8, "queue slots"},
+      {(void**)&q->sit, (size_t)D * 8, "queue slots"},
+      {(void**)&q->sub, (size_t)D * 8, "queue slots"},
+      {(void**)&q->slb, (size_t)D * 8, "queue slots"},
+      {(void**)&q->sbelow, (size_t)D * 8, "queue slots"},
+      {(void**)&q->sabove, (size_t)D * 8, "queue slots"},
+      {(void**)&q->skill, (size_t)D * 4, "queue slots"},
+      {(void**)&q->refill, (size_t)D * 16, "queue refills"},
+      {(void**)&q->ubsq, (size_t)D * 8, "queue bounds"},
+      {(void**)&q->pcidx, (size_t)M * 8, "queue row order"},
+      {(void**)&q->qlb, (size_t)total * 8, "queue results"},
+      {(void**)&q->qub, (size_t)total * 8, "queue results"},
+      {(void**)&q->qit, (size_t)total * 8, "queue results"},
+      {(void**)&q->qorder, (size_t)total * 8, "queue results"},
+      {(void**)&q->qstop, (size_t)total * 4, "queue results"}};
+  for (auto& a : al)
+    if (rc == RG_OK) rc = rg_malloc(a.p, a.bytes, a.what);
+  if (rc == RG_OK && (corr & 1)) rc = rg_malloc((void**)&q->rDx, (size_t)(cap * M * 8), "queue ring corrections");
+  if (rc == RG_OK && (corr & 2)) rc = rg_malloc((void**)&q->rDz, (size_t)(cap * M * 8), "queue ring corrections");
+  if (rc == RG_OK && hipHostMalloc((void**)&q->h_ctr, QC_N * 8) != hipSuccess) {
+    rg_set_error("bm_queue_start: pinned counters");
+    rc = RG_ENOMEM;
+  }
+  if (rc != RG_OK) { queue_free(b); return rc; }
+  hipStream_t st = b->ctx->stream;
+  hipLaunchKernelGGL((ew_fill<i64>), dim3(1), dim3(64), 0, st, q->ctr, (i64)0, (i64)QC_N);
+  hipLaunchKernelGGL((ew_fill<i64>), dim3(grid_for(D, 256)), dim3(256), 0, st, q->sdraw, (i64)-1, D);
+  hipLaunchKernelGGL((ew_fill<int>), dim3(grid_for(D, 256)), dim3(256), 0, st, q->skill, 0, D);
+  hipLaunchKernelGGL((ew_fill<int>), dim3(ew_grid(total)), dim3(256), 0, st, q->qstop, 0, (i64)total);
+  RG_LAUNCH_CHECK();
+  for (int k = 0; k < QC_N; ++k) q->h_ctr[k] = 0;
+  rc = rg_copy(st, q->pcidx, b->h_pcidx.data(), (size_t)M * 8, hipMemcpyHostToDevice) == hipSuccess ? RG_OK : RG_EHIP;
+  if (rc != RG_OK) { rg_set_error("bm_queue_start: copy of the row order failed"); queue_free(b); }
+  return rc;
+}
+
+// Appends k pending draws: p (k x M, input row order), the thresholds on the non-singleton part
+// (NaN: none) and, when the queue was started with corrections, dx2 / dz (k x M). The ring holds
+// `capacity` draws that no slot has taken yet.
+extern "C" int rumgpu_bm_queue_push(rumgpu_bm* b, int64_t k, const double* P, const double* below,
+                                    const double* above, const double* dx2, const double* dz) {
+  RG_CHECK(b && b->queue, RG_EINVAL, "bm_queue_push: no queue (bm_queue_start)");
+  BmQueue* q = b->queue;
+  RG_CHECK(k >= 0 && P && below && above, RG_EINVAL, "bm_queue_push: bad argument");
+  RG_CHECK(!(q->corr & 1) || dx2, RG_EINVAL, "bm_queue_push: the queue takes dx2 corrections");
+  RG_CHECK(!(q->corr & 2) || dz, RG_EINVAL, "bm_queue_push: the queue takes dz corrections");
+  RG_CHECK(q->tail + k <= q->total, RG_EINVAL, "bm_queue_push: %lld draws exceed the run's %lld",
+           (long long)(q->tail + k), (long long)q->total);
+  RG_CHECK(q->tail + k - q->h_ctr[QC_HEAD] <= q->cap, RG_EINVAL,
+           "bm_queue_push: the ring holds %lld pending draws (%lld taken of %lld pushed)", (long long)q->cap,
+           (long long)q->h_ctr[QC_HEAD], (long long)q->tail);
+  if (k == 0) return RG_OK;
+  RG_HIP(hipSetDevice(b->ctx->device));
+  hipStream_t st = b->ctx->stream;
+  const i64 M = b->M;
+  i64 done = 0;
+  while (done < k) {
+    const i64 e = (q->tail + done) % q->cap;
+    const i64 m = std::min(k - done, q->cap - e);
+    RG_HIP(hipMemcpyAsync(q->rP + e * M, P + done * M, (size_t)(m * M * 8), hipMemcpyHostToDevice, st));
+    RG_HIP(hipMemcpyAsync(q->rBelow + e, below + done, (size_t)m * 8, hipMemcpyHostToDevice, st));
+    RG_HIP(hipMemcpyAsync(q->rAbove + e, above + done, (size_t)m * 8, hipMemcpyHostToDevice, st));
+    if (q->corr & 1) RG_HIP(hipMemcpyAsync(q->rDx + e * M, dx2 + done * M, (size_t)(m * M * 8), hipMemcpyHostToDevice, st));
+    if (q->corr & 2) RG_HIP(hipMemcpyAsync(q->rDz + e * M, dz + done * M, (size_t)(m * M * 8), hipMemcpyHostToDevice, st));
+    done += m;
+  }
+  q->tail += k;
+  hipLaunchKernelGGL(k_set_i64, dim3(1), dim3(64), 0, st, q->ctr + QC_TAIL, q->tail);
+  RG_LAUNCH_CHECK();
+  RG_HIP(hipStreamSynchronize(st));
+  return RG_OK;
+}
+
+// Runs `cycles` cycles (the first call first gives every slot a draw) and returns, once they are
+// done, status[0..4] = draws taken, draws pushed, draws retired, live slots, checks run.
+extern "C" int rumgpu_bm_queue_run(rumgpu_bm* b, int64_t cycles, int64_t* status) {
+  RG_CHECK(b && b->queue && cycles >= 0 && status, RG_EINVAL, "bm_queue_run: bad argument");
+  RG_HIP(hipSetDevice(b->ctx->device));
+  RG_TRY(BM_DISPATCH(b, queue_run_t, b, cycles));
+  for (int k = 0; k < 5; ++k) status[k] = b->queue->h_ctr[k];
+  return RG_OK;
+}
+
+// Results of the draws with sequence numbers first .. first + count - 1: lb and ub of the
+// non-singleton part (running max / min at retirement), iterations, retirement rank and the stop
+// (0 open, 1 threshold, 2 tolerance, 3 time, 4 max_iter). Any output may be null.
+extern "C" int rumgpu_bm_queue_results(rumgpu_bm* b, int64_t first, int64_t count, double* lb, double* ub,
+                                       int64_t* iters, int64_t* order, int32_t* stop) {
+  RG_CHECK(b && b->queue, RG_EINVAL, "bm_queue_results: no queue");
+  BmQueue* q = b->queue;
+  RG_CHECK(first >= 0 && count >= 0 && first + count <= q->total, RG_EINVAL, "bm_queue_results: range outside the run");
+  if (count == 0) return RG_OK;
+  hipStream_t st = b->ctx->stream;
+  if (lb) RG_HIP(hipMemcpyAsync(lb, q->qlb + first, (size_t)count * 8, hipMemcpyDeviceToHost, st));
+  if (ub) RG_HIP(hipMemcpyAsync(ub, q->qub + first, (size_t)count * 8, hipMemcpyDeviceToHost, st));
+  if (iters) RG_HIP(hipMemcpyAsync(iters, q->qit + first, (size_t)count * 8, hipMemcpyDeviceToHost, st));
+  if (order) RG_HIP(hipMemcpyAsync(order, q->qorder + first, (size_t)count * 8, hipMemcpyDeviceToHost, st));
+  if (stop) RG_HIP(hipMemcpyAsync(stop, q->qstop + first, (size_t)count * 4, hipMemcpyDeviceToHost, st));
+  RG_HIP(hipStreamSynchronize(st));
+  return RG_OK;
+}
+
+// Per slot: the draw it holds (-1 idle) and that draw's iterations so far.
+extern "C" int rumgpu_bm_queue_slots(rumgpu_bm* b, int64_t* draw, int64_t* iters) {
+  RG_CHECK(b && b->queue && draw && iters, RG_EINVAL, "bm_queue_slots: bad argument");
+  hipStream_t st = b->ctx->stream;
+  RG_HIP(hipMemcpyAsync(draw, b->queue->sdraw, (size_t)b->D * 8, hipMemcpyDeviceToHost, st));
+  RG_HIP(hipMemcpyAsync(iters, b->queue->sit, (size_t)b->D * 8, hipMemcpyDeviceToHost, st));
+  RG_HIP(hipStreamSynchronize(st));
+  return RG_OK;
+}
+
+// The host's time stop: slot d's draw stops with stop 3 at the next decision when kill[d] != 0
+// (unless a threshold or the tolerance stops it first, as in the host driver).
+extern "C" int rumgpu_bm_queue_expire(rumgpu_bm* b, const int32_t* kill) {
+  RG_CHECK(b && b->queue && kill, RG_EINVAL, "bm_queue_expire: bad argument");
+  RG_HIP(rg_copy(b->ctx->stream, b->queue->skill, kill, (size_t)b->D * 4, hipMemcpyHostToDevice));
+  return RG_OK;
+}
+
+extern "C" int rumgpu_bm_queue_end(rumgpu_bm* b) {
+  RG_CHECK(b, RG_EINVAL, "bm_queue_end: null handle");
+  (void)hipStreamSynchronize(b->ctx->stream);
+  queue_free(b);
   return RG_OK;
 }
 

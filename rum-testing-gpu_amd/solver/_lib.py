@@ -55,6 +55,13 @@ SIGNATURES = {
     "rumgpu_bm_slot_get_obs": [_P, _I, _P, _P],
     "rumgpu_bm_certify": [_P, _I, _P, _P, _P],
     "rumgpu_bm_slot_fit": [_P, _I, _P],
+    "rumgpu_bm_queue_start": [_P, _P, _I64, _I64, _I, _I, _I64, _D, _D, _I],
+    "rumgpu_bm_queue_push": [_P, _I64, _P, _P, _P, _P, _P],
+    "rumgpu_bm_queue_run": [_P, _I64, _P],
+    "rumgpu_bm_queue_results": [_P, _I64, _I64, _P, _P, _P, _P, _P],
+    "rumgpu_bm_queue_slots": [_P, _P, _P],
+    "rumgpu_bm_queue_expire": [_P, _P],
+    "rumgpu_bm_queue_end": [_P],
     "rumgpu_butterfly64": [_P, _I, _P, _D, _D, _D, _D],
     "rumgpu_butterfly32": [_P, _I, _P, _F, _F, _F, _F],
     "rumgpu_prox_slab": [_P, _P, _P],

@@ -181,6 +181,58 @@ class Handle:
         _lib.check(self.lib.rumgpu_bm_slot_fit(self.h, int(slot), _lib.ptr(fit)), "bm_slot_fit")
         return fit
 
+    # ---------------------------------------------------------------- device draw queue
+    def queue_start(self, src, total, capacity, check, lb_every, max_iter, abs_tol, rel_tol, corr=0):
+        _lib.check(self.lib.rumgpu_bm_queue_start(self.h, src.h, int(total), int(capacity), int(check),
+                                                  int(lb_every), int(max_iter), float(abs_tol), float(rel_tol),
+                                                  int(corr)), "bm_queue_start")
+
+    def queue_push(self, P, below, above, dx2=None, dz=None):
+        P = _lib.c_array(P, np.float64)
+        k = P.shape[0] if P.ndim == 2 else 0
+        if P.ndim != 2 or P.shape[1] != self.M:
+            raise ValueError("queue_push: p must be k x %d" % self.M)
+        below, above = _lib.c_array(below, np.float64), _lib.c_array(above, np.float64)
+        cors = [None if c is None else _lib.c_array(c, np.float64) for c in (dx2, dz)]
+        for v in [below, above]:
+            if v.size != k:
+                raise ValueError("queue_push: %d thresholds for %d draws" % (v.size, k))
+        for c in cors:
+            if c is not None and c.shape != P.shape:
+                raise ValueError("queue_push: corrections must be k x %d" % self.M)
+        _lib.check(self.lib.rumgpu_bm_queue_push(self.h, k, _lib.ptr(P), _lib.ptr(below), _lib.ptr(above),
+                                                 _lib.ptr(cors[0]), _lib.ptr(cors[1])), "bm_queue_push")
+
+    def queue_run(self, cycles):
+        """-> (taken, pushed, retired, live, checks)"""
+        st = np.zeros(5, dtype=np.int64)
+        _lib.check(self.lib.rumgpu_bm_queue_run(self.h, int(cycles), _lib.ptr(st)), "bm_queue_run")
+        return tuple(int(v) for v in st)
+
+    def queue_results(self, first, count):
+        """-> (lb, ub, iterations, retirement rank, stop) of draws first .. first + count - 1"""
+        lb, ub = np.empty(count), np.empty(count)
+        it, order = np.empty(count, dtype=np.int64), np.empty(count, dtype=np.int64)
+        stop = np.empty(count, dtype=np.int32)
+        _lib.check(self.lib.rumgpu_bm_queue_results(self.h, int(first), int(count), _lib.ptr(lb), _lib.ptr(ub),
+                                                    _lib.ptr(it), _lib.ptr(order), _lib.ptr(stop)),
+                   "bm_queue_results")
+        return lb, ub, it, order, stop
+
+    def queue_slots(self):
+        draw, it = np.empty(self.slots, dtype=np.int64), np.empty(self.slots, dtype=np.int64)
+        _lib.check(self.lib.rumgpu_bm_queue_slots(self.h, _lib.ptr(draw), _lib.ptr(it)), "bm_queue_slots")
+        return draw, it
+
+    def queue_expire(self, kill):
+        k = np.ascontiguousarray(kill, dtype=np.int32)
+        if k.size != self.slots:
+            raise ValueError("queue_expire: one flag per slot")
+        _lib.check(self.lib.rumgpu_bm_queue_expire(self.h, _lib.ptr(k)), "bm_queue_expire")
+
+    def queue_end(self):
+        _lib.check(self.lib.rumgpu_bm_queue_end(self.h), "bm_queue_end")
+
     def prox_slab(self, xk):
         xk = _lib.c_array(xk, np.float64)
         out = np.empty_like(xk)

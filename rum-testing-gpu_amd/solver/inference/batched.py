@@ -73,9 +73,15 @@ class BatchedSolver:
     collect keep (dp, x2_obs - x2_hat_obs, z_obs - z_hat_obs, iterations) of every draw retired
             with a met tolerance (True) or of every retired draw ("all"), as training pairs for the
             warm-start model
+    queue   retire and refill on the device (rumgpu_bm_queue_*): the stop tests and refills run in
+            the replayed graphs and the host only tops up the pending draws between runs; False
+            takes every decision on the host after each check. Both retire every draw at the same
+            check with the same bracket. Collecting training pairs and cold slots (no kept p_hat
+            state) use the host loop.
     """
 
-    def __init__(self, boot, slots=None, dtype="float64", model=None, collect=False, max_slots=256, verbose=False):
+    def __init__(self, boot, slots=None, dtype="float64", model=None, collect=False, max_slots=256, verbose=False,
+                 queue=True):
         from ..bm_admm.engine import Handle
         sess = boot.session
         if sess.method != "bm_admm":
@@ -89,7 +95,7 @@ class BatchedSolver:
         self.n, self.menu, self.alt = rum.n, g.menu, g.alternative.astype(np.int32)
         self.rest, self.single = rum._rest, rum._single
         self.dtype = np.dtype(dtype)
-        self.model, self.collect, self.pairs = model, collect, []
+        self.model, self.collect, self.pairs, self.queue = model, collect, [], bool(queue)
         kept = sess._kept
         self.src = kept["handle"] if kept is not None else None
         if kept is not None and "beta" in kept:
@@ -140,7 +146,7 @@ class BatchedSolver:
         idx = [int(b) for b in idx]
         if not idx:
             return
-        D, h = self.D, self.h
+        h = self.h
         kept = self.boot.session._kept               # a refinement round may have re-kept p_hat
         self.src = kept["handle"] if kept is not None else None
         if kept is not None and "beta" in kept and float(kept["beta"]) != h.beta:
@@ -150,6 +156,93 @@ class BatchedSolver:
             h.slot_load(0, self.src)
             self.x_hat_obs, self.z_hat_obs = h.slot_get_obs(0)
         draw_time = float(defaults.BOOTSTRAP["draw_time"] if draw_time is None else draw_time)
+        if self.queue and self.src is not None and not self.collect:
+            yield from self._solve_on_device(idx, P_of, abs_tol, rel_tol, tau_lo, tau_hi, draw_time)
+        else:
+            yield from self._solve_on_host(idx, P_of, abs_tol, rel_tol, tau_lo, tau_hi, draw_time)
+
+    def _draw_rows(self, sel, P_of, tau_lo, tau_hi):
+        """p on the non-singleton rows, the singleton part's constant and the thresholds on the
+        non-singleton part, per draw of sel"""
+        P = np.asarray(P_of(sel), dtype=np.float64)
+        const = np.empty(len(sel))
+        for k in range(len(sel)):
+            r_s = 1.0 - P[k, self.single]
+            const[k] = float(r_s @ r_s)
+        below = np.array([_on_rest(tau_lo, c, False) for c in const])
+        above = np.array([_on_rest(tau_hi, c, True) for c in const])
+        return np.ascontiguousarray(P[:, self.rest]), const, below, above
+
+    def _solve_on_device(self, idx, P_of, abs_tol, rel_tol, tau_lo, tau_hi, draw_time):
+        """The draw queue: draws are pushed in order into a ring, the device retires and refills
+        slots after every check, and each run of lb_every cycles ends with the host reading the
+        retired draws (in retirement order), topping up the ring and applying time stops."""
+        D, h, total = self.D, self.h, len(idx)
+        cap = min(total, max(8 * D, 256))
+        h.queue_start(self.src, total, cap, self.check, self.lb_every, self.max_iter, abs_tol, rel_tol,
+                      corr=2 if self.model is not None else 0)
+        const = np.empty(total)
+        pushed = 0
+        done = np.zeros(total, dtype=bool)
+        lo = 0                                     # every draw before lo has been yielded
+        t_seen = {}
+        taken = 0
+        stops = {1: "threshold", 2: "tol", 3: "time", 4: "iters"}
+        t_all = time.perf_counter()
+        try:
+            while True:
+                upto = min(total, taken + cap)      # the ring holds cap draws no slot has taken
+                if upto > pushed:
+                    t0 = time.perf_counter()
+                    sel = idx[pushed:upto]
+                    Pr, c, below, above = self._draw_rows(sel, P_of, tau_lo, tau_hi)
+                    dz = None
+                    if self.model is not None:
+                        tm = time.perf_counter()
+                        _dx, dz = self.model.corrections(Pr - self.p_hat_rest)
+                        self.stats["model_s"] += time.perf_counter() - tm
+                    h.queue_push(Pr, below, above, None, dz)
+                    const[pushed:upto] = c
+                    pushed = upto
+                    self.stats["refill_s"] += time.perf_counter() - t0
+                t0 = time.perf_counter()
+                taken, _pushed, retired, _live, _checks = h.queue_run(self.lb_every)
+                self.stats["solve_s"] += time.perf_counter() - t0
+                self.stats["checks"] = self.stats.get("checks", 0) + self.lb_every
+                self.stats["iterations"] += self.check * self.lb_every
+                self.stats["slot_iterations"] += self.check * self.lb_every * D
+                if taken > lo:
+                    lb, ub, it, order, stop = h.queue_results(lo, taken - lo)
+                    fin = np.flatnonzero((stop != 0) & ~done[lo:taken])
+                    for k in fin[np.argsort(order[fin], kind="stable")]:
+                        g = lo + int(k)
+                        c = const[g]
+                        L = math.sqrt(max(lb[k], 0.0) ** 2 + c)
+                        U = math.sqrt(ub[k] ** 2 + c) if math.isfinite(ub[k]) else math.inf
+                        done[g] = True
+                        self.stats["draws"] += 1
+                        self.stats["busy_slot_iterations"] += int(it[k])
+                        yield idx[g], L, U, _flag(L, U, abs_tol, rel_tol, stops[int(stop[k])])
+                    while lo < taken and done[lo]:
+                        lo += 1
+                if retired >= total:
+                    break
+                if math.isfinite(draw_time):
+                    draw, _it = h.queue_slots()
+                    now = time.perf_counter()
+                    kill = np.zeros(D, dtype=np.int32)
+                    for s_, g in enumerate(draw):
+                        if g >= 0 and now - t_seen.setdefault(int(g), now) > draw_time:
+                            kill[s_] = 1
+                    if kill.any():
+                        h.queue_expire(kill)
+        finally:
+            h.queue_end()
+        self.stats["wall_s"] = self.stats.get("wall_s", 0.0) + time.perf_counter() - t_all
+        self.stats["slot_use"] = self.stats["busy_slot_iterations"] / max(1, self.stats["slot_iterations"])
+
+    def _solve_on_host(self, idx, P_of, abs_tol, rel_tol, tau_lo, tau_hi, draw_time):
+        D, h = self.D, self.h
         chunk = max(4 * D, 64)
         queue_pos, buf, buf_pos = 0, None, 0
         # per-slot bookkeeping

@@ -132,5 +132,10 @@ class WarmStart:
         z = self.net.forward((np.asarray(dp) / self.sx)[None, :])[0].astype(np.float64) * self.sz
         return None, z
 
+    def corrections(self, dP):
+        """correction() of many draws at once (rows of dP): one forward per max_batch rows"""
+        Z = self.net.forward(np.asarray(dP) / self.sx).astype(np.float64) * self.sz
+        return None, Z
+
     def close(self):
         self.net.close()

@@ -147,6 +147,11 @@ def test_batched_bootstrap_same_decisions():
     with rum.bootstrap(panel=panel, B=B, batch=dict(slots=16), **args) as bat:
         pv2 = bat.p_value()
         st = bat.batch_stats
+    with rum.bootstrap(panel=panel, B=B, batch=dict(slots=16, queue=False), **args) as host:
+        host.p_value()
+    # retire and refill on the device or on the host: the same brackets bit for bit
+    for a, b in ((bat.L, host.L), (bat.U, host.U), (bat.flag, host.flag), (bat.tol_stop, host.tol_stop)):
+        assert np.array_equal(a, b), np.flatnonzero(a != b)
     print(pv1)
     print(pv2)
     print(st)
@@ -308,8 +313,9 @@ def test_config4_alt0_slots_reach_per_draw_decisions():
     screen alone, see the module notes). p_hat is certified once; its open draws are then solved
     twice from the same screened brackets: one float64 Session.solve per draw (the reference's
     loop) and the draw slots of one float64 batch object (graph-replayed iterations, retire and
-    refill, float64 certificates per slot). Every draw's decision must agree, and every pair of
-    brackets must overlap."""
+    refill on the device, float64 certificates per slot). Every draw's decision must agree, and
+    every pair of brackets must overlap. A third pass with the host taking the retire and refill
+    decisions must give the device queue's brackets bit for bit."""
     import time
     import pandas as pd
     from solver.inference.bootstrap import _Batched, _PerDraw
@@ -339,3 +345,15 @@ def test_config4_alt0_slots_reach_per_draw_decisions():
         print("per draw %.1f s, slots %.1f s for %d draws; %s" % (t_one, t_bat, open_.size, boot.batch_stats))
         assert np.array_equal(d_one, d_bat), np.flatnonzero(d_one != d_bat)
         assert (boot.L[open_] <= U1[open_] + 1e-9).all() and (L1[open_] <= boot.U[open_] + 1e-9).all()
+        # the slots retired and refilled by the host after every check: the same draws retire at the
+        # same checks, so every bracket, flag and tolerance equals the device queue's bit for bit
+        queue = (boot.L.copy(), boot.U.copy(), boot.flag.copy())
+        boot.L[:], boot.U[:] = screened
+        runner = _Batched(boot, {"slots": 32, "queue": False})
+        t0 = time.perf_counter()
+        runner.run(open_, boot.abs_tol, boot.rel_tol, stops=True, until=None)
+        t_host = time.perf_counter() - t0
+        runner.close()
+        print("host-driven slots %.1f s; %s" % (t_host, boot.batch_stats))
+        for a, b in zip(queue, (boot.L, boot.U, boot.flag)):
+            assert np.array_equal(a, b), np.flatnonzero(a != b)
