@@ -198,8 +198,13 @@ __global__ __launch_bounds__(256, 4) void bm_prox_iter(const T* __restrict__ xk,
   const bool mixed = (om != 0) && (om != S);
   const double c_obs = (double)b2 + 2.0, c_un = (double)b2;
   // every member's value first, with no use in between, so all the loads are in flight together;
-  // key = v when the menu has no observation ((b2 v) / b2 = v)
-  double key[N];
+  // key = v when the menu has no observation ((b2 v) / b2 = v).
+  // The per-member arrays hold at least 17 elements: the AMDGPU backend turns a private array of
+  // up to 16 elements that is written under branches into one vector value, and every branch
+  // then copies the whole vector (n = 9-16 spilled 0.1-1.5 KB per lane to scratch and ran the
+  // prox about 30 times slower); from 17 elements on the array is split into scalar registers.
+  constexpr int NA = N < 17 ? 17 : N;
+  double key[NA];
 #pragma unroll
   for (int i = 0; i < N; ++i) {
     key[i] = 0.0;
@@ -209,7 +214,7 @@ __global__ __launch_bounds__(256, 4) void bm_prox_iter(const T* __restrict__ xk,
     // observed menus: the observed members' p, loaded without branches (a clamped index for the
     // others, whose value is not used) so the loads are in flight together, then the keys
     const int nob = __popc(om);
-    double xh[N];
+    double xh[NA];
 #pragma unroll
     for (int i = 0; i < N; ++i) {
       const int rk = __popc(om & ((1u << i) - 1u));
@@ -274,7 +279,7 @@ __global__ __launch_bounds__(256, 4) void bm_prox_iter(const T* __restrict__ xk,
       key[i] = (double)(yv > 0.0 ? (T)yv : (T)0.0);
     }
   }
-  T xo[N];
+  T xo[NA];
 #pragma unroll
   for (int i = 0; i < N; ++i) {
     xo[i] = (T)0;
