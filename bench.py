@@ -81,6 +81,10 @@ def _parse():
     ap.add_argument("--config3-only", action="store_true",
                     help="only the n=22 (config3) iteration roofline (prints its JSON object)")
     ap.add_argument("--no-config3", action="store_true", help="skip the n=22 iteration roofline")
+    ap.add_argument("--no-slots", action="store_true",
+                    help="skip the batched BM-ADMM bootstrap (n=16 alternative-0 panel: draw slots with the device "
+                         "queue, host-driven slots, per-draw solves, learned warm start)")
+    ap.add_argument("--slots-only", action="store_true", help="only the batched BM-ADMM bootstrap")
     ap.add_argument("--no-ai", action="store_true", help="skip the reference's stored AI run (n=18, 73,492 cells)")
     ap.add_argument("--pricing-sweeps", default="",
                     help="only slab pricing sweeps on one instance (n20_d1pct, n25_sparse or config3), "
@@ -144,6 +148,9 @@ def main():
         return
     if a.bootstrap_only:
         print(json.dumps(_bootstrap_bench(a)), flush=True)
+        return
+    if a.slots_only:
+        print(json.dumps(_slots_bench()), flush=True)
         return
     if a.dsr_all_alternatives:
         print(json.dumps(_dsr_all_alternatives()), flush=True)
@@ -290,6 +297,7 @@ def main():
     dsr = None if a.no_dsr else _dsr_bench()
     cfg3 = None if a.no_config3 else _config3_bench(a)
     ai = None if a.no_ai else _ai_bench()
+    slots = None if a.no_slots else _slots_bench()
 
     line = dict(metric=METRIC, value=round(value, 5), unit="certified solves/s", n_gpus=1, steps=a.steps,
                 warmup=a.warmup, ms_per_step=round(1e3 * elapsed / a.steps, 2), higher_is_better=True,
@@ -304,7 +312,7 @@ def main():
                             iters64=[r.info.get("iters64") for r in results],
                             bracket=[r_last.lb, r_last.ub], certified=certified),
                 roofline=roofline, cpu_baseline=cpu, plain=plain, bootstrap=boot, dsr_bootstrap=dsr,
-                config3_iteration=cfg3, ai_stored_run=ai,
+                config3_iteration=cfg3, ai_stored_run=ai, bm_admm_slots=slots,
                 tuned_vs_plain=round(value / plain["value"], 3) if plain and plain["value"] > 0 else None)
     print(json.dumps(line), flush=True)
 
@@ -423,6 +431,97 @@ def _reference_instances(only=()):
                 geomean_speedup=round(math.exp(sum(math.log(x) for x in sp) / len(sp)), 2),
                 seconds=round(sum(r["seconds"] for r in rows), 1),
                 stored_seconds=round(sum(r["stored_seconds"] for r in rows), 1))
+
+
+def _slots_bench(B=1024, per_draw=96, train=128, test=128, slots=32):
+    """Batched BM-ADMM bootstrap on the n = 16 alternative-0 panel (the n = 25 workload's generator at
+    n = 16: 301 cells of alternative 0, 520 respondents x 15 menus, seed 16002; B = 1,024
+    multinomial_reject draws, seed 0, abs 5e-4, float64 iterates and certificates). p_hat is
+    certified and the draws screened (untimed); then the open draws are solved with the band's
+    stops, warm from p_hat:
+      per_draw  one Session.solve per draw, the reference's loop (bootstrap.py:308-335), on the
+                first `per_draw` open draws
+      queue     draw slots of one batch object with retirement and refill on the device
+      host      the same slots with the host taking every decision after each check
+      learned   the MLP warm start trained on the GPU on the certified states of the first `train`
+                open draws, then the next `test` open draws solved by the queue with and without it
+    Decisions must agree everywhere; queue and host must give the same brackets bit for bit."""
+    import numpy as np
+    import pandas as pd
+    from solver import instances
+    from solver.inference import cells_from_panel, hong_li
+    from solver.inference.batched import BatchedSolver
+    from solver.learned import WarmStart
+    from solver.rum_solver import RUM
+    panel, n = instances.config4_alt0()
+    panel = pd.DataFrame(panel)
+    menu, alt, p = cells_from_panel(panel, n, alternatives=[0])
+    rum = RUM(menu, alt, p, num_alternatives=n)
+    out = dict(workload="bootstrap draws of the n=16 alternative-0 panel (seed 16002: 301 cells, 520 respondents "
+                        "x 15 menus), B=%d, bm_admm float64, abs 5e-4, band stops, warm from p_hat" % B,
+               unit="draws/s", slots=slots)
+    with rum.bootstrap(panel=panel, B=B, method="bm_admm", method_args={"dtype": "float64"}) as boot:
+        t0 = time.perf_counter()
+        boot._start()
+        und = np.flatnonzero(boot._undecided())
+        out.update(p_hat_s=round(time.perf_counter() - t0, 2), open_draws=int(und.size), screened=boot.n_screened)
+        print("[bench] slots: p_hat in %.1f s, %d open draws" % (out["p_hat_s"], und.size), file=sys.stderr,
+              flush=True)
+        tau = boot.tau
+        P_of = lambda sel: hong_li.perturb(boot.p_hat, boot.draws.rows(sel), boot.r_n, boot.eps_n)  # noqa: E731
+
+        def run(idx, **kw):
+            bs = BatchedSolver(boot, slots=slots, **kw)
+            t = time.perf_counter()
+            res = {b: (L, U, f) for b, L, U, f in bs.solve(idx, P_of, boot.abs_tol, 0.0, tau[0], tau[1], 120)}
+            t = time.perf_counter() - t
+            st, pairs = dict(bs.stats), bs.pairs
+            bs.close()
+            return res, t, st, pairs
+
+        legs = {}
+        for name, kw in (("queue", {}), ("host", {"queue": False})):
+            res, t, st, _ = run(und, **kw)
+            legs[name] = res
+            out[name] = round(und.size / t, 2)
+            out[name + "_slot_use"] = round(st["slot_use"], 3)
+        same_bits = all(legs["queue"][b] == legs["host"][b] for b in und)
+        # one Session.solve per draw on the first per_draw open draws
+        sess = boot.session
+        sample = und[:per_draw]
+        t = time.perf_counter()
+        one = {}
+        for b, P in zip(sample, P_of(sample)):
+            r = sess.solve(P, abs_tol=boot.abs_tol, rel_tol=0.0, max_time=120, warm=True, below=tau[0], above=tau[1])
+            one[b] = (r.lb, r.ub)
+        out["per_draw"] = round(sample.size / (time.perf_counter() - t), 2)
+        dq = hong_li.decide([legs["queue"][b][0] for b in sample], [legs["queue"][b][1] for b in sample], *tau)
+        d1 = hong_li.decide([one[b][0] for b in sample], [one[b][1] for b in sample], *tau)
+        out.update(queue_vs_per_draw=round(out["queue"] / out["per_draw"], 2), queue_equals_host_bitwise=same_bits,
+                   decisions_identical=bool(np.array_equal(dq, d1)))
+        # learned warm start: trained on the first `train` open draws, tested on the next `test`
+        if und.size >= train + test:
+            tr, te = und[:train], und[train:train + test]
+            _r, _t, _s, pairs = run(tr, collect="all")
+            ws = WarmStart(rum.M, 512, seed=0)
+            tt = time.perf_counter()
+            hist = ws.train(pairs, epochs=300, lr=1e-3, batch=64)
+            train_s = time.perf_counter() - tt
+            rp, tp, sp, _ = run(te)
+            rl, tl, sl, _ = run(te, model=ws)
+            ws.close()
+            dp = hong_li.decide([rp[b][0] for b in te], [rp[b][1] for b in te], *tau)
+            dl = hong_li.decide([rl[b][0] for b in te], [rl[b][1] for b in te], *tau)
+            ip, il = sp["busy_slot_iterations"] / te.size, sl["busy_slot_iterations"] / te.size
+            out["learned"] = dict(train_draws=int(tr.size), test_draws=int(te.size), train_s=round(train_s, 3),
+                                  loss=[round(hist[0], 5), round(hist[-1], 5)],
+                                  plain=round(te.size / tp, 2), learned=round(te.size / tl, 2),
+                                  iterations_per_draw=[round(ip, 1), round(il, 1)],
+                                  iterations_saved=round(1 - il / ip, 4),
+                                  decisions_identical=bool(np.array_equal(dp, dl)))
+    print("[bench] slots: per draw %.1f, queue %.1f, host %.1f draws/s; learned %s" % (
+        out["per_draw"], out["queue"], out["host"], out.get("learned")), file=sys.stderr, flush=True)
+    return out
 
 
 def _dsr_bench():

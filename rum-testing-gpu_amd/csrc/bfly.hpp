@@ -27,6 +27,7 @@ struct WinGeom {
   i64 Nloc;      // 2^nb
   i64 ntiles;    // items * 2^(nb - B - C)
   int nper;      // items per slot (n): a batch of D draws is n * D item blocks
+  const int* on; // per slot: 0 = idle, its tiles move no data (null: every slot runs)
 };
 
 // Host-side window plan for a slab of itemsize bytes: list of (s, B, C, RL).
@@ -103,6 +104,7 @@ inline WinGeom make_geom(int n, const WinPlan& w, int slots = 1) {
   g.TPB = g.TT >= 256 ? 1 : 256 / g.TT;
   g.Nloc = 1ll << g.nb;
   g.ntiles = ((i64)n * slots) << (g.nb - w.B - w.C);
+  g.on = nullptr;
   if (g.ntiles < g.TPB) g.TPB = (int)g.ntiles;
   if (g.TPB < 1) g.TPB = 1;
   return g;
@@ -169,6 +171,7 @@ struct Tile {
       x = tile >> outer_bits;
       o = tile & ((1ll << outer_bits) - 1);
     }
+    if (active && g.on && !g.on[x / g.nper]) active = false;   // an idle draw slot
     const int lowo = s - C;
     const i64 lowpart = o & ((1ll << lowo) - 1);
     const i64 highpart = o >> lowo;

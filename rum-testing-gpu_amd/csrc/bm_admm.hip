@@ -173,7 +173,8 @@ template <typename T, int N>
 __global__ __launch_bounds__(256, 4) void bm_prox_iter(const T* __restrict__ xk, T* __restrict__ out, T* __restrict__ z,
                                                     const u32* __restrict__ obsmask, const int* __restrict__ obsoff,
                                                     const double* __restrict__ pc, T b2, i64 m2slot, i64 Mslot,
-                                                    int* __restrict__ fresh) {
+                                                    int* __restrict__ fresh, const int* __restrict__ on) {
+  if (on && !on[blockIdx.y]) return;      // an idle draw slot
   // slot = blockIdx.y: its slab and its p
   xk += (i64)blockIdx.y * m2slot;
   out += (i64)blockIdx.y * m2slot;
@@ -838,13 +839,13 @@ int launch_pass(int kind, int RL, const PassArgs<T>& P, hipStream_t st) {
 
 template <typename T, int MODE>
 int launch_prox(int n, const T* xk, T* out, T* z, const u32* om, const int* oo, const double* pc, T b2,
-                hipStream_t st, int D = 1, i64 m2 = 0, i64 M = 0, int* fresh = nullptr) {
+                hipStream_t st, int D = 1, i64 m2 = 0, i64 M = 0, int* fresh = nullptr, const int* on = nullptr) {
   const i64 NN = 1ll << n;
   const dim3 grid(grid_for(NN, 256), (unsigned)D), block(256);
 #define RG_PROX_CASE(NV)                                                                                   \
   case NV:                                                                                                 \
     if constexpr (MODE == 0)                                                                               \
-      pass_launch((bm_prox_iter<T, NV>), grid, block, 0, st, xk, out, z, om, oo, pc, b2, m2, M, fresh); \
+      pass_launch((bm_prox_iter<T, NV>), grid, block, 0, st, xk, out, z, om, oo, pc, b2, m2, M, fresh, on); \
     else                                                                                                   \
       hipLaunchKernelGGL((bm_prox<T, NV, MODE>), grid, block, 0, st, xk, out, z, om, oo, pc, b2, m2, M);  \
     break;
@@ -874,6 +875,7 @@ struct rumgpu_bm {
   int n = 0, nb = 0, isz = 8, plain = 0;
   int D = 1;                  // draw slots: every slab, p, fit and node array holds D blocks
   int* fresh = nullptr;       // per slot: state loaded since the last iteration
+  int* on = nullptr;          // per slot: 1 = the iterations advance it (the draw queue idles empty slots)
   double* dwarm = nullptr;    // 2 M float64: warm-start corrections at the observed cells
   i64 Nloc = 0, m2 = 0, M = 0, NN = 0;
   double beta = 1, alpha = 1.95, pp = 1.4142135623730951;
@@ -921,13 +923,13 @@ struct rumgpu_bm {
   struct BmQueue* queue = nullptr;   // the device draw queue of a batch object (rumgpu_bm_queue_*)
 };
 
-static void queue_free(rumgpu_bm* b);
+static void queue_free(rumgpu_bm* b, bool reset_on = true);
 
 static void bm_free_all(rumgpu_bm* b) {
   void* bufs[] = {b->x2, b->s, b->y, b->z, b->tb, b->xk, b->inv_gl, b->kx2, b->ks, b->ky, b->kz,
                   b->pw, b->pAT, b->prhs, b->px, b->pAx, b->px2n, b->obsmask, b->obsoff, b->pc, b->obs,
-                  b->p, b->cert, b->outw, b->mass, b->order, b->part, b->res, b->fit, b->fresh, b->dwarm};
-  queue_free(b);
+                  b->p, b->cert, b->outw, b->mass, b->order, b->part, b->res, b->fit, b->fresh, b->dwarm, b->on};
+  queue_free(b, false);
   for (void* q : bufs) rg_free(q);
   if (b->h_res) (void)hipHostFree(b->h_res);
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
@@ -966,6 +968,7 @@ static PassArgs<T> pass_args(rumgpu_bm* b, int w) {
   P.cQt = coef<T>(b->Q[0], b->Q[2], b->Q[1], b->Q[3]);
   P.cT = P.cA;
   P.g = b->geoms[w];
+  P.g.on = b->on;
   P.fresh = b->fresh;
   return P;
 }
@@ -1002,7 +1005,7 @@ static int fused_iter(rumgpu_bm* b) {
   }
   prof_arm(b, 4);
   RG_TRY((launch_prox<T, 0>(b->n, (const T*)b->xk, (T*)b->x2, (T*)b->z, b->obsmask, b->obsoff, b->pc, (T)b->beta, st,
-                            b->D, b->m2, b->M, b->fresh)));
+                            b->D, b->m2, b->M, b->fresh, b->on)));
   for (int w = k - 2; w >= 1; --w) { prof_arm(b, 5); RG_TRY(run_pass<T>(b, K_T, w, &P0.cA)); }
   b->pending = 1;
   b->iters += 1;
@@ -1258,7 +1261,7 @@ struct QueueArgs {
   i64 *ctr, *sdraw, *sit, *refill, *qit, *qorder;
   double *sub, *slb, *sbelow, *sabove, *qlb, *qub;
   const double *rBelow, *rAbove, *ubsq, *res;
-  int *skill, *qstop;
+  int *skill, *qstop, *on;
   i64 cap, max_iter;
   double abs_tol, rel_tol;
   int D, check;
@@ -1329,6 +1332,7 @@ __global__ void k_queue_decide(QueueArgs a, int want_lb, int prime) {
         a.sdraw[d] = -1;
       }
     }
+    a.on[d] = a.sdraw[d] >= 0 ? 1 : 0;     // an idle slot's tiles move no data until it is refilled
     if (a.sdraw[d] >= 0) ++live;
   }
   a.ctr[QC_HEAD] = head;
@@ -1390,9 +1394,12 @@ __global__ void k_set_i64(i64* a, i64 v) {
   if (blockIdx.x == 0 && threadIdx.x == 0) *a = v;
 }
 
-static void queue_free(rumgpu_bm* b) {
+static void queue_free(rumgpu_bm* b, bool reset_on) {
   BmQueue* q = b->queue;
   if (!q) return;
+  if (reset_on && b->on)      // outside a queue every slot runs
+    hipLaunchKernelGGL((ew_fill<int>), dim3(grid_for((i64)b->D, 256)), dim3(256), 0, b->ctx->stream, b->on, 1,
+                       (i64)b->D);
   for (int k = 0; k < 2; ++k) {
     if (q->gx[k]) (void)hipGraphExecDestroy(q->gx[k]);
     if (q->g[k]) (void)hipGraphDestroy(q->g[k]);
@@ -1412,7 +1419,7 @@ static QueueArgs queue_args(rumgpu_bm* b) {
   a.ctr = q->ctr; a.sdraw = q->sdraw; a.sit = q->sit; a.refill = q->refill; a.qit = q->qit; a.qorder = q->qorder;
   a.sub = q->sub; a.slb = q->slb; a.sbelow = q->sbelow; a.sabove = q->sabove; a.qlb = q->qlb; a.qub = q->qub;
   a.rBelow = q->rBelow; a.rAbove = q->rAbove; a.ubsq = q->ubsq; a.res = b->res;
-  a.skill = q->skill; a.qstop = q->qstop;
+  a.skill = q->skill; a.qstop = q->qstop; a.on = b->on;
   a.cap = q->cap; a.max_iter = q->max_iter;
   a.abs_tol = q->abs_tol; a.rel_tol = q->rel_tol;
   a.D = b->D; a.check = q->check;
@@ -1610,6 +1617,7 @@ static int bm_create_impl(rumgpu_ctx* ctx, int n, const uint64_t* menu, const in
   if ((rc = rg_malloc((void**)&b->part, (size_t)256 * 4 * 8 * D, "partials")) != RG_OK) return fail(rc);
   if ((rc = rg_malloc((void**)&b->res, (16 + 4 * D) * 8, "scalars")) != RG_OK) return fail(rc);
   if ((rc = rg_malloc((void**)&b->fresh, D * 4, "slot flags")) != RG_OK) return fail(rc);
+  if ((rc = rg_malloc((void**)&b->on, D * 4, "slot flags")) != RG_OK) return fail(rc);
   if (hipHostMalloc((void**)&b->h_res, (16 + 4 * D) * 8, hipHostMallocDefault) != hipSuccess) {
     rg_set_error("hipHostMalloc of %zu bytes failed", (size_t)(16 + 4 * D) * 8);
     return fail(RG_ENOMEM);
@@ -1633,6 +1641,7 @@ static int bm_create_impl(rumgpu_ctx* ctx, int n, const uint64_t* menu, const in
     return fail(RG_EHIP);
   }
   hipLaunchKernelGGL((ew_fill<int>), dim3(grid_for((i64)D, 256)), dim3(256), 0, st, b->fresh, 0, (i64)D);
+  hipLaunchKernelGGL((ew_fill<int>), dim3(grid_for((i64)D, 256)), dim3(256), 0, st, b->on, 1, (i64)D);
   if (dtype_bytes == 8)
     hipLaunchKernelGGL(k_inv_gl<double>, dim3(grid_for(b->Nloc, 256)), dim3(256), 0, st, (double*)b->inv_gl, b->nb, e0, e1, gamma);
   else
