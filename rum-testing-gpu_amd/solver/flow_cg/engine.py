@@ -125,14 +125,17 @@ class Pricer:
         ry = math.fsum(r * y)
         return math.sqrt(max(0.0, f - 2.0 * (vmax - ry)))
 
-    SWEEPS = {"layers": 0, "auto": 1}
+    SWEEPS = {"layers": 0, "auto": 1, "window": 2}
     CLOSURE_MAX_N = 20      # the library refuses the closure sweep above this n
+    WINDOW_MAX_N = 26
 
     def set_sweep(self, mode):
-        """'layers' (the default): the layer-by-layer sweep for any rows (the reference's); 'auto': the
-        one-alternative closure sweep when every row belongs to one alternative (same results, 4 bytes
-        per node; opt-in, refused above n = 20: its first run at n = 25 ended in a GPU fault whose cause
-        has not been shown)"""
+        """'layers' (the default): the layer-by-layer sweep for any rows (the reference's). When every
+        row belongs to one alternative: 'window', the window-pass sweep (V as the subset maximum of
+        that alternative's row sums, 128 bytes per subset of the other alternatives, streamed in
+        passes of 8 bits; n <= 26), or 'auto', the closure sweep (4 bytes per node; refused above
+        n = 20: its first run at n = 25 ended in a GPU fault whose cause has not been shown). Both
+        return the layered sweep's values and rankings bit for bit."""
         _lib.check(self.lib.rumgpu_cg_set_sweep(self.h, self.SWEEPS[mode]), "cg_set_sweep")
 
     def set_slab_tile(self, bits=0):

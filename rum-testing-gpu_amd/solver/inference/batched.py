@@ -395,9 +395,13 @@ class BatchedCG:
         self.overlap = bool(overlap) and self.D >= 2
         from .. import machine as _machine
         n = boot.rum.n
-        one_alt = (np.unique(boot.rum._geom.alternative).size == 1 and sess._args.get("_sweep", "layers") == "auto"
-                   and n <= _Pricer.CLOSURE_MAX_N)
-        if one_alt:     # the closure sweep: 4 bytes per subset of the other alternatives, and the scores
+        one_alt = np.unique(boot.rum._geom.alternative).size == 1
+        mode = sess._args.get("_sweep", "layers")
+        if one_alt and mode == "window" and n <= _Pricer.WINDOW_MAX_N:
+            # the window-pass sweep: a 128-byte row per subset of the other alternatives, the scores
+            need = 128 * (1 << (n - 1)) + 32 * 4 * boot.rum.M + 64 * boot.rum.M
+        elif one_alt and mode == "auto" and n <= _Pricer.CLOSURE_MAX_N:
+            # the closure sweep: 4 bytes per subset of the other alternatives, and the scores
             need = 4 * (1 << (n - 1)) + 32 * 4 * boot.rum.M
         else:           # batched V, pred, float64 V (single and 8 draws), scores
             need = (32 * 5 + 8 + 64) * (1 << n) + 32 * 4 * boot.rum.M

@@ -864,9 +864,12 @@ def _cg_pricer(t):
                                                       dtype=a["dtype"])})
     pricer = eng["cg"].items["pricer"]
     if pricer.layout == "mask":
-        # "auto" opts in to the one-alternative closure sweep; "layers" (the default) is the
-        # reference's layer-by-layer sweep
-        pricer.set_sweep(a.get("_sweep", "layers") if g.num_alternatives <= Pricer.CLOSURE_MAX_N else "layers")
+        # "layers" (the default) is the reference's layer-by-layer sweep; for rows of one alternative
+        # "window" takes the window-pass sweep and "auto" the closure sweep (n <= 20)
+        mode = a.get("_sweep", "layers")
+        if mode == "auto" and g.num_alternatives > Pricer.CLOSURE_MAX_N:
+            mode = "layers"
+        pricer.set_sweep(mode)
     return pricer
 
 

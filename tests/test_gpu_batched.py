@@ -228,6 +228,49 @@ def test_one_alternative_sweep_equals_layers(n, a, D, quant):
     pr.close()
 
 
+@pytest.mark.parametrize("n,a,D,quant", [(2, 1, 3, False), (5, 0, 7, True), (9, 4, 32, True), (9, 8, 40, False),
+                                          (13, 0, 33, True), (14, 6, 5, False), (16, 15, 32, True),
+                                          (20, 0, 32, False), (22, 3, 35, True), (25, 0, 32, True)])
+def test_window_sweep_equals_layers(n, a, D, quant):
+    """Rows of one alternative: the window-pass sweep (V as the subset maximum of a's row sums,
+    formed in passes of 8 bits) returns the layered sweep's vmax and ranking bit for bit, batched
+    (32 draws per table, padded) and single, and the same float64 longest path, single and for
+    up to 8 draws at once; quantized scores (multiples of 1/4) make ties common."""
+    from oracle import oracle as orc
+    from solver.flow_cg.engine import Pricer, xmajor
+    rng = np.random.default_rng(950 + n)
+    menu, alt, _p = _medium(n, min(6000, max(2, (n << n) >> 3)), rng)
+    keep = alt == a
+    menu, alt = menu[keep], alt[keep]
+    perm, masks_x, _alts, offsets = xmajor(menu, alt, n)
+    pr = Pricer(n, masks_x, offsets, layout="mask")
+    pr.set_sweep("window")
+    assert pr.one_alternative() == a
+    S = rng.standard_normal((D, menu.size)).astype(np.float32)
+    if quant:
+        S = (np.round(S * 4) / 4).astype(np.float32)
+    R64 = rng.standard_normal((5, menu.size))
+    vb, rb = pr.price_batch(S)
+    v1 = [pr.price(S[d])[:2] for d in range(min(D, 6))]
+    l1 = pr.vmax64(R64[0])
+    lb1 = pr.vmax64_batch(R64)
+    pr.set_sweep("layers")
+    assert pr.one_alternative() == -1
+    vb0, rb0 = pr.price_batch(S)
+    l0 = pr.vmax64(R64[0])
+    lb0 = pr.vmax64_batch(R64)
+    for d in range(D):
+        assert vb[d] == vb0[d] and np.array_equal(rb[d], rb0[d]), d
+        if d < len(v1):
+            v0, r0, _m = pr.price(S[d])
+            assert v1[d][0] == v0 and np.array_equal(v1[d][1], r0), d
+        if n <= 14 and d < 3:
+            vo, ro, _V, _pred = orc.price_mask(n, masks_x, S[d], offsets)
+            assert vo == vb[d] and np.array_equal(ro, rb[d]), d
+    assert l1 == l0 and np.array_equal(lb1, lb0)
+    pr.close()
+
+
 def test_one_alternative_sweep_refused_above_n20():
     """Above n = 20 the library refuses the closure sweep (status 4) and the layered sweep stays on;
     nothing is launched: the refusal happens on the host."""
