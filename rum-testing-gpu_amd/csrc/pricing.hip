@@ -473,137 +473,144 @@ __global__ void cl_backtrace(const u32* __restrict__ F, const u32* __restrict__ 
 // One-alternative sweep by window passes (sweep mode 2). With every row in alternative a, the
 // identity above gives V[D] = max over T' subset of T of W[T'] for every D containing a: V on those
 // subsets is the subset maximum of W over the lattice of the other n - 1 alternatives. This sweep
-// forms that table for 32 draws, V[T][d] (one 128-byte row per T): the first pass computes W for
-// 2^8 consecutive T (a's rows staged in LDS, tested once per T for all draws, each draw's matches
-// summed in row order) and takes the maximum over those 8 bits in LDS; each further pass takes the
-// maximum over up to 8 more bits of T, moving whole 128-byte rows. A maximum returns one of its
-// operands, so the order of the bits changes no value and the table is the layered sweep's V bit
-// for bit. The ranking is a walk from the full set that re-evaluates each visited node's
-// candidates in ascending member order with the layered sweep's strict > (W of the node itself
-// summed again in row order for x = a, V of the predecessor read from the table for x != a).
+// forms that table for 32 draws, V[T][d] (one 128-byte row per T): oa_w computes W (a thread per T,
+// the subset tests shared by the draws, each draw's matches summed in row order), then window
+// passes take the maximum over 8 bits of T at a time with the butterfly tiles (register rounds,
+// one LDS exchange, whole 128-byte rows moved). A maximum returns one of its operands, so the
+// order of the bits changes no value and the table is the layered sweep's V bit for bit. The
+// ranking is a walk from the full set that re-evaluates each visited node's candidates in
+// ascending member order with the layered sweep's strict > (W of the node itself summed again in
+// row order for x = a, V of the predecessor read from the table for x != a).
 // Memory: 128 bytes per subset of the other alternatives (2.1 GB at n = 25), streamed in order,
 // against the layered batched sweep's 128-byte row gathers of every (subset, member) pair.
-#define OA_TB 8          // bits of T per tile of the first pass, and per later window
+#define OA_TB 8          // bits of T per window pass
 #define OA_DB 32         // draws per table row (a batch is padded to 32)
-#define OA_ROWS 192      // rows of a staged in LDS at a time
 
-__global__ __launch_bounds__(256) void oa_first(const u32* __restrict__ cm, const float* __restrict__ sc, int ra,
-                                                int tb, float* __restrict__ Vt) {
-  __shared__ float tile[(1 << OA_TB) * (OA_DB + 1)];
-  __shared__ float srow[OA_ROWS * OA_DB];
-  __shared__ u32 scm[OA_ROWS];
+// W[T][d] for every T: one thread per T. a's menus come 32 at a time through uniform (scalar)
+// loads (padded to a multiple of 32 with a menu no T contains); the rows' scores are staged in LDS
+// with a 33-float stride (a lane's row is its own, so the stride spreads the lanes over the banks),
+// and each lane adds its own matches, rows in order.
+#define OA_ROWS 256
+__device__ __forceinline__ void oa_w_rows(const u32* __restrict__ cm, const float* __restrict__ sc, int ra, u32 T,
+                                          bool own, float* srow, float (&w)[OA_DB]) {
   const int tid = threadIdx.x;
-  const int span = 1 << tb;                          // T per tile: 2^min(8, n - 1)
-  const u32 T = (u32)blockIdx.x * (u32)span + (u32)tid;
-  const bool own = tid < span;
-  float w[OA_DB];
 #pragma unroll
   for (int d = 0; d < OA_DB; ++d) w[d] = 0.0f;
   for (int base = 0; base < ra; base += OA_ROWS) {
     const int cnt = min(OA_ROWS, ra - base);
     __syncthreads();
-    for (int i = tid; i < cnt * OA_DB; i += 256) srow[i] = sc[(size_t)base * OA_DB + i];
-    for (int i = tid; i < cnt; i += 256) scm[i] = cm[base + i];
+    for (int i = tid; i < cnt * OA_DB; i += blockDim.x)
+      srow[(i / OA_DB) * (OA_DB + 1) + (i % OA_DB)] = sc[(size_t)base * OA_DB + i];
     __syncthreads();
-    if (own) {
-      for (int sub = 0; sub < cnt; sub += 32) {
-        const int c32 = min(32, cnt - sub);
-        u32 mm = 0;
-        for (int t = 0; t < c32; ++t) {
-          const u32 m = scm[sub + t];
-          mm |= ((m & T) == m ? 1u : 0u) << t;
-        }
-        while (mm) {                                 // this lane's matches, ascending rows
-          const int t = __ffs((int)mm) - 1;
-          mm &= mm - 1u;
-          const float* sj = srow + (sub + t) * OA_DB;
+    for (int sub = 0; sub < cnt; sub += 32) {
+      u32 mm = 0;
 #pragma unroll
-          for (int d = 0; d < OA_DB; ++d) w[d] += sj[d];
-        }
+      for (int t = 0; t < 32; ++t) {
+        const u32 m = cm[base + sub + t];
+        mm |= ((m & T) == m ? 1u : 0u) << t;
+      }
+      if (!own) mm = 0;
+      while (mm) {                                   // this lane's matches, ascending rows
+        const int t = __ffs((int)mm) - 1;
+        mm &= mm - 1u;
+        const float* sj = srow + (sub + t) * (OA_DB + 1);
+#pragma unroll
+        for (int d = 0; d < OA_DB; ++d) w[d] += sj[d];
       }
     }
   }
+}
+
+// W: the 128-byte row of each T stored whole
+__global__ __launch_bounds__(256) void oa_w(const u32* __restrict__ cm, const float* __restrict__ sc, int ra, u32 NT,
+                                            float* __restrict__ Vt) {
+  __shared__ float srow[OA_ROWS * (OA_DB + 1)];
+  const u32 T = (u32)blockIdx.x * 256u + (u32)threadIdx.x;
+  const bool own = T < NT;
+  float w[OA_DB];
+  oa_w_rows(cm, sc, ra, T, own, srow, w);
   if (own) {
+    float4* out = reinterpret_cast<float4*>(Vt + (size_t)T * OA_DB);
 #pragma unroll
-    for (int d = 0; d < OA_DB; ++d) tile[tid * (OA_DB + 1) + d] = w[d];
-  }
-  for (int b = 0; b < tb; ++b) {
-    __syncthreads();
-    for (int e = tid; e < span * OA_DB; e += 256) {
-      const int t = e / OA_DB, d = e - t * OA_DB;
-      if ((t >> b) & 1) {
-        const float u = tile[(t ^ (1 << b)) * (OA_DB + 1) + d], v = tile[t * (OA_DB + 1) + d];
-        tile[t * (OA_DB + 1) + d] = u > v ? u : v;
-      }
-    }
-  }
-  __syncthreads();
-  float* out = Vt + (size_t)blockIdx.x * (size_t)span * OA_DB;
-  for (int e = tid; e < span * OA_DB; e += 256) out[e] = tile[(e / OA_DB) * (OA_DB + 1) + (e % OA_DB)];
-}
-
-// the maximum over T bits s .. s+w-1 (w <= 8): a block takes the 2^w rows T = g + k 2^s + hi 2^(s+w)
-// (k = 0 .. 2^w - 1) for one g < 2^s and one hi; grid = 2^(n - 1 - w) blocks
-__global__ __launch_bounds__(256) void oa_close(float* __restrict__ Vt, int s, int w) {
-  __shared__ float buf[(1 << OA_TB) * OA_DB];
-  const int tid = threadIdx.x;
-  const u64 lowg = 1ull << s;
-  const u64 g = (u64)blockIdx.x & (lowg - 1ull), hi = (u64)blockIdx.x >> s;
-  const u64 origin = g + (hi << (s + w));
-  const int cnt = (1 << w) * OA_DB;
-  for (int e = tid; e < cnt; e += 256) {
-    const int k = e / OA_DB, d = e - k * OA_DB;
-    buf[e] = Vt[(origin + ((u64)k << s)) * OA_DB + d];
-  }
-  for (int b = 0; b < w; ++b) {
-    __syncthreads();
-    for (int e = tid; e < cnt; e += 256) {
-      if (((e / OA_DB) >> b) & 1) {
-        const float u = buf[e ^ (OA_DB << b)], v = buf[e];
-        buf[e] = u > v ? u : v;
-      }
-    }
-  }
-  __syncthreads();
-  for (int e = tid; e < cnt; e += 256) {
-    const int k = e / OA_DB, d = e - k * OA_DB;
-    Vt[(origin + ((u64)k << s)) * OA_DB + d] = buf[e];
+    for (int q = 0; q < OA_DB / 4; ++q) out[q] = make_float4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
   }
 }
 
-// one thread per draw: vmax = V of the full set, then the walk from the full set
-__global__ void oa_backtrace(const float* __restrict__ Vt, const u32* __restrict__ cm, const float* __restrict__ sc,
-                             int ra, int nd, int n, int a, float* __restrict__ vmax, int32_t* __restrict__ rank_pos) {
-  const int d = threadIdx.x;
-  if (d >= nd) return;
+// the subset maximum over one window of T bits: the table as a slab of (n - 1) + 5 bits whose low
+// 5 bits (the draws, one 128-byte row) are the columns; register rounds and one LDS exchange
+template <int RL>
+__global__ __launch_bounds__(512) void oa_max_pass(float* __restrict__ Vt, WinGeom g) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  Tile<float, RL> t;
+  t.init(g, reinterpret_cast<float*>(smem));
+  t.load(Vt);
+  t.transform_max();
+  t.store(Vt);
+}
+
+// one block of 64 lanes per draw: vmax = V of the full set, then the walk from the full set. Per
+// step the lanes read V of the predecessors (lane y for member y != a), test a's rows for the node
+// (64 rows per ballot) and lane 0 sums the matches in row order; lane 0 then takes the first member
+// in ascending order with the strict > of the layered sweep. a's menus and this draw's scores sit
+// in LDS (ra <= OA_BT_ROWS, checked when the sweep mode is set).
+#define OA_BT_ROWS 4096
+__global__ __launch_bounds__(64) void oa_backtrace(const float* __restrict__ Vt, const u32* __restrict__ cm,
+                                                   const float* __restrict__ sc, int ra, int n, int a,
+                                                   float* __restrict__ vmax, int32_t* __restrict__ rank_pos) {
+  __shared__ u32 s_cm[OA_BT_ROWS];
+  __shared__ float s_sc[OA_BT_ROWS];
+  __shared__ float cand[32];
+  __shared__ int pick;
+  const int d = blockIdx.x, lane = threadIdx.x;
+  for (int j = lane; j < ra; j += 64) {
+    s_cm[j] = cm[j];
+    s_sc[j] = sc[(size_t)j * OA_DB + d];
+  }
   u32 T = (1u << (n - 1)) - 1u, D = (1u << n) - 1u;
-  vmax[d] = Vt[(size_t)T * OA_DB + d];
+  if (lane == 0) vmax[d] = Vt[(size_t)T * OA_DB + d];
+  __syncthreads();
   bool has_a = true;
   for (int i = 0; i < n; ++i) {
-    int x = 0;
+    int x;
     if (has_a) {
-      float best = 0.0f;
-      bool seen = false;
-      for (int y = 0; y < n; ++y) {
-        if (!((D >> y) & 1u)) continue;
-        float cand;
-        if (y == a) {
-          cand = 0.0f;
-          for (int j = 0; j < ra; ++j)
-            if ((cm[j] & T) == cm[j]) cand += sc[(size_t)j * OA_DB + d];
-        } else {
-          cand = Vt[(size_t)(T & ~(1u << (y < a ? y : y - 1))) * OA_DB + d];
-        }
-        if (!seen || cand > best) {
-          best = cand;
-          x = y;
-          seen = true;
+      if (lane < n && lane != a && ((D >> lane) & 1u))
+        cand[lane] = Vt[(size_t)(T & ~(1u << (lane < a ? lane : lane - 1))) * OA_DB + d];
+      float wv = 0.0f;                                 // lane 0: W of the node, rows in order
+      for (int base = 0; base < ra; base += 64) {
+        const int j = base + lane;
+        const bool hit = j < ra && (s_cm[j] & T) == s_cm[j];
+        u64 mm = __ballot(hit);
+        if (lane == 0) {
+          while (mm) {
+            const int t = __ffsll((unsigned long long)mm) - 1;
+            mm &= mm - 1ull;
+            wv += s_sc[base + t];
+          }
         }
       }
+      if (lane == 0) cand[a] = wv;
+      __syncthreads();
+      if (lane == 0) {
+        float best = 0.0f;
+        bool seen = false;
+        int bx = 0;
+        for (int y = 0; y < n; ++y) {
+          if (!((D >> y) & 1u)) continue;
+          if (!seen || cand[y] > best) {
+            best = cand[y];
+            bx = y;
+            seen = true;
+          }
+        }
+        pick = bx;
+      }
+      __syncthreads();
+      x = pick;
+      __syncthreads();
     } else {
       x = __ffs((int)D) - 1;
     }
-    rank_pos[(size_t)d * n + x] = i;
+    if (lane == 0) rank_pos[(size_t)d * n + x] = i;
     D &= ~(1u << x);
     if (x == a) has_a = false;
     else if (has_a) T &= ~(1u << (x < a ? x : x - 1));
@@ -615,16 +622,18 @@ __global__ void oa_backtrace(const float* __restrict__ Vt, const u32* __restrict
 template <int DB>
 __global__ __launch_bounds__(256) void oa_wmax64(const u32* __restrict__ cm, const double* __restrict__ r, int ra,
                                                  u32 NT, double* __restrict__ bmax) {
-  __shared__ double srow[OA_ROWS * DB];
-  __shared__ u32 scm[OA_ROWS];
+  __shared__ double srow[OA_ROWS * (DB + 1)];
   __shared__ double red[4][DB];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   double run[DB];
 #pragma unroll
   for (int d = 0; d < DB; ++d) run[d] = -INFINITY;
-  const u32 stride = gridDim.x * 256u;
-  // the T loop is uniform per block: every block runs the same number of rounds
-  for (u32 T0 = (u32)blockIdx.x * 256u; T0 < NT; T0 += stride) {
+  const bool one_chunk = ra <= OA_ROWS;
+  if (one_chunk) {
+    for (int i = tid; i < ra * DB; i += 256) srow[(i / DB) * (DB + 1) + (i % DB)] = r[i];
+    __syncthreads();
+  }
+  for (u32 T0 = (u32)blockIdx.x * 256u; T0 < NT; T0 += gridDim.x * 256u) {
     const u32 T = T0 + (u32)tid;
     const bool own = T < NT;
     double w[DB];
@@ -632,24 +641,24 @@ __global__ __launch_bounds__(256) void oa_wmax64(const u32* __restrict__ cm, con
     for (int d = 0; d < DB; ++d) w[d] = 0.0;
     for (int base = 0; base < ra; base += OA_ROWS) {
       const int cnt = min(OA_ROWS, ra - base);
-      __syncthreads();
-      for (int i = tid; i < cnt * DB; i += 256) srow[i] = r[(size_t)base * DB + i];
-      for (int i = tid; i < cnt; i += 256) scm[i] = cm[base + i];
-      __syncthreads();
-      if (own) {
-        for (int sub = 0; sub < cnt; sub += 32) {
-          const int c32 = min(32, cnt - sub);
-          u32 mm = 0;
-          for (int t = 0; t < c32; ++t) {
-            const u32 m = scm[sub + t];
-            mm |= ((m & T) == m ? 1u : 0u) << t;
-          }
-          while (mm) {
-            const int t = __ffs((int)mm) - 1;
-            mm &= mm - 1u;
+      if (!one_chunk) {
+        __syncthreads();
+        for (int i = tid; i < cnt * DB; i += 256) srow[(i / DB) * (DB + 1) + (i % DB)] = r[(size_t)base * DB + i];
+        __syncthreads();
+      }
+      for (int sub = 0; sub < cnt; sub += 32) {
+        u32 mm = 0;
 #pragma unroll
-            for (int d = 0; d < DB; ++d) w[d] += srow[(sub + t) * DB + d];
-          }
+        for (int t = 0; t < 32; ++t) {
+          const u32 m = cm[base + sub + t];
+          mm |= ((m & T) == m ? 1u : 0u) << t;
+        }
+        if (!own) mm = 0;
+        while (mm) {
+          const int t = __ffs((int)mm) - 1;
+          mm &= mm - 1u;
+#pragma unroll
+          for (int d = 0; d < DB; ++d) w[d] += srow[(sub + t) * (DB + 1) + d];
         }
       }
     }
@@ -676,12 +685,21 @@ __global__ __launch_bounds__(256) void oa_wmax64(const u32* __restrict__ cm, con
   }
 }
 
-__global__ void oa_max64_final(const double* __restrict__ bmax, int nblk, int DB, double* __restrict__ out) {
-  const int d = threadIdx.x;
-  if (d >= DB) return;
+// one block: thread (g, d) takes every (256 / DB)-th block's maximum of draw d, then LDS
+__global__ __launch_bounds__(256) void oa_max64_final(const double* __restrict__ bmax, int nblk, int DB,
+                                                      double* __restrict__ out) {
+  __shared__ double red[256];
+  const int tid = threadIdx.x, G = 256 / DB, d = tid % DB, g = tid / DB;
   double m = -INFINITY;
-  for (int b = 0; b < nblk; ++b) m = bmax[(size_t)b * DB + d] > m ? bmax[(size_t)b * DB + d] : m;
-  out[d] = m;
+  if (g < G)
+    for (int b = g; b < nblk; b += G) m = bmax[(size_t)b * DB + d] > m ? bmax[(size_t)b * DB + d] : m;
+  red[tid] = m;
+  __syncthreads();
+  if (tid < DB) {
+    double v = red[tid];
+    for (int q = 1; q < G; ++q) v = red[q * DB + tid] > v ? red[q * DB + tid] : v;
+    out[tid] = v;
+  }
 }
 
 // Slab layout, first window of the subset-sum butterfly (at bit 0, 2^B consecutive positions per
@@ -1286,11 +1304,14 @@ extern "C" int rumgpu_cg_create(rumgpu_ctx* ctx, int n, const uint64_t* masks_x,
     RG_HIP(rg_copy(ctx->stream, c->nodes, ord.data(), (size_t)c->NN * 4, hipMemcpyHostToDevice));
     if (c->single_a >= 0) {
       const int a = c->single_a;
-      std::vector<u32> cm(c->ra);
+      // padded to a multiple of 32 with bit 31 set: a menu no subset of the other n - 1 <= 25
+      // alternatives contains (the window-pass sweep reads 32 menus at a time)
+      const int rpad = (c->ra + 31) / 32 * 32;
+      std::vector<u32> cm(rpad, 0x80000000u);
       for (int j = 0; j < c->ra; ++j)
         cm[j] = (u32)compress_bit(masks_x[offsets[a] + j] & ~(1ull << a), a);
-      if ((rc = cg_alloc(c, (void**)&c->clcm, (size_t)c->ra * 4, "one-alternative menus")) != RG_OK) return fail(rc);
-      RG_HIP(rg_copy(ctx->stream, c->clcm, cm.data(), (size_t)c->ra * 4, hipMemcpyHostToDevice));
+      if ((rc = cg_alloc(c, (void**)&c->clcm, (size_t)rpad * 4, "one-alternative menus")) != RG_OK) return fail(rc);
+      RG_HIP(rg_copy(ctx->stream, c->clcm, cm.data(), (size_t)rpad * 4, hipMemcpyHostToDevice));
     }
     if (layout == 1) {
       RG_CHECK(n >= 2, RG_EUNSUP, "slab pricing needs n >= 2");
@@ -1520,14 +1541,39 @@ static int oa_ready(rumgpu_cg* c) {
 // columns 0): rank (nd x n) and vmax (nd floats) on the device
 static int oa_sweep(rumgpu_cg* c, hipStream_t st, const float* sc, int nd, int32_t* rank, float* vmax) {
   RG_TRY(oa_ready(c));
-  const int n = c->n, nb = n - 1, tb = std::min(OA_TB, nb);
+  const int n = c->n, nb = n - 1;
   const u64 NT = (u64)c->Nloc;
-  hipLaunchKernelGGL(oa_first, dim3((unsigned)(NT >> tb)), dim3(256), 0, st, c->clcm, sc, c->ra, tb, c->oaV);
-  for (int s0 = tb; s0 < nb; s0 += OA_TB) {
-    const int w = std::min(OA_TB, nb - s0);
-    hipLaunchKernelGGL(oa_close, dim3((unsigned)(1ull << (nb - w))), dim3(256), 0, st, c->oaV, s0, w);
+  // the table as a slab of nb + 5 bits, the 32 draws as its 5 column bits, one item
+  auto geom = [&](int s0, int RL) {
+    WinGeom g;
+    g.nper = 1;
+    g.nb = nb + 5;
+    g.s = 5 + s0;
+    g.B = std::min(OA_TB, nb - s0);
+    g.C = 5;
+    g.TT = 1 << (g.B + g.C - RL);
+    g.TPB = g.TT >= 256 ? 1 : 256 / g.TT;
+    g.Nloc = 1ll << g.nb;
+    g.ntiles = 1ll << (g.nb - g.B - g.C);
+    if (g.ntiles < g.TPB) g.TPB = (int)g.ntiles;
+    g.on = nullptr;
+    return g;
+  };
+  hipLaunchKernelGGL(oa_w, dim3((unsigned)((NT + 255) / 256)), dim3(256), 0, st, c->clcm, sc, c->ra, (u32)NT,
+                     c->oaV);
+  for (int s0 = 0; s0 < nb; s0 += OA_TB) {
+    const int RL = std::min(4, std::min(OA_TB, nb - s0));
+    const WinGeom g = geom(s0, RL);
+    const size_t shm = (size_t)g.TPB * tile_lds_elems(g) * 4;
+    const dim3 grid((unsigned)((g.ntiles + g.TPB - 1) / g.TPB)), block((unsigned)(g.TT * g.TPB));
+    switch (RL) {
+      case 1: hipLaunchKernelGGL(oa_max_pass<1>, grid, block, shm, st, c->oaV, g); break;
+      case 2: hipLaunchKernelGGL(oa_max_pass<2>, grid, block, shm, st, c->oaV, g); break;
+      case 3: hipLaunchKernelGGL(oa_max_pass<3>, grid, block, shm, st, c->oaV, g); break;
+      default: hipLaunchKernelGGL(oa_max_pass<4>, grid, block, shm, st, c->oaV, g); break;
+    }
   }
-  hipLaunchKernelGGL(oa_backtrace, dim3(1), dim3(64), 0, st, c->oaV, c->clcm, sc, c->ra, nd, n, c->single_a, vmax,
+  hipLaunchKernelGGL(oa_backtrace, dim3(nd), dim3(64), 0, st, c->oaV, c->clcm, sc, c->ra, n, c->single_a, vmax,
                      rank);
   RG_LAUNCH_CHECK();
   return RG_OK;
@@ -1544,7 +1590,7 @@ static int oa_vmax64(rumgpu_cg* c, hipStream_t st, const double* r, int DB, doub
     case 4: hipLaunchKernelGGL(oa_wmax64<4>, dim3(nblk), dim3(256), 0, st, c->clcm, r, c->ra, NT, c->oab64); break;
     default: hipLaunchKernelGGL(oa_wmax64<8>, dim3(nblk), dim3(256), 0, st, c->clcm, r, c->ra, NT, c->oab64); break;
   }
-  hipLaunchKernelGGL(oa_max64_final, dim3(1), dim3(64), 0, st, c->oab64, nblk, DB, c->oav64);
+  hipLaunchKernelGGL(oa_max64_final, dim3(1), dim3(256), 0, st, c->oab64, nblk, DB, c->oav64);
   RG_LAUNCH_CHECK();
   RG_HIP(rg_copy(st, out_host, c->oav64, (size_t)nd * 8, hipMemcpyDeviceToHost));
   return RG_OK;
@@ -1553,8 +1599,9 @@ static int oa_vmax64(rumgpu_cg* c, hipStream_t st, const double* r, int DB, doub
 extern "C" int rumgpu_cg_set_sweep(rumgpu_cg* c, int mode) {
   RG_CHECK(c && mode >= 0 && mode <= 2, RG_EINVAL,
            "cg_set_sweep: mode must be 0 (layers), 1 (closure) or 2 (window passes)");
-  RG_CHECK(mode != 2 || (c->n >= 2 && c->n <= 26), RG_EUNSUP,
-           "cg_set_sweep: the window-pass sweep handles 2 <= n <= 26 (n = %d)", c->n);
+  RG_CHECK(mode != 2 || (c->n >= 2 && c->n <= 26 && c->ra <= OA_BT_ROWS), RG_EUNSUP,
+           "cg_set_sweep: the window-pass sweep handles 2 <= n <= 26 and up to %d rows (n = %d, %d rows)",
+           OA_BT_ROWS, c->n, c->ra);
   // The closure sweep's first run at n = 25 ended in a GPU fault whose cause has not been shown; it
   // passed at n <= 20, so it is refused above that until the cause is known.
   RG_CHECK(mode != 1 || c->n <= CL_MAX_N, RG_EUNSUP,

@@ -128,6 +128,7 @@ class Pricer:
     SWEEPS = {"layers": 0, "auto": 1, "window": 2}
     CLOSURE_MAX_N = 20      # the library refuses the closure sweep above this n
     WINDOW_MAX_N = 26
+    WINDOW_MAX_ROWS = 4096  # rows of the one alternative the window-pass sweep takes
 
     def set_sweep(self, mode):
         """'layers' (the default): the layer-by-layer sweep for any rows (the reference's). When every

@@ -396,7 +396,11 @@ class BatchedCG:
         from .. import machine as _machine
         n = boot.rum.n
         one_alt = np.unique(boot.rum._geom.alternative).size == 1
-        mode = sess._args.get("_sweep", "layers")
+        mode = sess._args.get("_sweep")
+        if one_alt and mode is None and n <= _Pricer.WINDOW_MAX_N and boot.rum._geom.M <= _Pricer.WINDOW_MAX_ROWS:
+            # rows of one alternative (the bootstrap panels' shape): the window-pass sweep, whose
+            # values and rankings are the layered sweep's bit for bit; "_sweep": "layers" keeps the latter
+            sess._args["_sweep"] = mode = "window"
         if one_alt and mode == "window" and n <= _Pricer.WINDOW_MAX_N:
             # the window-pass sweep: a 128-byte row per subset of the other alternatives, the scores
             need = 128 * (1 << (n - 1)) + 32 * 4 * boot.rum.M + 64 * boot.rum.M
