@@ -235,36 +235,6 @@ struct Tile {
       }
     }
   }
-  // The subset maximum on the window's bits: for every bit, the value of a set containing the bit
-  // becomes the larger of itself and the value of the set without it (exact: a maximum returns an
-  // operand, so the order of the bits does not matter).
-  __device__ __forceinline__ void stages_max(int r) {
-    const int lo = lo_of(r);
-    const int q0 = r * RL - lo;
-    const int q1e = (r + 1) * RL < B ? (r + 1) * RL : B;
-    const int q1 = q1e - lo;
-#pragma unroll
-    for (int q = 0; q < RL; ++q) {
-      if (q >= q0 && q < q1) {
-#pragma unroll
-        for (int j = 0; j < R; ++j) {
-          if (!(j & (1 << q))) {
-            const T X = v[j], Y = v[j | (1 << q)];
-            v[j | (1 << q)] = X > Y ? X : Y;
-          }
-        }
-      }
-    }
-  }
-  __device__ __forceinline__ void transform_max() {
-    const bool fwd = (cur == 0);
-#pragma unroll
-    for (int i = 0; i < nr; ++i) {
-      const int r = fwd ? i : nr - 1 - i;
-      if (r != cur) exchange(r);
-      stages_max(r);
-    }
-  }
   // For a window at bit 0, move to the last round's layout, where a wavefront's lanes hold
   // consecutive slab positions for every register (one contiguous 512 B run per access for
   // float64); windows with columns are contiguous along the lanes in every layout already.

@@ -470,27 +470,28 @@ __global__ void cl_backtrace(const u32* __restrict__ F, const u32* __restrict__ 
 }
 
 // ---------------------------------------------------------------------------------------------
-// One-alternative sweep by window passes (sweep mode 2). With every row in alternative a, the
-// identity above gives V[D] = max over T' subset of T of W[T'] for every D containing a: V on those
-// subsets is the subset maximum of W over the lattice of the other n - 1 alternatives. This sweep
-// forms that table for 32 draws, V[T][d] (one 128-byte row per T): oa_w computes W (a thread per T,
-// the subset tests shared by the draws, each draw's matches summed in row order), then window
-// passes take the maximum over 8 bits of T at a time with the butterfly tiles (register rounds,
-// one LDS exchange, whole 128-byte rows moved). A maximum returns one of its operands, so the
-// order of the bits changes no value and the table is the layered sweep's V bit for bit. The
-// ranking is a walk from the full set that re-evaluates each visited node's candidates in
-// ascending member order with the layered sweep's strict > (W of the node itself summed again in
-// row order for x = a, V of the predecessor read from the table for x != a).
-// Memory: 128 bytes per subset of the other alternatives (2.1 GB at n = 25), streamed in order,
-// against the layered batched sweep's 128-byte row gathers of every (subset, member) pair.
-#define OA_TB 8          // bits of T per window pass
-#define OA_DB 32         // draws per table row (a batch is padded to 32)
+// One-alternative sweep in window passes (sweep mode 2). It rests on the identity and the path
+// rule above (V[D] = max over T' subset of T of W[T'] for D containing a; along the path from the
+// full set V stays at vmax until a is chosen; the choice at D is the first member x, ascending,
+// with x = a and W[T] = vmax, or x != a and some T' subset of T minus x with W[T'] = vmax), in
+// kernels of their own:
+//   oa_wbits   W for 256 consecutive T per block (a thread per T, a's menus through uniform loads,
+//              the rows' scores staged in LDS with a 33-float stride, each lane's matches summed in
+//              row order per draw), the block's maximum per draw, and per T the bits "W = block max"
+//   oa_gmax    the global maximum per draw (two levels)
+//   oa_or_lo   per 4096 T: drop the bits of blocks whose maximum is not the global one, then close the
+//              bits under supersets over T bits 0..11 (OR, in LDS)
+//   oa_or_hi   the closure over 8 more bits at a time, 32 consecutive T (128 bytes) per row
+//   oa_bt      a block per draw walks the path (W of the node summed again in row order)
+// 4 bytes per subset of the other alternatives (64 MB at n = 25) for 32 draws. Every value compared
+// is a float32 sum in the layered sweep's order and a maximum returns an operand, so the rankings
+// and vmax are the layered sweep's bit for bit.
+#define OA_DB 32         // draws per sweep (a batch is padded to 32)
+#define OA_ROWS 256      // a's rows staged in LDS at a time
+#define OA_LO 12         // T bits closed by oa_or_lo (4096 T per block)
+#define OA_HI 8          // T bits per oa_or_hi pass
+#define OA_BT_ROWS 4096  // a's rows the path walk holds in LDS (the sweep mode is refused above)
 
-// W[T][d] for every T: one thread per T. a's menus come 32 at a time through uniform (scalar)
-// loads (padded to a multiple of 32 with a menu no T contains); the rows' scores are staged in LDS
-// with a 33-float stride (a lane's row is its own, so the stride spreads the lanes over the banks),
-// and each lane adds its own matches, rows in order.
-#define OA_ROWS 256
 __device__ __forceinline__ void oa_w_rows(const u32* __restrict__ cm, const float* __restrict__ sc, int ra, u32 T,
                                           bool own, float* srow, float (&w)[OA_DB]) {
   const int tid = threadIdx.x;
@@ -505,7 +506,7 @@ __device__ __forceinline__ void oa_w_rows(const u32* __restrict__ cm, const floa
     for (int sub = 0; sub < cnt; sub += 32) {
       u32 mm = 0;
 #pragma unroll
-      for (int t = 0; t < 32; ++t) {
+      for (int t = 0; t < 32; ++t) {                  // padded to 32: no T contains a padding menu
         const u32 m = cm[base + sub + t];
         mm |= ((m & T) == m ? 1u : 0u) << t;
       }
@@ -521,60 +522,134 @@ __device__ __forceinline__ void oa_w_rows(const u32* __restrict__ cm, const floa
   }
 }
 
-// W: the 128-byte row of each T stored whole
-__global__ __launch_bounds__(256) void oa_w(const u32* __restrict__ cm, const float* __restrict__ sc, int ra, u32 NT,
-                                            float* __restrict__ Vt) {
-  __shared__ float srow[OA_ROWS * (OA_DB + 1)];
-  const u32 T = (u32)blockIdx.x * 256u + (u32)threadIdx.x;
+__global__ __launch_bounds__(256) void oa_wbits(const u32* __restrict__ cm, const float* __restrict__ sc, int ra,
+                                                u32 NT, u32* __restrict__ F, float* __restrict__ bmax) {
+  __shared__ float srow[OA_ROWS * (OA_DB + 1)];      // the staged rows, then the block's W rows
+  __shared__ float part[8][OA_DB];
+  __shared__ float bm[OA_DB];
+  const int tid = threadIdx.x;
+  const u32 T = (u32)blockIdx.x * 256u + (u32)tid;
   const bool own = T < NT;
   float w[OA_DB];
   oa_w_rows(cm, sc, ra, T, own, srow, w);
-  if (own) {
-    float4* out = reinterpret_cast<float4*>(Vt + (size_t)T * OA_DB);
+  // the block's maximum per draw: W rows through LDS, thread (g, d) takes rows 32 g .. 32 g + 31
+  __syncthreads();
 #pragma unroll
-    for (int q = 0; q < OA_DB / 4; ++q) out[q] = make_float4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
+  for (int d = 0; d < OA_DB; ++d) srow[tid * (OA_DB + 1) + d] = own ? w[d] : -INFINITY;
+  __syncthreads();
+  {
+    const int d = tid % OA_DB, g = tid / OA_DB;
+    float m = -INFINITY;
+    for (int k = 0; k < 32; ++k) {
+      const float v = srow[(g * 32 + k) * (OA_DB + 1) + d];
+      m = v > m ? v : m;
+    }
+    part[g][d] = m;
+  }
+  __syncthreads();
+  if (tid < OA_DB) {
+    float m = part[0][tid];
+    for (int g = 1; g < 8; ++g) m = part[g][tid] > m ? part[g][tid] : m;
+    bm[tid] = m;
+    bmax[(size_t)blockIdx.x * OA_DB + tid] = m;
+  }
+  __syncthreads();
+  u32 bits = 0;
+#pragma unroll
+  for (int d = 0; d < OA_DB; ++d) bits |= (w[d] == bm[d] ? 1u : 0u) << d;
+  if (own) F[T] = bits;
+}
+
+// maxima per draw over blocks: block g of the grid takes blocks g, g + G, ... (one thread per
+// (group of 8, draw)); called twice, the second time with one block
+__global__ __launch_bounds__(256) void oa_gmax(const float* __restrict__ in, int nin, float* __restrict__ out) {
+  __shared__ float red[256];
+  const int tid = threadIdx.x, d = tid % OA_DB, q = tid / OA_DB;       // 8 groups of 32
+  float m = -INFINITY;
+  for (int b = (int)blockIdx.x * 8 + q; b < nin; b += (int)gridDim.x * 8) {
+    const float v = in[(size_t)b * OA_DB + d];
+    m = v > m ? v : m;
+  }
+  red[tid] = m;
+  __syncthreads();
+  if (tid < OA_DB) {
+    for (int k = 1; k < 8; ++k) m = red[k * OA_DB + tid] > m ? red[k * OA_DB + tid] : m;
+    out[(size_t)blockIdx.x * OA_DB + tid] = m;
   }
 }
 
-// the subset maximum over one window of T bits: the table as a slab of (n - 1) + 5 bits whose low
-// 5 bits (the draws, one 128-byte row) are the columns; register rounds and one LDS exchange
-template <int RL>
-__global__ __launch_bounds__(512) void oa_max_pass(float* __restrict__ Vt, WinGeom g) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  Tile<float, RL> t;
-  t.init(g, reinterpret_cast<float*>(smem));
-  t.load(Vt);
-  t.transform_max();
-  t.store(Vt);
+// per span = 2^lo consecutive T (lo = min(12, n - 1)): a T's bit for draw d stands only where its
+// block's maximum is the global one; then F[T] |= F[T minus b] for the bits b < lo
+__global__ __launch_bounds__(256) void oa_or_lo(u32* __restrict__ F, const float* __restrict__ bmax,
+                                                const float* __restrict__ vmax, int lo) {
+  __shared__ u32 s[1 << OA_LO];
+  __shared__ u32 keep[(1 << OA_LO) / 256];
+  const int tid = threadIdx.x;
+  const u32 span = 1u << lo, base = (u32)blockIdx.x * span;
+  const u32 nsub = (span + 255u) / 256u;
+  if (tid < (int)nsub) {
+    const size_t blk = base / 256u + (u32)tid;
+    u32 k = 0;
+    for (int d = 0; d < OA_DB; ++d) k |= (bmax[blk * OA_DB + d] == vmax[d] ? 1u : 0u) << d;
+    keep[tid] = k;
+  }
+  __syncthreads();
+  for (u32 e = tid; e < span; e += 256u) s[e] = F[base + e] & keep[e / 256u];
+  for (int b = 0; b < lo; ++b) {
+    __syncthreads();
+    for (u32 e = tid; e < span; e += 256u)
+      if ((e >> b) & 1u) s[e] |= s[e ^ (1u << b)];
+  }
+  __syncthreads();
+  for (u32 e = tid; e < span; e += 256u) F[base + e] = s[e];
 }
 
-// one block of 64 lanes per draw: vmax = V of the full set, then the walk from the full set. Per
-// step the lanes read V of the predecessors (lane y for member y != a), test a's rows for the node
-// (64 rows per ballot) and lane 0 sums the matches in row order; lane 0 then takes the first member
-// in ascending order with the strict > of the layered sweep. a's menus and this draw's scores sit
-// in LDS (ra <= OA_BT_ROWS, checked when the sweep mode is set).
-#define OA_BT_ROWS 4096
-__global__ __launch_bounds__(64) void oa_backtrace(const float* __restrict__ Vt, const u32* __restrict__ cm,
-                                                   const float* __restrict__ sc, int ra, int n, int a,
-                                                   float* __restrict__ vmax, int32_t* __restrict__ rank_pos) {
+// the closure over T bits b0 .. b0+w-1 (b0 >= 12, w <= 8): a block holds 2^w rows of 32 consecutive
+// T (128 bytes each): T = g 32 + j + k 2^b0 + hi 2^(b0 + w); grid = 2^(n - 1 - w - 5) blocks
+__global__ __launch_bounds__(256) void oa_or_hi(u32* __restrict__ F, int b0, int w) {
+  __shared__ u32 s[(1 << OA_HI) * 32];
+  const int tid = threadIdx.x;
+  const u32 groups = 1u << (b0 - 5);
+  const u32 g = (u32)blockIdx.x & (groups - 1u), hi = (u32)blockIdx.x >> (b0 - 5);
+  const u32 origin = g * 32u + (hi << (b0 + w));
+  const int cnt = (1 << w) * 32;
+  for (int e = tid; e < cnt; e += 256) s[e] = F[origin + (u32)(e & 31) + ((u32)(e >> 5) << b0)];
+  for (int b = 0; b < w; ++b) {
+    __syncthreads();
+    for (int e = tid; e < cnt; e += 256)
+      if (((e >> 5) >> b) & 1) s[e] |= s[e ^ (32 << b)];
+  }
+  __syncthreads();
+  for (int e = tid; e < cnt; e += 256) F[origin + (u32)(e & 31) + ((u32)(e >> 5) << b0)] = s[e];
+}
+
+// a block of 64 lanes per draw: the walk from the full set. Per step the lanes read the closed
+// bits of the predecessors (lane y for member y != a), test a's rows for the node (64 rows per
+// ballot; lane 0 sums the matches in row order and compares with vmax), and lane 0 takes the
+// first member in ascending order that qualifies. err is set if none does (it cannot when the
+// sums are the sweep's).
+__global__ __launch_bounds__(64) void oa_bt(const u32* __restrict__ F, const u32* __restrict__ cm,
+                                            const float* __restrict__ sc, int ra, int n, int a,
+                                            const float* __restrict__ vmax, int32_t* __restrict__ rank_pos,
+                                            int* __restrict__ err) {
   __shared__ u32 s_cm[OA_BT_ROWS];
   __shared__ float s_sc[OA_BT_ROWS];
-  __shared__ float cand[32];
+  __shared__ int ok[32];
   __shared__ int pick;
   const int d = blockIdx.x, lane = threadIdx.x;
   for (int j = lane; j < ra; j += 64) {
     s_cm[j] = cm[j];
     s_sc[j] = sc[(size_t)j * OA_DB + d];
   }
+  const float vm = vmax[d];
   u32 T = (1u << (n - 1)) - 1u, D = (1u << n) - 1u;
-  if (lane == 0) vmax[d] = Vt[(size_t)T * OA_DB + d];
   __syncthreads();
   bool has_a = true;
   for (int i = 0; i < n; ++i) {
     int x;
     if (has_a) {
-      if (lane < n && lane != a && ((D >> lane) & 1u))
-        cand[lane] = Vt[(size_t)(T & ~(1u << (lane < a ? lane : lane - 1))) * OA_DB + d];
+      if (lane < n && lane != a)
+        ok[lane] = ((D >> lane) & 1u) ? (int)((F[T & ~(1u << (lane < a ? lane : lane - 1))] >> d) & 1u) : 0;
       float wv = 0.0f;                                 // lane 0: W of the node, rows in order
       for (int base = 0; base < ra; base += 64) {
         const int j = base + lane;
@@ -588,19 +663,15 @@ __global__ __launch_bounds__(64) void oa_backtrace(const float* __restrict__ Vt,
           }
         }
       }
-      if (lane == 0) cand[a] = wv;
+      if (lane == 0) ok[a] = wv == vm ? 1 : 0;
       __syncthreads();
       if (lane == 0) {
-        float best = 0.0f;
-        bool seen = false;
-        int bx = 0;
-        for (int y = 0; y < n; ++y) {
-          if (!((D >> y) & 1u)) continue;
-          if (!seen || cand[y] > best) {
-            best = cand[y];
-            bx = y;
-            seen = true;
-          }
+        int bx = -1;
+        for (int y = 0; y < n && bx < 0; ++y)
+          if (((D >> y) & 1u) && ok[y]) bx = y;
+        if (bx < 0) {
+          atomicOr(err, 1);
+          bx = a;
         }
         pick = bx;
       }
@@ -874,6 +945,9 @@ __global__ void cg_fill_f64(double* a, double v, i64 m) {
 __global__ void cg_fill_u8(uint8_t* a, uint8_t v, i64 m) {
   for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (i64)gridDim.x * blockDim.x) a[i] = v;
 }
+__global__ void cg_fill_i32(int* a, int v, i64 m) {
+  for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (i64)gridDim.x * blockDim.x) a[i] = v;
+}
 
 // ---------------------------------------------------------------------------------------------
 // stream layout: one layer k of colex ranks, 1024 ranks per block (256 threads x 4).
@@ -1082,7 +1156,10 @@ struct rumgpu_cg {
   u32 clNT = 0;
   int cl_grid = 0, cl_tpb = 0;
   // window-pass one-alternative sweep (sweep mode 2)
-  float* oaV = nullptr;      // V[T][32]: 128 bytes per subset of the other alternatives
+  u32* oaF = nullptr;        // per T: the closed bits of 32 draws (4 bytes per subset of the others)
+  float* oabm = nullptr;     // per block of 256 T and draw: the block's maximum of W
+  float* oagp = nullptr;     // per block of the first oa_gmax level and draw
+  int* oaerr = nullptr;
   float* oavmax = nullptr;   // 32 draws
   double* oab64 = nullptr;   // per block and draw (8) float64 maxima
   double* oav64 = nullptr;   // 8 draws
@@ -1170,7 +1247,7 @@ static void cg_free_all(rumgpu_cg* c) {
                   c->binom, c->vbuf[0], c->vbuf[1], c->lbuf[0], c->lbuf[1], c->mbuf[0], c->mbuf[1], c->ebuf[0],
                   c->ebuf[1], c->lstep, c->subV, c->subpred, c->subnodes, c->submasks, c->subscores, c->suboffsets,
                   c->Vb, c->predb, c->scoresb, c->vmaxb, c->rankb, c->V64, c->r64, c->V64b, c->r64b, c->clcm, c->clF, c->cltmax,
-                  c->clbmax, c->clvmax, c->clbmax64, c->clv64, c->clerr, c->oaV, c->oavmax, c->oab64, c->oav64, c->cbits, c->cp, c->cpout, c->cq,
+                  c->clbmax, c->clvmax, c->clbmax64, c->clv64, c->clerr, c->oaF, c->oabm, c->oagp, c->oaerr, c->oavmax, c->oab64, c->oav64, c->cbits, c->cp, c->cpout, c->cq,
                   c->cidx, c->clam, c->crank, c->cpart, c->thlist, c->tlord};
   for (void* q : bufs) rg_free(q);
   for (auto* p : c->predk) rg_free(p);
@@ -1521,16 +1598,21 @@ static int oa_nblk64(const rumgpu_cg* c) {
 
 // buffers of the window-pass sweep, at its first use
 static int oa_ready(rumgpu_cg* c) {
-  if (c->oaV) return RG_OK;
+  if (c->oaF) return RG_OK;
+  const size_t nblk = (size_t)((c->Nloc + 255) / 256);
   int rc = RG_OK;
-  if ((rc = cg_alloc(c, (void**)&c->oaV, (size_t)c->Nloc * OA_DB * 4, "one-alternative table")) != RG_OK ||
+  if ((rc = cg_alloc(c, (void**)&c->oaF, (size_t)c->Nloc * 4, "one-alternative bits")) != RG_OK ||
+      (rc = cg_alloc(c, (void**)&c->oabm, nblk * OA_DB * 4, "one-alternative block maxima")) != RG_OK ||
+      (rc = cg_alloc(c, (void**)&c->oagp, (size_t)256 * OA_DB * 4, "one-alternative maxima")) != RG_OK ||
+      (rc = cg_alloc(c, (void**)&c->oaerr, 4, "one-alternative status")) != RG_OK ||
       (rc = cg_alloc(c, (void**)&c->oavmax, (size_t)OA_DB * 4, "one-alternative vmax")) != RG_OK ||
       (rc = cg_alloc(c, (void**)&c->oab64, (size_t)2048 * 8 * 8, "float64 block maxima")) != RG_OK ||
       (rc = cg_alloc(c, (void**)&c->oav64, (size_t)8 * 8, "float64 vmax")) != RG_OK) {
-    void** bufs[] = {(void**)&c->oaV, (void**)&c->oavmax, (void**)&c->oab64, (void**)&c->oav64};
-    for (void** b : bufs) {
-      rg_free(*b);
-      *b = nullptr;
+    void** bufs[] = {(void**)&c->oaF, (void**)&c->oabm, (void**)&c->oagp, (void**)&c->oaerr, (void**)&c->oavmax,
+                     (void**)&c->oab64, (void**)&c->oav64};
+    for (void** q : bufs) {
+      rg_free(*q);
+      *q = nullptr;
     }
     return rc;
   }
@@ -1538,44 +1620,31 @@ static int oa_ready(rumgpu_cg* c) {
 }
 
 // the window-pass sweep for nd <= 32 draws whose scores of a's rows are sc ([ra][32], unused
-// columns 0): rank (nd x n) and vmax (nd floats) on the device
+// columns 0): rank (nd x n) and vmax (nd floats) on the device; check oaerr afterwards
 static int oa_sweep(rumgpu_cg* c, hipStream_t st, const float* sc, int nd, int32_t* rank, float* vmax) {
   RG_TRY(oa_ready(c));
   const int n = c->n, nb = n - 1;
-  const u64 NT = (u64)c->Nloc;
-  // the table as a slab of nb + 5 bits, the 32 draws as its 5 column bits, one item
-  auto geom = [&](int s0, int RL) {
-    WinGeom g;
-    g.nper = 1;
-    g.nb = nb + 5;
-    g.s = 5 + s0;
-    g.B = std::min(OA_TB, nb - s0);
-    g.C = 5;
-    g.TT = 1 << (g.B + g.C - RL);
-    g.TPB = g.TT >= 256 ? 1 : 256 / g.TT;
-    g.Nloc = 1ll << g.nb;
-    g.ntiles = 1ll << (g.nb - g.B - g.C);
-    if (g.ntiles < g.TPB) g.TPB = (int)g.ntiles;
-    g.on = nullptr;
-    return g;
-  };
-  hipLaunchKernelGGL(oa_w, dim3((unsigned)((NT + 255) / 256)), dim3(256), 0, st, c->clcm, sc, c->ra, (u32)NT,
-                     c->oaV);
-  for (int s0 = 0; s0 < nb; s0 += OA_TB) {
-    const int RL = std::min(4, std::min(OA_TB, nb - s0));
-    const WinGeom g = geom(s0, RL);
-    const size_t shm = (size_t)g.TPB * tile_lds_elems(g) * 4;
-    const dim3 grid((unsigned)((g.ntiles + g.TPB - 1) / g.TPB)), block((unsigned)(g.TT * g.TPB));
-    switch (RL) {
-      case 1: hipLaunchKernelGGL(oa_max_pass<1>, grid, block, shm, st, c->oaV, g); break;
-      case 2: hipLaunchKernelGGL(oa_max_pass<2>, grid, block, shm, st, c->oaV, g); break;
-      case 3: hipLaunchKernelGGL(oa_max_pass<3>, grid, block, shm, st, c->oaV, g); break;
-      default: hipLaunchKernelGGL(oa_max_pass<4>, grid, block, shm, st, c->oaV, g); break;
-    }
+  const u32 NT = (u32)c->Nloc;
+  const int nblk = (int)((NT + 255u) / 256u), g1 = std::min(256, nblk), lo = std::min(OA_LO, nb);
+  hipLaunchKernelGGL(oa_wbits, dim3(nblk), dim3(256), 0, st, c->clcm, sc, c->ra, NT, c->oaF, c->oabm);
+  hipLaunchKernelGGL(oa_gmax, dim3(g1), dim3(256), 0, st, c->oabm, nblk, c->oagp);
+  hipLaunchKernelGGL(oa_gmax, dim3(1), dim3(256), 0, st, c->oagp, g1, vmax);
+  hipLaunchKernelGGL(oa_or_lo, dim3(NT >> lo), dim3(256), 0, st, c->oaF, c->oabm, vmax, lo);
+  for (int b0 = lo; b0 < nb; b0 += OA_HI) {
+    const int w = std::min(OA_HI, nb - b0);
+    hipLaunchKernelGGL(oa_or_hi, dim3(1u << (nb - w - 5)), dim3(256), 0, st, c->oaF, b0, w);
   }
-  hipLaunchKernelGGL(oa_backtrace, dim3(nd), dim3(64), 0, st, c->oaV, c->clcm, sc, c->ra, n, c->single_a, vmax,
-                     rank);
+  hipLaunchKernelGGL(cg_fill_i32, dim3(1), dim3(64), 0, st, c->oaerr, 0, (i64)1);
+  hipLaunchKernelGGL(oa_bt, dim3(nd), dim3(64), 0, st, c->oaF, c->clcm, sc, c->ra, n, c->single_a, vmax, rank,
+                     c->oaerr);
   RG_LAUNCH_CHECK();
+  return RG_OK;
+}
+
+static int oa_check(rumgpu_cg* c) {
+  int err = 0;
+  RG_HIP(rg_copy(c->ctx->stream, &err, c->oaerr, 4, hipMemcpyDeviceToHost));
+  RG_CHECK(err == 0, RG_EHIP, "cg pricing: the window-pass sweep found no choice matching vmax");
   return RG_OK;
 }
 
@@ -1698,7 +1767,7 @@ extern "C" int rumgpu_cg_price(rumgpu_cg* c, const float* scores_x, double* vmax
     float vf = 0.0f;
     RG_HIP(hipMemcpyAsync(&vf, c->oavmax, 4, hipMemcpyDeviceToHost, st));
     RG_HIP(hipMemcpyAsync(rank_pos, c->rank, (size_t)n * 4, hipMemcpyDeviceToHost, st));
-    RG_HIP(hipStreamSynchronize(st));
+    RG_TRY(oa_check(c));
     *vmax = (double)vf;
     c->last_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return RG_OK;
@@ -1902,7 +1971,7 @@ extern "C" int rumgpu_cg_price_batch(rumgpu_cg* c, int D, const float* scores, d
       RG_TRY(oa_sweep(c, st, c->scoresb + (size_t)c->h_offsets[c->single_a] * OA_DB, nd, c->rankb, c->vmaxb));
       RG_HIP(hipMemcpyAsync(vf.data(), c->vmaxb, (size_t)nd * 4, hipMemcpyDeviceToHost, st));
       RG_HIP(hipMemcpyAsync(rank_pos + (size_t)first * n, c->rankb, (size_t)nd * n * 4, hipMemcpyDeviceToHost, st));
-      RG_HIP(hipStreamSynchronize(st));
+      RG_TRY(oa_check(c));
       for (int d = 0; d < nd; ++d) vmax[first + d] = (double)vf[d];
       continue;
     }
