@@ -210,9 +210,11 @@ int rumgpu_cg_info(rumgpu_cg* cg, size_t* device_bytes, double* last_kernel_ms);
  * mode 1 = when every row belongs to one alternative a (a panel of one alternative's choice
  * shares), the same V[full] and ranking from a max over W[T] = a's row-order float32 sums on the
  * 2^(n-1) subsets T of the other alternatives, with one bit per node and draw closed under
- * supersets for the path; 4 bytes per node instead of V and pred for 2^n nodes. No reference
- * counterpart: the reference's sweep does not special-case one alternative. sweep_info reports the
- * alternative the closure sweep uses, or -1 when the layered sweep runs. */
+ * supersets for the path; 4 bytes per node instead of V and pred for 2^n nodes (refused above
+ * n = 20); mode 2 = the same rule in window passes of other kernels (W with block maxima, the bits
+ * closed in 12- and 8-bit windows, a block per draw for the path; n <= 26, at most 4,096 rows).
+ * No reference counterpart: the reference's sweep does not special-case one alternative.
+ * sweep_info reports the alternative mode 1 or 2 uses, or -1 when the layered sweep runs. */
 int rumgpu_cg_set_sweep(rumgpu_cg* cg, int mode);
 int rumgpu_cg_sweep_info(rumgpu_cg* cg, int* one_alternative);
 /* Tile bits L of the slab layout's layered max (2..10, 0 = the built-in choice): the subsets are
