@@ -258,6 +258,25 @@ int rumgpu_qp_destroy(rumgpu_qp* state);
 int rumgpu_qp_simplex_warm(rumgpu_qp* state, int K, const double* Q, int64_t ldq, const double* h,
                            const int32_t* A_init, int nA, int max_outer, double tol, double reg, double* lam,
                            int32_t* support, int* n_support);
+/* ---- The host side of one column-generation run between two sweeps (the loop body of
+ * flow_cg/solver.py:92-240, with qp_bitset_incremental.py:36-130's bitset Gram rows): ranking
+ * columns as bitsets over the x-major rows (cg_core.py:63-80: a row is 1 iff its alternative is
+ * its menu's best-ranked member), the integer Gram matrix, h = V'p (rows ascending), the master
+ * QP warm from the last working set and factor (rumgpu_qp_simplex_warm), p_out = V lam over lam's
+ * support (columns ascending), and the next sweep's scores float32(p - p_out), one call per round.
+ * Host code; no device memory. */
+typedef struct rumgpu_cgr rumgpu_cgr;
+int rumgpu_cgr_create(int n, int64_t M, const uint64_t* menu_x, const int32_t* alt_x, const double* p_x, int max_K,
+                      rumgpu_cgr** out);
+int rumgpu_cgr_destroy(rumgpu_cgr* c);
+int rumgpu_cgr_add_ranking(rumgpu_cgr* c, const int32_t* rank_pos);
+int rumgpu_cgr_add_column(rumgpu_cgr* c, const uint8_t* col);
+/* dst's columns become src's (same rows), with h recomputed from dst's p */
+int rumgpu_cgr_copy_columns(rumgpu_cgr* dst, const rumgpu_cgr* src);
+int rumgpu_cgr_set_lambda(rumgpu_cgr* c, const double* lam, int K, const int32_t* working_set, int n_working);
+/* out[0] = lam'Q lam, out[1] = h'lam after the master; scores[M] for the next sweep */
+int rumgpu_cgr_master(rumgpu_cgr* c, int max_outer, double tol, double reg, double* out, float* scores);
+int rumgpu_cgr_get(rumgpu_cgr* c, int* K, double* lam, double* p_out, uint8_t* columns);
 
 /* ---- Learned warm start: in -> hidden -> hidden -> out MLP (ReLU), bf16 matrix-core forward and
  * backward, Adam on a float32 master copy. No reference counterpart: it maps a bootstrap draw's

@@ -655,3 +655,235 @@ static int qp_simplex_impl(int K, const double* Q, int64_t ldq, const double* h,
   for (int i = 0; i < K; ++i) lam_out[i] = lam[i];
   return RG_OK;
 }
+
+// ---------------------------------------------------------------------------------------------
+// The host side of one column-generation run between two pricing sweeps (the loop body of
+// flow_cg/solver.py:92-240 of the reference, as the engine's cg_steps runs it): the ranking
+// columns as bitsets over the x-major rows, their integer Gram matrix Q and h = V'p, the master QP
+// warm from the last working set and factor, p_out = V lam over lam's support, and the float32
+// scores of the next sweep, in one call per round. A batched bootstrap runs tens of these per
+// round beside one batched sweep, so these steps bound it once the sweeps are shared.
+struct rumgpu_cgr {
+  int n = 0, W = 0;
+  int64_t M = 0;
+  std::vector<double> p;          // x-major rows: float32-rounded p as double
+  std::vector<uint32_t> menu;     // the rows' menus
+  std::vector<int32_t> alt;       // the rows' alternatives
+  std::vector<uint64_t> bits;     // cap x W
+  std::vector<double> Q;          // cap x cap (row i at i * cap)
+  std::vector<double> h, lam;     // cap
+  std::vector<double> p_out;      // M
+  std::vector<int32_t> work;      // the last master's support, the next one's start
+  rumgpu_qp qp;
+  int K = 0, cap = 0, max_K = 0;
+};
+
+static void cgr_grow(rumgpu_cgr* c, int cap) {
+  cap = std::min(std::max(cap, 1), c->max_K);
+  if (cap <= c->cap) return;
+  std::vector<uint64_t> bits((size_t)cap * c->W, 0);
+  std::vector<double> Q((size_t)cap * cap, 0.0), h(cap, 0.0), lam(cap, 0.0);
+  std::copy(c->bits.begin(), c->bits.begin() + (size_t)c->K * c->W, bits.begin());
+  for (int i = 0; i < c->K; ++i)
+    std::copy(c->Q.begin() + (size_t)i * c->cap, c->Q.begin() + (size_t)i * c->cap + c->K, Q.begin() + (size_t)i * cap);
+  std::copy(c->h.begin(), c->h.begin() + c->K, h.begin());
+  std::copy(c->lam.begin(), c->lam.begin() + c->K, lam.begin());
+  c->bits.swap(bits), c->Q.swap(Q), c->h.swap(h), c->lam.swap(lam);
+  c->cap = cap;
+}
+
+// append a column given as a bitset: its Gram row (integer overlaps), h (its rows' p in ascending
+// row order) and lam = 0
+static int cgr_append(rumgpu_cgr* c, const uint64_t* b) {
+  RG_CHECK(c->K < c->max_K, RG_EINVAL, "cg rounds: max_K = %d columns exceeded", c->max_K);
+  if (c->K >= c->cap) cgr_grow(c, 2 * c->cap);
+  const int k = c->K, W = c->W, cap = c->cap;
+  uint64_t* bk = c->bits.data() + (size_t)k * W;
+  std::copy(b, b + W, bk);
+  double* Qk = c->Q.data() + (size_t)k * cap;
+  for (int j = 0; j < k; ++j) {
+    const uint64_t* bj = c->bits.data() + (size_t)j * W;
+    int64_t cnt = 0;
+    for (int w = 0; w < W; ++w) cnt += __builtin_popcountll(bj[w] & bk[w]);
+    Qk[j] = (double)cnt;
+    c->Q[(size_t)j * cap + k] = (double)cnt;
+  }
+  int64_t self = 0;
+  double hs = 0.0;
+  for (int w = 0; w < W; ++w) {
+    self += __builtin_popcountll(bk[w]);
+    uint64_t m = bk[w];
+    while (m) {
+      const int t = __builtin_ctzll(m);
+      m &= m - 1;
+      hs += c->p[(size_t)w * 64 + t];
+    }
+  }
+  Qk[k] = (double)self;
+  c->h[k] = hs;
+  c->lam[k] = 0.0;
+  c->K = k + 1;
+  return RG_OK;
+}
+
+// p_out = sum over lam's nonzero entries (ascending column) of lam_k times column k
+static void cgr_pout(rumgpu_cgr* c) {
+  std::fill(c->p_out.begin(), c->p_out.end(), 0.0);
+  for (int k = 0; k < c->K; ++k) {
+    const double l = c->lam[k];
+    if (l == 0.0) continue;
+    const uint64_t* bk = c->bits.data() + (size_t)k * c->W;
+    for (int w = 0; w < c->W; ++w) {
+      uint64_t m = bk[w];
+      while (m) {
+        const int t = __builtin_ctzll(m);
+        m &= m - 1;
+        c->p_out[(size_t)w * 64 + t] += l;
+      }
+    }
+  }
+}
+
+extern "C" int rumgpu_cgr_create(int n, int64_t M, const uint64_t* menu, const int32_t* alt, const double* p,
+                                 int max_K, rumgpu_cgr** out) {
+  RG_CHECK(out && menu && alt && p && M >= 1 && max_K >= 1, RG_EINVAL, "cgr_create: bad argument");
+  RG_CHECK(n >= 1 && n <= 32, RG_EUNSUP, "cgr_create: n = %d outside 1..32", n);
+  rumgpu_cgr* c = new rumgpu_cgr();
+  c->n = n;
+  c->M = M;
+  c->W = (int)((M + 63) / 64);
+  c->max_K = max_K;
+  c->p.assign(p, p + M);
+  c->menu.resize(M);
+  c->alt.assign(alt, alt + M);
+  for (int64_t r = 0; r < M; ++r) {
+    c->menu[r] = (uint32_t)menu[r];
+    if (alt[r] < 0 || alt[r] >= n || !((menu[r] >> alt[r]) & 1ull)) {
+      delete c;
+      RG_CHECK(false, RG_EINVAL, "cgr_create: row %lld's alternative is not in its menu", (long long)r);
+    }
+  }
+  c->p_out.assign(M, 0.0);
+  cgr_grow(c, 64);
+  *out = c;
+  return RG_OK;
+}
+
+extern "C" int rumgpu_cgr_destroy(rumgpu_cgr* c) {
+  delete c;
+  return RG_OK;
+}
+
+// the column of a ranking (rank_pos[x] = position of x, 0 = most preferred): row r is 1 iff its
+// alternative is the best-ranked member of its menu (cg_core.py:63-80 of the reference)
+extern "C" int rumgpu_cgr_add_ranking(rumgpu_cgr* c, const int32_t* rank_pos) {
+  RG_CHECK(c && rank_pos, RG_EINVAL, "cgr_add_ranking: null argument");
+  std::vector<uint64_t> b(c->W, 0);
+  for (int64_t r = 0; r < c->M; ++r) {
+    uint32_t m = c->menu[r];
+    int top = -1, best = 1 << 30;
+    while (m) {
+      const int x = __builtin_ctz(m);
+      m &= m - 1u;
+      if (rank_pos[x] < best) {
+        best = rank_pos[x];
+        top = x;
+      }
+    }
+    if (top == c->alt[r]) b[r >> 6] |= 1ull << (r & 63);
+  }
+  return cgr_append(c, b.data());
+}
+
+// a column given as 0/1 bytes over the rows
+extern "C" int rumgpu_cgr_add_column(rumgpu_cgr* c, const uint8_t* col) {
+  RG_CHECK(c && col, RG_EINVAL, "cgr_add_column: null argument");
+  std::vector<uint64_t> b(c->W, 0);
+  for (int64_t r = 0; r < c->M; ++r)
+    if (col[r]) b[r >> 6] |= 1ull << (r & 63);
+  return cgr_append(c, b.data());
+}
+
+// dst's columns = src's (same rows): the bitsets and Gram matrix copied, h from dst's p
+extern "C" int rumgpu_cgr_copy_columns(rumgpu_cgr* dst, const rumgpu_cgr* src) {
+  RG_CHECK(dst && src && dst->M == src->M, RG_EINVAL, "cgr_copy_columns: incompatible runs");
+  RG_CHECK(src->K <= dst->max_K, RG_EINVAL, "cgr_copy_columns: %d columns exceed max_K = %d", src->K, dst->max_K);
+  dst->K = 0;
+  cgr_grow(dst, src->K + 64);
+  const int k = src->K, W = dst->W;
+  std::copy(src->bits.begin(), src->bits.begin() + (size_t)k * W, dst->bits.begin());
+  for (int i = 0; i < k; ++i)
+    std::copy(src->Q.begin() + (size_t)i * src->cap, src->Q.begin() + (size_t)i * src->cap + k,
+              dst->Q.begin() + (size_t)i * dst->cap);
+  for (int i = 0; i < k; ++i) {
+    const uint64_t* bk = dst->bits.data() + (size_t)i * W;
+    double hs = 0.0;
+    for (int w = 0; w < W; ++w) {
+      uint64_t m = bk[w];
+      while (m) {
+        const int t = __builtin_ctzll(m);
+        m &= m - 1;
+        hs += dst->p[(size_t)w * 64 + t];
+      }
+    }
+    dst->h[i] = hs;
+    dst->lam[i] = 0.0;
+  }
+  dst->K = k;
+  return RG_OK;
+}
+
+// lam of the first K columns (the rest 0), the next master's starting working set, and p_out
+extern "C" int rumgpu_cgr_set_lambda(rumgpu_cgr* c, const double* lam, int K, const int32_t* A, int nA) {
+  RG_CHECK(c && lam && K >= 0 && K <= c->K && (A || nA == 0), RG_EINVAL, "cgr_set_lambda: bad argument");
+  std::fill(c->lam.begin(), c->lam.end(), 0.0);
+  std::copy(lam, lam + K, c->lam.begin());
+  c->work.assign(A, A + nA);
+  cgr_pout(c);
+  return RG_OK;
+}
+
+// One master: the QP over the K columns from the last working set (its factor kept), then p_out,
+// out[0] = lam'Q lam and out[1] = h'lam (sums over the support, ascending), and the next sweep's
+// scores float32(p - p_out).
+extern "C" int rumgpu_cgr_master(rumgpu_cgr* c, int max_outer, double tol, double reg, double* out, float* scores) {
+  RG_CHECK(c && out && scores && c->K >= 1, RG_EINVAL, "cgr_master: bad argument");
+  const int K = c->K;
+  std::vector<double> lam(K);
+  std::vector<int32_t> sup(K);
+  int ns = 0;
+  RG_TRY(qp_simplex_impl(K, c->Q.data(), c->cap, c->h.data(), c->work.data(), (int)c->work.size(), max_outer, tol, reg,
+                         lam.data(), sup.data(), &ns, &c->qp));
+  c->work.assign(sup.begin(), sup.begin() + ns);
+  std::copy(lam.begin(), lam.end(), c->lam.begin());
+  cgr_pout(c);
+  double lql = 0.0, hl = 0.0;
+  for (int i = 0; i < K; ++i) {
+    const double li = lam[i];
+    if (li == 0.0) continue;
+    const double* Qi = c->Q.data() + (size_t)i * c->cap;
+    double s = 0.0;
+    for (int j = 0; j < K; ++j)
+      if (lam[j] != 0.0) s += Qi[j] * lam[j];
+    lql += li * s;
+    hl += c->h[i] * li;
+  }
+  out[0] = lql;
+  out[1] = hl;
+  for (int64_t r = 0; r < c->M; ++r) scores[r] = (float)(c->p[r] - c->p_out[r]);
+  return RG_OK;
+}
+
+// lam (K), p_out (M) and the columns as 0/1 bytes (K x M); any output may be null
+extern "C" int rumgpu_cgr_get(rumgpu_cgr* c, int* K, double* lam, double* p_out, uint8_t* cols) {
+  RG_CHECK(c, RG_EINVAL, "cgr_get: null handle");
+  if (K) *K = c->K;
+  if (lam) std::copy(c->lam.begin(), c->lam.begin() + c->K, lam);
+  if (p_out) std::copy(c->p_out.begin(), c->p_out.end(), p_out);
+  if (cols)
+    for (int k = 0; k < c->K; ++k) {
+      const uint64_t* bk = c->bits.data() + (size_t)k * c->W;
+      for (int64_t r = 0; r < c->M; ++r) cols[(size_t)k * c->M + r] = (uint8_t)((bk[r >> 6] >> (r & 63)) & 1ull);
+    }
+  return RG_OK;
+}

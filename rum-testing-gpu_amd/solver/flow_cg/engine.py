@@ -347,6 +347,91 @@ class DeviceColumns:
 
 RESIDENT = "resident"      # a request to price the scores DeviceColumns.set_lambda left on the device
 
+
+class NativeRounds:
+    """A run's host steps between two sweeps in the library (rumgpu_cgr_*): the ranking columns as
+    bitsets with their integer Gram matrix and h, the master QP warm from the last working set and
+    factor, p_out over lam's support and the next sweep's float32 scores, one call per round."""
+
+    def __init__(self, n, masks_x, alt_x, p, max_K):
+        self.lib = _lib.load()
+        self.M = int(np.asarray(p).size)
+        m = np.ascontiguousarray(masks_x, dtype=np.uint64)
+        a = np.ascontiguousarray(alt_x, dtype=np.int32)
+        pv = np.ascontiguousarray(p, dtype=np.float64)
+        h = ctypes.c_void_p()
+        _lib.check(self.lib.rumgpu_cgr_create(int(n), self.M, _lib.ptr(m), _lib.ptr(a), _lib.ptr(pv), int(max_K),
+                                              ctypes.byref(h)), "cgr_create")
+        self.h = h
+        self._out = np.empty(2)
+        self._scores = np.empty(self.M, dtype=np.float32)
+        self._k = ctypes.c_int()
+
+    def add_ranking(self, rank_pos):
+        rp = np.ascontiguousarray(rank_pos, dtype=np.int32)
+        _lib.check(self.lib.rumgpu_cgr_add_ranking(self.h, _lib.ptr(rp)), "cgr_add_ranking")
+
+    def add_column(self, col):
+        c = np.ascontiguousarray(col, dtype=np.uint8)
+        _lib.check(self.lib.rumgpu_cgr_add_column(self.h, _lib.ptr(c)), "cgr_add_column")
+
+    def copy_columns(self, src):
+        _lib.check(self.lib.rumgpu_cgr_copy_columns(self.h, src.h), "cgr_copy_columns")
+
+    def set_lambda(self, lam, working_set):
+        lv = np.ascontiguousarray(lam, dtype=np.float64)
+        A = np.ascontiguousarray(working_set, dtype=np.int32)
+        _lib.check(self.lib.rumgpu_cgr_set_lambda(self.h, _lib.ptr(lv), int(lv.size), _lib.ptr(A), int(A.size)),
+                   "cgr_set_lambda")
+
+    def master(self, max_outer=200, tol=1e-10, reg=1e-12):
+        """the master QP, p_out and the next scores: (lam'Q lam, h'lam, scores)"""
+        _lib.check(self.lib.rumgpu_cgr_master(self.h, int(max_outer), float(tol), float(reg), _lib.ptr(self._out),
+                                              _lib.ptr(self._scores)), "cgr_master")
+        return float(self._out[0]), float(self._out[1]), self._scores.copy()
+
+    @property
+    def K(self):
+        _lib.check(self.lib.rumgpu_cgr_get(self.h, ctypes.byref(self._k), None, None, None), "cgr_get")
+        return self._k.value
+
+    def result(self):
+        """(lam, p_out, columns as 0/1 uint8 arrays)"""
+        K = self.K
+        lam, p_out = np.empty(K), np.empty(self.M)
+        cols = np.empty((K, self.M), dtype=np.uint8)
+        _lib.check(self.lib.rumgpu_cgr_get(self.h, None, _lib.ptr(lam), _lib.ptr(p_out), _lib.ptr(cols)), "cgr_get")
+        return lam, p_out, list(cols)
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.lib.rumgpu_cgr_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_NATIVE_TEMPLATES = []
+
+
+def _native_template(init_columns, n, masks_x, alt_x):
+    """A warm-start column list as native rounds' columns (bitsets and integer Gram), built once per
+    list; each draw copies it and sums h over its own p."""
+    M = int(np.asarray(alt_x).size)
+    for lst, tpl in _NATIVE_TEMPLATES:
+        if lst is init_columns and tpl.M == M:
+            return tpl
+    tpl = NativeRounds(n, masks_x, alt_x, np.zeros(M), max(1, len(init_columns)))
+    for c in init_columns:
+        tpl.add_column(c)
+    _NATIVE_TEMPLATES.insert(0, (init_columns, tpl))
+    del _NATIVE_TEMPLATES[4:]
+    return tpl
+
 _TEMPLATES = []
 
 
@@ -369,6 +454,87 @@ def ranking_column(memb, alt_x, rank_pos, n):
     (cg_core.py:63-80 semantics). memb: (M, n) bool membership of the x-major rows."""
     top = np.argmin(np.where(memb, np.asarray(rank_pos)[None, :], n + 1), axis=1)
     return (top == alt_x).astype(np.uint8)
+
+
+def _cg_steps_native(n, masks_x, alt_x, perm, p_obs, pricer, t_setup, rc_tol, max_iters, time_limit_s, seed, verbose,
+                     phi_ub_target, gap_rel_tol, gap_abs_tol, init_columns, init_lambda, early_ub_check, lb_stop):
+    """cg_steps with the host columns and the native master (NativeRounds): the same loop, requests,
+    stop tests and result; each round's host steps are one library call."""
+    M = p_obs.size
+    p_sq = float(p_obs @ p_obs)
+    n_init = len(init_columns) if init_columns is not None else 0
+    cols = NativeRounds(n, masks_x, alt_x, p_obs, max_iters + 2 + n_init)
+    try:
+        if n_init:
+            cols.copy_columns(_native_template(init_columns, n, masks_x, alt_x))
+            lam = (np.asarray(init_lambda, dtype=np.float64).copy() if init_lambda is not None
+                   else np.full(n_init, 1.0 / n_init))
+            lam = np.maximum(lam, 0.0)
+            lam /= lam.sum()
+            cols.set_lambda(lam, [int(i) for i in np.flatnonzero(lam > 1e-12)] or [0])
+        else:
+            cols.add_ranking(np.random.RandomState(seed).permutation(n))
+            cols.set_lambda(np.array([1.0]), [0])
+        setup_s = time.perf_counter() - t_setup
+        UB, lb_best, gap_rel, rc_final = float("inf"), 0.0, float("inf"), float("nan")
+        stop = "max_iters"
+        margins, trace = [], []
+        qp_s = price_s = 0.0
+        t0 = time.perf_counter()
+        it = 0
+        K = cols.K
+        for it in range(1, max_iters + 1):
+            if time_limit_s > 0 and time.perf_counter() - t0 > time_limit_s:
+                stop = "time_limit"
+                break
+            tq = time.perf_counter()
+            lQl, hl, req = cols.master()
+            err2 = lQl - 2.0 * hl + p_sq
+            inner = hl - lQl
+            qp_s += time.perf_counter() - tq
+            if early_ub_check and phi_ub_target is not None:
+                UB = float(np.sqrt(max(0.0, err2)))
+                if UB <= phi_ub_target:
+                    stop = "phi_ub"
+                    break
+            tpz = time.perf_counter()
+            vmax, rank_pos, margin = yield req
+            price_s += time.perf_counter() - tpz
+            if margin:
+                margins.append(margin)
+            fw_gap = (vmax + margin) - inner
+            rc_final = fw_gap
+            UB = float(np.sqrt(max(0.0, err2)))
+            lb_best = max(lb_best, float(np.sqrt(max(0.0, err2 - 2.0 * fw_gap))))
+            gap_rel = (UB - lb_best) / lb_best if lb_best > 0 else float("inf")
+            trace.append((it, time.perf_counter() - t0, UB, lb_best, K))
+            if verbose and (it <= 5 or it % 25 == 0):
+                print("[flow_cg %4d] K=%4d dist=%.6f rc=%.3g gap=%.2e" % (it, K, UB, fw_gap, gap_rel), flush=True)
+            if phi_ub_target is not None and UB <= phi_ub_target:
+                stop = "phi_ub"
+                break
+            if lb_stop is not None and lb_best >= lb_stop:
+                stop = "lb_stop"
+                break
+            if gap_rel_tol is not None and gap_rel <= gap_rel_tol:
+                stop = "gap_rel"
+                break
+            if gap_abs_tol is not None and UB - lb_best <= gap_abs_tol:
+                stop = "gap_abs"
+                break
+            if fw_gap < rc_tol:
+                stop = "rc"
+                break
+            cols.add_ranking(rank_pos)
+            K += 1
+        wall = time.perf_counter() - t0
+        lam, p_out, columns = cols.result()
+    finally:
+        cols.close()
+    return dict(wall_s=wall, setup_s=setup_s, trace=trace, margins=margins, iters=it, n_cols=lam.size, lam=lam,
+                p_out=p_out, perm=perm, columns=columns, rc_final=rc_final, distance=UB, lb=lb_best, gap_rel=gap_rel,
+                stop_reason=stop, qp_s=qp_s, price_s=price_s, device_bytes=pricer.info()[0], layout=pricer.layout,
+                dtype=pricer.dtype)
 
 
 def solve_cg(n, menu, alt, p, pricer=None, layout="mask", dtype="float32", **kw):
@@ -417,14 +583,18 @@ def cg_steps(n, menu, alt, p, pricer, rc_tol=0.0, max_iters=4000, time_limit_s=0
     yields RESIDENT instead of the scores: the column of each ranking, its Gram row, p_out and the
     scores are then computed on the GPU."""
     t_setup = time.perf_counter()
-    qp_state = QPState() if qp == "native" else None
-    qp_solve = {"native": lambda Q, h, A_init: solve_qp_native(Q, h, A_init=A_init, state=qp_state),
-                "numpy": solve_qp_numpy}[qp]
     perm, masks_x, alt_x, offsets = xmajor(menu, alt, n)
     p32 = np.asarray(p, dtype=np.float64)[perm].astype(np.float32)
     p_obs = p32.astype(np.float64)
     M = p_obs.size
     dev = bool(resident) and M > Columns.DENSE_ROWS
+    if not dev and qp == "native":
+        return (yield from _cg_steps_native(n, masks_x, alt_x, perm, p_obs, pricer, t_setup, rc_tol, max_iters,
+                                            time_limit_s, seed, verbose, phi_ub_target, gap_rel_tol, gap_abs_tol,
+                                            init_columns, init_lambda, early_ub_check, lb_stop))
+    qp_state = QPState() if qp == "native" else None
+    qp_solve = {"native": lambda Q, h, A_init: solve_qp_native(Q, h, A_init=A_init, state=qp_state),
+                "numpy": solve_qp_numpy}[qp]
     memb = None if dev else ((masks_x[:, None] >> np.arange(n, dtype=np.uint64)[None, :]) & np.uint64(1)).astype(bool)
     setup_s = time.perf_counter() - t_setup
     p_sq = float(p_obs @ p_obs)

@@ -492,3 +492,63 @@ def test_out_of_memory_gives_exit_flag_minus_3(monkeypatch):
     with rum.session(method="bm_admm", machine=prof) as s:
         r2 = s.solve(rum.choice_probability)
     assert r2.exit_flag == -3
+
+
+class _OraclePricer:
+    """pricing through the oracle's mask DP (CPU), for the host-side column generation tests"""
+    layout, dtype = "mask", "float32"
+
+    def __init__(self, n, masks_x, offsets):
+        self.n, self.masks_x, self.offsets = n, masks_x, offsets
+
+    def price(self, scores):
+        from oracle import oracle as O
+        vmax, rank, _V, _pred = O.price_mask(self.n, self.masks_x, np.asarray(scores, dtype=np.float32), self.offsets)
+        return float(vmax), rank, 0.0
+
+    def info(self):
+        return 0, 0.0
+
+
+def test_native_rounds_reach_the_exact_projection():
+    """Column generation with the library's host rounds (rumgpu_cgr_*: bitset columns, integer Gram,
+    warm master QP, p_out, scores) against the numpy master, both priced by the oracle's mask DP on
+    the CPU: both reach the exact distance (oracle projection), with the same columns' Gram counts,
+    and a warm restart from the kept columns lands on the same distance."""
+    from oracle import oracle as O
+    from solver.flow_cg.engine import cg_steps, drive, xmajor
+    rng = np.random.default_rng(11)
+    n = 6
+    menus = [m for m in range(1 << n) if bin(m).count("1") >= 2 and rng.random() < 0.5]
+    om, ox = [], []
+    for m in menus:
+        for a in range(n):
+            if (m >> a) & 1:
+                om.append(m)
+                ox.append(a)
+    menu, alt = np.array(om, dtype=np.uint64), np.array(ox, dtype=np.int32)
+    p = rng.random(menu.size) + 0.05
+    _, inv = np.unique(menu, return_inverse=True)
+    p = p / np.bincount(inv, weights=p)[inv]
+    p = p.astype(np.float32).astype(np.float64)            # the engine works on float32(p)
+    d_exact, _lb, _x = O.exact_projection(n, menu.astype(np.int64), alt.astype(np.int64), p, tol=1e-14)
+    perm, masks_x, _ax, offsets = xmajor(menu, alt, n)
+    pricer = _OraclePricer(n, masks_x, offsets)
+    res = {}
+    for qp in ("native", "numpy"):
+        r = drive(cg_steps(n, menu, alt, p, pricer, gap_abs_tol=1e-7, max_iters=800, qp=qp), pricer)
+        res[qp] = r
+        assert r["stop_reason"] in ("gap_abs", "rc"), r["stop_reason"]
+        # the lower bound is limited by the float32 scores (about 1e-8 here), hence abs 1e-7
+        assert abs(r["distance"] - d_exact) < 1e-7 and r["lb"] <= d_exact + 1e-9, (qp, r["distance"], d_exact)
+        # p_out is the mixture of the columns with weights lam
+        C = np.asarray(r["columns"], dtype=np.float64)
+        assert np.allclose(r["lam"] @ C, r["p_out"], atol=1e-12) and abs(r["lam"].sum() - 1) < 1e-12
+    # warm restart from the native run's columns: the same distance within its tolerance
+    rn = res["native"]
+    r2 = drive(cg_steps(n, menu, alt, p, pricer, gap_abs_tol=1e-7, max_iters=800, qp="native",
+                        init_columns=rn["columns"], init_lambda=rn["lam"]), pricer)
+    assert abs(r2["distance"] - d_exact) < 1e-7 and r2["iters"] <= 3
+    # the two masters follow the same trajectory (same QP; h and p_out summed in other orders)
+    assert res["native"]["n_cols"] == res["numpy"]["n_cols"]
+    assert abs(res["native"]["distance"] - res["numpy"]["distance"]) < 1e-12
