@@ -194,8 +194,10 @@ __global__ __launch_bounds__(256, 4) void bm_prox_iter(const T* __restrict__ xk,
     z[pos] = b2 * ((T)1 - xk[pos]);
     return;
   }
+  // the offset is loaded with the mask (not after it), so an observed menu's p follows one round trip
   const u32 om = obsmask[S];
-  const int off = om ? obsoff[S] : 0;
+  const int off_all = obsoff[S];
+  const int off = om ? off_all : 0;
   const bool mixed = (om != 0) && (om != S);
   const double c_obs = (double)b2 + 2.0, c_un = (double)b2;
   // every member's value first, with no use in between, so all the loads are in flight together;
@@ -211,6 +213,10 @@ __global__ __launch_bounds__(256, 4) void bm_prox_iter(const T* __restrict__ xk,
     key[i] = 0.0;
     if ((S >> i) & 1u) key[i] = (double)xk[(u32)i * Nloc + (u32)compress_bit(S & ~(1u << i), i)];
   }
+  // n <= 19: an observed menu keeps its members' xk in LDS for z = b2 (x2 - xk) at the end (four
+  // blocks of 256 threads fit a CU's LDS); larger n read xk again
+  constexpr bool XO_LDS = N <= 19;
+  __shared__ T sxo[XO_LDS ? N : 1][256];
   if (om != 0) {
     // observed menus: the observed members' p, loaded without branches (a clamped index for the
     // others, whose value is not used) so the loads are in flight together, then the keys
@@ -226,6 +232,7 @@ __global__ __launch_bounds__(256, 4) void bm_prox_iter(const T* __restrict__ xk,
     for (int i = 0; i < N; ++i) {
       if ((S >> i) & 1u) {
         const T xv = (T)key[i];
+        if constexpr (XO_LDS) sxo[i][threadIdx.x] = xv;
         const bool ob = (om >> i) & 1u;
         const T xhv = ob ? (T)xh[i] : (T)0;
         key[i] = mixed ? (double)b2 * (double)xv + 2.0 * (ob ? 1.0 : 0.0) * (double)xhv
@@ -271,7 +278,7 @@ __global__ __launch_bounds__(256, 4) void bm_prox_iter(const T* __restrict__ xk,
     }
     return;
   }
-  // observed menus: the outputs replace the keys, xk is read again (all loads in flight together)
+  // observed menus: the outputs replace the keys; xk from LDS, or read again (all loads in flight together)
 #pragma unroll
   for (int i = 0; i < N; ++i) {
     if ((S >> i) & 1u) {
@@ -284,7 +291,10 @@ __global__ __launch_bounds__(256, 4) void bm_prox_iter(const T* __restrict__ xk,
 #pragma unroll
   for (int i = 0; i < N; ++i) {
     xo[i] = (T)0;
-    if ((S >> i) & 1u) xo[i] = xk[(u32)i * Nloc + (u32)compress_bit(S & ~(1u << i), i)];
+    if ((S >> i) & 1u) {
+      if constexpr (XO_LDS) xo[i] = sxo[i][threadIdx.x];
+      else xo[i] = xk[(u32)i * Nloc + (u32)compress_bit(S & ~(1u << i), i)];
+    }
   }
 #pragma unroll
   for (int i = 0; i < N; ++i) {
