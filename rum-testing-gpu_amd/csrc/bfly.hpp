@@ -253,18 +253,25 @@ struct Tile {
       stages<FMA>(r, k);
     }
   }
-  template <typename S>
+  // NT: non-temporal accesses, for slabs too large for the caches to give anything back
+  template <bool NT = false, typename S>
   __device__ __forceinline__ void load(const S* __restrict__ p) {
     if (active) {
 #pragma unroll
-      for (int j = 0; j < R; ++j) v[j] = (T)p[gidx(j)];
+      for (int j = 0; j < R; ++j) {
+        if constexpr (NT) v[j] = (T)__builtin_nontemporal_load(p + gidx(j));
+        else v[j] = (T)p[gidx(j)];
+      }
     }
   }
-  template <typename S>
+  template <bool NT = false, typename S>
   __device__ __forceinline__ void store(S* __restrict__ p) const {
     if (active) {
 #pragma unroll
-      for (int j = 0; j < R; ++j) p[gidx(j)] = (S)v[j];
+      for (int j = 0; j < R; ++j) {
+        if constexpr (NT) __builtin_nontemporal_store((S)v[j], p + gidx(j));
+        else p[gidx(j)] = (S)v[j];
+      }
     }
   }
 };

@@ -46,11 +46,27 @@ struct PassArgs {
   Coef<T> cA, cAp, cQ, cQt, cT;
   WinGeom g;
   const int* fresh;   // per slot: 1 = state just loaded, skip the deferred s, y update once
+  int nt;             // 1: non-temporal slab accesses (a working set the caches cannot hold)
 };
 
-template <typename T, int RL, int KIND, int BT = 0, int CT = -1>
+// slab element access of the fused passes and the prox; NT: non-temporal
+template <bool NT, typename T>
+__device__ __forceinline__ T ldg(const T* p) {
+  if constexpr (NT) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+template <bool NT, typename T>
+__device__ __forceinline__ void stg(T* p, T v) {
+  if constexpr (NT) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
+
+template <typename T, int RL, int KIND, int BT = 0, int CT = -1, bool NT = false>
 __global__ __launch_bounds__(512) void bm_pass(PassArgs<T> P) {
   extern __shared__ __align__(16) unsigned char smem[];
+  // NT (slabs too large for the caches): non-temporal accesses for every slab but xk, which the
+  // prox reads right after this pass and finds in the Infinity Cache when it is stored normally
+  // (at n = 20: 342 us per iteration, 353 with xk non-temporal too, 386 with no NT at all)
   Tile<T, RL, BT, CT> t;
   t.init(P.g, reinterpret_cast<T*>(smem), true, KIND == K_D);
   const T one = (T)1;
@@ -61,11 +77,11 @@ __global__ __launch_bounds__(512) void bm_pass(PassArgs<T> P) {
 #pragma unroll
       for (int j = 0; j < R; ++j) {
         const i64 i = t.gidx(j);
-        t.v[j] = P.y[i] - P.b1 * P.s[i];                                    // fuse_w
+        t.v[j] = ldg<NT>(P.y + i) - P.b1 * ldg<NT>(P.s + i);                // fuse_w
       }
     }
   } else {
-    t.load(P.tb);
+    t.template load<NT>(P.tb);
   }
   if constexpr (KIND == K_SY || KIND == K_SYONLY || KIND == K_FULL) {
     t.template transform<true>(P.cA);                                                      // Ax (raw A)
@@ -75,7 +91,7 @@ __global__ __launch_bounds__(512) void bm_pass(PassArgs<T> P) {
 #pragma unroll
       for (int j = 0; j < R; ++j) {
         const i64 i = t.gidx(j);
-        const T sj = P.s[i], yj = P.y[i];
+        const T sj = ldg<NT>(P.s + i), yj = ldg<NT>(P.y + i);
         if (fresh) {                                                        // a refilled slot
           t.v[j] = yj - P.b1 * sj;
           continue;
@@ -84,8 +100,8 @@ __global__ __launch_bounds__(512) void bm_pass(PassArgs<T> P) {
         T sn = P.ib1 * yj - axor;
         sn = sn > (T)0 ? sn : (T)0;
         const T yn = yj - P.b1 * (axor + sn);
-        P.s[i] = sn;
-        P.y[i] = yn;
+        stg<NT>(P.s + i, sn);
+        stg<NT>(P.y + i, yn);
         t.v[j] = yn - P.b1 * sn;                                            // next fuse_w
       }
     }
@@ -99,7 +115,7 @@ __global__ __launch_bounds__(512) void bm_pass(PassArgs<T> P) {
 #pragma unroll
       for (int j = 0; j < R; ++j) {
         const i64 i = t.gidx(j);
-        t.v[j] = (P.z[i] + P.b2 * P.x2[i] - t.v[j]) * P.ib1;                // fuse_rhs
+        t.v[j] = (ldg<NT>(P.z + i) + P.b2 * ldg<NT>(P.x2 + i) - t.v[j]) * P.ib1;  // fuse_rhs
       }
     }
     t.template transform<true>(P.cQt);
@@ -120,14 +136,14 @@ __global__ __launch_bounds__(512) void bm_pass(PassArgs<T> P) {
 #pragma unroll
       for (int j = 0; j < R; ++j) {
         const i64 i = t.gidx(j);
-        P.xk[i] = P.a * t.v[j] + (one - P.a) * P.x2[i] - P.ib2 * P.z[i];    // fuse_xk
+        P.xk[i] = P.a * t.v[j] + (one - P.a) * ldg<NT>(P.x2 + i) - P.ib2 * ldg<NT>(P.z + i);  // fuse_xk
       }
     }
   }
   if constexpr (KIND == K_XK) t.template transform<true>(P.cA);
   if constexpr (KIND == K_T) t.template transform<true>(P.cT);
   t.coalesce();
-  t.store(P.tb);
+  t.template store<NT>(P.tb);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -187,6 +203,9 @@ __global__ __launch_bounds__(256, 4) void bm_prox_iter(const T* __restrict__ xk,
   }
   if (S >= (1u << N)) return;
   constexpr u32 Nloc = 1u << (N - 1);     // slab positions fit 32 bits for n <= 26
+  // float64 slabs from n = 21 on: non-temporal accesses (at n = 20 the prox's x2 and z stores are
+  // still found in the caches by the next passes: 342 against 369 us per iteration)
+  constexpr bool NT = sizeof(T) == 8 && N >= 21;
   const int k = __popc(S);
   if (k == 1) {
     const u32 pos = (u32)(__ffs((int)S) - 1) * Nloc;
@@ -211,7 +230,7 @@ __global__ __launch_bounds__(256, 4) void bm_prox_iter(const T* __restrict__ xk,
 #pragma unroll
   for (int i = 0; i < N; ++i) {
     key[i] = 0.0;
-    if ((S >> i) & 1u) key[i] = (double)xk[(u32)i * Nloc + (u32)compress_bit(S & ~(1u << i), i)];
+    if ((S >> i) & 1u) key[i] = (double)ldg<NT>(xk + (u32)i * Nloc + (u32)compress_bit(S & ~(1u << i), i));
   }
   // n <= 19: an observed menu keeps its members' xk in LDS for z = b2 (x2 - xk) at the end (four
   // blocks of 256 threads fit a CU's LDS); larger n read xk again
@@ -272,8 +291,8 @@ __global__ __launch_bounds__(256, 4) void bm_prox_iter(const T* __restrict__ xk,
         const double yv = (key[i] - thr) * 1.0;
         const T o = yv > 0.0 ? (T)yv : (T)0.0;
         const u32 pos = (u32)i * Nloc + (u32)compress_bit(S & ~(1u << i), i);
-        out[pos] = o;
-        z[pos] = b2 * (o - (T)key[i]);
+        stg<NT>(out + pos, o);
+        stg<NT>(z + pos, b2 * (o - (T)key[i]));
       }
     }
     return;
@@ -293,7 +312,7 @@ __global__ __launch_bounds__(256, 4) void bm_prox_iter(const T* __restrict__ xk,
     xo[i] = (T)0;
     if ((S >> i) & 1u) {
       if constexpr (XO_LDS) xo[i] = sxo[i][threadIdx.x];
-      else xo[i] = xk[(u32)i * Nloc + (u32)compress_bit(S & ~(1u << i), i)];
+      else xo[i] = ldg<NT>(xk + (u32)i * Nloc + (u32)compress_bit(S & ~(1u << i), i));
     }
   }
 #pragma unroll
@@ -301,8 +320,8 @@ __global__ __launch_bounds__(256, 4) void bm_prox_iter(const T* __restrict__ xk,
     if ((S >> i) & 1u) {
       const T o = (T)key[i];
       const u32 pos = (u32)i * Nloc + (u32)compress_bit(S & ~(1u << i), i);
-      out[pos] = o;
-      z[pos] = b2 * (o - xo[i]);
+      stg<NT>(out + pos, o);
+      stg<NT>(z + pos, b2 * (o - xo[i]));
     }
   }
 }
@@ -762,6 +781,20 @@ void pass_launch(F kernel, const dim3& grid, const dim3& block, size_t shm, hipS
   }
 }
 
+// A fused pass; with P.nt the non-temporal variant, which exists for float64 on the geometries of
+// slabs too large for the caches (n >= 20: 10-13 bits at bit 0, 6-8 bits with 16 columns above, and
+// the runtime-geometry kernels); elsewhere P.nt is ignored (the results are the same either way).
+template <typename T, int RL, int KIND, int BT = 0, int CT = -1>
+void launch_bm(const PassArgs<T>& P, const dim3& grid, const dim3& block, size_t shm, hipStream_t st) {
+  if constexpr (sizeof(T) == 8 && (BT == 0 || (CT == 0 && BT >= 10) || (CT == 4 && BT >= 6))) {
+    if (P.nt) {
+      pass_launch((bm_pass<T, RL, KIND, BT, CT, true>), grid, block, shm, st, P);
+      return;
+    }
+  }
+  pass_launch((bm_pass<T, RL, KIND, BT, CT, false>), grid, block, shm, st, P);
+}
+
 template <typename T, int KIND>
 int launch_pass_rl(int RL, const PassArgs<T>& P, hipStream_t st) {
   const WinGeom& g = P.g;
@@ -773,28 +806,28 @@ int launch_pass_rl(int RL, const PassArgs<T>& P, hipStream_t st) {
     // 13 bits at bit 0 (4 or 5 register bits, 512 or 256 threads): with 8 bits above it, two windows cover
     // n = 22; the float64 tile needs 68 KiB of LDS, which gfx950's 160 KiB per workgroup allows
     if ((RL == 4 || RL == 5) && g.s == 0 && g.C == 0 && g.B == 13) {
-      if (RL == 4) pass_launch((bm_pass<T, 4, KIND, 13, 0>), grid, block, shm, st, P);
-      else pass_launch((bm_pass<T, 5, KIND, 13, 0>), grid, block, shm, st, P);
+      if (RL == 4) launch_bm<T, 4, KIND, 13, 0>(P, grid, block, shm, st);
+      else launch_bm<T, 5, KIND, 13, 0>(P, grid, block, shm, st);
       RG_LAUNCH_CHECK();
       return RG_OK;
     }
     // 4 register bits on 10-12 bits at bit 0: one LDS exchange fewer per transform than 3
     if (RL == 4 && g.s == 0 && g.C == 0 && g.B >= 10 && g.B <= 12) {
       switch (g.B) {
-        case 10: pass_launch((bm_pass<T, 4, KIND, 10, 0>), grid, block, shm, st, P); break;
-        case 11: pass_launch((bm_pass<T, 4, KIND, 11, 0>), grid, block, shm, st, P); break;
-        default: pass_launch((bm_pass<T, 4, KIND, 12, 0>), grid, block, shm, st, P); break;
+        case 10: launch_bm<T, 4, KIND, 10, 0>(P, grid, block, shm, st); break;
+        case 11: launch_bm<T, 4, KIND, 11, 0>(P, grid, block, shm, st); break;
+        default: launch_bm<T, 4, KIND, 12, 0>(P, grid, block, shm, st); break;
       }
       RG_LAUNCH_CHECK();
       return RG_OK;
     }
     if (RL == 3 && g.s == 0 && g.C == 0 && g.B >= 8 && g.B <= 12) {
       switch (g.B) {
-        case 8: pass_launch((bm_pass<T, 3, KIND, 8, 0>), grid, block, shm, st, P); break;
-        case 9: pass_launch((bm_pass<T, 3, KIND, 9, 0>), grid, block, shm, st, P); break;
-        case 10: pass_launch((bm_pass<T, 3, KIND, 10, 0>), grid, block, shm, st, P); break;
-        case 11: pass_launch((bm_pass<T, 3, KIND, 11, 0>), grid, block, shm, st, P); break;
-        default: pass_launch((bm_pass<T, 3, KIND, 12, 0>), grid, block, shm, st, P); break;
+        case 8: launch_bm<T, 3, KIND, 8, 0>(P, grid, block, shm, st); break;
+        case 9: launch_bm<T, 3, KIND, 9, 0>(P, grid, block, shm, st); break;
+        case 10: launch_bm<T, 3, KIND, 10, 0>(P, grid, block, shm, st); break;
+        case 11: launch_bm<T, 3, KIND, 11, 0>(P, grid, block, shm, st); break;
+        default: launch_bm<T, 3, KIND, 12, 0>(P, grid, block, shm, st); break;
       }
       RG_LAUNCH_CHECK();
       return RG_OK;
@@ -806,8 +839,8 @@ int launch_pass_rl(int RL, const PassArgs<T>& P, hipStream_t st) {
     if ((RL == 3 || RL == 4) && g.C == CW && g.B >= 4 && g.B <= 8) {
 #define RG_HI_CASE(BB)                                                                                   \
   case BB:                                                                                               \
-    if (RL == 3) pass_launch((bm_pass<T, 3, KIND, BB, CW>), grid, block, shm, st, P);             \
-    else pass_launch((bm_pass<T, 4, KIND, BB, CW>), grid, block, shm, st, P);                     \
+    if (RL == 3) launch_bm<T, 3, KIND, BB, CW>(P, grid, block, shm, st);             \
+    else launch_bm<T, 4, KIND, BB, CW>(P, grid, block, shm, st);                     \
     break;
       switch (g.B) {
         RG_HI_CASE(4) RG_HI_CASE(5) RG_HI_CASE(6) RG_HI_CASE(7)
@@ -819,11 +852,11 @@ int launch_pass_rl(int RL, const PassArgs<T>& P, hipStream_t st) {
     }
   }
   switch (RL) {
-    case 1: pass_launch((bm_pass<T, 1, KIND>), grid, block, shm, st, P); break;
-    case 2: pass_launch((bm_pass<T, 2, KIND>), grid, block, shm, st, P); break;
-    case 3: pass_launch((bm_pass<T, 3, KIND>), grid, block, shm, st, P); break;
-    case 4: pass_launch((bm_pass<T, 4, KIND>), grid, block, shm, st, P); break;
-    case 5: pass_launch((bm_pass<T, 5, KIND>), grid, block, shm, st, P); break;
+    case 1: launch_bm<T, 1, KIND>(P, grid, block, shm, st); break;
+    case 2: launch_bm<T, 2, KIND>(P, grid, block, shm, st); break;
+    case 3: launch_bm<T, 3, KIND>(P, grid, block, shm, st); break;
+    case 4: launch_bm<T, 4, KIND>(P, grid, block, shm, st); break;
+    case 5: launch_bm<T, 5, KIND>(P, grid, block, shm, st); break;
     default: rg_set_error("bad RL %d", RL); return RG_EINVAL;
   }
   RG_LAUNCH_CHECK();
@@ -886,6 +919,7 @@ struct rumgpu_bm {
   int D = 1;                  // draw slots: every slab, p, fit and node array holds D blocks
   int* fresh = nullptr;       // per slot: state loaded since the last iteration
   int* on = nullptr;          // per slot: 1 = the iterations advance it (the draw queue idles empty slots)
+  int nt = 0;                 // non-temporal slab accesses in the fused passes
   double* dwarm = nullptr;    // 2 M float64: warm-start corrections at the observed cells
   i64 Nloc = 0, m2 = 0, M = 0, NN = 0;
   double beta = 1, alpha = 1.95, pp = 1.4142135623730951;
@@ -980,6 +1014,7 @@ static PassArgs<T> pass_args(rumgpu_bm* b, int w) {
   P.g = b->geoms[w];
   P.g.on = b->on;
   P.fresh = b->fresh;
+  P.nt = b->nt;
   return P;
 }
 
@@ -1551,6 +1586,9 @@ static int bm_create_impl(rumgpu_ctx* ctx, int n, const uint64_t* menu, const in
   b->D = slots;
   b->Nloc = 1ll << (n - 1);
   b->m2 = (i64)n * b->Nloc;
+  // a float64 object whose six iteration slabs pass 384 MiB (n >= 20; the Infinity Cache holds
+  // 256 MiB) streams them with non-temporal accesses: measured faster from n = 20 on, slower below
+  b->nt = (dtype_bytes == 8 && slots == 1 && 6 * b->m2 * 8 > (384ll << 20)) ? 1 : 0;
   b->NN = 1ll << n;
   b->M = M;
   b->beta = beta;
