@@ -30,6 +30,10 @@ namespace {
 // the first large gradient and kept until the process ends. A worker polls for the next run for
 // about a millisecond before it sleeps (a column generation's masters come a fraction of a
 // millisecond apart, and waking a sleeping thread costs about as much as one gradient).
+// set on a thread while it runs one part of a pool run that is itself a whole solve (a batch of
+// masters): that solve's gradient then stays on its thread instead of starting a run of its own
+thread_local bool tl_in_pool_run = false;
+
 class HostPool {
  public:
   static HostPool& get() {
@@ -581,7 +585,8 @@ static int qp_simplex_impl(int K, const double* Q, int64_t ldq, const double* h,
         for (int i = 0; i < k; ++i) lam_full[work[i]] = lam_w[i];
       }
       const double work_fma = (double)K * (double)std::min(k, K - k);
-      const int parts = work_fma < kParallelFma ? 1 : std::min(HostPool::get().size(), K / 128);
+      const int parts =
+          (work_fma < kParallelFma || tl_in_pool_run) ? 1 : std::min(HostPool::get().size(), K / 128);
       auto part = [&](int j0, int j1) {
         if (q32)
           grad_range<float>(j0, j1, K, k, q32, ld32, h, in.data(), work.data(), lam_w.data(), lam_full.data(), mu,
@@ -871,6 +876,27 @@ extern "C" int rumgpu_cgr_master(rumgpu_cgr* c, int max_outer, double tol, doubl
   out[0] = lql;
   out[1] = hl;
   for (int64_t r = 0; r < c->M; ++r) scores[r] = (float)(c->p[r] - c->p_out[r]);
+  return RG_OK;
+}
+
+// The masters of `count` runs with the same rows, one per host thread of the pool at a time: run i's
+// out[2 i .. 2 i + 1] and scores[i M ..] are those of rumgpu_cgr_master on that run alone, status[i]
+// its return code (the last error message belongs to one of the failed runs).
+extern "C" int rumgpu_cgr_master_batch(rumgpu_cgr* const* runs, int count, int max_outer, double tol, double reg,
+                                       double* out, float* scores, int* status) {
+  RG_CHECK(runs && count >= 0 && out && scores && status, RG_EINVAL, "cgr_master_batch: bad argument");
+  if (count == 0) return RG_OK;
+  RG_CHECK(runs[0], RG_EINVAL, "cgr_master_batch: null run");
+  const int64_t M = runs[0]->M;
+  for (int i = 0; i < count; ++i)
+    RG_CHECK(runs[i] && runs[i]->M == M, RG_EINVAL, "cgr_master_batch: runs of different rows");
+  HostPool::get().run(count, [&](int i) {
+    tl_in_pool_run = true;
+    status[i] = rumgpu_cgr_master(runs[i], max_outer, tol, reg, out + 2 * (size_t)i, scores + (size_t)i * M);
+    tl_in_pool_run = false;
+  });
+  for (int i = 0; i < count; ++i)
+    if (status[i] != RG_OK) return status[i];
   return RG_OK;
 }
 

@@ -384,10 +384,13 @@ class NativeRounds:
         _lib.check(self.lib.rumgpu_cgr_set_lambda(self.h, _lib.ptr(lv), int(lv.size), _lib.ptr(A), int(A.size)),
                    "cgr_set_lambda")
 
-    def master(self, max_outer=200, tol=1e-10, reg=1e-12):
+    MASTER = (200, 1e-10, 1e-12)      # active-set passes, KKT tolerance, ridge of the master QP
+
+    def master(self):
         """the master QP, p_out and the next scores: (lam'Q lam, h'lam, scores)"""
-        _lib.check(self.lib.rumgpu_cgr_master(self.h, int(max_outer), float(tol), float(reg), _lib.ptr(self._out),
-                                              _lib.ptr(self._scores)), "cgr_master")
+        mo, tol, reg = self.MASTER
+        _lib.check(self.lib.rumgpu_cgr_master(self.h, mo, tol, reg, _lib.ptr(self._out), _lib.ptr(self._scores)),
+                   "cgr_master")
         return float(self._out[0]), float(self._out[1]), self._scores.copy()
 
     @property
@@ -413,6 +416,51 @@ class NativeRounds:
             self.close()
         except Exception:
             pass
+
+
+class MasterCall:
+    """A request of the native loop: its round's master QP, p_out and next scores (NativeRounds.master).
+    The loop yields it instead of calling the library itself, so that a driver of many loops can run
+    their masters on host threads together (the library call releases the GIL); drive() runs it at
+    once. The reply is master()'s (lam'Q lam, h'lam, scores)."""
+    __slots__ = ("cols",)
+
+    def __init__(self, cols):
+        self.cols = cols
+
+    def run(self):
+        return self.cols.master()
+
+
+def master_batch(calls):
+    """MasterCall.run() of runs on the same rows in one library call, on the library's host threads
+    (rumgpu_cgr_master_batch; each run's result is the one it gets alone): per call (reply, None),
+    or (None, the exception its own call would raise)."""
+    if len(calls) == 1:
+        try:
+            return [(calls[0].run(), None)]
+        except Exception as e:        # noqa: BLE001 -- handed to the caller's loop
+            return [(None, e)]
+    lib = calls[0].cols.lib
+    M = calls[0].cols.M
+    hs = np.array([c.cols.h.value for c in calls], dtype=np.uint64)
+    out = np.empty((len(calls), 2))
+    sc = np.empty((len(calls), M), dtype=np.float32)
+    st = np.zeros(len(calls), dtype=np.int32)
+    mo, tol, reg = NativeRounds.MASTER
+    rc = lib.rumgpu_cgr_master_batch(_lib.ptr(hs), len(calls), mo, tol, reg, _lib.ptr(out), _lib.ptr(sc), _lib.ptr(st))
+    if rc != _lib.OK and not st.any():
+        _lib.check(rc, "cgr_master_batch")
+    res = []
+    for i in range(len(calls)):
+        if st[i] != _lib.OK:
+            try:
+                _lib.check(int(st[i]), "cgr_master")
+            except Exception as e:    # noqa: BLE001
+                res.append((None, e))
+        else:
+            res.append(((float(out[i, 0]), float(out[i, 1]), sc[i].copy()), None))
+    return res
 
 
 _NATIVE_TEMPLATES = []
@@ -459,7 +507,8 @@ def ranking_column(memb, alt_x, rank_pos, n):
 def _cg_steps_native(n, masks_x, alt_x, perm, p_obs, pricer, t_setup, rc_tol, max_iters, time_limit_s, seed, verbose,
                      phi_ub_target, gap_rel_tol, gap_abs_tol, init_columns, init_lambda, early_ub_check, lb_stop):
     """cg_steps with the host columns and the native master (NativeRounds): the same loop, requests,
-    stop tests and result; each round's host steps are one library call."""
+    stop tests and result; each round's host steps are one library call, which the loop asks its
+    driver to make (a MasterCall request before each pricing request)."""
     M = p_obs.size
     p_sq = float(p_obs @ p_obs)
     n_init = len(init_columns) if init_columns is not None else 0
@@ -488,7 +537,7 @@ def _cg_steps_native(n, masks_x, alt_x, perm, p_obs, pricer, t_setup, rc_tol, ma
                 stop = "time_limit"
                 break
             tq = time.perf_counter()
-            lQl, hl, req = cols.master()
+            lQl, hl, req = yield MasterCall(cols)
             err2 = lQl - 2.0 * hl + p_sq
             inner = hl - lQl
             qp_s += time.perf_counter() - tq
@@ -554,7 +603,9 @@ def solve_cg(n, menu, alt, p, pricer=None, layout="mask", dtype="float32", **kw)
 
 def answer(pricer, req):
     """The reply to one request of a runner: a pricing sweep for float32 scores, the float64
-    longest path for ("vmax64", r)."""
+    longest path for ("vmax64", r), the master QP for a MasterCall."""
+    if isinstance(req, MasterCall):
+        return req.run()
     if isinstance(req, tuple):
         return pricer.vmax64(req[1])
     if isinstance(req, str) and req == RESIDENT:
@@ -581,7 +632,8 @@ def cg_steps(n, menu, alt, p, pricer, rc_tol=0.0, max_iters=4000, time_limit_s=0
     resident=True (a driver that gives this loop the pricer to itself) keeps the columns of a run
     with more than Columns.DENSE_ROWS rows in the pricer's device column store (DeviceColumns) and
     yields RESIDENT instead of the scores: the column of each ranking, its Gram row, p_out and the
-    scores are then computed on the GPU."""
+    scores are then computed on the GPU. With the native master on host columns the loop also
+    yields a MasterCall before each pricing request (answer() runs it)."""
     t_setup = time.perf_counter()
     perm, masks_x, alt_x, offsets = xmajor(menu, alt, n)
     p32 = np.asarray(p, dtype=np.float64)[perm].astype(np.float32)

@@ -378,6 +378,8 @@ class BatchedCG:
     slots   draws in flight (a multiple of 32 fills the sweeps)
     overlap two halves of the slots alternate, so one half's host steps run while the GPU prices
             the other half
+    A half's master QPs (each draw's round of the native loop) run in one library call on its host
+    threads; each draw's results are the ones it gets alone.
     """
 
     def __init__(self, boot, slots=64, overlap=True, verbose=False, **_ignored):
@@ -419,6 +421,7 @@ class BatchedCG:
     def solve(self, idx, P_of, abs_tol, rel_tol, tau_lo, tau_hi, draw_time=None):
         """Yields (b, L, U, exit_flag) per draw of idx, as Bootstrap._solve records them."""
         from ..rum_solver import Result, _flow_cg_steps, _cg_pricer, _TimeUp, _Decided, _is_oom
+        from ..flow_cg.engine import MasterCall, master_batch
         idx = [int(b) for b in idx]
         if not idx:
             return
@@ -430,12 +433,16 @@ class BatchedCG:
         pricer = None
         t_all = time.perf_counter()
 
-        def step(slot, reply):
-            """advance one draw's runner; returns a finished (b, L, U, flag) or None"""
+        def step(slot, reply, exc=None):
+            """advance one draw's runner (exc: raise that in it); returns a finished (b, L, U, flag)
+            or None"""
             b, task, gen = live[slot][:3]
             msg, stop = "", None
             try:
-                live[slot][3] = next(gen) if reply is None else gen.send(reply)
+                if exc is not None:
+                    live[slot][3] = gen.throw(exc)
+                else:
+                    live[slot][3] = next(gen) if reply is None else gen.send(reply)
                 return None
             except StopIteration as e:
                 stop = e.value
@@ -472,8 +479,35 @@ class BatchedCG:
                 self.stats["draws"] += 1
                 yield done
 
+        def masters(slots):
+            """the master QPs that the loops of `slots` asked for, in one library call on host threads
+            (each loop's own sequence of requests and its results are unchanged), each result sent to
+            its loop (an error raised in it); yields the draws that finish, refilling their slots"""
+            todo = [q for q in slots if q in live and isinstance(live[q][3], MasterCall)]
+            while todo:
+                res = master_batch([live[q][3] for q in todo])
+                nxt = []
+                for q, (r, e) in zip(todo, res):
+                    done = step(q, r, e)
+                    if done is not None:
+                        self.stats["draws"] += 1
+                        yield done
+                        yield from start(q)
+                    if q in live and isinstance(live[q][3], MasterCall):
+                        nxt.append(q)
+                todo = nxt
+
+        yield from self._rounds(live, step, start, masters, pricer_of=lambda: pricer)
+        self.stats["wall_s"] += time.perf_counter() - t_all
+
+    def _rounds(self, live, step, start, masters, pricer_of):
+        """the rounds of BatchedCG.solve once its closures exist: every slot started, then batched
+        sweeps for the slots' pricing requests and their replies, until no draw is left"""
+        D = self.D
         for s in range(D):
             yield from start(s)
+        yield from masters(list(live))
+        pricer = pricer_of()
         if not self.overlap:
             while live:
                 slots = sorted(live)
@@ -487,6 +521,7 @@ class BatchedCG:
                         self.stats["draws"] += 1
                         yield done
                         yield from start(s)
+                yield from masters(slots)
                 self.stats["host_s"] += time.perf_counter() - t1
         else:
             # two halves of the slots take turns: while the GPU prices one half (a worker thread;
@@ -521,12 +556,12 @@ class BatchedCG:
                             self.stats["draws"] += 1
                             yield done
                             yield from start(s)
+                    yield from masters(slots)
                     futs[g] = launch(g)
                     self.stats["host_s"] += time.perf_counter() - t1
                     g ^= 1
             finally:
                 ex.shutdown(wait=True)
-        self.stats["wall_s"] += time.perf_counter() - t_all
 
     def _serve(self, pricer, slots, reqs):
         """Replies to the requests of `slots` (reqs: slot -> request): one batched sweep for every
