@@ -510,6 +510,49 @@ class _OraclePricer:
         return 0, 0.0
 
 
+def test_master_batch_equals_single_masters():
+    """rumgpu_cgr_master_batch (a bootstrap half's masters on host threads) gives every run the
+    result of its own rumgpu_cgr_master bit for bit, cold and warm, from 5 to 900 columns; a run that
+    fails gets its own error while the others get their results."""
+    from solver.flow_cg.engine import MasterCall, NativeRounds, master_batch, xmajor
+    rng = np.random.default_rng(5)
+    n, M = 12, 180
+    menus = set()
+    while len(menus) < M:
+        k = int(rng.integers(2, 7))
+        menus.add(int(sum(1 << int(i) for i in rng.choice(np.arange(1, n), k - 1, replace=False))) | 1)
+    menu = np.array(sorted(menus), dtype=np.uint64)
+    _perm, masks_x, alt_x, _off = xmajor(menu, np.zeros(M, dtype=np.int32), n)
+
+    def runs(K, seed, D=9):
+        r2, out = np.random.default_rng(seed), []
+        for _ in range(D):
+            r = NativeRounds(n, masks_x, alt_x, r2.uniform(0.05, 0.95, M), K + 2)
+            for _k in range(K):
+                r.add_ranking(r2.permutation(n).astype(np.int32))
+            r.set_lambda(np.array([1.0]), [0])
+            out.append(r)
+        return out
+
+    for K, D in ((5, 9), (40, 9), (900, 4)):
+        a, b = runs(K, K, D), runs(K, K, D)
+        for _rnd in range(2):               # the second master is warm from the first one's factor
+            one = [r.master() for r in a]
+            many = master_batch([MasterCall(r) for r in b])
+            for x, (y, e) in zip(one, many):
+                assert e is None and x[0] == y[0] and x[1] == y[1] and np.array_equal(x[2], y[2])
+            for r in a + b:
+                r.add_ranking(np.arange(n, dtype=np.int32)[::-1].copy())
+        for r in a + b:
+            r.close()
+    good = runs(5, 1, D=3)
+    empty = NativeRounds(n, masks_x, alt_x, np.full(M, 0.5), 4)        # no column: the master refuses
+    res = master_batch([MasterCall(good[0]), MasterCall(empty), MasterCall(good[1])])
+    assert res[0][1] is None and res[2][1] is None and isinstance(res[1][1], ValueError) and res[1][0] is None
+    for r in good + [empty]:
+        r.close()
+
+
 def test_native_rounds_reach_the_exact_projection():
     """Column generation with the library's host rounds (rumgpu_cgr_*: bitset columns, integer Gram,
     warm master QP, p_out, scores) against the numpy master, both priced by the oracle's mask DP on
