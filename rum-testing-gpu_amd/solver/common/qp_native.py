@@ -90,14 +90,19 @@ def solve_qp_numpy(Q, h, A_init=None, max_outer=200, tol=1e-10, reg=1e-12):
         idx = np.asarray(work, dtype=np.int64)
         k = idx.size
         Qw = Q[np.ix_(idx, idx)] + reg * np.eye(k)
-        b = np.column_stack([h[idx], np.ones(k)])
+        ones = np.ones(k)
         try:
-            sol = cho_solve(cho_factor(Qw, lower=True, check_finite=False), b, check_finite=False)
+            sol = cho_solve(cho_factor(Qw, lower=True, check_finite=False), np.column_stack([h[idx], ones]),
+                            check_finite=False)
+            qh, qe = sol[:, 0], sol[:, 1]
         except np.linalg.LinAlgError:
-            sol = np.linalg.lstsq(Qw, b, rcond=None)[0]
-        qh, qe = sol[:, 0], sol[:, 1]
-        se = float(qe.sum())
-        mu = (float(qh.sum()) - 1.0) / se if abs(se) > 1e-30 else 0.0
+            # one least-squares solve per right-hand side, as the reference does (its minimum-norm
+            # solutions, bit for bit)
+            qh = np.linalg.lstsq(Qw, h[idx], rcond=None)[0]
+            qe = np.linalg.lstsq(Qw, ones, rcond=None)[0]
+        # 1'x as a dot product with a ones vector (the reference's summation order)
+        se = float(ones @ qe)
+        mu = (float(ones @ qh) - 1.0) / se if abs(se) > 1e-30 else 0.0
         lam_w = qh - mu * qe
         if k and float(lam_w.min()) < -tol:
             work.pop(int(np.argmin(lam_w)))

@@ -249,9 +249,8 @@ class Columns:
         self.K = k
 
     def apply_delta(self, dlam):
-        if self.dense:
-            self.p_out += dlam[:self.K] @ self.C[:self.K]
-            return
+        """p_out += dlam_a * column a for every |dlam_a| > 1e-12, columns ascending: the reference's
+        accumulation (qp_bitset_incremental.py:127-141), so p_out and the next scores are its bits"""
         for a in np.flatnonzero(np.abs(dlam) > 1e-12):
             self.p_out[self.pos[int(a)]] += float(dlam[a])
 

@@ -510,6 +510,76 @@ class _OraclePricer:
         return 0, 0.0
 
 
+def _qp_vectors():
+    """the reference's master QP and p_out golden vectors (tests/golden/make_qp_native_vectors.py)"""
+    return np.load(os.path.join(GOLDEN, "qp_native_vectors.npz"))
+
+
+def test_master_qp_matches_the_reference_vectors():
+    """The reference's own solve_qp_native (common/qp_native.py:37-100) on 40 random and degenerate
+    masters (duplicated columns, more columns than rows, ties in h, p a mixture of two columns, warm
+    working sets valid, out of range and empty): the numpy form returns its lam and support bit for
+    bit; the library's form (a growing Cholesky factor) the same support in the same order, and lam
+    within 1e-13."""
+    from solver.common.qp_native import solve_qp_native, solve_qp_numpy
+    z = _qp_vectors()
+    assert z["K"].size == 40
+    for i, K in enumerate(z["K"]):
+        K = int(K)
+        Q = z["Q"][z["off_q"][i]:z["off_q"][i + 1]].reshape(K, K)
+        h = z["h"][z["off_h"][i]:z["off_h"][i + 1]]
+        A0 = None if z["a_none"][i] else [int(a) for a in z["a_init"][z["off_a"][i]:z["off_a"][i + 1]]]
+        lam = z["lam"][z["off_h"][i]:z["off_h"][i + 1]]
+        sup = [int(a) for a in z["support"][z["off_s"][i]:z["off_s"][i + 1]]]
+        ln, sn = solve_qp_numpy(Q, h, A_init=A0)
+        assert np.array_equal(ln, lam) and list(sn) == sup, i
+        lc, sc = solve_qp_native(Q, h, A_init=A0)
+        assert [int(a) for a in sc] == sup and np.abs(lc - lam).max() < 1e-13, (i, np.abs(lc - lam).max())
+
+
+def test_pout_accumulation_matches_the_reference_vectors():
+    """Two growing masters (12 and 20 columns) through the reference's BitsetGramMaintainer and
+    PoutMaintainer (flow_cg/qp_bitset_incremental.py:36-141), warm from the last support at every
+    step: Columns gives the same Q and h, the numpy master the same lam, and p_out (moved by the
+    change of lam, columns ascending) the same bits at all 32 steps. The library's rounds (p_out
+    summed afresh from lam over the support) stay within 1e-15 of it, lam within 1e-13."""
+    from solver.common.qp_native import solve_qp_numpy
+    from solver.flow_cg.engine import Columns, NativeRounds
+    z = _qp_vectors()
+    o_cols = o_lam = 0                      # start of the sequence in pout_cols / pout_steps, pout_lam
+    for j, (M, K) in enumerate(zip(z["pout_M"], z["pout_K"])):
+        M, K = int(M), int(K)
+        C = z["pout_cols"][o_cols:o_cols + K * M].reshape(K, M).astype(np.uint8)
+        p = z["pout_p"][z["pout_off"][j]:z["pout_off"][j + 1]]
+        cols = Columns(M, p, K)
+        nat = NativeRounds(2, np.full(M, 3, dtype=np.uint64), np.zeros(M, dtype=np.int32), p, K)
+        lam_prev, sup = np.zeros(0), None
+        for k in range(1, K + 1):
+            cols.add(C[k - 1])
+            nat.add_column(C[k - 1])
+            Qv, hv = cols.view()
+            Cf = C[:k].astype(np.float64)
+            assert np.array_equal(Qv, Cf @ Cf.T)
+            assert np.array_equal(hv, np.array([p[np.flatnonzero(c)].sum() for c in C[:k]]))
+            lam, sup = solve_qp_numpy(Qv, hv, A_init=sup)
+            ref_lam = z["pout_lam"][o_lam + (k - 1) * k // 2:][:k]
+            assert np.array_equal(lam, ref_lam), (j, k)
+            d = lam.copy()
+            d[:lam_prev.size] -= lam_prev
+            cols.apply_delta(d)
+            lam_prev = lam
+            want = z["pout_steps"][o_cols + (k - 1) * M:][:M]
+            assert np.array_equal(cols.p_out, want), (j, k)
+            if k == 1:
+                nat.set_lambda(np.array([1.0]), [0])
+            nat.master()
+            lam_n, pout_n, _c = nat.result()
+            assert np.abs(pout_n - want).max() <= 1e-15 and np.abs(lam_n - lam).max() < 1e-13, (j, k)
+        nat.close()
+        o_cols += K * M
+        o_lam += K * (K + 1) // 2
+
+
 def test_master_batch_equals_single_masters():
     """rumgpu_cgr_master_batch (a bootstrap half's masters on host threads) gives every run the
     result of its own rumgpu_cgr_master bit for bit, cold and warm, from 5 to 900 columns; a run that
