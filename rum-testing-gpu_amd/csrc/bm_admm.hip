@@ -749,15 +749,22 @@ __global__ __launch_bounds__(256) void k_lb_rows(const double* __restrict__ yv, 
 }
 
 // Final reduction of block partials: out[q] = sum over blocks of (sum + comp).
-__global__ void k_final(const double* part, int nblk, int Q, double* out) {
-  const int q = threadIdx.x;
-  if (q >= Q) return;
+// nblk <= 256 partial pairs per quantity, Q <= 2 quantities (every caller). The block's 64 threads
+// stage the partials in LDS first, so the serial compensated sums below, the same adds in the same
+// block order as a loop over global memory, do not wait on a load per step.
+__global__ __launch_bounds__(64) void k_final(const double* part, int nblk, int Q, double* out) {
+  __shared__ double sp[256 * 2 * 2];
   part += (size_t)blockIdx.x * nblk * 2 * Q;        // slot = blockIdx.x
   out += (size_t)blockIdx.x * Q;
+  const int tot = nblk * 2 * Q;
+  for (int i = threadIdx.x; i < tot; i += 64) sp[i] = part[i];
+  __syncthreads();
+  const int q = threadIdx.x;
+  if (q >= Q) return;
   double s = 0.0, c = 0.0;
   for (int b = 0; b < nblk; ++b) {
-    two_sum(s, c, part[(size_t)b * 2 * Q + 2 * q]);
-    c += part[(size_t)b * 2 * Q + 2 * q + 1];
+    two_sum(s, c, sp[b * 2 * Q + 2 * q]);
+    c += sp[b * 2 * Q + 2 * q + 1];
   }
   out[q] = s + c;
 }
