@@ -238,7 +238,7 @@ def main():
     if os.path.exists(tf) and dom_kernel:
         try:
             table = json.load(open(tf))
-            hits = [v for k, v in table.items() if k.startswith(dom_kernel)]
+            hits = [v for k, v in table.items() if _same_kernel(k, dom_kernel)]
             traffic = hits[0] if len(hits) == 1 else None
         except Exception:
             traffic = None
@@ -602,23 +602,32 @@ def _dsr_all_alternatives():
 
 def _kernel_name(kind, dtype, plan, n):
     """rocprof's name (prefix) of the kernel that runs pass `kind` (0 SY, 1 RHS, 2 D, 3 XK, 4 prox) for
-    window plan (lo_bits, lo_rl, hi_rl): bm_pass<T, RL, KIND, B, C> with the geometry fixed at compile
-    time where the launcher has that variant, else bm_pass<T, RL, KIND, 0, -1>."""
+    window plan (lo_bits, lo_rl, hi_rl): bm_pass<T, RL, KIND, B, C, NT> with the geometry fixed at
+    compile time where the launcher has that variant, else bm_pass<T, RL, KIND, 0, -1, NT>; NT is the
+    non-temporal variant of float64 objects from n = 20 on."""
     if not plan:
         return None
     T = "double" if dtype == "float64" else "float"
+    nt = "true" if (T == "double" and n >= 20) else "false"
     lo, lrl, hrl = (int(v) for v in plan)
     if kind == 4:
         return "void bm_prox_iter<%s, %d>" % (T, n)
     if kind in (0, 2):
         K = {0: 1, 2: 4}[kind]
         if lrl == 3 and 8 <= lo <= 12:
-            return "void bm_pass<%s, 3, %d, %d, 0>(PassArgs<%s>)" % (T, K, lo, T)
-        return "void bm_pass<%s, %d, %d, 0, -1>(PassArgs<%s>)" % (T, lrl, K, T)
+            return "void bm_pass<%s, 3, %d, %d, 0, %s>(PassArgs<%s>)" % (T, K, lo, nt, T)
+        return "void bm_pass<%s, %d, %d, 0, -1, %s>(PassArgs<%s>)" % (T, lrl, K, nt, T)
     K, B, CW = {1: 3, 3: 5}[kind], n - 1 - lo, (4 if T == "double" else 5)
     if hrl in (3, 4) and 4 <= B <= 8:
-        return "void bm_pass<%s, %d, %d, %d, %d>(PassArgs<%s>)" % (T, hrl, K, B, CW, T)
-    return "void bm_pass<%s, %d, %d, 0, -1>(PassArgs<%s>)" % (T, hrl, K, T)
+        return "void bm_pass<%s, %d, %d, %d, %d, %s>(PassArgs<%s>)" % (T, hrl, K, B, CW, nt, T)
+    return "void bm_pass<%s, %d, %d, 0, -1, %s>(PassArgs<%s>)" % (T, hrl, K, nt, T)
+
+
+def _same_kernel(a, b):
+    """a starts with b once the NT argument (absent from tables taken before it existed; the non-NT
+    kernels are the same code) is left out of both names"""
+    strip = lambda k: k.replace(", false>", ">").replace(", true>", ">")      # noqa: E731
+    return strip(a).startswith(strip(b))
 
 
 def _config3_bench(a):

@@ -732,7 +732,7 @@ def _bm_handles(t):
             if defaults.AUTO["bm_autotune"] and n >= defaults.AUTO["bm_autotune_min_n"]:
                 from .bm_admm.autotune import tune
                 plan, _us = tune(h, device=_machine.gpu_name())
-                t.state["info"]["window_plan"] = list(plan)
+                h.window_plan = list(plan)
                 h.init_cold()               # the timing runs moved the state
             if a["use_graph"]:
                 h.set_graph(int(a["check"]))
@@ -741,6 +741,9 @@ def _bm_handles(t):
     hs = [eng["bm"].items[d] for d in dts]
     for h in hs:
         h.set_p(t.p)
+    plan = getattr(hs[-1], "window_plan", None)
+    if plan is not None:                    # every solve on these handles reports their tuned plan
+        t.state["info"]["window_plan"] = list(plan)
     return hs
 
 
