@@ -10,7 +10,7 @@ h = C p), random and degenerate: duplicated columns, more columns than rows, tie
 working sets (valid, partly out of range, empty). Writes qp_native_vectors.npz next to this file
 (flat arrays with offsets; np.load with allow_pickle=False reads it).
 
-Run in this repository's container only (the reference is not on the GPU box):
+Run where the reference's sources are at /root/reference (the tests read only the .npz):
     python3 tests/golden/make_qp_native_vectors.py
 """
 import importlib.util
