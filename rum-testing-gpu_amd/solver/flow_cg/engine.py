@@ -435,6 +435,8 @@ def master_batch(calls):
     """MasterCall.run() of runs on the same rows in one library call, on the library's host threads
     (rumgpu_cgr_master_batch; each run's result is the one it gets alone): per call (reply, None),
     or (None, the exception its own call would raise)."""
+    if not calls:
+        return []
     if len(calls) == 1:
         try:
             return [(calls[0].run(), None)]

@@ -617,6 +617,7 @@ def test_master_batch_equals_single_masters():
             r.close()
     good = runs(5, 1, D=3)
     empty = NativeRounds(n, masks_x, alt_x, np.full(M, 0.5), 4)        # no column: the master refuses
+    assert master_batch([]) == []
     res = master_batch([MasterCall(good[0]), MasterCall(empty), MasterCall(good[1])])
     assert res[0][1] is None and res[2][1] is None and isinstance(res[1][1], ValueError) and res[1][0] is None
     for r in good + [empty]:
