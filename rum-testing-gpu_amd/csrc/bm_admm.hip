@@ -789,11 +789,12 @@ void pass_launch(F kernel, const dim3& grid, const dim3& block, size_t shm, hipS
 }
 
 // A fused pass; with P.nt the non-temporal variant, which exists for float64 on the geometries of
-// slabs too large for the caches (n >= 20: 10-13 bits at bit 0, 6-8 bits with 16 columns above, and
-// the runtime-geometry kernels); elsewhere P.nt is ignored (the results are the same either way).
+// slabs too large for the caches (8-13 bits at bit 0, 4-8 bits with 16 columns above: single draws
+// from n = 20 on and batches of draw slots; and the runtime-geometry kernels); elsewhere P.nt is
+// ignored (the results are the same either way).
 template <typename T, int RL, int KIND, int BT = 0, int CT = -1>
 void launch_bm(const PassArgs<T>& P, const dim3& grid, const dim3& block, size_t shm, hipStream_t st) {
-  if constexpr (sizeof(T) == 8 && (BT == 0 || (CT == 0 && BT >= 10) || (CT == 4 && BT >= 6))) {
+  if constexpr (sizeof(T) == 8 && (BT == 0 || (CT == 0 && BT >= 8) || (CT == 4 && BT >= 4))) {
     if (P.nt) {
       pass_launch((bm_pass<T, RL, KIND, BT, CT, true>), grid, block, shm, st, P);
       return;
@@ -1593,9 +1594,10 @@ static int bm_create_impl(rumgpu_ctx* ctx, int n, const uint64_t* menu, const in
   b->D = slots;
   b->Nloc = 1ll << (n - 1);
   b->m2 = (i64)n * b->Nloc;
-  // a float64 object whose six iteration slabs pass 384 MiB (n >= 20; the Infinity Cache holds
-  // 256 MiB) streams them with non-temporal accesses: measured faster from n = 20 on, slower below
-  b->nt = (dtype_bytes == 8 && slots == 1 && 6 * b->m2 * 8 > (384ll << 20)) ? 1 : 0;
+  // a float64 object whose six iteration slabs (all draw slots) pass the 256 MiB Infinity Cache
+  // streams them with non-temporal accesses: measured faster for one draw from n = 20 on (n = 19,
+  // 239 MB, is slower) and for 16-64 slots at n = 16 (403-1,611 MB)
+  b->nt = (dtype_bytes == 8 && 6 * b->m2 * 8 * slots > (256ll << 20)) ? 1 : 0;
   b->NN = 1ll << n;
   b->M = M;
   b->beta = beta;
