@@ -204,6 +204,12 @@ int rumgpu_cg_vmax64(rumgpu_cg* cg, const double* r_x, double* vmax);
  * counterpart: the batched bootstrap takes the final bounds of the draws that finish in one round
  * together instead of one fw_lb_rows_gpu per draw (transforms.py:101-114). */
 int rumgpu_cg_vmax64_batch(rumgpu_cg* cg, int D, const double* r, double* vmax);
+/* rumgpu_cg_price_batch for D draws and rumgpu_cg_vmax64_batch for L more, the same values, in one
+ * call: on the one-alternative window-pass sweep (D <= 32, L <= 8) the float64 maxima of the draws
+ * that finish come out of the same sweep that prices the others. No reference counterpart (the
+ * reference prices each draw and takes each draw's final bound apart, bootstrap.py:308-335). */
+int rumgpu_cg_price_batch_lb(rumgpu_cg* cg, int D, const float* scores, double* vmax, int32_t* rank_pos, int L,
+                             const double* r, double* vmax64);
 int rumgpu_cg_info(rumgpu_cg* cg, size_t* device_bytes, double* last_kernel_ms);
 /* How the mask layout sweeps (rumgpu_cg_price, _price_batch, _vmax64): mode 0 (the default) =
  * layer by layer for any rows, in the reference's order (bf_update, flow_cg/bf_pricing.py:7-47);

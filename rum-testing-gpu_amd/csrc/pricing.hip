@@ -492,16 +492,25 @@ __global__ void cl_backtrace(const u32* __restrict__ F, const u32* __restrict__ 
 #define OA_HI 8          // T bits per oa_or_hi pass
 #define OA_BT_ROWS 4096  // a's rows the path walk holds in LDS (the sweep mode is refused above)
 
-__device__ __forceinline__ void oa_w_rows(const u32* __restrict__ cm, const float* __restrict__ sc, int ra, u32 T,
-                                          bool own, float* srow, float (&w)[OA_DB]) {
+// W of this lane's T for 32 draws (w, float32) and, with L > 0, for L more draws in float64 (w64,
+// the float64 bound's row sums of the residuals r, [ra][L]): the same matches, each added in row
+// order, so w64 equals oa_wmax64's sums bit for bit
+template <int L>
+__device__ __forceinline__ void oa_w_rows(const u32* __restrict__ cm, const float* __restrict__ sc,
+                                          const double* __restrict__ r, int ra, u32 T, bool own, float* srow,
+                                          double* srow64, float (&w)[OA_DB], double (&w64)[L > 0 ? L : 1]) {
   const int tid = threadIdx.x;
 #pragma unroll
   for (int d = 0; d < OA_DB; ++d) w[d] = 0.0f;
+#pragma unroll
+  for (int l = 0; l < (L > 0 ? L : 1); ++l) w64[l] = 0.0;
   for (int base = 0; base < ra; base += OA_ROWS) {
     const int cnt = min(OA_ROWS, ra - base);
     __syncthreads();
     for (int i = tid; i < cnt * OA_DB; i += blockDim.x)
       srow[(i / OA_DB) * (OA_DB + 1) + (i % OA_DB)] = sc[(size_t)base * OA_DB + i];
+    if constexpr (L > 0)
+      for (int i = tid; i < cnt * L; i += blockDim.x) srow64[i] = r[(size_t)base * L + i];
     __syncthreads();
     for (int sub = 0; sub < cnt; sub += 32) {
       u32 mm = 0;
@@ -517,21 +526,46 @@ __device__ __forceinline__ void oa_w_rows(const u32* __restrict__ cm, const floa
         const float* sj = srow + (sub + t) * (OA_DB + 1);
 #pragma unroll
         for (int d = 0; d < OA_DB; ++d) w[d] += sj[d];
+        if constexpr (L > 0) {
+          const double* rj = srow64 + (sub + t) * L;
+#pragma unroll
+          for (int l = 0; l < L; ++l) w64[l] += rj[l];
+        }
       }
     }
   }
 }
 
-__global__ __launch_bounds__(256) void oa_wbits(const u32* __restrict__ cm, const float* __restrict__ sc, int ra,
-                                                u32 NT, u32* __restrict__ F, float* __restrict__ bmax) {
+// L > 0: also the float64 maxima of L more draws' row sums over the block's T (bmax64[block][L]),
+// the float64 bound of the draws that finish beside this sweep's pricing, on the same subset tests
+template <int L>
+__global__ __launch_bounds__(256) void oa_wbits(const u32* __restrict__ cm, const float* __restrict__ sc,
+                                                const double* __restrict__ r, int ra, u32 NT, u32* __restrict__ F,
+                                                float* __restrict__ bmax, double* __restrict__ bmax64) {
   __shared__ float srow[OA_ROWS * (OA_DB + 1)];      // the staged rows, then the block's W rows
+  __shared__ double srow64[OA_ROWS * (L > 0 ? L : 1)];
+  __shared__ double red64[4][L > 0 ? L : 1];
   __shared__ float part[8][OA_DB];
   __shared__ float bm[OA_DB];
   const int tid = threadIdx.x;
   const u32 T = (u32)blockIdx.x * 256u + (u32)tid;
   const bool own = T < NT;
   float w[OA_DB];
-  oa_w_rows(cm, sc, ra, T, own, srow, w);
+  double w64[L > 0 ? L : 1];
+  oa_w_rows<L>(cm, sc, r, ra, T, own, srow, srow64, w, w64);
+  if constexpr (L > 0) {
+    const int lane = tid & 63, wv = tid >> 6;
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+      double v = own ? w64[l] : -INFINITY;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        const double u = __shfl_xor(v, off);
+        v = u > v ? u : v;
+      }
+      if (lane == 0) red64[wv][l] = v;
+    }
+  }
   // the block's maximum per draw: W rows through LDS, thread (g, d) takes rows 32 g .. 32 g + 31
   __syncthreads();
 #pragma unroll
@@ -552,6 +586,14 @@ __global__ __launch_bounds__(256) void oa_wbits(const u32* __restrict__ cm, cons
     for (int g = 1; g < 8; ++g) m = part[g][tid] > m ? part[g][tid] : m;
     bm[tid] = m;
     bmax[(size_t)blockIdx.x * OA_DB + tid] = m;
+  }
+  if constexpr (L > 0) {
+    if (tid >= 64 && tid < 64 + L) {               // red64 was written before the syncs above
+      const int l = tid - 64;
+      double m = red64[0][l];
+      for (int q = 1; q < 4; ++q) m = red64[q][l] > m ? red64[q][l] : m;
+      bmax64[(size_t)blockIdx.x * L + l] = m;
+    }
   }
   __syncthreads();
   u32 bits = 0;
@@ -757,6 +799,25 @@ __global__ __launch_bounds__(256) void oa_wmax64(const u32* __restrict__ cm, con
 }
 
 // one block: thread (g, d) takes every (256 / DB)-th block's maximum of draw d, then LDS
+// block g of the grid: the maxima over inputs g, g + G, ... per draw (DB <= 8), out[g][DB]; a grid
+// of G blocks before oa_max64_final over many block maxima
+__global__ __launch_bounds__(256) void oa_max64_part(const double* __restrict__ in, int nin, int DB,
+                                                     double* __restrict__ out) {
+  __shared__ double red[256];
+  const int tid = threadIdx.x, G = 256 / DB, d = tid % DB, q = tid / DB;
+  double m = -INFINITY;
+  if (q < G)
+    for (int b = (int)blockIdx.x * G + q; b < nin; b += (int)gridDim.x * G)
+      m = in[(size_t)b * DB + d] > m ? in[(size_t)b * DB + d] : m;
+  red[tid] = m;
+  __syncthreads();
+  if (tid < DB) {
+    double v = red[tid];
+    for (int k = 1; k < G; ++k) v = red[k * DB + tid] > v ? red[k * DB + tid] : v;
+    out[(size_t)blockIdx.x * DB + tid] = v;
+  }
+}
+
 __global__ __launch_bounds__(256) void oa_max64_final(const double* __restrict__ bmax, int nblk, int DB,
                                                       double* __restrict__ out) {
   __shared__ double red[256];
@@ -1162,6 +1223,7 @@ struct rumgpu_cg {
   int* oaerr = nullptr;
   float* oavmax = nullptr;   // 32 draws
   double* oab64 = nullptr;   // per block and draw (8) float64 maxima
+  double* oafb64 = nullptr;  // per block of 256 T and draw (8): float64 maxima of a fused sweep
   double* oav64 = nullptr;   // 8 draws
   std::vector<i64> bounds;
   // device rows
@@ -1247,7 +1309,7 @@ static void cg_free_all(rumgpu_cg* c) {
                   c->binom, c->vbuf[0], c->vbuf[1], c->lbuf[0], c->lbuf[1], c->mbuf[0], c->mbuf[1], c->ebuf[0],
                   c->ebuf[1], c->lstep, c->subV, c->subpred, c->subnodes, c->submasks, c->subscores, c->suboffsets,
                   c->Vb, c->predb, c->scoresb, c->vmaxb, c->rankb, c->V64, c->r64, c->V64b, c->r64b, c->clcm, c->clF, c->cltmax,
-                  c->clbmax, c->clvmax, c->clbmax64, c->clv64, c->clerr, c->oaF, c->oabm, c->oagp, c->oaerr, c->oavmax, c->oab64, c->oav64, c->cbits, c->cp, c->cpout, c->cq,
+                  c->clbmax, c->clvmax, c->clbmax64, c->clv64, c->clerr, c->oaF, c->oabm, c->oagp, c->oaerr, c->oavmax, c->oab64, c->oav64, c->oafb64, c->cbits, c->cp, c->cpout, c->cq,
                   c->cidx, c->clam, c->crank, c->cpart, c->thlist, c->tlord};
   for (void* q : bufs) rg_free(q);
   for (auto* p : c->predk) rg_free(p);
@@ -1607,9 +1669,10 @@ static int oa_ready(rumgpu_cg* c) {
       (rc = cg_alloc(c, (void**)&c->oaerr, 4, "one-alternative status")) != RG_OK ||
       (rc = cg_alloc(c, (void**)&c->oavmax, (size_t)OA_DB * 4, "one-alternative vmax")) != RG_OK ||
       (rc = cg_alloc(c, (void**)&c->oab64, (size_t)2048 * 8 * 8, "float64 block maxima")) != RG_OK ||
-      (rc = cg_alloc(c, (void**)&c->oav64, (size_t)8 * 8, "float64 vmax")) != RG_OK) {
+      (rc = cg_alloc(c, (void**)&c->oav64, (size_t)8 * 8, "float64 vmax")) != RG_OK ||
+      (rc = cg_alloc(c, (void**)&c->oafb64, nblk * 8 * 8, "fused float64 block maxima")) != RG_OK) {
     void** bufs[] = {(void**)&c->oaF, (void**)&c->oabm, (void**)&c->oagp, (void**)&c->oaerr, (void**)&c->oavmax,
-                     (void**)&c->oab64, (void**)&c->oav64};
+                     (void**)&c->oab64, (void**)&c->oav64, (void**)&c->oafb64};
     for (void** q : bufs) {
       rg_free(*q);
       *q = nullptr;
@@ -1620,13 +1683,42 @@ static int oa_ready(rumgpu_cg* c) {
 }
 
 // the window-pass sweep for nd <= 32 draws whose scores of a's rows are sc ([ra][32], unused
-// columns 0): rank (nd x n) and vmax (nd floats) on the device; check oaerr afterwards
-static int oa_sweep(rumgpu_cg* c, hipStream_t st, const float* sc, int nd, int32_t* rank, float* vmax) {
+// columns 0): rank (nd x n) and vmax (nd floats) on the device; check oaerr afterwards. With L64 > 0
+// (1, 2, 4 or 8) the same sweep also takes the float64 maxima of L64 more draws whose residuals of
+// a's rows are r64 ([ra][L64], device) into oav64 (the values oa_vmax64 gives)
+static int oa_sweep(rumgpu_cg* c, hipStream_t st, const float* sc, int nd, int32_t* rank, float* vmax,
+                    const double* r64 = nullptr, int L64 = 0) {
   RG_TRY(oa_ready(c));
   const int n = c->n, nb = n - 1;
   const u32 NT = (u32)c->Nloc;
   const int nblk = (int)((NT + 255u) / 256u), g1 = std::min(256, nblk), lo = std::min(OA_LO, nb);
-  hipLaunchKernelGGL(oa_wbits, dim3(nblk), dim3(256), 0, st, c->clcm, sc, c->ra, NT, c->oaF, c->oabm);
+  switch (L64) {
+    case 0:
+      hipLaunchKernelGGL(oa_wbits<0>, dim3(nblk), dim3(256), 0, st, c->clcm, sc, nullptr, c->ra, NT, c->oaF, c->oabm,
+                         nullptr);
+      break;
+    case 1:
+      hipLaunchKernelGGL(oa_wbits<1>, dim3(nblk), dim3(256), 0, st, c->clcm, sc, r64, c->ra, NT, c->oaF, c->oabm,
+                         c->oafb64);
+      break;
+    case 2:
+      hipLaunchKernelGGL(oa_wbits<2>, dim3(nblk), dim3(256), 0, st, c->clcm, sc, r64, c->ra, NT, c->oaF, c->oabm,
+                         c->oafb64);
+      break;
+    case 4:
+      hipLaunchKernelGGL(oa_wbits<4>, dim3(nblk), dim3(256), 0, st, c->clcm, sc, r64, c->ra, NT, c->oaF, c->oabm,
+                         c->oafb64);
+      break;
+    case 8:
+      hipLaunchKernelGGL(oa_wbits<8>, dim3(nblk), dim3(256), 0, st, c->clcm, sc, r64, c->ra, NT, c->oaF, c->oabm,
+                         c->oafb64);
+      break;
+    default: rg_set_error("cg pricing: %d float64 draws in a sweep (1, 2, 4 or 8)", L64); return RG_EINVAL;
+  }
+  if (L64 > 0) {                               // block maxima -> 256 partial maxima -> the maxima
+    hipLaunchKernelGGL(oa_max64_part, dim3(g1), dim3(256), 0, st, c->oafb64, nblk, L64, c->oab64);
+    hipLaunchKernelGGL(oa_max64_final, dim3(1), dim3(256), 0, st, c->oab64, g1, L64, c->oav64);
+  }
   hipLaunchKernelGGL(oa_gmax, dim3(g1), dim3(256), 0, st, c->oabm, nblk, c->oagp);
   hipLaunchKernelGGL(oa_gmax, dim3(1), dim3(256), 0, st, c->oagp, g1, vmax);
   hipLaunchKernelGGL(oa_or_lo, dim3(NT >> lo), dim3(256), 0, st, c->oaF, c->oabm, vmax, lo);
@@ -2142,6 +2234,56 @@ extern "C" int rumgpu_cg_vmax64_batch(rumgpu_cg* c, int D, const double* r, doub
     RG_HIP(rg_copy(st, out.data(), c->V64b + full * DB, (size_t)nd * 8, hipMemcpyDeviceToHost));
     for (int d = 0; d < nd; ++d) vmax[first + d] = out[d];
   }
+  return RG_OK;
+}
+
+// rumgpu_cg_price_batch for D draws and rumgpu_cg_vmax64_batch for L more in one call; on the
+// one-alternative window-pass sweep (D <= 32, L <= 8) the float64 maxima ride in the pricing sweep,
+// which tests every subset once for both; otherwise the two calls one after the other. The values
+// are those of the two calls either way.
+extern "C" int rumgpu_cg_price_batch_lb(rumgpu_cg* c, int D, const float* scores, double* vmax, int32_t* rank_pos,
+                                        int L, const double* r, double* vmax64) {
+  RG_CHECK(c && scores && vmax && rank_pos && D >= 1 && L >= 0 && (L == 0 || (r && vmax64)), RG_EINVAL,
+           "cg_price_batch_lb: bad argument");
+  if (L == 0) return rumgpu_cg_price_batch(c, D, scores, vmax, rank_pos);
+  if (c->layout != 0 || !use_window(c) || D > CG_BATCH || L > 8) {
+    RG_TRY(rumgpu_cg_price_batch(c, D, scores, vmax, rank_pos));
+    return rumgpu_cg_vmax64_batch(c, L, r, vmax64);
+  }
+  RG_HIP(hipSetDevice(c->ctx->device));
+  hipStream_t st = c->ctx->stream;
+  const int n = c->n;
+  const i64 M = c->M;
+  auto t0 = std::chrono::steady_clock::now();
+  if (!c->scoresb) {
+    int rc = RG_OK;
+    if ((rc = cg_alloc(c, (void**)&c->scoresb, (size_t)M * CG_BATCH * 4, "batched scores")) != RG_OK ||
+        (rc = cg_alloc(c, (void**)&c->vmaxb, (size_t)CG_BATCH * 4, "batched vmax")) != RG_OK ||
+        (rc = cg_alloc(c, (void**)&c->rankb, (size_t)CG_BATCH * 64 * 4, "batched ranks")) != RG_OK) {
+      rg_free(c->scoresb), rg_free(c->vmaxb), rg_free(c->rankb);
+      c->scoresb = nullptr, c->vmaxb = nullptr, c->rankb = nullptr;
+      return rc;
+    }
+  }
+  if (!c->r64b) RG_TRY(cg_alloc(c, (void**)&c->r64b, (size_t)M * CG_LB_BATCH * 8, "batched float64 residuals"));
+  const int DL = L <= 1 ? 1 : L <= 2 ? 2 : L <= 4 ? 4 : 8;
+  std::vector<float> tr((size_t)M * OA_DB), vf(OA_DB);
+  for (i64 j = 0; j < M; ++j)
+    for (int d = 0; d < OA_DB; ++d) tr[(size_t)j * OA_DB + d] = d < D ? scores[(size_t)d * M + j] : 0.0f;
+  std::vector<double> tr64((size_t)M * DL), out64(8);
+  for (i64 j = 0; j < M; ++j)
+    for (int d = 0; d < DL; ++d) tr64[(size_t)j * DL + d] = d < L ? r[(size_t)d * M + j] : 0.0;
+  RG_HIP(hipMemcpyAsync(c->scoresb, tr.data(), (size_t)M * OA_DB * 4, hipMemcpyHostToDevice, st));
+  RG_HIP(hipMemcpyAsync(c->r64b, tr64.data(), (size_t)M * DL * 8, hipMemcpyHostToDevice, st));
+  const size_t off = (size_t)c->h_offsets[c->single_a];
+  RG_TRY(oa_sweep(c, st, c->scoresb + off * OA_DB, D, c->rankb, c->vmaxb, c->r64b + off * DL, DL));
+  RG_HIP(hipMemcpyAsync(vf.data(), c->vmaxb, (size_t)D * 4, hipMemcpyDeviceToHost, st));
+  RG_HIP(hipMemcpyAsync(rank_pos, c->rankb, (size_t)D * n * 4, hipMemcpyDeviceToHost, st));
+  RG_HIP(hipMemcpyAsync(out64.data(), c->oav64, (size_t)L * 8, hipMemcpyDeviceToHost, st));
+  RG_TRY(oa_check(c));                         // synchronizes the stream
+  for (int d = 0; d < D; ++d) vmax[d] = (double)vf[d];
+  for (int d = 0; d < L; ++d) vmax64[d] = out64[d];
+  c->last_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return RG_OK;
 }
 

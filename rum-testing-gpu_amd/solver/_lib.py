@@ -81,6 +81,7 @@ SIGNATURES = {
     "rumgpu_cg_price_batch": [_P, _I, _P, _P, _P],
     "rumgpu_cg_vmax64": [_P, _P, _P],
     "rumgpu_cg_vmax64_batch": [_P, _I, _P, _P],
+    "rumgpu_cg_price_batch_lb": [_P, _I, _P, _P, _P, _I, _P, _P],
     "rumgpu_cg_set_sweep": [_P, _I],
     "rumgpu_cg_sweep_info": [_P, _P],
     "rumgpu_cg_set_slab_tile": [_P, _I],

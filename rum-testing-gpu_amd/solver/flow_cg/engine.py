@@ -100,6 +100,18 @@ class Pricer:
                    "cg_price_batch")
         return vmax, rank
 
+    def price_batch_lb(self, scores, R):
+        """price_batch(scores) and vmax64_batch(R) in one call (the same values): on the window-pass
+        sweep the float64 maxima come out of the pricing sweep -> vmax[D], rank_pos (D, n), vmax64[L]"""
+        s = np.ascontiguousarray(scores, dtype=np.float32)
+        r = np.ascontiguousarray(R, dtype=np.float64)
+        D, L = s.shape[0], r.shape[0]
+        vmax, v64 = np.empty(D), np.empty(L)
+        rank = np.empty((D, self.n), dtype=np.int32)
+        _lib.check(self.lib.rumgpu_cg_price_batch_lb(self.h, D, _lib.ptr(s), _lib.ptr(vmax), _lib.ptr(rank), L,
+                                                     _lib.ptr(r), _lib.ptr(v64)), "cg_price_batch_lb")
+        return vmax, rank, v64
+
     def vmax64(self, r_x):
         """float64 longest path for residuals r per x-major row (mask or stream layout)"""
         r = _lib.c_array(r_x, np.float64)

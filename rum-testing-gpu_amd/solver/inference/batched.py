@@ -569,15 +569,25 @@ class BatchedCG:
         (each draw's value equals its own sweep's)."""
         get = reqs.__getitem__
         price = [q for q in slots if not isinstance(get(q), tuple)]
+        lbs = [q for q in slots if isinstance(get(q), tuple)]
         out = {}
         if price:
-            vmax, rank = pricer.price_batch(np.stack([get(q) for q in price]))
+            # up to 8 float64 bounds ride in a sweep of at most 32 draws (the values of separate calls)
+            fused = lbs[:8] if (len(price) <= 32 and pricer.layout == "mask") else []
+            if fused:
+                vmax, rank, v64 = pricer.price_batch_lb(np.stack([get(q) for q in price]),
+                                                        np.stack([get(q)[1] for q in fused]))
+                for i, q in enumerate(fused):
+                    out[q] = float(v64[i])
+                lbs = lbs[len(fused):]
+                self.stats["lb_fused"] = self.stats.get("lb_fused", 0) + len(fused)
+            else:
+                vmax, rank = pricer.price_batch(np.stack([get(q) for q in price]))
             self.stats["rounds"] += 1
             self.stats["sweeps"] += -(-len(price) // 32)
             self.stats["draw_rounds"] += len(price)
             for i, q in enumerate(price):
                 out[q] = (float(vmax[i]), rank[i].copy(), 0.0)
-        lbs = [q for q in slots if q not in out]
         if len(lbs) == 1 or (lbs and pricer.layout != "mask"):
             for q in lbs:
                 out[q] = pricer.vmax64(get(q)[1])

@@ -271,6 +271,36 @@ def test_window_sweep_equals_layers(n, a, D, quant):
     pr.close()
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,a,D,L", [(2, 1, 3, 1), (9, 4, 32, 8), (12, 0, 5, 3), (16, 15, 32, 2), (20, 0, 17, 8),
+                                     (25, 0, 32, 5)])
+def test_fused_float64_bounds_equal_separate_calls(n, a, D, L):
+    """price_batch_lb (the float64 maxima of L draws computed in the window-pass sweep that prices D
+    others, on the same subset tests) returns price_batch's vmax and rankings and vmax64_batch's
+    values bit for bit, with tied and untied scores; on the layered sweep it is the two calls"""
+    from solver.flow_cg.engine import Pricer, xmajor
+    rng = np.random.default_rng(970 + n)
+    menu, alt, _p = _medium(n, min(6000, max(2, (n << n) >> 3)), rng)
+    keep = alt == a
+    menu, alt = menu[keep], alt[keep]
+    perm, masks_x, _alts, offsets = xmajor(menu, alt, n)
+    pr = Pricer(n, masks_x, offsets, layout="mask")
+    for mode in ("window", "layers"):
+        pr.set_sweep(mode)
+        for quant in (False, True):
+            S = rng.standard_normal((D, menu.size)).astype(np.float32)
+            R64 = rng.standard_normal((L, menu.size))
+            if quant:
+                S = (np.round(S * 4) / 4).astype(np.float32)
+                R64 = np.round(R64 * 8) / 8
+            vf, rf, lf = pr.price_batch_lb(S, R64)
+            vb, rb = pr.price_batch(S)
+            lb = pr.vmax64_batch(R64)
+            assert np.array_equal(vf, vb) and np.array_equal(rf, rb) and np.array_equal(lf, lb), (mode, quant)
+            assert np.array_equal(lf, [pr.vmax64(R64[i]) for i in range(L)]), (mode, quant)
+    pr.close()
+
+
 def test_one_alternative_sweep_refused_above_n20():
     """Above n = 20 the library refuses the closure sweep (status 4) and the layered sweep stays on;
     nothing is launched: the refusal happens on the host."""
