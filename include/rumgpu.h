@@ -284,9 +284,11 @@ int rumgpu_cgr_set_lambda(rumgpu_cgr* c, const double* lam, int K, const int32_t
 int rumgpu_cgr_master(rumgpu_cgr* c, int max_outer, double tol, double reg, double* out, float* scores);
 /* the masters of `count` runs on the same rows (a batch of bootstrap draws, each draw's round of
  * Bootstrap._solve's per-draw loop, bootstrap.py:308-335) on host threads: run i's results in
- * out[2i..2i+1] and scores[i*M..], equal to rumgpu_cgr_master on that run alone; status[i] its code */
-int rumgpu_cgr_master_batch(rumgpu_cgr* const* runs, int count, int max_outer, double tol, double reg, double* out,
-                            float* scores, int* status);
+ * out[2i..2i+1] and scores[i*M..], equal to rumgpu_cgr_master on that run alone; status[i] its code.
+ * ranks (count x n) may be null; else run i with has_rank[i] first adds that ranking's column
+ * (rumgpu_cgr_add_ranking). */
+int rumgpu_cgr_master_batch(rumgpu_cgr* const* runs, int count, const int32_t* ranks, const int32_t* has_rank,
+                            int max_outer, double tol, double reg, double* out, float* scores, int* status);
 int rumgpu_cgr_get(rumgpu_cgr* c, int* K, double* lam, double* p_out, uint8_t* columns);
 
 /* ---- Learned warm start: in -> hidden -> hidden -> out MLP (ReLU), bf16 matrix-core forward and

@@ -881,18 +881,26 @@ extern "C" int rumgpu_cgr_master(rumgpu_cgr* c, int max_outer, double tol, doubl
 
 // The masters of `count` runs with the same rows, one per host thread of the pool at a time: run i's
 // out[2 i .. 2 i + 1] and scores[i M ..] are those of rumgpu_cgr_master on that run alone, status[i]
-// its return code (the last error message belongs to one of the failed runs).
-extern "C" int rumgpu_cgr_master_batch(rumgpu_cgr* const* runs, int count, int max_outer, double tol, double reg,
-                                       double* out, float* scores, int* status) {
-  RG_CHECK(runs && count >= 0 && out && scores && status, RG_EINVAL, "cgr_master_batch: bad argument");
+// its return code (the last error message belongs to one of the failed runs). With ranks (count x n,
+// may be null), a run whose has_rank[i] is set first takes that ranking's column
+// (rumgpu_cgr_add_ranking), as the column generation does before each master.
+extern "C" int rumgpu_cgr_master_batch(rumgpu_cgr* const* runs, int count, const int32_t* ranks, const int32_t* has_rank,
+                                       int max_outer, double tol, double reg, double* out, float* scores, int* status) {
+  RG_CHECK(runs && count >= 0 && out && scores && status && (!ranks || has_rank), RG_EINVAL,
+           "cgr_master_batch: bad argument");
   if (count == 0) return RG_OK;
   RG_CHECK(runs[0], RG_EINVAL, "cgr_master_batch: null run");
   const int64_t M = runs[0]->M;
+  const int n = runs[0]->n;
   for (int i = 0; i < count; ++i)
-    RG_CHECK(runs[i] && runs[i]->M == M, RG_EINVAL, "cgr_master_batch: runs of different rows");
+    RG_CHECK(runs[i] && runs[i]->M == M && runs[i]->n == n, RG_EINVAL, "cgr_master_batch: runs of different rows");
   HostPool::get().run(count, [&](int i) {
     tl_in_pool_run = true;
-    status[i] = rumgpu_cgr_master(runs[i], max_outer, tol, reg, out + 2 * (size_t)i, scores + (size_t)i * M);
+    int st = RG_OK;
+    if (ranks && has_rank[i]) st = rumgpu_cgr_add_ranking(runs[i], ranks + (size_t)i * n);
+    if (st == RG_OK)
+      st = rumgpu_cgr_master(runs[i], max_outer, tol, reg, out + 2 * (size_t)i, scores + (size_t)i * M);
+    status[i] = st;
     tl_in_pool_run = false;
   });
   for (int i = 0; i < count; ++i)

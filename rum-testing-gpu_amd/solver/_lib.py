@@ -62,7 +62,7 @@ SIGNATURES = {
     "rumgpu_cgr_copy_columns": [_P, _P],
     "rumgpu_cgr_set_lambda": [_P, _P, _I, _P, _I],
     "rumgpu_cgr_master": [_P, _I, _D, _D, _P, _P],
-    "rumgpu_cgr_master_batch": [_P, _I, _I, _D, _D, _P, _P, _P],
+    "rumgpu_cgr_master_batch": [_P, _I, _P, _P, _I, _D, _D, _P, _P, _P],
     "rumgpu_cgr_get": [_P, _P, _P, _P, _P],
     "rumgpu_bm_queue_start": [_P, _P, _I64, _I64, _I, _I, _I64, _D, _D, _I],
     "rumgpu_bm_queue_push": [_P, _I64, _P, _P, _P, _P, _P],

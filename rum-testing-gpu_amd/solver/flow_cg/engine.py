@@ -366,7 +366,7 @@ class NativeRounds:
 
     def __init__(self, n, masks_x, alt_x, p, max_K):
         self.lib = _lib.load()
-        self.M = int(np.asarray(p).size)
+        self.M, self.n = int(np.asarray(p).size), int(n)
         m = np.ascontiguousarray(masks_x, dtype=np.uint64)
         a = np.ascontiguousarray(alt_x, dtype=np.int32)
         pv = np.ascontiguousarray(p, dtype=np.float64)
@@ -430,16 +430,19 @@ class NativeRounds:
 
 
 class MasterCall:
-    """A request of the native loop: its round's master QP, p_out and next scores (NativeRounds.master).
-    The loop yields it instead of calling the library itself, so that a driver of many loops can run
-    their masters on host threads together (the library call releases the GIL); drive() runs it at
-    once. The reply is master()'s (lam'Q lam, h'lam, scores)."""
-    __slots__ = ("cols",)
+    """A request of the native loop: its round's new ranking column, if any (NativeRounds.add_ranking),
+    then the master QP, p_out and next scores (NativeRounds.master). The loop yields it instead of
+    calling the library itself, so that a driver of many loops can run these steps on host threads
+    together (the library call releases the GIL); drive() runs it at once. The reply is master()'s
+    (lam'Q lam, h'lam, scores)."""
+    __slots__ = ("cols", "rank")
 
-    def __init__(self, cols):
-        self.cols = cols
+    def __init__(self, cols, rank=None):
+        self.cols, self.rank = cols, rank
 
     def run(self):
+        if self.rank is not None:
+            self.cols.add_ranking(self.rank)
         return self.cols.master()
 
 
@@ -455,13 +458,19 @@ def master_batch(calls):
         except Exception as e:        # noqa: BLE001 -- handed to the caller's loop
             return [(None, e)]
     lib = calls[0].cols.lib
-    M = calls[0].cols.M
+    M, n = calls[0].cols.M, calls[0].cols.n
     hs = np.array([c.cols.h.value for c in calls], dtype=np.uint64)
+    has = np.array([c.rank is not None for c in calls], dtype=np.int32)
+    ranks = np.zeros((len(calls), n), dtype=np.int32)
+    for i, c in enumerate(calls):
+        if c.rank is not None:
+            ranks[i] = c.rank
     out = np.empty((len(calls), 2))
     sc = np.empty((len(calls), M), dtype=np.float32)
     st = np.zeros(len(calls), dtype=np.int32)
     mo, tol, reg = NativeRounds.MASTER
-    rc = lib.rumgpu_cgr_master_batch(_lib.ptr(hs), len(calls), mo, tol, reg, _lib.ptr(out), _lib.ptr(sc), _lib.ptr(st))
+    rc = lib.rumgpu_cgr_master_batch(_lib.ptr(hs), len(calls), _lib.ptr(ranks), _lib.ptr(has), mo, tol, reg,
+                                     _lib.ptr(out), _lib.ptr(sc), _lib.ptr(st))
     if rc != _lib.OK and not st.any():
         _lib.check(rc, "cgr_master_batch")
     res = []
@@ -545,12 +554,14 @@ def _cg_steps_native(n, masks_x, alt_x, perm, p_obs, pricer, t_setup, rc_tol, ma
         t0 = time.perf_counter()
         it = 0
         K = cols.K
+        pending = None                  # the last priced ranking, added with the next master
         for it in range(1, max_iters + 1):
             if time_limit_s > 0 and time.perf_counter() - t0 > time_limit_s:
                 stop = "time_limit"
                 break
             tq = time.perf_counter()
-            lQl, hl, req = yield MasterCall(cols)
+            lQl, hl, req = yield MasterCall(cols, pending)
+            pending = None
             err2 = lQl - 2.0 * hl + p_sq
             inner = hl - lQl
             qp_s += time.perf_counter() - tq
@@ -587,8 +598,10 @@ def _cg_steps_native(n, masks_x, alt_x, perm, p_obs, pricer, t_setup, rc_tol, ma
             if fw_gap < rc_tol:
                 stop = "rc"
                 break
-            cols.add_ranking(rank_pos)
+            pending = rank_pos
             K += 1
+        if pending is not None:         # a stop before the next master: the column is kept all the same
+            cols.add_ranking(pending)
         wall = time.perf_counter() - t0
         lam, p_out, columns = cols.result()
     finally:

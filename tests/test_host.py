@@ -582,8 +582,9 @@ def test_pout_accumulation_matches_the_reference_vectors():
 
 def test_master_batch_equals_single_masters():
     """rumgpu_cgr_master_batch (a bootstrap half's masters on host threads) gives every run the
-    result of its own rumgpu_cgr_master bit for bit, cold and warm, from 5 to 900 columns; a run that
-    fails gets its own error while the others get their results."""
+    result of its own rumgpu_cgr_master bit for bit, cold and warm, from 5 to 900 columns, with and
+    without a new ranking to add first; a run that fails gets its own error while the others get
+    their results."""
     from solver.flow_cg.engine import MasterCall, NativeRounds, master_batch, xmajor
     rng = np.random.default_rng(5)
     n, M = 12, 180
@@ -606,13 +607,17 @@ def test_master_batch_equals_single_masters():
 
     for K, D in ((5, 9), (40, 9), (900, 4)):
         a, b = runs(K, K, D), runs(K, K, D)
-        for _rnd in range(2):               # the second master is warm from the first one's factor
-            one = [r.master() for r in a]
-            many = master_batch([MasterCall(r) for r in b])
+        r3 = np.random.default_rng(K + 1)
+        for rnd in range(3):                # warm from the last factor; from the second round on, a new
+            new = [r3.permutation(n).astype(np.int32) for _ in a] if rnd else [None] * len(a)
+            one = []                        # ranking's column first (the batch adds it on its thread)
+            for r, q in zip(a, new):
+                if q is not None:
+                    r.add_ranking(q)
+                one.append(r.master())
+            many = master_batch([MasterCall(r, q) for r, q in zip(b, new)])
             for x, (y, e) in zip(one, many):
                 assert e is None and x[0] == y[0] and x[1] == y[1] and np.array_equal(x[2], y[2])
-            for r in a + b:
-                r.add_ranking(np.arange(n, dtype=np.int32)[::-1].copy())
         for r in a + b:
             r.close()
     good = runs(5, 1, D=3)
