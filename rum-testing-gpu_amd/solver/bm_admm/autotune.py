@@ -4,7 +4,8 @@ A plan (lo_bits, lo_rl, hi_rl) splits the n-1 rest bits into the window at bit 0
 windows of at most 8 bits above it; each window is one fused pass per Kronecker transform, and the register
 bits set how many rounds (LDS exchanges) a window takes. Every plan computes the same iteration up
 to rounding order. The tuner times graph-replayed iterations of each candidate near the built-in
-plan on the caller's own object and keeps the fastest; results are cached in memory and in
+plan on the caller's own object, a short run each and a full run for the fastest few, and keeps
+the fastest; results are cached in memory and in
 ~/.cache/rum-testing-gpu_amd/plans.json, keyed by device, n and dtype. The reference plans its
 butterfly passes with a calibrated DP instead (common/butterfly.py:125-195).
 """
@@ -65,8 +66,22 @@ def _save(d):
         pass
 
 
-def tune(h, device="gpu", iters=200, warm=50):
-    """The fastest plan for handle h (its n and dtype), applied to h; returns (plan, timings)."""
+def _time(h, plan, warm, iters):
+    """microseconds per graph-replayed iteration of plan on h, None where h cannot take the plan"""
+    try:
+        h.set_plan(*plan)
+    except ValueError:               # the plan exceeds the pass's thread or LDS limits
+        return None
+    h.run(warm)
+    t0 = time.perf_counter()
+    h.run(iters)
+    return (time.perf_counter() - t0) / iters * 1e6
+
+
+def tune(h, device="gpu", iters=200, warm=50, screen=(25, 50), keep=3):
+    """The fastest plan for handle h (its n and dtype), applied to h; returns (plan, timings).
+    Every candidate is screened on a short run (screen = warm, timed iterations); the `keep`
+    fastest are timed again on the full run (warm, iters), which picks the plan."""
     key = _key(device, h.n, str(h.dtype))
     if key in _MEM:
         plan = tuple(_MEM[key]["plan"])
@@ -79,22 +94,19 @@ def tune(h, device="gpu", iters=200, warm=50):
         h.set_plan(*plan)
         return plan, disk[key]["us"]
     graph = h.graph_len
-    if not graph:
-        h.set_graph(warm)
+    h.set_graph(screen[0])
     times = {}
     for plan in candidates(h.n - 1):
-        try:
-            h.set_plan(*plan)
-        except ValueError:           # the plan exceeds the pass's thread or LDS limits
-            continue
-        h.run(warm)
-        t0 = time.perf_counter()
-        h.run(iters)
-        times[plan] = (time.perf_counter() - t0) / iters * 1e6
-    best = min(times, key=times.get)
+        us = _time(h, plan, *screen)
+        if us is not None:
+            times[plan] = us
+    h.set_graph(warm)
+    finalists = sorted(times, key=times.get)[:keep]
+    for plan in finalists:
+        times[plan] = _time(h, plan, warm, iters)
+    best = min(finalists, key=times.get)
     h.set_plan(*best)
-    if not graph:
-        h.set_graph(0)
+    h.set_graph(graph)
     rec = dict(plan=list(best), us={"%d,%d,%d" % k: round(v, 2) for k, v in times.items()})
     _MEM[key] = rec
     disk = _load()

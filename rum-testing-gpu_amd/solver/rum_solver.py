@@ -726,18 +726,23 @@ def _bm_handles(t):
         if "bm" in eng:
             eng.pop("bm").close()
         items = {}
+        tuned_s = None
         for d in dts:
             h = Handle(n, g.menu, g.alternative.astype(np.int32), t.p, dtype=d, beta=a["beta"], alpha=a["alpha"],
                        precond_p=a["precond_p"])
             if defaults.AUTO["bm_autotune"] and n >= defaults.AUTO["bm_autotune_min_n"]:
                 from .bm_admm.autotune import tune
+                t_tune = time.perf_counter()
                 plan, _us = tune(h, device=_machine.gpu_name())
+                tuned_s = (tuned_s or 0.0) + time.perf_counter() - t_tune
                 h.window_plan = list(plan)
                 h.init_cold()               # the timing runs moved the state
             if a["use_graph"]:
                 h.set_graph(int(a["check"]))
             items[d] = h
         eng["bm"] = _EngineSet(key, items)
+        if tuned_s is not None:             # plan lookup (or timing, on a cache miss) inside this call
+            t.state["info"]["autotune_s"] = round(tuned_s, 3)
     hs = [eng["bm"].items[d] for d in dts]
     for h in hs:
         h.set_p(t.p)

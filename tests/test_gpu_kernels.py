@@ -430,7 +430,7 @@ def test_autotuner_picks_a_valid_plan(tmp_path, monkeypatch):
     menu, alt, p = _medium(n, 3000, rng)
     h = Handle(n, menu, alt, p, dtype="float64", beta=1.0)
     plan, us = autotune.tune(h, device="test", iters=20, warm=5)
-    assert tuple(plan) in autotune.candidates(n - 1) and len(us) >= 2
+    assert tuple(plan) in autotune.candidates(n - 1) and len(us) >= 2 and h.graph_len == 0
     assert (tmp_path / "plans.json").exists()
     # a second object of the same n and dtype takes the cached plan without timing
     h2 = Handle(n, menu, alt, p, dtype="float64", beta=1.0)

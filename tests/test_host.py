@@ -671,3 +671,47 @@ def test_native_rounds_reach_the_exact_projection():
     # the two masters follow the same trajectory (same QP; h and p_out summed in other orders)
     assert res["native"]["n_cols"] == res["numpy"]["n_cols"]
     assert abs(res["native"]["distance"] - res["numpy"]["distance"]) < 1e-12
+
+
+def test_autotuner_screens_then_times_the_fastest_few(tmp_path, monkeypatch):
+    """The tuner's selection on a handle stand-in with a modelled clock: every candidate gets a short
+    run, the three fastest a full run, and the full-run times pick the plan (a finalist that screened
+    fastest but runs slower loses; a plan the handle rejects is skipped); the graph length is restored."""
+    from solver.bm_admm import autotune
+    monkeypatch.setattr(autotune, "CACHE", str(tmp_path / "plans.json"))
+    monkeypatch.setattr(autotune, "_MEM", {})
+    clock = [0.0]
+    monkeypatch.setattr(autotune, "time", type("T", (), {"perf_counter": staticmethod(lambda: clock[0])}))
+    nb = 15
+    cands = autotune.candidates(nb)
+    cost = {c: 100.0 + i for i, c in enumerate(cands)}        # steady per-iteration microseconds
+    screen_bias = {cands[5]: -50.0, cands[7]: -40.0, cands[9]: -30.0}   # fast on the short run only
+    cost[cands[7]] = 99.0                                     # the fastest on the full run
+    rejected = cands[0]
+
+    class FakeHandle:
+        n, dtype = nb + 1, "float64"
+
+        def __init__(self):
+            self.graph_len, self.plan, self.log = 0, None, []
+
+        def set_graph(self, length):
+            self.graph_len = length
+
+        def set_plan(self, *plan):
+            if plan == rejected:
+                raise ValueError("too much LDS")
+            self.plan = plan
+
+        def run(self, iters):
+            full = self.graph_len == 50
+            self.log.append((self.plan, iters, full))
+            clock[0] += iters * 1e-6 * (cost[self.plan] + (0.0 if full else screen_bias.get(self.plan, 0.0)))
+
+    h = FakeHandle()
+    plan, us = autotune.tune(h, device="fake")
+    assert plan == cands[7] and h.plan == cands[7] and h.graph_len == 0
+    full_runs = {p for p, it, full in h.log if full and it == 200}
+    assert full_runs == {cands[5], cands[7], cands[9]}
+    assert "%d,%d,%d" % rejected not in us and len(us) == len(cands) - 1
+    assert us["%d,%d,%d" % cands[7]] == pytest.approx(99.0)
