@@ -1016,8 +1016,9 @@ __global__ void cg_fill_i32(int* a, int v, i64 m) {
 #define CG_THREADS 256
 #define CG_VPT 4
 
-// float64 value of one colex layer (sweep_max64, pricing_stream.py:190-248): the same unranking
-// and predecessor ranks as cg_stream_layer, values and row sums in float64, no choices kept.
+// float64 value of one colex layer (sweep_max64, pricing_stream.py:190-248): the same unranking,
+// predecessor ranks and block-uniform element loop as cg_stream_layer (one rank per thread), values
+// and row sums in float64, no choices kept.
 __global__ __launch_bounds__(CG_THREADS) void cg_stream_layer64(const u64* __restrict__ binom, int n, int k,
                                                               u64 count, const double* __restrict__ vprev,
                                                               double* __restrict__ vcur,
@@ -1025,36 +1026,41 @@ __global__ __launch_bounds__(CG_THREADS) void cg_stream_layer64(const u64* __res
                                                               const double* __restrict__ r,
                                                               const u64* __restrict__ offsets) {
   const u64 rk = (u64)blockIdx.x * CG_THREADS + threadIdx.x;
-  if (rk >= count) return;
   const int np1 = n + 1;
-  u64 rr = rk, D = 0;
-  int b = n - 1;
-  for (int t = k; t >= 1; --t) {
-    while (binom[b * np1 + t] > rr) --b;
-    D |= 1ull << b;
-    rr -= binom[b * np1 + t];
-    --b;
+  u64 D = 0;
+  if (rk < count) {
+    u64 rr = rk;
+    int b = n - 1;
+    for (int t = k; t >= 1; --t) {
+      while (binom[b * np1 + t] > rr) --b;
+      D |= 1ull << b;
+      rr -= binom[b * np1 + t];
+      --b;
+    }
   }
   double best = 0.0;
   bool seen = false;
-  u64 suf = 0, remain = rk, left = D;
-  for (int t = k - 1; t >= 0; --t) {
-    const int x = 63 - __clzll((long long)left);
-    left ^= 1ull << x;
-    remain -= binom[x * np1 + (t + 1)];
-    const u64 rp = remain + suf;
-    suf += binom[x * np1 + t];
+  u64 suf = 0, remain = rk;
+  int t = k - 1;
+  for (int x = n - 1; x >= 0; --x) {
     double w = 0.0;
     const u64 hi = offsets[x + 1];
+#pragma unroll 8
     for (u64 i = offsets[x]; i < hi; ++i) {
       const u64 Dp = masks[i];
       if ((Dp & D) == Dp) w += r[i];
     }
-    const double cand = vprev[rp] + w;
-    if (!seen || cand > best) best = cand;
-    seen = true;
+    if ((D >> x) & 1ull) {
+      remain -= binom[x * np1 + (t + 1)];
+      const u64 rp = remain + suf;
+      suf += binom[x * np1 + t];
+      --t;
+      const double cand = vprev[rp] + w;
+      if (!seen || cand > best) best = cand;
+      seen = true;
+    }
   }
-  vcur[rk] = best;
+  if (rk < count) vcur[rk] = best;
 }
 
 template <int MODE>
@@ -1071,6 +1077,12 @@ __device__ __forceinline__ float cg_load_prev(const void* vprev, const uint8_t* 
   }
 }
 
+// One colex layer of the row-order sweep. Each thread owns CG_VPT ranks; the element x runs from
+// n-1 down to 0 in step across the block, so the rows of alternative x (masks, scores) are the same
+// for every lane: they come through scalar loads once per wave and each is tested against the
+// thread's CG_VPT subsets in registers. A lane meets its own members in the same order as a walk
+// from its highest member down, so predecessor ranks, row-order sums and the tie rule (a later,
+// lower member wins ties) are those of that walk.
 template <int MODE>
 __global__ __launch_bounds__(CG_THREADS) void cg_stream_layer(const u64* __restrict__ binom, int n, int k, u64 count,
                                                             const void* vprev, const uint8_t* lprev, const float* mprev,
@@ -1081,53 +1093,69 @@ __global__ __launch_bounds__(CG_THREADS) void cg_stream_layer(const u64* __restr
                                                             const u64* __restrict__ offsets) {
   const u64 base = (u64)blockIdx.x * CG_BLOCK;
   const int np1 = n + 1;
+  u64 D[CG_VPT], remain[CG_VPT], suf[CG_VPT];
+  int t[CG_VPT];
   float vals[CG_VPT];
   uint8_t preds[CG_VPT];
-  float tmin = 3.0e38f, tmax = -3.0e38f;
+  bool seen[CG_VPT];
 #pragma unroll
   for (int j = 0; j < CG_VPT; ++j) {
     const u64 r = base + (u64)j * CG_THREADS + threadIdx.x;
+    D[j] = 0;
+    remain[j] = r;
+    suf[j] = 0;
+    t[j] = k - 1;
     vals[j] = 0.0f;
     preds[j] = 0;
+    seen[j] = false;
     if (r < count) {
       // colex unranking: the largest element is the largest b with C(b, t) <= r
-      u64 rr = r, D = 0;
+      u64 rr = r;
       int b = n - 1;
-      for (int t = k; t >= 1; --t) {
-        while (binom[b * np1 + t] > rr) --b;
-        D |= 1ull << b;
-        rr -= binom[b * np1 + t];
+      for (int tt = k; tt >= 1; --tt) {
+        while (binom[b * np1 + tt] > rr) --b;
+        D[j] |= 1ull << b;
+        rr -= binom[b * np1 + tt];
         --b;
       }
-      // members from the highest down; predecessor rank from prefix/suffix binomial sums
-      float best = 0.0f;
-      bool seen = false;
-      int bestx = 0;
-      u64 suf = 0, remain = r, left = D;
-      for (int t = k - 1; t >= 0; --t) {
-        const int x = 63 - __clzll((long long)left);
-        left ^= 1ull << x;
-        remain -= binom[x * np1 + (t + 1)];
-        const u64 rp = remain + suf;
-        suf += binom[x * np1 + t];
-        const float vp = cg_load_prev<MODE>(vprev, lprev, mprev, eprev, rp);
-        float w = 0.0f;
-        const u64 hi = offsets[x + 1];
-        for (u64 i = offsets[x]; i < hi; ++i) {
-          const u64 Dp = masks[i];
-          if ((Dp & D) == Dp) w += scores[i];
-        }
-        const float cand = vp + w;
-        if (!seen || cand >= best) {
-          best = cand;
-          seen = true;
-          bestx = x;
+    }
+  }
+  for (int x = n - 1; x >= 0; --x) {
+    float w[CG_VPT];
+#pragma unroll
+    for (int j = 0; j < CG_VPT; ++j) w[j] = 0.0f;
+    const u64 hi = offsets[x + 1];
+#pragma unroll 8
+    for (u64 i = offsets[x]; i < hi; ++i) {
+      const u64 Dp = masks[i];
+      const float sc = scores[i];
+#pragma unroll
+      for (int j = 0; j < CG_VPT; ++j)
+        if ((Dp & D[j]) == Dp) w[j] += sc;
+    }
+#pragma unroll
+    for (int j = 0; j < CG_VPT; ++j) {
+      if ((D[j] >> x) & 1ull) {
+        // predecessor rank from prefix/suffix binomial sums
+        remain[j] -= binom[x * np1 + (t[j] + 1)];
+        const u64 rp = remain[j] + suf[j];
+        suf[j] += binom[x * np1 + t[j]];
+        --t[j];
+        const float cand = cg_load_prev<MODE>(vprev, lprev, mprev, eprev, rp) + w[j];
+        if (!seen[j] || cand >= vals[j]) {
+          vals[j] = cand;
+          seen[j] = true;
+          preds[j] = (uint8_t)x;
         }
       }
-      vals[j] = best;
-      preds[j] = (uint8_t)bestx;
-      tmin = best < tmin ? best : tmin;
-      tmax = best > tmax ? best : tmax;
+    }
+  }
+  float tmin = 3.0e38f, tmax = -3.0e38f;
+#pragma unroll
+  for (int j = 0; j < CG_VPT; ++j) {
+    if (base + (u64)j * CG_THREADS + threadIdx.x < count) {
+      tmin = vals[j] < tmin ? vals[j] : tmin;
+      tmax = vals[j] > tmax ? vals[j] : tmax;
     }
   }
   if constexpr (MODE == 0) {
