@@ -87,9 +87,11 @@ def _parse():
     ap.add_argument("--slots-only", action="store_true", help="only the batched BM-ADMM bootstrap")
     ap.add_argument("--no-ai", action="store_true", help="skip the reference's stored AI run (n=18, 73,492 cells)")
     ap.add_argument("--pricing-sweeps", default="",
-                    help="only slab pricing sweeps on one instance (n20_d1pct, n25_sparse or config3), "
-                         "--sweeps of them: the workload of the slab pricing profile")
+                    help="only pricing sweeps on one instance (n20_d1pct, n25_sparse, config2 or config3), "
+                         "--sweeps of them: the workload of the pricing profiles")
     ap.add_argument("--sweeps", type=int, default=20)
+    ap.add_argument("--pricing-layout", default="slab",
+                    help="with --pricing-sweeps: slab, mask, or stream_f32 / stream_u24 / stream_u16")
     ap.add_argument("--slab-tile", type=int, default=0,
                     help="with --pricing-sweeps: low elements per layered-max tile (2..10; 0 = the built-in 7)")
     ap.add_argument("--summarize-stats", default="",
@@ -141,7 +143,7 @@ def main():
                 json.dump(per_kernel, f, indent=1)
         return
     if a.pricing_sweeps:
-        _pricing_sweeps(a.pricing_sweeps, a.sweeps, a.slab_tile)
+        _pricing_sweeps(a.pricing_sweeps, a.sweeps, a.slab_tile, a.pricing_layout)
         return
     if a.config3_only:
         print(json.dumps(_config3_bench(a)), flush=True)
@@ -743,21 +745,24 @@ def _ai_bench():
     return out
 
 
-def _pricing_sweeps(key, sweeps, tile_bits=0):
-    """Slab-layout pricing sweeps on one instance (random float32 scores), for a kernel profile."""
+def _pricing_sweeps(key, sweeps, tile_bits=0, layout="slab"):
+    """Pricing sweeps on one instance (random float32 scores) in one layout, for a kernel profile."""
     import numpy as np
     from solver import instances
     from solver.flow_cg.engine import Pricer, xmajor
-    if key == "config3":
-        cd, n = instances.config3()
+    lay, sdt = {"slab": ("slab", "float32"), "mask": ("mask", "float32"), "stream_f32": ("stream", "float32"),
+               "stream_u24": ("stream", "uint24"), "stream_u16": ("stream", "uint16")}[layout]
+    if key in ("config2", "config3"):
+        cd, n = getattr(instances, key)()
     else:
         n, cat = int(key.split("_")[0][1:]), key.split("_", 1)[1]
         cd, _mp = instances.get_instance(n, cat)
     menu, alt = cd[:, 0].astype(np.uint64), cd[:, 1].astype(np.int32)
     perm, masks_x, _a, offsets = xmajor(menu, alt, n)
     rng = np.random.default_rng(1)
-    pr = Pricer(n, masks_x, offsets, layout="slab")
-    pr.set_slab_tile(tile_bits)
+    pr = Pricer(n, masks_x, offsets, layout=lay, dtype=sdt)
+    if lay == "slab":
+        pr.set_slab_tile(tile_bits)
     S = rng.standard_normal((2, menu.size)).astype(np.float32)
     pr.price(S[0])
     t0 = time.perf_counter()
@@ -765,8 +770,9 @@ def _pricing_sweeps(key, sweeps, tile_bits=0):
         pr.price(S[k % 2])
     dt = (time.perf_counter() - t0) / sweeps
     pr.close()
-    print("[bench] slab pricing %s (n=%d, %d rows, tile bits %s): %.3f ms per sweep over %d sweeps"
-          % (key, n, menu.size, tile_bits or "7", dt * 1e3, sweeps), flush=True)
+    print("[bench] %s pricing %s (n=%d, %d rows%s): %.3f ms per sweep over %d sweeps"
+          % (layout, key, n, menu.size, ", tile bits %s" % (tile_bits or "7") if lay == "slab" else "",
+             dt * 1e3, sweeps), flush=True)
 
 
 def _summarize_stats(db):
